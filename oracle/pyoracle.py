@@ -1,0 +1,275 @@
+"""ctypes wrapper over the oracle libraries.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, as the checker. The product never imports
+this module.
+
+  liboracle.so  C restatement of the reference substep (xf_oracle.c)
+  libxfref.so   the unmodified reference compiled from its own sources plus
+                ref_harness.cpp (present only where it was built)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libxfref.so")
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class XoParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("gravity", C.c_float * 2),
+        ("drag", C.c_float),
+        ("energy", C.c_int),
+        ("simultaneous", C.c_int),
+        ("yeoh", C.c_float * 3),
+        ("collide", C.c_int),
+        ("ground_y", C.c_float),
+        ("friction", C.c_float),
+        ("sphere_c0", C.c_float * 3),
+        ("sphere_v", C.c_float * 3),
+        ("sphere_r", C.c_float),
+    ]
+
+
+class XrSettings(C.Structure):
+    _fields_ = [
+        ("compliance", C.c_float),
+        ("poissons_ratio", C.c_float),
+        ("gravity", C.c_float * 2),
+        ("drag", C.c_float),
+        ("energy", C.c_uint32),
+        ("simultaneous", C.c_uint32),
+    ]
+
+
+SKIN_YEOH = (0.1095, 14.95, 4.595)  # Fem.cpp:526
+
+
+def make_params(dt, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True, yeoh=SKIN_YEOH, collide=0,
+                ground_y=0.0, friction=0.0, sphere_c0=(0, 0, 0), sphere_v=(0, 0, 0), sphere_r=0.0):
+    p = XoParams()
+    p.dt = dt
+    p.gravity[:] = gravity
+    p.drag = drag
+    p.energy = energy
+    p.simultaneous = 1 if simultaneous else 0
+    p.yeoh[:] = yeoh
+    p.collide = collide
+    p.ground_y = ground_y
+    p.friction = friction
+    p.sphere_c0[:] = sphere_c0
+    p.sphere_v[:] = sphere_v
+    p.sphere_r = sphere_r
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run __graft_entry__.build())")
+        L = C.CDLL(ORACLE_SO)
+        L.xo_tet_block.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, C.c_int, _f32p, _u32p]
+        L.xo_minstd_order.argtypes = [C.c_uint32, _u32p]
+        L.xo_init_elements.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float, _f32p, _f32p, _f32p, _f32p,
+                                       _f32p]
+        L.xo_material.argtypes = [C.c_uint32, _f32p, C.c_float, C.c_void_p, C.c_float, _f32p, _f32p, _f32p]
+        L.xo_colour_greedy.argtypes = [C.c_uint32, _u32p, C.c_uint32, _u32p, _u32p]
+        L.xo_colour_greedy.restype = C.c_uint32
+        for sfx, sp in (("_ref64", _f64p), ("_f32", _f32p), ("_f64", _f64p)):
+            fn = getattr(L, "xo_substep" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), C.c_uint32, sp, sp, sp, _f32p, C.c_void_p, C.c_uint32, _u32p,
+                           _u32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+            fn = getattr(L, "xo_solve_tet" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), sp, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
+        L.xo_ldlt_solve.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
+        _lib = L
+    return _lib
+
+
+_ref = None
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            raise RuntimeError(f"reference harness missing: {REF_SO}")
+        L = C.CDLL(REF_SO)
+        L.xr_create.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float]
+        L.xr_create.restype = C.c_void_p
+        L.xr_destroy.argtypes = [C.c_void_p]
+        L.xr_vert_count.argtypes = [C.c_void_p]
+        L.xr_vert_count.restype = C.c_uint32
+        L.xr_element_count.argtypes = [C.c_void_p]
+        L.xr_element_count.restype = C.c_uint32
+        L.xr_get_order.argtypes = [C.c_void_p, _u32p]
+        L.xr_set_order.argtypes = [C.c_void_p, _u32p]
+        L.xr_set_pins.argtypes = [C.c_void_p, _u8p]
+        L.xr_get_elements.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, _f32p, _f32p]
+        L.xr_get_tets.argtypes = [C.c_void_p, _u32p]
+        L.xr_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.xr_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.xr_substeps.argtypes = [C.c_void_p, C.POINTER(XrSettings), C.c_float, C.c_uint32]
+        L.xr_solve_tet.argtypes = [C.POINTER(XrSettings), C.c_float, _f64p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+        L.xr_gen_tet_block.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                       C.c_uint32, C.c_void_p, C.c_void_p]
+        L.xr_gen_tet_block.restype = C.c_uint32
+        L.xr_ldlt.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
+        L.xr_volume.argtypes = [C.c_void_p]
+        L.xr_volume.restype = C.c_float
+        _ref = L
+    return _ref
+
+
+# ------------------------------------------------------------------ helpers
+
+def tet_block(W, H, D, scale, origin, mirrored=False):
+    n = (W + 1) * (H + 1) * (D + 1)
+    xyz = np.zeros(3 * n, np.float32)
+    tets = np.zeros(24 * W * H * D, np.uint32)
+    lib().xo_tet_block(W, H, D, np.asarray(scale, np.float32), np.asarray(origin, np.float32), int(mirrored), xyz,
+                       tets)
+    return xyz, tets
+
+
+def minstd_order(n):
+    o = np.zeros(n, np.uint32)
+    lib().xo_minstd_order(n, o)
+    return o
+
+
+def init_elements(xyz, tets, density):
+    n = xyz.size // 3
+    t = tets.size // 4
+    qi = np.zeros(9 * t, np.float32)
+    qq = np.zeros(3 * t, np.float32)
+    qr = np.zeros(3 * t, np.float32)
+    vol = np.zeros(t, np.float32)
+    w = np.zeros(n, np.float32)
+    lib().xo_init_elements(xyz, n, tets, t, density, qi, qq, qr, vol, w)
+    return dict(qi=qi, qq=qq, qr=qr, vol=vol, w=w)
+
+
+def material(vol, compliance=None, tet_mu=None, nu=0.5):
+    t = vol.size
+    c0 = np.zeros(t, np.float32)
+    c1 = np.zeros(t, np.float32)
+    a = np.zeros(t, np.float32)
+    mu_ptr = None
+    if tet_mu is not None:
+        tet_mu = np.ascontiguousarray(tet_mu, np.float32)
+        mu_ptr = tet_mu.ctypes.data
+    lib().xo_material(t, vol, 0.0 if compliance is None else compliance, mu_ptr, nu, c0, c1, a)
+    return c0, c1, a
+
+
+def colour_greedy(n_nodes, tets, visit):
+    t = tets.size // 4
+    col = np.zeros(t, np.uint32)
+    nc = lib().xo_colour_greedy(n_nodes, tets, t, np.ascontiguousarray(visit, np.uint32), col)
+    return nc, col
+
+
+class OracleBody:
+    """Node state + element constants of one body for the oracle substep."""
+
+    def __init__(self, xyz, tets, density, pinned=None, compliance=None, tet_mu=None, nu=0.5, variant="f32"):
+        self.variant = variant
+        st = np.float32 if variant == "f32" else np.float64
+        self.n = xyz.size // 3
+        self.t = tets.size // 4
+        self.tets = np.ascontiguousarray(tets, np.uint32)
+        el = init_elements(xyz, self.tets, density)
+        self.qi, self.qq, self.qr, self.vol, self.w = el["qi"], el["qq"], el["qr"], el["vol"], el["w"]
+        self.pinned = None
+        if pinned is not None:
+            self.pinned = np.ascontiguousarray(pinned, np.uint8)
+            self.w = self.w.copy()
+            self.w[self.pinned != 0] = 0.0
+        self.c0, self.c1, self.a = material(self.vol, compliance, tet_mu, nu)
+        self.X = xyz.astype(st).copy()
+        self.O = self.X.copy()
+        self.V = np.zeros_like(self.X)
+
+    def substep(self, params, order, sphere_t=0.0, count=1, dt=None):
+        fn = getattr(lib(), "xo_substep_" + self.variant)
+        pin = self.pinned.ctypes.data if self.pinned is not None else None
+        order = np.ascontiguousarray(order, np.uint32)
+        for k in range(count):
+            fn(C.byref(params), self.n, self.X, self.O, self.V, self.w, pin, self.t, order, self.tets, self.qi,
+               self.qq, self.qr, self.c0, self.c1, self.a, sphere_t + k * params.dt)
+
+
+class RefBody:
+    """The compiled reference's GeoLinear3d driven through ref_harness."""
+
+    def __init__(self, xyz, tets, density, pinned=None, order=None):
+        L = ref()
+        self.n = xyz.size // 3
+        self.t = tets.size // 4
+        self.h = L.xr_create(np.ascontiguousarray(xyz, np.float32), self.n, np.ascontiguousarray(tets, np.uint32),
+                             self.t, density)
+        L.xr_set_pins(self.h, np.ascontiguousarray(pinned if pinned is not None else np.zeros(self.n), np.uint8))
+        if order is not None:
+            L.xr_set_order(self.h, np.ascontiguousarray(order, np.uint32))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().xr_destroy(self.h)
+            self.h = None
+
+    def settings(self, compliance, nu=0.5, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True):
+        s = XrSettings()
+        s.compliance = compliance
+        s.poissons_ratio = nu
+        s.gravity[:] = gravity
+        s.drag = drag
+        s.energy = energy
+        s.simultaneous = 1 if simultaneous else 0
+        return s
+
+    def substeps(self, settings, dt, count):
+        ref().xr_substeps(self.h, C.byref(settings), dt, count)
+
+    def state(self):
+        X = np.zeros(3 * self.n)
+        V = np.zeros(3 * self.n)
+        ref().xr_get_state(self.h, X.ctypes.data, V.ctypes.data)
+        return X, V
+
+    def set_state(self, X=None, V=None):
+        Xp = np.ascontiguousarray(X, np.float64) if X is not None else None
+        Vp = np.ascontiguousarray(V, np.float64) if V is not None else None
+        ref().xr_set_state(self.h, Xp.ctypes.data if Xp is not None else None,
+                           Vp.ctypes.data if Vp is not None else None)
+
+    def elements(self):
+        qi = np.zeros(9 * self.t, np.float32)
+        qq = np.zeros(3 * self.t, np.float32)
+        qr = np.zeros(3 * self.t, np.float32)
+        vol = np.zeros(self.t, np.float32)
+        w = np.zeros(self.n, np.float32)
+        ref().xr_get_elements(self.h, qi, qq, qr, vol, w)
+        return dict(qi=qi, qq=qq, qr=qr, vol=vol, w=w)
+
+    def order(self):
+        o = np.zeros(self.t, np.uint32)
+        ref().xr_get_order(self.h, o)
+        return o

@@ -1,0 +1,310 @@
+/*
+ * xf_oracle.c -- CPU restatement of xpbd-fem's small-step XPBD substep for
+ * linear tetrahedra (reference: XPBDFEM/Geo.cpp, Fem.cpp, Xpbd.h,
+ * MeshGen.cpp, vectormath.cpp).
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: tests/, the
+ * smoke() entry and the cpu_baseline leg of bench.py may load it; the
+ * product (xpbd-fem_amd/) never links or calls it.
+ *
+ * Build: gcc -std=c11 -O2 -ffp-contract=off -fPIC -shared (see Makefile).
+ * Never build it with -ffast-math: Poisson's ratio 0.5 relies on IEEE
+ * infinity (lambda = inf -> a = 1, comp1 = 0; Fem.cpp:445-449).
+ *
+ * Pinning of each function (what checks it):
+ *   xo_tet_block      == reference GenerateTetBlock, bit-exact (test_oracle_ref)
+ *                        + closed-form counts and volumes (test_oracle)
+ *   xo_minstd_order   == reference GeoLinear3d::Init tOrder, exact
+ *   xo_init_elements  == reference InitFiniteElement Qi/QQ/QR/volume and
+ *                        w = 1/m, bit-exact; Qi*Ds^T = I and
+ *                        sum(m) = rho*V closed forms
+ *   xo_colour_greedy  validity (no shared node inside a colour) and
+ *                        determinism; no reference counterpart
+ *   xo_substep_ref64  == reference Geo3d::Substep on the same sweep order,
+ *                        bit-exact, energies 3/4/5/6/7, serial and
+ *                        simultaneous (committed golden fixtures + live)
+ *   xo_substep_f32    == GPU path bit-exact; ~= ref64 within tolerance
+ *   xo_substep_f64    error-reporting oracle; ~= ref64 within tolerance
+ *   collision pass    no reference counterpart: closed-form tests only
+ *                        (ground/sphere non-penetration, static and kinetic
+ *                        friction cases)
+ *   xo_ldlt_solve     == Ldlt.h usage example answer x = (-3, 3, 1)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct xo_params {
+    float dt;           /* substep length (1 / substepsPerSecond) */
+    float gravity[2];   /* Settings::gravity, z is always 0 (Settings.h:83) */
+    float drag;         /* Settings::timeCorrectedDrag (Geo.cpp:309) */
+    int energy;         /* 3, 4, 5, 6, 7 (Settings.h:41-53) */
+    int simultaneous;   /* Settings_XpbdSolveBit (Settings.h:15) */
+    float yeoh[3];      /* Yeoh C (Fem.cpp:526 uses 0.1095, 14.95, 4.595) */
+    int collide;        /* bit0 ground plane, bit1 kinematic sphere */
+    float ground_y;
+    float friction;
+    float sphere_c0[3];
+    float sphere_v[3];
+    float sphere_r;
+} xo_params;
+
+/* ---------------------------------------------------------------- mesh */
+
+/* GenerateHexBlock + GenerateTetBlock (MeshGen.cpp:156-244) with wonkiness 0,
+ * node (x,y,z) at origin + scale * (x,y,z), index x + y(W+1) + z(W+1)(H+1).
+ * tets: 4 indices per tet, 6 tets per hex in hex order. */
+void xo_tet_block(uint32_t W, uint32_t H, uint32_t D, const float scale[3], const float origin[3], int mirrored,
+                  float* xyz, uint32_t* tets) {
+    for (uint32_t z = 0; z <= D; z++)
+        for (uint32_t y = 0; y <= H; y++)
+            for (uint32_t x = 0; x <= W; x++) {
+                uint32_t i = x + y * (W + 1) + z * (W + 1) * (H + 1);
+                xyz[3 * i + 0] = origin[0] + scale[0] * (float)x;
+                xyz[3 * i + 1] = origin[1] + scale[1] * (float)y;
+                xyz[3 * i + 2] = origin[2] + scale[2] * (float)z;
+            }
+    static const int split[6][4] = {{0, 1, 5, 7}, {0, 7, 3, 1}, {0, 2, 3, 7}, {0, 7, 6, 2}, {0, 4, 6, 7}, {0, 7, 5, 4}};
+    uint32_t t = 0;
+    for (uint32_t z = 0; z < D; z++)
+        for (uint32_t y = 0; y < H; y++)
+            for (uint32_t x = 0; x < W; x++) {
+                uint32_t ih[8];
+                for (uint32_t k = 0; k < 2; k++)
+                    for (uint32_t j = 0; j < 2; j++)
+                        for (uint32_t i = 0; i < 2; i++) {
+                            uint32_t hi = i + j * 2 + k * 4;
+                            uint32_t xx = i, yy = j, zz = k;
+                            if (mirrored) {
+                                xx = x % 2 == 0 ? i : 1 - i;
+                                yy = y % 2 == 0 ? j : 1 - j;
+                                zz = z % 2 == 0 ? k : 1 - k;
+                                if ((x + y + z) % 2 == 1) hi = 7 - hi;
+                            }
+                            ih[hi] = (x + xx) + (y + yy) * (W + 1) + (z + zz) * (W + 1) * (H + 1);
+                        }
+                for (int s = 0; s < 6; s++) {
+                    for (int c = 0; c < 4; c++) tets[4 * t + c] = ih[split[s][c]];
+                    ++t;
+                }
+            }
+}
+
+/* Sweep order of GeoLinear3d::Init (Geo.cpp:759-771): identity, then
+ * swap(order[i], order[Rand() % n]) with MINSTD Rand, state 1. */
+void xo_minstd_order(uint32_t n, uint32_t* order) {
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    uint32_t s = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        s = (uint32_t)(((uint64_t)s * 48271u) % 0x7fffffffu);
+        uint32_t j = s % n;
+        uint32_t c = order[i];
+        order[i] = order[j];
+        order[j] = c;
+    }
+}
+
+/* ------------------------------------------------------ element setup */
+
+/* inverse(mat3) (vectormath.cpp:41-60): adjugate in double, determinant and
+ * scaling in float. m[c][r] is column c, row r. */
+static void inverse3(const float m[3][3], float out[3][3]) {
+    double m00 = m[0][0], m01 = m[0][1], m02 = m[0][2];
+    double m10 = m[1][0], m11 = m[1][1], m12 = m[1][2];
+    double m20 = m[2][0], m21 = m[2][1], m22 = m[2][2];
+    float adj[3][3];
+    adj[0][0] = (float)+(m11 * m22 - m12 * m21);
+    adj[0][1] = (float)-(m01 * m22 - m02 * m21);
+    adj[0][2] = (float)+(m01 * m12 - m02 * m11);
+    adj[1][0] = (float)-(m10 * m22 - m12 * m20);
+    adj[1][1] = (float)+(m00 * m22 - m02 * m20);
+    adj[1][2] = (float)-(m00 * m12 - m02 * m10);
+    adj[2][0] = (float)+(m10 * m21 - m11 * m20);
+    adj[2][1] = (float)-(m00 * m21 - m01 * m20);
+    adj[2][2] = (float)+(m00 * m11 - m01 * m10);
+    float det = m[0][0] * adj[0][0] + m[0][1] * adj[1][0] + m[0][2] * adj[2][0];
+    float inv = 1.0f / det;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) out[c][r] = adj[c][r] * inv;
+}
+
+/* determinant(mat3) (vectormath.cpp:34-39), cofactor form in float */
+static float det3(const float m[3][3]) {
+    float a = m[0][0], b = m[1][0], c = m[2][0];
+    float d = m[0][1], e = m[1][1], f = m[2][1];
+    float g = m[0][2], h = m[1][2], i = m[2][2];
+    return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+
+/* InitTriTetFiniteElement for T4 (Fem.cpp:197-224, 269) with
+ * GeneratePrefactoredCoefficients (Fem.cpp:132-161) using 4 quadrature
+ * points, and w = 1/m (Geo.cpp:744). Node data enters as float and the
+ * reference holds it in double (Geo.cpp:713), so P = float(double - double).
+ * Outputs per tet: qi9 (column-major Qi), qq3, qr3, vol; per node: w. */
+void xo_init_elements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                      float* qi9, float* qq3, float* qr3, float* vol, float* w) {
+    for (uint32_t i = 0; i < n_nodes; i++) w[i] = 0.0f;
+    const float vmult = 1.0f / 6.0f;
+    for (uint32_t t = 0; t < n_tets; t++) {
+        const uint32_t* is = tets + 4 * t;
+        float P[3][3];
+        for (int n = 0; n < 3; n++)
+            for (int r = 0; r < 3; r++) P[n][r] = (float)((double)xyz[3 * is[n] + r] - (double)xyz[3 * is[3] + r]);
+        float J[3][3]; /* MatMul(dN, P): column k = (P0[k], P1[k], P2[k]) */
+        for (int k = 0; k < 3; k++)
+            for (int n = 0; n < 3; n++) J[k][n] = P[n][k];
+        float Qi[3][3];
+        inverse3(J, Qi);
+        /* prefactored coefficients, 4 identical points for a linear tet */
+        float QQ[3] = {0.0f, 0.0f, 0.0f}, QR[3] = {0.0f, 0.0f, 0.0f};
+        float volume = 0.0f;
+        for (int p = 0; p < 4; p++) {
+            float Ji[3][3];
+            inverse3(J, Ji);
+            float wq = det3(J) * vmult * 0.25f;
+            volume += wq;
+            for (int i = 0; i < 3; i++) {
+                const float* qa = Ji[i];
+                QQ[i] += wq * (qa[0] * qa[0] + qa[1] * qa[1] + qa[2] * qa[2]);
+            }
+            int head = 0;
+            for (int i = 0; i < 3; i++)
+                for (int j = i + 1; j < 3; j++) {
+                    const float* qa = Ji[i];
+                    const float* qb = Ji[j];
+                    QR[head++] += wq * 2.0f * (qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2]);
+                }
+        }
+        for (int i = 0; i < 3; i++) { QQ[i] /= volume; QR[i] /= volume; }
+        float v = vmult * det3(J);
+        for (int c = 0; c < 3; c++)
+            for (int r = 0; r < 3; r++) qi9[9 * t + 3 * c + r] = Qi[c][r];
+        for (int i = 0; i < 3; i++) { qq3[3 * t + i] = QQ[i]; qr3[3 * t + i] = QR[i]; }
+        vol[t] = v;
+        for (int n = 0; n < 4; n++) w[is[n]] += 0.25f * density * v;
+    }
+    for (uint32_t i = 0; i < n_nodes; i++) w[i] = 1.0f / w[i];
+}
+
+/* Material constants of SolveElementMixed (Fem.cpp:445-449):
+ * mu = 1/compliance; lambda = 2 mu nu / (1 - 2 nu); a = 1 + mu/lambda;
+ * comp = {1/mu/V, 1/lambda/V}. tet_mu (nullable) gives mu per tet instead. */
+void xo_material(uint32_t n_tets, const float* vol, float compliance, const float* tet_mu, float nu, float* comp0,
+                 float* comp1, float* a) {
+    for (uint32_t t = 0; t < n_tets; t++) {
+        float mu = tet_mu ? tet_mu[t] : 1.0f / compliance;
+        float lambda = (2.0f * mu * nu) / (1.0f - 2.0f * nu);
+        a[t] = 1.0f + mu / lambda;
+        comp0[t] = 1.0f / mu / vol[t];
+        comp1[t] = 1.0f / lambda / vol[t];
+    }
+}
+
+/* ------------------------------------------------------------ colouring */
+
+/* Greedy colouring: tets visited in `visit` order, each takes the smallest
+ * colour not yet used by any of its 4 nodes (at most 256 colours). Returns
+ * the colour count, or 0 if more than 256 would be needed. */
+uint32_t xo_colour_greedy(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, const uint32_t* visit,
+                          uint32_t* colour) {
+    uint64_t* mask = (uint64_t*)calloc((size_t)n_nodes * 4, sizeof(uint64_t));
+    uint32_t ncol = 0;
+    for (uint32_t k = 0; k < n_tets; k++) {
+        uint32_t t = visit[k];
+        const uint32_t* is = tets + 4 * t;
+        uint32_t c = 256;
+        for (int word = 0; word < 4 && c == 256; word++) {
+            uint64_t used = mask[4 * is[0] + word] | mask[4 * is[1] + word] | mask[4 * is[2] + word] |
+                            mask[4 * is[3] + word];
+            if (~used) {
+                uint64_t freeb = ~used;
+                uint32_t b = 0;
+                while (!(freeb & 1)) { freeb >>= 1; b++; }
+                c = 64 * word + b;
+            }
+        }
+        if (c == 256) { free(mask); return 0; }
+        colour[t] = c;
+        for (int n = 0; n < 4; n++) mask[4 * is[n] + c / 64] |= (uint64_t)1 << (c % 64);
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    free(mask);
+    return ncol;
+}
+
+/* --------------------------------------------------------------- substeps */
+
+#define STATE double
+#define REAL float
+#define RC(x) ((float)(x))
+#define FMAX fmaxf
+#define SQRT sqrtf
+#define SFX _ref64
+#include "xo_substep.inc"
+#undef STATE
+#undef REAL
+#undef RC
+#undef FMAX
+#undef SQRT
+#undef SFX
+
+#define STATE float
+#define REAL float
+#define RC(x) ((float)(x))
+#define FMAX fmaxf
+#define SQRT sqrtf
+#define SFX _f32
+#include "xo_substep.inc"
+#undef STATE
+#undef REAL
+#undef RC
+#undef FMAX
+#undef SQRT
+#undef SFX
+
+#define STATE double
+#define REAL double
+#define RC(x) ((double)(x))
+#define FMAX fmax
+#define SQRT sqrt
+#define SFX _f64
+#include "xo_substep.inc"
+#undef STATE
+#undef REAL
+#undef RC
+#undef FMAX
+#undef SQRT
+#undef SFX
+
+/* ----------------------------------------------------------- small solve */
+
+/* LDLTDecomposeAndSolve (Ldlt.h:35-95): packed lower-triangular symmetric
+ * A (row i holds i+1 entries), no pivoting. Used only as a host utility. */
+void xo_ldlt_solve(uint32_t n, const float* A, const float* b, float* x) {
+    float L[25 * 25];
+    float D[25];
+    uint32_t r[26];
+    r[0] = 0;
+    for (uint32_t i = 1; i <= n; i++) r[i] = r[i - 1] + i;
+    /* row offset of packed row i is i(i+1)/2; L uses the same packing */
+    for (uint32_t j = 0; j < n; j++) {
+        D[j] = A[r[j] + j];
+        for (uint32_t k = 0; k < j; k++) D[j] -= L[r[j] + k] * L[r[j] + k] * D[k];
+        for (uint32_t i = j + 1; i < n; i++) {
+            L[r[i] + j] = A[r[i] + j];
+            for (uint32_t k = 0; k < j; k++) L[r[i] + j] -= L[r[i] + k] * L[r[j] + k] * D[k];
+            L[r[i] + j] /= D[j];
+        }
+    }
+    float invD[25];
+    for (uint32_t j = 0; j < n; j++) invD[j] = 1.0f / D[j];
+    /* LDLTSolve (Ldlt.h:66-86): forward, scale by 1/D, backward from the end */
+    for (uint32_t i = 0; i < n; i++) {
+        x[i] = b[i];
+        for (uint32_t k = 0; k < i; k++) x[i] -= L[r[i] + k] * x[k];
+    }
+    for (uint32_t i = 0; i < n; i++) x[i] *= invD[i];
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < i; j++) x[n - 1 - i] -= L[r[n - 1 - j] + n - 1 - i] * x[n - 1 - j];
+}
