@@ -1,0 +1,137 @@
+"""Benchmark: tet-substeps/s of the XPBD substep on one MI355X.
+
+Workload: the 96x24x24-cell Yeoh beam (331,776 linear tets, 60,625 nodes),
+1/60 s frames of 30 substeps, ground plane and kinematic sphere, inputs
+resident in HBM. One step = one frame (30 substeps, captured as one hipGraph).
+
+Prints one JSON line: value = whole-job tet-substeps/s, the roofline of the
+projection kernels (algorithmic bytes per launch / measured launch time,
+HIP events on the library stream), and the reference CPU path timed on a
+bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "xpbd-fem_amd"), os.path.join(ROOT, "oracle")]
+
+import workloads  # noqa: E402
+import xpbdfem as xf  # noqa: E402
+
+METRIC = "tet-substeps/s plain vs batched at ≤1e-4 rel pos err; adjoint:forward time"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per tet projection with the reference's precision (FP64
+# positions, FP32 inverse mass and constants): node ids 16 + constants (Qi 36,
+# QQ 12, QR 12, material 4) 64 + gather 4 x (xyz FP64 24 + w 4) 112 +
+# scatter 4 x xyz FP64 96.
+BYTES_PER_TET = 16 + 64 + 112 + 96
+# Per node and substep (predict + collisions + velocity): read X, V; write X, V (FP64).
+BYTES_PER_NODE = 96
+
+
+def cpu_baseline(wl, budget_s=12.0):
+    """The reference's own Geo3d::Substep (oracle/_ref, compiled from the
+    reference sources) on the same mesh, pins and velocities, 1 thread,
+    YeohSkinFast; falls back to the oracle's reference-precision port."""
+    import pyoracle as po
+
+    xyz, tets = wl["xyz"], wl["tets"]
+    t = tets.size // 4
+    dt = np.float32(wl["dt"])
+    s = wl["settings"]
+    if po.have_ref():
+        body = po.RefBody(xyz, tets, wl["density"], wl["pinned"])
+        body.set_state(X=xyz.astype(np.float64), V=wl["velocity"].astype(np.float64))
+        st = body.settings(s["compliance"], nu=s["poissons_ratio"], energy=7, simultaneous=True)
+        run = lambda k: body.substeps(st, dt, k)  # noqa: E731
+        kind, what = "reference", "reference Geo3d::Substep (no collision pass: the reference has none)"
+    else:
+        ob = po.body_from_workload(wl, variant="ref64")
+        order = po.minstd_order(t)
+        p = po.params_from_settings(dt, s)
+        run = lambda k: ob.substep(p, order, count=k)  # noqa: E731
+        kind, what = "port", "oracle reference-precision restatement"
+    run(1)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        run(2)
+        n += 2
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return dict(value=t * n / el, unit="tet-substeps/s", cores=1, kind=kind,
+                sample=f"{what}: {n} substeps of the {t}-tet beam, single thread, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
+              file=sys.stderr)
+        sys.exit(2)
+
+    wl = workloads.beam_yeoh_96x24x24()
+    ctx = xf.Context(0)
+    sc = workloads.make_scene(ctx, wl, use_graph=True)
+    info = sc.info()
+    T, N, S = info["n_tets"], info["n_nodes"], wl["substeps"]
+    dt = float(wl["dt"])
+
+    for _ in range(args.warmup):
+        sc.step(S, dt)
+    sc.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sc.step(S, dt)
+    sc.sync()
+    el = time.perf_counter() - t0
+    bad = sc.count_nonfinite()
+    if bad:
+        print(f"non-finite positions after the run: {bad}", file=sys.stderr)
+        sys.exit(1)
+
+    # Per-launch device time of the projection kernels (HIP events on the
+    # library stream, eager launches of one frame).
+    ms_proj, ms_node, n_launch = sc.profile(S, dt)
+    bytes_proj = BYTES_PER_TET * T * S
+    achieved = bytes_proj / (ms_proj * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": T * S * args.steps / el,
+        "unit": "tet-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 element math, fp64 node state",
+        "data": "synthetic",
+        "config": {"workload": f"{wl['name']}: {T} tets, {N} nodes, {S} substeps/frame, {info['n_colours']} "
+                               f"colours, ground + sphere", "step": "one frame"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "xf_project_colour", "launches": n_launch,
+                     "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
+                     "bytes_per_launch": bytes_proj / max(n_launch, 1)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,212 @@
+/*
+ * xpbdfem.h -- C ABI of the MI355X (gfx950) small-step XPBD substep for
+ * linear tetrahedra (Neo-Hookean / Yeoh, incompressible mixed form).
+ *
+ * The reference (jak-xyz/xpbd-fem) has no C ABI for this path: its step
+ * API is the C++ virtual interface Geo::Substep / Constrain / Damp
+ * (Geo.h:15-35) over GeoLinear3d::Init (Geo.h:147, Geo.cpp:697-772), with
+ * the per-element entry SolveElement (Fem.h:124, Fem.cpp:892). The entry
+ * points below replace those; the C++ adapter in
+ * xpbd-fem_amd/csrc/xf_geo_gpu.h keeps the Geo3d-shaped interface on top.
+ *
+ * Conventions: plain pointers and sizes, host buffers are caller-owned and
+ * copied, device memory is library-owned. Every call returns an xf_status;
+ * no call aborts or throws. xf_last_error() gives the message of the last
+ * failing call on a context. A context owns one HIP stream and is not
+ * thread-safe; separate contexts may be used from separate threads.
+ */
+#ifndef XPBDFEM_H
+#define XPBDFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XF_API __attribute__((visibility("default")))
+
+typedef enum xf_status {
+    XF_OK = 0,
+    XF_E_INVALID = -1,     /* bad argument or mesh */
+    XF_E_HIP = -2,         /* HIP runtime error */
+    XF_E_NOMEM = -3,       /* device or host allocation failed */
+    XF_E_STATE = -4,       /* call not valid in the object's current state */
+    XF_E_UNSUPPORTED = -5, /* option outside the implemented scope */
+    XF_E_NONFINITE = -6    /* NaN/Inf detected in the node state */
+} xf_status;
+
+/* Energy ids (Settings.h:41-53); the linear-tet subset of SolveElementMixed
+ * (Fem.cpp:437-564). Pixar (0-2), continuous (8-10) and cube (11-12)
+ * energies are not implemented (XF_E_UNSUPPORTED). */
+enum {
+    XF_ENERGY_MIXED = 3,
+    XF_ENERGY_MIXED_SEL = 4,
+    XF_ENERGY_YEOH_SKIN = 5,
+    XF_ENERGY_YEOH_SKIN_SEL = 6,
+    XF_ENERGY_YEOH_SKIN_FAST = 7
+};
+
+/* Collision flags (no reference counterpart: the reference has no
+ * collision code). */
+enum { XF_COLLIDE_GROUND = 1, XF_COLLIDE_SPHERE = 2 };
+
+/* Colouring visit orders for the greedy node-disjoint tet colouring. */
+enum {
+    XF_COLOUR_NATURAL = 0, /* tets in input order */
+    XF_COLOUR_MINSTD = 1   /* the reference's MINSTD sweep order (Geo.cpp:759-771) */
+};
+
+typedef struct xf_ctx xf_ctx;
+typedef struct xf_scene xf_scene;
+
+/* Per-body runtime settings: the hot-path fields of struct Settings
+ * (Settings.h:79-102) plus Yeoh coefficients and collisions.
+ *   energy, simultaneous  <- flags bits 6-10 and bit 11
+ *   compliance            <- Settings::compliance (mu = 1/compliance,
+ *                            Fem.cpp:445); ignored for tets with a per-tet mu
+ *   poissons_ratio        <- Settings::poissonsRatio (0.5 = incompressible)
+ *   gravity[2]            <- Settings::gravity (z is always 0)
+ *   drag                  <- Settings::timeCorrectedDrag (velocity scale
+ *                            1 - drag per substep, Geo.cpp:309)
+ *   yeoh[3]               Yeoh C used by energies 5/6/7; the reference
+ *                         hard-codes {0.1095, 14.95, 4.595} (Fem.cpp:471,526).
+ *                         A Yeoh material C10, C20, C30 maps to
+ *                         compliance = 1/C10, yeoh = {1, C20/C10, C30/C10}.
+ * damping and pbd_damping must be 0 (Rayleigh / PBD damping passes,
+ * Geo.cpp:790-811, are not implemented). volume_passes must be 0. */
+typedef struct xf_settings {
+    uint32_t energy;
+    uint32_t simultaneous;
+    float compliance;
+    float poissons_ratio;
+    float gravity[2];
+    float drag;
+    float damping;
+    float pbd_damping;
+    uint32_t volume_passes;
+    float yeoh[3];
+    uint32_t collide;      /* XF_COLLIDE_* bits */
+    float ground_y;        /* plane y = ground_y, normal +y */
+    float friction;        /* Coulomb coefficient (static = kinetic) */
+    float sphere_c0[3];    /* sphere centre at simulation time 0 */
+    float sphere_v[3];     /* sphere velocity (kinematic) */
+    float sphere_r;
+} xf_settings;
+
+/* Fills defaults: MixedSel, simultaneous, compliance 0.01, nu 0.45,
+ * gravity (0, -9.81), skin Yeoh constants, no collisions (the
+ * Settings.h:79-92 defaults with the solve bit set). */
+XF_API void xf_settings_default(xf_settings* s);
+
+/* One body: a linear-tet mesh. Mirrors GeoLinear3d::Init (Geo.cpp:697):
+ * xyz[3*n_nodes], tets[4*n_tets] (the index stream [4,i0..i3]* of the
+ * reference without the leading 4s), density -> lumped masses rho*V/4
+ * (Fem.cpp:217-223), w = 1/m (Geo.cpp:744).
+ * pinned[n_nodes] (nullable): explicit pins with the Settings_LockLeft
+ * semantics (X reset to O after the sweep, w = 0 from the first substep).
+ * velocity[3*n_nodes] (nullable): initial velocity, else 0.
+ * tet_mu[n_tets] (nullable): per-tet shear modulus mu (compliance 1/mu);
+ * overrides settings.compliance for this body. */
+typedef struct xf_body_desc {
+    const float* xyz;
+    uint32_t n_nodes;
+    const uint32_t* tets;
+    uint32_t n_tets;
+    float density;
+    const uint8_t* pinned;
+    const float* velocity;
+    const float* tet_mu;
+} xf_body_desc;
+
+typedef struct xf_scene_opts {
+    uint32_t colour_order;   /* XF_COLOUR_* */
+    uint32_t block_size;     /* projection workgroup size, 0 = default (64) */
+    uint32_t use_graph;      /* capture each xf_scene_step call shape in a hipGraph */
+} xf_scene_opts;
+
+typedef struct xf_scene_info {
+    uint32_t n_bodies;
+    uint32_t n_nodes;
+    uint32_t n_tets;
+    uint32_t n_colours;      /* colour phases per substep (max over bodies) */
+    uint64_t device_bytes;   /* device memory held by the scene */
+} xf_scene_info;
+
+/* ------------------------------------------------------------ context */
+XF_API int xf_ctx_create(int device, xf_ctx** out);
+XF_API void xf_ctx_destroy(xf_ctx* ctx);
+XF_API const char* xf_last_error(const xf_ctx* ctx);
+XF_API const char* xf_version(void);
+
+/* -------------------------------------------------------------- scene */
+XF_API int xf_scene_create(xf_ctx* ctx, xf_scene** out);
+XF_API void xf_scene_destroy(xf_scene* scene);
+/* Adds a body (before xf_scene_finalize). *body_id receives its index. */
+XF_API int xf_scene_add_body(xf_scene* scene, const xf_body_desc* desc, uint32_t* body_id);
+/* Element constants, colouring, device layout and upload. opts nullable. */
+XF_API int xf_scene_finalize(xf_scene* scene, const xf_scene_opts* opts);
+XF_API int xf_scene_info_get(const xf_scene* scene, xf_scene_info* info);
+/* Runtime settings of one body (any time after add_body). */
+XF_API int xf_scene_set_settings(xf_scene* scene, uint32_t body, const xf_settings* s);
+/* Replaces the per-tet mu of a body (n_tets values). */
+XF_API int xf_scene_set_tet_mu(xf_scene* scene, uint32_t body, const float* tet_mu);
+
+/* Advances every body by `substeps` calls of the substep of Geo3d::Substep
+ * (Geo.cpp:305-356): predict, tet sweep in colour order, pin, collisions,
+ * velocity. dt is the substep length (Demo.cpp:39: 1/substepsPerSecond).
+ * Returns when the work is enqueued; xf_scene_sync waits for it. */
+XF_API int xf_scene_step(xf_scene* scene, uint32_t substeps, float dt);
+XF_API int xf_scene_sync(xf_scene* scene);
+
+/* Node state of one body: x[3*n], v[3*n] (either nullable). The state is
+ * FP64 like the reference's dvec3 X, V (Geo.h:61-64). Synchronous. */
+XF_API int xf_scene_get_state(xf_scene* scene, uint32_t body, double* x, double* v);
+XF_API int xf_scene_set_state(xf_scene* scene, uint32_t body, const double* x, const double* v);
+/* Simulation clock (seconds) that drives kinematic colliders. */
+XF_API int xf_scene_get_time(const xf_scene* scene, float* t);
+XF_API int xf_scene_set_time(xf_scene* scene, float t);
+
+/* The sweep order of one body: its tets (body-local ids) in colour order.
+ * Sweeping this order serially reproduces the GPU result. */
+XF_API int xf_scene_get_order(const xf_scene* scene, uint32_t body, uint32_t* order, uint32_t* colour_of_tet);
+/* Element constants of one body as computed by InitFiniteElement
+ * (Fem.cpp:269): qi9 column-major inverse rest map, qq3, qr3, vol per tet,
+ * w per node (0 where pinned). Any pointer may be null. */
+XF_API int xf_scene_get_elements(const xf_scene* scene, uint32_t body, float* qi9, float* qq3, float* qr3,
+                                 float* vol, float* w);
+
+/* Runs `substeps` substeps eagerly with a HIP event pair around every
+ * launch and reports device time: ms_project = sum over projection
+ * kernels, ms_node = sum over node kernels, launches_project = number of
+ * projection launches. */
+XF_API int xf_scene_profile(xf_scene* scene, uint32_t substeps, float dt, float* ms_project, float* ms_node,
+                            uint32_t* launches_project);
+/* Counts non-finite node positions (0 = all finite). */
+XF_API int xf_scene_count_nonfinite(xf_scene* scene, uint32_t* count);
+
+/* ---------------------------------------------------- host utilities */
+/* Linear-tet block generator (GenerateHexBlock + GenerateTetBlock,
+ * MeshGen.cpp:156-244, wonkiness 0): node (x,y,z) at origin + scale*(x,y,z),
+ * index x + y(W+1) + z(W+1)(H+1); 6 tets per cell. xyz[3*(W+1)(H+1)(D+1)],
+ * tets[24*W*H*D]. mirrored selects Pattern_Mirrored. */
+XF_API int xf_mesh_tet_block(uint32_t W, uint32_t H, uint32_t D, const float scale[3], const float origin[3],
+                             uint32_t mirrored, float* xyz, uint32_t* tets);
+/* The reference's MINSTD sweep order for n tets (Geo.cpp:759-771). */
+XF_API int xf_minstd_order(uint32_t n, uint32_t* order);
+/* n normal deviates N(0, sigma^2) from SplitMix64(seed) + Box-Muller. */
+XF_API int xf_normal_fill(uint64_t seed, float sigma, uint32_t n, float* out);
+/* The greedy node-disjoint colouring xf_scene_finalize applies, on the host:
+ * order[n_tets] = tets in sweep order (colour-major, visit order inside a
+ * colour), colour[n_tets] (nullable), *n_colours. No reference counterpart:
+ * the reference sweeps its MINSTD order serially (Geo.cpp:759-777). */
+XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, uint32_t colour_order,
+                          uint32_t* order, uint32_t* colour, uint32_t* n_colours);
+/* LDLTDecomposeAndSolve (Ldlt.h:88-95): packed lower-triangular A, n <= 25. */
+XF_API int xf_ldlt_solve(uint32_t n, const float* A, const float* b, float* x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPBDFEM_H */

@@ -92,6 +92,8 @@ def lib():
             fn = getattr(L, "xo_substep" + sfx)
             fn.argtypes = [C.POINTER(XoParams), C.c_uint32, sp, sp, sp, _f32p, C.c_void_p, C.c_uint32, _u32p,
                            _u32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+            fn = getattr(L, "xo_collide" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), C.c_float, sp, sp]
             fn = getattr(L, "xo_solve_tet" + sfx)
             fn.argtypes = [C.POINTER(XoParams), sp, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
         L.xo_ldlt_solve.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
@@ -190,7 +192,8 @@ def colour_greedy(n_nodes, tets, visit):
 class OracleBody:
     """Node state + element constants of one body for the oracle substep."""
 
-    def __init__(self, xyz, tets, density, pinned=None, compliance=None, tet_mu=None, nu=0.5, variant="f32"):
+    def __init__(self, xyz, tets, density, pinned=None, compliance=None, tet_mu=None, nu=0.5, variant="f32",
+                 velocity=None):
         self.variant = variant
         st = np.float32 if variant == "f32" else np.float64
         self.n = xyz.size // 3
@@ -207,14 +210,39 @@ class OracleBody:
         self.X = xyz.astype(st).copy()
         self.O = self.X.copy()
         self.V = np.zeros_like(self.X)
+        if velocity is not None:
+            self.V[:] = np.asarray(velocity, np.float32).reshape(-1)
+        self.clock = np.float32(0.0)
 
-    def substep(self, params, order, sphere_t=0.0, count=1, dt=None):
+    def substep(self, params, order, count=1):
+        """`count` substeps; the collider clock advances by dt in float32 per
+        substep (as the device clock does) and the substep sees its end time."""
         fn = getattr(lib(), "xo_substep_" + self.variant)
         pin = self.pinned.ctypes.data if self.pinned is not None else None
         order = np.ascontiguousarray(order, np.uint32)
-        for k in range(count):
+        dt = np.float32(params.dt)
+        for _ in range(count):
+            self.clock = np.float32(self.clock + dt)
             fn(C.byref(params), self.n, self.X, self.O, self.V, self.w, pin, self.t, order, self.tets, self.qi,
-               self.qq, self.qr, self.c0, self.c1, self.a, sphere_t + k * params.dt)
+               self.qq, self.qr, self.c0, self.c1, self.a, float(self.clock))
+
+
+def params_from_settings(dt, settings):
+    """xo_params from the keyword settings of a workload."""
+    s = dict(settings)
+    return make_params(float(dt), gravity=s.get("gravity", (0.0, -9.81)), drag=s.get("drag", 0.0),
+                       energy=s.get("energy", 4), simultaneous=bool(s.get("simultaneous", 1)),
+                       yeoh=s.get("yeoh", SKIN_YEOH), collide=s.get("collide", 0), ground_y=s.get("ground_y", 0.0),
+                       friction=s.get("friction", 0.0), sphere_c0=s.get("sphere_c0", (0, 0, 0)),
+                       sphere_v=s.get("sphere_v", (0, 0, 0)), sphere_r=s.get("sphere_r", 0.0))
+
+
+def body_from_workload(wl, variant="f32"):
+    """OracleBody with a workload's mesh, pins, velocity and material."""
+    s = wl["settings"]
+    return OracleBody(wl["xyz"], wl["tets"], wl["density"], wl["pinned"], compliance=s.get("compliance"),
+                      tet_mu=wl["tet_mu"], nu=s.get("poissons_ratio", 0.5), variant=variant,
+                      velocity=wl["velocity"])
 
 
 class RefBody:

@@ -1,0 +1,113 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+Run in a container that holds the reference sources (oracle/Makefile builds
+oracle/_ref/libxfref.so from them):  python tests/golden/make_golden.py
+
+Fixtures (numpy .npz, no pickles):
+  ref_beam12.npz   12x4x4 beam (0.1 m cells, x from 0): the reference's
+                   element constants (Qi, QQ, QR, volume, w), its MINSTD
+                   tOrder, GenerateTetBlock output for a centred 12x4x4 block,
+                   the greedy natural-order colour sweep order, and node
+                   positions/velocities of GeoLinear3d::Substep (x = 0 face
+                   pinned, rho 1000, compliance 1e-5, nu 0.5, g = -9.81,
+                   dt = 1/1200) at frames 1, 10 and 120 (20 substeps each)
+                   for MixedSel and YeohSkinFast (simultaneous) and MixedSel
+                   serial.
+  ref_tet_kats.npz 64 random deformed tets: inputs (X, w, element constants,
+                   energy, solve mode) and the reference SolveElement output.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "oracle")]
+
+import pyoracle as po  # noqa: E402
+
+FRAMES = (1, 10, 120)
+RUNS = ((4, True), (7, True), (4, False))
+DT = np.float32(1.0) / np.float32(1200.0)
+
+
+def natural_colour_order(n, tets):
+    t = tets.size // 4
+    nc, col = po.colour_greedy(n, tets, np.arange(t, dtype=np.uint32))
+    return np.concatenate([np.nonzero(col == c)[0] for c in range(nc)]).astype(np.uint32)
+
+
+def beam12():
+    xyz, tets = po.tet_block(12, 4, 4, (0.1, 0.1, 0.1), (0.0, -0.2, -0.2))
+    n, t = xyz.size // 3, tets.size // 4
+    pinned = (xyz[0::3] == 0.0).astype(np.uint8)
+    out = dict(xyz=xyz, tets=tets, pinned=pinned, dt=np.array([DT], np.float32))
+    rb = po.RefBody(xyz, tets, 1000.0, None)
+    el = rb.elements()
+    for k, v in el.items():
+        out["el_" + k] = v
+    out["minstd_order"] = rb.order()
+    cx = np.zeros(3 * 13 * 5 * 5, np.float32)
+    ct = np.zeros(4 * t, np.uint32)
+    po.ref().xr_gen_tet_block(12, 4, 4, 0.1, 0.1, 0.1, 0, cx.ctypes.data, ct.ctypes.data)
+    out["gen_xyz"], out["gen_tets"] = cx, ct
+    order = natural_colour_order(n, tets)
+    out["sweep_order"] = order
+    for energy, sim in RUNS:
+        tag = f"e{energy}{'s' if sim else 'g'}"
+        rb = po.RefBody(xyz, tets, 1000.0, pinned, order)
+        st = rb.settings(1e-5, nu=0.5, energy=energy, simultaneous=sim)
+        done = 0
+        for f in FRAMES:
+            rb.substeps(st, DT, 20 * f - done)
+            done = 20 * f
+            X, V = rb.state()
+            out[f"{tag}_X{f}"] = X
+            out[f"{tag}_V{f}"] = V
+    np.savez_compressed(os.path.join(HERE, "ref_beam12.npz"), **out)
+
+
+def tet_kats(count=64, seed=7):
+    rng = np.random.default_rng(seed)
+    rows = dict(X=[], w=[], qi=[], qq=[], qr=[], vol=[], energy=[], sim=[], Xout=[])
+    for k in range(count):
+        # random well-shaped rest tet, then a random deformation of it
+        rest = (np.eye(3, dtype=np.float64) * 0.1 + rng.normal(0, 0.02, (3, 3))).astype(np.float32)
+        if np.linalg.det(rest) < 0:
+            rest[0] *= -1
+        xyz = np.vstack([rest, np.zeros((1, 3), np.float32)]).astype(np.float32).reshape(-1)
+        el = po.init_elements(xyz, np.arange(4, dtype=np.uint32), 1000.0)
+        F = np.eye(3) + rng.normal(0, 0.05, (3, 3))
+        X = (xyz.reshape(4, 3).astype(np.float64) @ F.T + rng.normal(0, 1e-3, 3)).reshape(-1)
+        w = (el["w"] * rng.uniform(0.5, 2.0, 4)).astype(np.float32)
+        if k % 8 == 0:
+            w[rng.integers(0, 4)] = 0.0
+        energy = [3, 4, 5, 6, 7][k % 5]
+        sim = (k // 5) % 2 == 0
+        st = po.XrSettings()
+        st.compliance = 1e-5
+        st.poissons_ratio = 0.5 if k % 3 else 0.45
+        st.gravity[:] = (0.0, -9.81)
+        st.drag = 0.0
+        st.energy = energy
+        st.simultaneous = 1 if sim else 0
+        Xo = X.copy()
+        po.ref().xr_solve_tet(st, DT, Xo, w, el["qi"], el["qq"], el["qr"], float(el["vol"][0]))
+        for key, val in (("X", X), ("w", w), ("qi", el["qi"]), ("qq", el["qq"]), ("qr", el["qr"]),
+                         ("vol", el["vol"]), ("energy", energy), ("sim", sim), ("Xout", Xo)):
+            rows[key].append(val)
+        rows.setdefault("nu", []).append(st.poissons_ratio)
+    out = {k: np.asarray(v) for k, v in rows.items()}
+    out["dt"] = np.array([DT], np.float32)
+    out["compliance"] = np.array([1e-5], np.float32)
+    np.savez_compressed(os.path.join(HERE, "ref_tet_kats.npz"), **out)
+
+
+if __name__ == "__main__":
+    if not po.have_ref():
+        sys.exit("oracle/_ref/libxfref.so is missing: build the oracle with the reference present")
+    beam12()
+    tet_kats()
+    for f in ("ref_beam12.npz", "ref_tet_kats.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -1,0 +1,147 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Bar: bit-exact. With the same colour order, GPU positions and velocities
+equal those of
+  * the oracle's reference-precision sweep (xo_substep_ref64: FP64 node
+    state, FP32 element math), and
+  * the compiled reference itself (oracle/_ref/libxfref.so driving the
+    reference's GeoLinear3d::Substep), where that build is present,
+for every energy (3-7) and both solve modes. The error against the FP64
+oracle is reported; it is bounded by FP64_REPORT_BOUND only to catch gross
+failures, since the substep amplifies rounding (the reference's own result
+differs from FP64 by ~4e-4 of the bbox diagonal after one frame).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import workloads
+import xpbdfem as xf
+
+pytestmark = pytest.mark.gpu
+
+FP64_REPORT_BOUND = 5e-3
+
+
+def bbox_diag(xyz):
+    p = xyz.reshape(-1, 3)
+    return float(np.linalg.norm(p.max(0) - p.min(0)))
+
+
+def rel_err(x, ref, xyz0):
+    return float(np.abs(x.reshape(-1, 3) - ref.reshape(-1, 3)).max() / bbox_diag(xyz0))
+
+
+def run_gpu(ctx, wl, substeps, use_graph=True, colour_order=xf.COLOUR_NATURAL, chunks=1):
+    sc = workloads.make_scene(ctx, wl, use_graph=use_graph, colour_order=colour_order)
+    for _ in range(chunks):
+        sc.step(substeps, float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, _ = sc.order(0)
+    sc.close()
+    return x, v, order
+
+
+def run_oracle(wl, order, substeps, variant="ref64"):
+    ob = po.body_from_workload(wl, variant=variant)
+    ob.substep(po.params_from_settings(wl["dt"], wl["settings"]), order, count=substeps)
+    return ob
+
+
+@pytest.mark.parametrize("energy", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("simultaneous", [True, False])
+def test_one_frame_bit_exact(ctx, energy, simultaneous):
+    wl = workloads.beam_neo_12x4x4(energy=energy, simultaneous=simultaneous)
+    x, v, order = run_gpu(ctx, wl, wl["substeps"])
+    ob = run_oracle(wl, order, wl["substeps"])
+    print(f"\nenergy {energy} sim {simultaneous}: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
+    assert np.array_equal(x, ob.X)
+    assert np.array_equal(v, ob.V)
+
+
+@pytest.mark.skipif(not po.have_ref(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("energy", [4, 7])
+def test_frames_bit_exact_vs_compiled_reference(ctx, energy):
+    """No collisions (the reference has none): 5 frames, GPU == reference."""
+    wl = workloads.beam_neo_12x4x4(energy=energy, collide=False)
+    x, v, order = run_gpu(ctx, wl, wl["substeps"], chunks=5)
+    rb = po.RefBody(wl["xyz"], wl["tets"], wl["density"], wl["pinned"], order)
+    s = wl["settings"]
+    rb.substeps(rb.settings(s["compliance"], nu=s["poissons_ratio"], energy=energy), wl["dt"], 5 * wl["substeps"])
+    xr, vr = rb.state()
+    assert np.array_equal(x, xr)
+    assert np.array_equal(v, vr)
+
+
+def test_120_frames_bit_exact_and_fp64_error(ctx):
+    wl = workloads.beam_neo_12x4x4()
+    sc = workloads.make_scene(ctx, wl)
+    order, _ = sc.order(0)
+    ob = po.body_from_workload(wl, "ref64")
+    f64 = po.body_from_workload(wl, "f64")
+    p = po.params_from_settings(wl["dt"], wl["settings"])
+    for frame in range(1, wl["frames"] + 1):
+        sc.step(wl["substeps"], float(wl["dt"]))
+        ob.substep(p, order, count=wl["substeps"])
+        f64.substep(p, order, count=wl["substeps"])
+        if frame in (1, 2, 5, 10, 30, 60, 120):
+            x, v = sc.get_state(0)
+            assert np.array_equal(x, ob.X), f"frame {frame}"
+            assert np.array_equal(v, ob.V), f"frame {frame}"
+            e = rel_err(x, f64.X, wl["xyz"])
+            print(f"\nframe {frame}: rel pos err vs FP64 oracle {e:.3e}; min y {x[1::3].min():.4f}")
+            if frame == 1:
+                assert e <= FP64_REPORT_BOUND
+    assert x[1::3].min() >= wl["settings"]["ground_y"] - 1e-9
+    sc.close()
+
+
+def test_graph_equals_eager(ctx):
+    wl = workloads.beam_neo_12x4x4(energy=7)
+    xg, vg, _ = run_gpu(ctx, wl, 2 * wl["substeps"], use_graph=True)
+    xe, ve, _ = run_gpu(ctx, wl, 2 * wl["substeps"], use_graph=False)
+    assert np.array_equal(xg, xe) and np.array_equal(vg, ve)
+
+
+def test_graph_replays_match_eager(ctx):
+    """Several replays of one captured frame, with host reads between them."""
+    wl = workloads.beam_neo_12x4x4()
+    sg = workloads.make_scene(ctx, wl, use_graph=True)
+    se = workloads.make_scene(ctx, wl, use_graph=False)
+    for _ in range(4):
+        sg.step(wl["substeps"], float(wl["dt"]))
+        se.step(wl["substeps"], float(wl["dt"]))
+        xg, vg = sg.get_state(0)
+        xe, ve = se.get_state(0)
+        assert np.array_equal(xg, xe) and np.array_equal(vg, ve)
+    sg.close()
+    se.close()
+
+
+def test_minstd_colouring_bit_exact(ctx):
+    wl = workloads.beam_neo_12x4x4()
+    x, v, order = run_gpu(ctx, wl, wl["substeps"], colour_order=xf.COLOUR_MINSTD)
+    ob = run_oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_block_sizes_bit_exact(ctx, block):
+    wl = workloads.beam_neo_12x4x4(energy=7)
+    sc = workloads.make_scene(ctx, wl, block_size=block)
+    sc.step(wl["substeps"], float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, _ = sc.order(0)
+    ob = run_oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+    sc.close()
+
+
+def test_big_yeoh_beam_one_frame_bit_exact(ctx):
+    """The 96x24x24 Yeoh beam (331,776 tets) with ground and moving sphere."""
+    wl = workloads.beam_yeoh_96x24x24()
+    x, v, order = run_gpu(ctx, wl, wl["substeps"])
+    ob = run_oracle(wl, order, wl["substeps"])
+    print(f"\nbig beam: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
+    assert np.array_equal(x, ob.X)
+    assert np.array_equal(v, ob.V)

@@ -1,0 +1,224 @@
+"""Closed-form checks of the oracle (and of the host utilities of the
+product), independent of the reference's own code.
+
+* mesh counts, rest volumes and total lumped mass;
+* Qi * Ds^T = I;
+* the simultaneous projection against a textbook restatement:
+  F = Ds Dm^-1, U0 = |F|_F^2 (MixedSel), U0 = C0 IM + C1 IM^2 + C2 IM^3
+  (Yeoh), U1 = (det F - a)^2, analytic gradients, A lambda = -2U solved
+  with numpy (catches a dropped term, a wrong sign or a transposed operand);
+* linear momentum and angular momentum of a projection are conserved;
+* energies 3 and 5 equal 4 and 7 for a linear tet (one quadrature point);
+* collision pass: non-penetration, static and kinetic friction;
+* colouring: node-disjoint colours, product == oracle;
+* LDLT usage example of Ldlt.h: x = (-3, 3, 1).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import xpbdfem as xf
+
+DT = float(np.float32(1.0) / np.float32(1200.0))
+
+
+def rest_tet(rng):
+    rest = np.eye(3) * 0.1 + rng.normal(0, 0.02, (3, 3))
+    if np.linalg.det(rest) < 0:
+        rest[0] *= -1
+    return np.vstack([rest, np.zeros((1, 3))]).astype(np.float32)
+
+
+def textbook(X, rest, w, energy, comp0, comp1, a, C=po.SKIN_YEOH):
+    """Independent restatement of one simultaneous projection in float64."""
+    Dm = (rest[:3] - rest[3]).T.astype(np.float64)  # columns = rest edges
+    Ds = (X[:3] - X[3]).T
+    Dmi = np.linalg.inv(Dm)
+    F = Ds @ Dmi
+    I1 = np.sum(F * F)
+    dI1 = 2.0 * F @ Dmi.T  # dI1/dDs
+    if energy == 4:
+        U0, dU0 = I1, dI1
+    else:
+        IM = I1 - 3.0
+        U0 = C[0] * IM + C[1] * IM ** 2 + C[2] * IM ** 3
+        dU0 = (C[0] + 2 * C[1] * IM + 3 * C[2] * IM ** 2) * dI1
+    J = np.linalg.det(F)
+    dJ = J * np.linalg.inv(F).T @ Dmi.T
+    U1 = (J - a) ** 2
+    dU1 = 2.0 * (J - a) * dJ
+
+    def node_grads(dD):
+        g = np.zeros((4, 3))
+        g[:3] = dD.T
+        g[3] = -g[:3].sum(0)
+        return g
+
+    g0, g1 = node_grads(dU0), node_grads(dU1)
+    al0, al1 = comp0 / DT ** 2, comp1 / DT ** 2
+    A = np.array([[np.sum(w[:, None] * g0 * g0) + 2 * U0 * al0, np.sum(w[:, None] * g0 * g1)],
+                  [np.sum(w[:, None] * g0 * g1), np.sum(w[:, None] * g1 * g1) + 2 * U1 * al1]])
+    lam = np.linalg.solve(A, [-2 * U0, -2 * U1])
+    return X + w[:, None] * (lam[0] * g0 + lam[1] * g1)
+
+
+def oracle_solve(X, rest, w, energy, comp0, comp1, a, sim=True, variant="f64"):
+    el = po.init_elements(rest.reshape(-1), np.arange(4, dtype=np.uint32), 1000.0)
+    p = po.make_params(DT, energy=energy, simultaneous=sim)
+    Xo = X.reshape(-1).astype(np.float64).copy()
+    getattr(po.lib(), "xo_solve_tet_" + variant)(p, Xo, w.astype(np.float32), el["qi"], el["qq"], el["qr"], comp0,
+                                                 comp1, a)
+    return Xo.reshape(4, 3)
+
+
+@pytest.mark.parametrize("energy", [4, 7])
+@pytest.mark.parametrize("seed", range(6))
+def test_projection_matches_textbook(energy, seed):
+    rng = np.random.default_rng(seed)
+    rest = rest_tet(rng)
+    F = np.eye(3) + rng.normal(0, 0.08, (3, 3))
+    if energy == 7:
+        F = F * 1.05  # keep I1 > 3 so the 1e-4 clamp stays inactive
+    X = rest.astype(np.float64) @ F.T + rng.normal(0, 0.01, 3)
+    w = rng.uniform(0.5, 2.0, 4)
+    comp0, comp1, a = 0.02, 0.003, 1.0
+    got = oracle_solve(X, rest, w, energy, comp0, comp1, a)
+    want = textbook(X, rest, w, energy, comp0, comp1, a)
+    dx = want - X
+    assert np.abs(got - want).max() <= 1e-4 * np.abs(dx).max() + 1e-12
+
+
+def test_textbook_catches_a_planted_mistake():
+    """A wrong sign in the volume term is detected by the same comparison."""
+    rng = np.random.default_rng(3)
+    rest = rest_tet(rng)
+    X = rest.astype(np.float64) @ (np.eye(3) + rng.normal(0, 0.08, (3, 3))).T
+    w = rng.uniform(0.5, 2.0, 4)
+    got = oracle_solve(X, rest, w, 4, 0.02, 0.003, 1.0)
+    wrong = textbook(X, rest, w, 4, 0.02, 0.003, -1.0)  # (J + 1)^2 instead of (J - 1)^2
+    assert np.abs(got - wrong).max() > 1e-3 * np.abs(wrong - X).max()
+
+
+@pytest.mark.parametrize("energy", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("sim", [True, False])
+def test_projection_conserves_momentum(energy, sim):
+    rng = np.random.default_rng(11 + energy)
+    rest = rest_tet(rng)
+    X = rest.astype(np.float64) @ (1.1 * np.eye(3) + rng.normal(0, 0.05, (3, 3))).T + 0.3
+    w = rng.uniform(0.5, 2.0, 4).astype(np.float32).astype(np.float64)  # the oracle takes FP32 w
+    m = 1.0 / w
+    Xo = oracle_solve(X, rest, w, energy, 0.02, 0.0, 1.0, sim=sim)
+    d = Xo - X
+    scale = np.abs(d).max() * m.max()
+    assert scale > 0
+    assert np.abs((m[:, None] * d).sum(0)).max() <= 1e-9 * scale
+    if sim:  # one projection: isotropic energies give torque-free corrections
+        torque = np.cross(X - X.mean(0), m[:, None] * d).sum(0)
+        assert np.abs(torque).max() <= 1e-9 * scale * np.abs(X).max()
+
+
+@pytest.mark.parametrize("pair", [(3, 4), (5, 7)])
+def test_quadrature_energies_equal_prefactored(pair):
+    rng = np.random.default_rng(5)
+    rest = rest_tet(rng)
+    X = rest.astype(np.float64) @ (1.08 * np.eye(3) + rng.normal(0, 0.05, (3, 3))).T
+    w = rng.uniform(0.5, 2.0, 4)
+    A = oracle_solve(X, rest, w, pair[0], 0.02, 0.0, 1.0)
+    B = oracle_solve(X, rest, w, pair[1], 0.02, 0.0, 1.0)
+    assert np.abs(A - B).max() <= 1e-7 * np.abs(B - X).max()
+
+
+def test_mesh_counts_volumes_and_mass():
+    W, H, D, h = 7, 3, 2, 0.05
+    xyz, tets = xf.tet_block(W, H, D, (h, h, h), (0.0, 0.0, 0.0))
+    n, t = xyz.size // 3, tets.size // 4
+    assert n == (W + 1) * (H + 1) * (D + 1) and t == 6 * W * H * D
+    el = po.init_elements(xyz, tets, 1200.0)
+    assert np.allclose(el["vol"], h ** 3 / 6, rtol=1e-5)
+    assert np.isclose((1.0 / el["w"]).astype(np.float64).sum(), 1200.0 * W * H * D * h ** 3, rtol=1e-5)
+    qi = el["qi"].reshape(-1, 3, 3)  # [tet][column][row]
+    P = (xyz.reshape(-1, 3)[tets.reshape(-1, 4)[:, :3]] - xyz.reshape(-1, 3)[tets.reshape(-1, 4)[:, 3:]])
+    # Qi is the inverse of the map J whose row n is P_n
+    for k in range(0, t, 17):
+        Jm = P[k]  # rows = P_n
+        Qi = qi[k].T  # row/col matrix from column-major storage
+        assert np.allclose(Qi @ Jm, np.eye(3), atol=1e-5)
+
+
+def test_ground_collision_cases():
+    p = po.make_params(DT, collide=1, ground_y=-0.5, friction=0.5)
+    o = np.array([0.0, -0.49, 0.0])
+    # static: tangential motion 0.001 <= mu * depth = 0.5 * 0.01
+    x = np.array([0.001, -0.51, 0.0])
+    po.lib().xo_collide_f64(p, 0.0, o, x)
+    assert x[1] == -0.5 and x[0] == o[0] and x[2] == o[2]
+    # kinetic: tangential 0.1 > 0.005, reduced by exactly mu * depth along its direction
+    x = np.array([0.06, -0.51, 0.08])
+    po.lib().xo_collide_f64(p, 0.0, o, x)
+    assert x[1] == -0.5
+    assert np.isclose(np.hypot(x[0], x[2]), 0.1 - 0.005, rtol=1e-12)
+    assert np.isclose(x[0] / x[2], 0.06 / 0.08, rtol=1e-12)
+    # above the plane: untouched
+    x = np.array([0.3, -0.2, 0.1])
+    po.lib().xo_collide_f64(p, 0.0, o, x.copy())
+    y = x.copy()
+    po.lib().xo_collide_f64(p, 0.0, o, y)
+    assert np.array_equal(x, y)
+
+
+def test_sphere_collision_cases():
+    p = po.make_params(DT, collide=2, friction=0.5, sphere_c0=(0.0, 0.0, 0.0), sphere_v=(1.0, 0.0, 0.0),
+                       sphere_r=0.3)
+    t = 0.25
+    c = np.array([0.25, 0.0, 0.0])
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        x = c + d * rng.uniform(0.2, 0.299)
+        o = x + rng.normal(0, 0.01, 3)
+        po.lib().xo_collide_f64(p, t, o, x)
+        assert np.linalg.norm(x - c) >= 0.3 - 1e-6
+    # static friction: tangential motion relative to the sphere surface is removed
+    x = c + np.array([0.0, 0.295, 0.0])
+    n = (x - c) / np.linalg.norm(x - c)  # contact normal
+    o = x - np.array([1.0 * DT - 1e-5, 0.0, 0.0])  # moved with the sphere, slightly less
+    po.lib().xo_collide_f64(p, t, o, x)
+    rel = (x - o) - np.array([1.0, 0.0, 0.0]) * DT
+    assert np.abs(rel - n * rel.dot(n)).max() <= 1e-9
+
+
+@pytest.mark.parametrize("order", [xf.COLOUR_NATURAL, xf.COLOUR_MINSTD])
+@pytest.mark.parametrize("dims,mirrored", [((12, 4, 4), False), ((5, 6, 7), True)])
+def test_colouring_valid_and_product_equals_oracle(order, dims, mirrored):
+    xyz, tets = xf.tet_block(*dims, (0.1, 0.1, 0.1), (0, 0, 0), mirrored)
+    n, t = xyz.size // 3, tets.size // 4
+    sweep, colour, nc = xf.mesh_colour(n, tets, order)
+    visit = np.arange(t, dtype=np.uint32) if order == xf.COLOUR_NATURAL else po.minstd_order(t)
+    onc, ocol = po.colour_greedy(n, tets, visit)
+    assert nc == onc and np.array_equal(colour, ocol)
+    oorder = np.concatenate([visit[ocol[visit] == c] for c in range(onc)])
+    assert np.array_equal(sweep, oorder)
+    assert sorted(sweep.tolist()) == list(range(t))
+    T4 = tets.reshape(-1, 4)
+    valence = np.bincount(T4.reshape(-1), minlength=n).max()
+    assert nc >= valence
+    for c in range(nc):
+        nodes = T4[colour == c].reshape(-1)
+        assert np.unique(nodes).size == nodes.size, f"colour {c} shares a node"
+
+
+def test_ldlt_usage_example():
+    A = np.array([1, 2, 3, 4, 5, 6], np.float32)
+    b = np.array([7, 8, 9], np.float32)
+    xo = np.zeros(3, np.float32)
+    po.lib().xo_ldlt_solve(3, A, b, xo)
+    assert np.allclose(xo, [-3, 3, 1], atol=1e-5)
+    assert np.array_equal(xf.ldlt_solve(A, b), xo)
+    # against numpy on a random SPD system
+    rng = np.random.default_rng(0)
+    M = rng.normal(size=(6, 6))
+    S = (M @ M.T + 6 * np.eye(6)).astype(np.float32)
+    packed = np.concatenate([S[i, :i + 1] for i in range(6)])
+    rhs = rng.normal(size=6).astype(np.float32)
+    assert np.allclose(xf.ldlt_solve(packed, rhs), np.linalg.solve(S.astype(np.float64), rhs), rtol=1e-4)

@@ -1,0 +1,239 @@
+// xf_host_util.cpp -- host-side mesh, element-constant and colouring code.
+#include "xf_host_util.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace xf {
+
+void TetBlock(uint32_t W, uint32_t H, uint32_t D, const float scale[3], const float origin[3], bool mirrored,
+              float* xyz, uint32_t* tets) {
+    // Node placement and numbering of GenerateHexBlock (MeshGen.cpp:171-185)
+    // with an explicit origin instead of the centred one.
+    for (uint32_t z = 0; z <= D; z++)
+        for (uint32_t y = 0; y <= H; y++)
+            for (uint32_t x = 0; x <= W; x++) {
+                size_t i = x + (size_t)y * (W + 1) + (size_t)z * (W + 1) * (H + 1);
+                xyz[3 * i + 0] = origin[0] + scale[0] * (float)x;
+                xyz[3 * i + 1] = origin[1] + scale[1] * (float)y;
+                xyz[3 * i + 2] = origin[2] + scale[2] * (float)z;
+            }
+    // Hex corner numbering (MeshGen.cpp:189-208) and the 6-tet split around
+    // the 0-7 diagonal (MeshGen.cpp:235-240).
+    static const uint8_t kSplit[6][4] = {{0, 1, 5, 7}, {0, 7, 3, 1}, {0, 2, 3, 7},
+                                         {0, 7, 6, 2}, {0, 4, 6, 7}, {0, 7, 5, 4}};
+    size_t t = 0;
+    for (uint32_t z = 0; z < D; z++)
+        for (uint32_t y = 0; y < H; y++)
+            for (uint32_t x = 0; x < W; x++) {
+                uint32_t corner[8];
+                for (uint32_t c = 0; c < 8; c++) {
+                    uint32_t i = c & 1, j = (c >> 1) & 1, k = c >> 2;
+                    uint32_t slot = c, ox = i, oy = j, oz = k;
+                    if (mirrored) {
+                        ox = (x & 1) ? 1 - i : i;
+                        oy = (y & 1) ? 1 - j : j;
+                        oz = (z & 1) ? 1 - k : k;
+                        if ((x + y + z) & 1) slot = 7 - c;
+                    }
+                    corner[slot] = (x + ox) + (y + oy) * (W + 1) + (z + oz) * (W + 1) * (H + 1);
+                }
+                for (int s = 0; s < 6; s++, t++)
+                    for (int c = 0; c < 4; c++) tets[4 * t + c] = corner[kSplit[s][c]];
+            }
+}
+
+void MinstdOrder(uint32_t n, uint32_t* order) {
+    // GeoLinear3d::Init sweep order (Geo.cpp:759-771).
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    uint32_t state = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        state = (uint32_t)(((uint64_t)state * 48271u) % 0x7fffffffu);
+        uint32_t j = state % n;
+        uint32_t tmp = order[i];
+        order[i] = order[j];
+        order[j] = tmp;
+    }
+}
+
+void NormalFill(uint64_t seed, float sigma, uint32_t n, float* out) {
+    // SplitMix64 stream, 53-bit uniforms, Box-Muller pairs (cos branch first).
+    uint64_t s = seed;
+    auto next = [&]() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    const double kTwoPi = 6.283185307179586476925286766559;
+    for (uint32_t i = 0; i < n; i += 2) {
+        double u1 = ((next() >> 11) + 1.0) * (1.0 / 9007199254740992.0);  // (0, 1]
+        double u2 = (next() >> 11) * (1.0 / 9007199254740992.0);          // [0, 1)
+        double r = std::sqrt(-2.0 * std::log(u1));
+        out[i] = (float)(sigma * r * std::cos(kTwoPi * u2));
+        if (i + 1 < n) out[i + 1] = (float)(sigma * r * std::sin(kTwoPi * u2));
+    }
+}
+
+namespace {
+// inverse(mat3) of vectormath.cpp:41-60: adjugate in double, scale in float.
+void Inverse3(const float m[3][3], float out[3][3]) {
+    double a00 = m[0][0], a01 = m[0][1], a02 = m[0][2];
+    double a10 = m[1][0], a11 = m[1][1], a12 = m[1][2];
+    double a20 = m[2][0], a21 = m[2][1], a22 = m[2][2];
+    float adj[3][3];
+    adj[0][0] = (float)+(a11 * a22 - a12 * a21);
+    adj[0][1] = (float)-(a01 * a22 - a02 * a21);
+    adj[0][2] = (float)+(a01 * a12 - a02 * a11);
+    adj[1][0] = (float)-(a10 * a22 - a12 * a20);
+    adj[1][1] = (float)+(a00 * a22 - a02 * a20);
+    adj[1][2] = (float)-(a00 * a12 - a02 * a10);
+    adj[2][0] = (float)+(a10 * a21 - a11 * a20);
+    adj[2][1] = (float)-(a00 * a21 - a01 * a20);
+    adj[2][2] = (float)+(a00 * a11 - a01 * a10);
+    float det = m[0][0] * adj[0][0] + m[0][1] * adj[1][0] + m[0][2] * adj[2][0];
+    float s = 1.0f / det;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) out[c][r] = adj[c][r] * s;
+}
+
+// determinant(mat3) of vectormath.cpp:34-39.
+float Det3(const float m[3][3]) {
+    float a = m[0][0], b = m[1][0], c = m[2][0];
+    float d = m[0][1], e = m[1][1], f = m[2][1];
+    float g = m[0][2], h = m[1][2], i = m[2][2];
+    return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+
+float Dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+}  // namespace
+
+bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                  ElementConstants* out) {
+    out->qi.assign((size_t)9 * n_tets, 0.0f);
+    out->qq.assign((size_t)3 * n_tets, 0.0f);
+    out->qr.assign((size_t)3 * n_tets, 0.0f);
+    out->vol.assign(n_tets, 0.0f);
+    out->w.assign(n_nodes, 0.0f);
+    const float kVolMult = 1.0f / 6.0f;  // VolumeMult<Tet> (Fem.cpp:80)
+    const float kQuadW = 0.25f;          // QuadratureWeights<Tet, 4> (Fem.cpp:11)
+    for (uint32_t t = 0; t < n_tets; t++) {
+        const uint32_t* is = tets + 4 * (size_t)t;
+        for (int c = 0; c < 4; c++)
+            if (is[c] >= n_nodes) return false;
+        // InitTriTetFiniteElement (Fem.cpp:197-224): P in float from double positions.
+        float P[3][3];
+        for (int nd = 0; nd < 3; nd++)
+            for (int r = 0; r < 3; r++)
+                P[nd][r] = (float)((double)xyz[3 * (size_t)is[nd] + r] - (double)xyz[3 * (size_t)is[3] + r]);
+        float J[3][3];  // MatMul(dN, P): column k holds (P0[k], P1[k], P2[k])
+        for (int k = 0; k < 3; k++)
+            for (int nd = 0; nd < 3; nd++) J[k][nd] = P[nd][k];
+        float Qi[3][3];
+        Inverse3(J, Qi);
+        // GeneratePrefactoredCoefficients (Fem.cpp:132-161), 4 points.
+        float QQ[3] = {0.0f, 0.0f, 0.0f}, QR[3] = {0.0f, 0.0f, 0.0f}, volume = 0.0f;
+        for (int p = 0; p < 4; p++) {
+            float Ji[3][3];
+            Inverse3(J, Ji);
+            float wq = Det3(J) * kVolMult * kQuadW;
+            volume += wq;
+            for (int i = 0; i < 3; i++) QQ[i] += wq * Dot3(Ji[i], Ji[i]);
+            int head = 0;
+            for (int i = 0; i < 3; i++)
+                for (int j = i + 1; j < 3; j++) QR[head++] += wq * 2.0f * Dot3(Ji[i], Ji[j]);
+        }
+        for (int i = 0; i < 3; i++) {
+            QQ[i] /= volume;
+            QR[i] /= volume;
+        }
+        float v = kVolMult * Det3(J);
+        for (int c = 0; c < 3; c++)
+            for (int r = 0; r < 3; r++) out->qi[9 * (size_t)t + 3 * c + r] = Qi[c][r];
+        for (int i = 0; i < 3; i++) {
+            out->qq[3 * (size_t)t + i] = QQ[i];
+            out->qr[3 * (size_t)t + i] = QR[i];
+        }
+        out->vol[t] = v;
+        // Lumped mass, LumpsTet4 = 1/4 each (Fem.cpp:220-223).
+        for (int c = 0; c < 4; c++) out->w[is[c]] += 0.25f * density * v;
+    }
+    for (uint32_t i = 0; i < n_nodes; i++) out->w[i] = 1.0f / out->w[i];  // Geo.cpp:744
+    return true;
+}
+
+uint32_t GreedyColour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, const uint32_t* visit,
+                      std::vector<uint32_t>* colour) {
+    // Each tet, in visit order, takes the lowest colour none of its 4 nodes
+    // has seen; node masks are kWords x 64 bits.
+    const int kWords = 4;
+    std::vector<uint64_t> mask((size_t)n_nodes * kWords, 0);
+    colour->assign(n_tets, 0);
+    uint32_t ncol = 0;
+    for (uint32_t k = 0; k < n_tets; k++) {
+        uint32_t t = visit[k];
+        const uint32_t* is = tets + 4 * (size_t)t;
+        uint32_t c = kMaxColours;
+        for (int word = 0; word < kWords && c == kMaxColours; word++) {
+            uint64_t used = mask[(size_t)is[0] * kWords + word] | mask[(size_t)is[1] * kWords + word] |
+                            mask[(size_t)is[2] * kWords + word] | mask[(size_t)is[3] * kWords + word];
+            if (used != ~0ull) c = 64 * word + (uint32_t)__builtin_ctzll(~used);
+        }
+        if (c == kMaxColours) return 0;
+        (*colour)[t] = c;
+        for (int i = 0; i < 4; i++) mask[(size_t)is[i] * kWords + c / 64] |= 1ull << (c % 64);
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    return ncol;
+}
+
+uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, bool minstd_visit,
+                          std::vector<uint32_t>* colour, std::vector<uint32_t>* order) {
+    std::vector<uint32_t> visit(n_tets);
+    if (minstd_visit) MinstdOrder(n_tets, visit.data());
+    else
+        for (uint32_t t = 0; t < n_tets; t++) visit[t] = t;
+    uint32_t ncol = GreedyColour(n_nodes, tets, n_tets, visit.data(), colour);
+    if (ncol == 0) return 0;
+    // counting sort by colour, stable in visit order
+    std::vector<uint32_t> start(ncol + 1, 0);
+    for (uint32_t t = 0; t < n_tets; t++) start[(*colour)[t] + 1]++;
+    for (uint32_t c = 0; c < ncol; c++) start[c + 1] += start[c];
+    order->assign(n_tets, 0);
+    for (uint32_t k = 0; k < n_tets; k++) {
+        uint32_t t = visit[k];
+        (*order)[start[(*colour)[t]]++] = t;
+    }
+    return ncol;
+}
+
+int LdltSolve(uint32_t n, const float* A, const float* b, float* x) {
+    // LDLTDecomposition / LDLTSolve (Ldlt.h:35-86) on packed lower-triangular A.
+    if (n == 0 || n > 25) return -1;
+    uint32_t rowA[26], rowL[26];
+    for (uint32_t i = 0; i <= n; i++) {
+        rowA[i] = i * (i + 1) / 2;  // packed row start including the diagonal
+        rowL[i] = i * (i - 1) / 2;  // strictly lower part
+    }
+    float L[325], D[25], invD[25];
+    for (uint32_t j = 0; j < n; j++) {
+        D[j] = A[rowA[j] + j];
+        for (uint32_t k = 0; k < j; k++) D[j] -= L[rowL[j] + k] * L[rowL[j] + k] * D[k];
+        for (uint32_t i = j + 1; i < n; i++) {
+            L[rowL[i] + j] = A[rowA[i] + j];
+            for (uint32_t k = 0; k < j; k++) L[rowL[i] + j] -= L[rowL[i] + k] * L[rowL[j] + k] * D[k];
+            L[rowL[i] + j] /= D[j];
+        }
+    }
+    for (uint32_t j = 0; j < n; j++) invD[j] = 1.0f / D[j];
+    for (uint32_t i = 0; i < n; i++) {
+        x[i] = b[i];
+        for (uint32_t j = 0; j < i; j++) x[i] -= L[rowL[i] + j] * x[j];
+    }
+    for (uint32_t i = 0; i < n; i++) x[i] *= invD[i];
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < i; j++) x[n - 1 - i] -= L[rowL[n - 1 - j] + n - 1 - i] * x[n - 1 - j];
+    return 0;
+}
+
+}  // namespace xf

@@ -1,0 +1,48 @@
+// xf_host_util.h -- host-side mesh, element-constant and colouring code
+// (no HIP dependency).
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace xf {
+
+constexpr uint32_t kMaxColours = 256;
+
+// Linear-tet block (GenerateHexBlock + GenerateTetBlock, MeshGen.cpp:156-244).
+void TetBlock(uint32_t W, uint32_t H, uint32_t D, const float scale[3], const float origin[3], bool mirrored,
+              float* xyz, uint32_t* tets);
+
+// The reference's MINSTD sweep order (Geo.cpp:759-771).
+void MinstdOrder(uint32_t n, uint32_t* order);
+
+// SplitMix64 + Box-Muller normal deviates.
+void NormalFill(uint64_t seed, float sigma, uint32_t n, float* out);
+
+struct ElementConstants {
+    std::vector<float> qi;   // 9 per tet, column-major inverse rest map
+    std::vector<float> qq;   // 3 per tet
+    std::vector<float> qr;   // 3 per tet
+    std::vector<float> vol;  // rest volume per tet
+    std::vector<float> w;    // inverse lumped mass per node
+};
+
+// InitFiniteElement for T4 over a whole mesh (Fem.cpp:197-224, 269;
+// Geo.cpp:732-744). Returns false if a tet index is out of range.
+bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                  ElementConstants* out);
+
+// Greedy node-disjoint colouring in visit order. Returns the colour count
+// (0 if more than kMaxColours would be needed).
+uint32_t GreedyColour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, const uint32_t* visit,
+                      std::vector<uint32_t>* colour);
+
+// Colour-major sweep order of a greedy colouring: tets of colour 0 in visit
+// order, then colour 1, ... Returns the colour count (0 on failure).
+uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, bool minstd_visit,
+                          std::vector<uint32_t>* colour, std::vector<uint32_t>* order);
+
+// LDLTDecomposeAndSolve (Ldlt.h:88-95). Returns 0 on success.
+int LdltSolve(uint32_t n, const float* A, const float* b, float* x);
+
+}  // namespace xf

@@ -1,0 +1,794 @@
+// xf_scene.cpp -- C ABI of the gfx950 XPBD substep (include/xpbdfem.h).
+//
+// A scene holds any number of bodies in one device layout: their nodes are
+// concatenated, every body is coloured on its own, and colour c of the
+// scene sweeps colour c of all bodies in one launch. One body is the plain
+// method; many bodies are the batched one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xpbdfem.h"
+#include "xf_device.h"
+#include "xf_host_util.h"
+
+extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, uint32_t block,
+                                        uint32_t begin, uint32_t count, const uint4* tidx, const float4* tg0,
+                                        const float4* tg1, const float4* tg2, const float4* tg3, const float4* taux,
+                                        double4* X, const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
+                                        float* clock);
+extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
+                                        const XfBodyParams* bodies);
+extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
+                                       double4* V, const XfBodyParams* bodies, const float* clock);
+extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const double4* X, uint32_t* count);
+
+struct xf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int Fail(xf_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define XF_HIP(ctx, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return Fail((ctx), XF_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));         \
+    } while (0)
+
+struct Body {
+    uint32_t n = 0, t = 0;
+    float density = 0.0f;
+    std::vector<float> xyz, vel, tet_mu;
+    std::vector<uint32_t> tets;
+    std::vector<uint8_t> pinned;
+    xf_settings settings{};
+    // finalize products
+    xf::ElementConstants ec;
+    std::vector<uint32_t> colour;     // per local tet
+    std::vector<uint32_t> order;      // local tets in sweep order
+    std::vector<uint32_t> sched_pos;  // scene schedule position of each local tet
+    uint32_t ncol = 0;
+    uint32_t node_base = 0;
+};
+
+template <typename T>
+int DevAlloc(xf_ctx* ctx, T** p, size_t count, uint64_t* total) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) return Fail(ctx, XF_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", sizeof(T) * count,
+                                     hipGetErrorString(e));
+    *total += sizeof(T) * count;
+    return XF_OK;
+}
+
+bool EnergySupported(uint32_t e) { return e >= 3 && e <= 7; }
+
+}  // namespace
+
+struct xf_scene {
+    xf_ctx* ctx = nullptr;
+    std::vector<Body> bodies;
+    bool finalized = false;
+    xf_scene_opts opts{};
+    uint32_t N = 0, T = 0, ncol = 0;
+    std::vector<uint32_t> colour_begin;  // ncol + 1 offsets into the schedule
+    // device
+    double4 *X = nullptr, *O = nullptr, *V = nullptr;
+    uint4* tidx = nullptr;
+    float4* tg[4] = {nullptr, nullptr, nullptr, nullptr};
+    float4* taux = nullptr;
+    XfBodyParams* bp = nullptr;
+    float* clock = nullptr;
+    uint32_t* counter = nullptr;
+    uint64_t device_bytes = 0;
+    // host mirrors
+    std::vector<float4> h_tg3, h_taux;
+    std::vector<double> h_w;       // per node inverse mass
+    std::vector<double> h_oflags;  // per node O.w: (body << 1) | pinned
+    bool material_dirty = true, params_dirty = true;
+    bool aux = false;
+    float comp1_u = 0.0f, a_u = 1.0f;
+    float params_dt = -1.0f;
+    float time_host = 0.0f;
+    // graph cache
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_substeps = 0;
+    float graph_dt = 0.0f;
+    int graph_energy = -1;
+    bool graph_sim = false, graph_aux = false;
+};
+
+namespace {
+
+void DropGraph(xf_scene* s) {
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+    s->graph_substeps = 0;
+}
+
+// Scene-wide projection variant: one energy for all bodies, or per-body
+// MixedSel / YeohSkinFast (energy 0). Returns XF_OK or an error.
+int ResolveMode(xf_scene* s, int* energy, bool* sim) {
+    uint32_t e0 = s->bodies[0].settings.energy;
+    uint32_t s0 = s->bodies[0].settings.simultaneous ? 1u : 0u;
+    bool same = true, all47 = true;
+    for (const Body& b : s->bodies) {
+        uint32_t e = b.settings.energy;
+        uint32_t si = b.settings.simultaneous ? 1u : 0u;
+        if (e != e0 || si != s0) same = false;
+        if (!((e == 4 || e == 7) && si == 1)) all47 = false;
+    }
+    if (same) {
+        *energy = (int)e0;
+        *sim = s0 != 0;
+        return XF_OK;
+    }
+    if (all47) {
+        *energy = 0;
+        *sim = true;
+        return XF_OK;
+    }
+    return Fail(s->ctx, XF_E_UNSUPPORTED,
+                "bodies of one scene must share energy and solve mode, or all use simultaneous MixedSel/YeohSkinFast");
+}
+
+// Per-tet material constants of SolveElementMixed (Fem.cpp:445-449).
+void ComputeMaterial(xf_scene* s) {
+    bool compressible = false;
+    for (Body& b : s->bodies) {
+        float nu = b.settings.poissons_ratio;
+        for (uint32_t t = 0; t < b.t; t++) {
+            float mu = b.tet_mu.empty() ? 1.0f / b.settings.compliance : b.tet_mu[t];
+            float lambda = (2.0f * mu * nu) / (1.0f - 2.0f * nu);
+            float a = 1.0f + mu / lambda;
+            float comp0 = 1.0f / mu / b.ec.vol[t];
+            float comp1 = 1.0f / lambda / b.ec.vol[t];
+            uint32_t pos = b.sched_pos[t];
+            s->h_tg3[pos].w = comp0;
+            s->h_taux[pos].x = comp1;
+            s->h_taux[pos].y = a;
+            if (comp1 != 0.0f || a != 1.0f) compressible = true;
+        }
+    }
+    s->aux = compressible || s->bodies.size() > 1;
+}
+
+int UploadMaterial(xf_scene* s) {
+    xf_ctx* ctx = s->ctx;
+    bool had_aux = s->aux;
+    ComputeMaterial(s);
+    if (s->aux != had_aux) DropGraph(s);
+    if (s->aux && !s->taux) {
+        int rc = DevAlloc(ctx, &s->taux, s->T, &s->device_bytes);
+        if (rc) return rc;
+    }
+    XF_HIP(ctx, hipMemcpyAsync(s->tg[3], s->h_tg3.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, ctx->stream));
+    if (s->aux)
+        XF_HIP(ctx, hipMemcpyAsync(s->taux, s->h_taux.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->material_dirty = false;
+    return XF_OK;
+}
+
+int UploadParams(xf_scene* s, float dt) {
+    xf_ctx* ctx = s->ctx;
+    std::vector<XfBodyParams> hp(s->bodies.size());
+    for (size_t i = 0; i < s->bodies.size(); i++) {
+        const xf_settings& st = s->bodies[i].settings;
+        XfBodyParams& p = hp[i];
+        memset(&p, 0, sizeof(p));
+        p.gdx = st.gravity[0] * dt;
+        p.gdy = st.gravity[1] * dt;
+        p.dragf = 1.0f - st.drag;
+        p.dt = dt;
+        p.invdt = 1.0f / dt;
+        p.collide = st.collide;
+        p.ground_y = st.ground_y;
+        p.friction = st.friction;
+        for (int r = 0; r < 3; r++) {
+            p.sc0[r] = st.sphere_c0[r];
+            p.sv[r] = st.sphere_v[r];
+            p.yeoh[r] = st.yeoh[r];
+        }
+        p.sr = st.sphere_r;
+        p.energy = st.energy;
+    }
+    XF_HIP(ctx, hipMemcpyAsync(s->bp, hp.data(), sizeof(XfBodyParams) * hp.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->params_dt = dt;
+    s->params_dirty = false;
+    return XF_OK;
+}
+
+// Enqueues `substeps` substeps on the context stream. ev (nullable) gets an
+// event pair per launch for profiling: [2*k] before, [2*k+1] after.
+struct LaunchTimer {
+    std::vector<hipEvent_t> ev_proj, ev_node;
+};
+
+int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, LaunchTimer* timer) {
+    xf_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    uint32_t block = s->opts.block_size ? s->opts.block_size : 64;
+    auto node = [&](int kind, bool predict_next) -> int {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timer) {
+            XF_HIP(ctx, hipEventCreate(&e0));
+            XF_HIP(ctx, hipEventCreate(&e1));
+            XF_HIP(ctx, hipEventRecord(e0, st));
+        }
+        if (kind == 0) XF_HIP(ctx, xf_launch_predict(st, s->N, s->X, s->O, s->V, s->bp));
+        else XF_HIP(ctx, xf_launch_finish(st, predict_next, s->N, s->X, s->O, s->V, s->bp, s->clock));
+        if (timer) {
+            XF_HIP(ctx, hipEventRecord(e1, st));
+            timer->ev_node.push_back(e0);
+            timer->ev_node.push_back(e1);
+        }
+        return XF_OK;
+    };
+    int rc = node(0, false);
+    if (rc) return rc;
+    for (uint32_t k = 0; k < substeps; k++) {
+        for (uint32_t c = 0; c < s->ncol; c++) {
+            uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timer) {
+                XF_HIP(ctx, hipEventCreate(&e0));
+                XF_HIP(ctx, hipEventCreate(&e1));
+                XF_HIP(ctx, hipEventRecord(e0, st));
+            }
+            XF_HIP(ctx, xf_launch_project(st, energy, sim, s->aux, block, begin, count, s->tidx, s->tg[0], s->tg[1],
+                                          s->tg[2], s->tg[3], s->taux, s->X, s->bp, s->comp1_u, s->a_u, dt,
+                                          c == 0 ? s->clock : nullptr));
+            if (timer) {
+                XF_HIP(ctx, hipEventRecord(e1, st));
+                timer->ev_proj.push_back(e0);
+                timer->ev_proj.push_back(e1);
+            }
+        }
+        rc = node(1, k + 1 < substeps);
+        if (rc) return rc;
+    }
+    return XF_OK;
+}
+
+int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return Fail(s->ctx, XF_E_INVALID, "dt must be positive and finite");
+    int rc = ResolveMode(s, energy, sim);
+    if (rc) return rc;
+    if (s->material_dirty) {
+        rc = UploadMaterial(s);
+        if (rc) return rc;
+    }
+    if (s->params_dirty || s->params_dt != dt) {
+        rc = UploadParams(s, dt);
+        if (rc) return rc;
+    }
+    return XF_OK;
+}
+
+int CheckSettings(xf_ctx* ctx, const xf_settings* st) {
+    if (!EnergySupported(st->energy))
+        return Fail(ctx, XF_E_UNSUPPORTED, "energy %u not implemented (3..7 are)", st->energy);
+    if (st->damping != 0.0f || st->pbd_damping != 0.0f)
+        return Fail(ctx, XF_E_UNSUPPORTED, "Rayleigh/PBD damping passes are not implemented (must be 0)");
+    if (st->volume_passes != 0) return Fail(ctx, XF_E_UNSUPPORTED, "volume passes are not implemented (must be 0)");
+    if (!(st->compliance > 0.0f)) return Fail(ctx, XF_E_INVALID, "compliance must be positive");
+    if (!(st->poissons_ratio > 0.0f && st->poissons_ratio <= 0.5f))
+        return Fail(ctx, XF_E_INVALID, "poissons_ratio must be in (0, 0.5]");
+    if (st->collide & ~3u) return Fail(ctx, XF_E_INVALID, "unknown collide bits");
+    if ((st->collide & XF_COLLIDE_SPHERE) && !(st->sphere_r > 0.0f))
+        return Fail(ctx, XF_E_INVALID, "sphere radius must be positive");
+    return XF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+XF_API const char* xf_version(void) { return "xpbdfem-gfx950 0.1"; }
+
+XF_API void xf_settings_default(xf_settings* s) {
+    memset(s, 0, sizeof(*s));
+    s->energy = XF_ENERGY_MIXED_SEL;
+    s->simultaneous = 1;
+    s->compliance = 0.01f;
+    s->poissons_ratio = 0.45f;
+    s->gravity[0] = 0.0f;
+    s->gravity[1] = -9.81f;
+    s->yeoh[0] = 0.1095f;
+    s->yeoh[1] = 14.95f;
+    s->yeoh[2] = 4.595f;
+}
+
+XF_API int xf_ctx_create(int device, xf_ctx** out) {
+    if (!out) return XF_E_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return XF_E_HIP;
+    if (device < 0 || device >= count) return XF_E_INVALID;
+    xf_ctx* ctx = new xf_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+                                                  hipSuccess) {
+        delete ctx;
+        return XF_E_HIP;
+    }
+    *out = ctx;
+    return XF_OK;
+}
+
+XF_API void xf_ctx_destroy(xf_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+XF_API const char* xf_last_error(const xf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+XF_API int xf_scene_create(xf_ctx* ctx, xf_scene** out) {
+    if (!ctx || !out) return XF_E_INVALID;
+    xf_scene* s = new xf_scene();
+    s->ctx = ctx;
+    s->opts.colour_order = XF_COLOUR_NATURAL;
+    s->opts.block_size = 64;
+    s->opts.use_graph = 1;
+    *out = s;
+    return XF_OK;
+}
+
+XF_API void xf_scene_destroy(xf_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    DropGraph(s);
+    void* ptrs[] = {s->X, s->O, s->V, s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->bp, s->clock,
+                    s->counter};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+XF_API int xf_scene_add_body(xf_scene* s, const xf_body_desc* d, uint32_t* body_id) {
+    if (!s || !d) return XF_E_INVALID;
+    xf_ctx* ctx = s->ctx;
+    if (s->finalized) return Fail(ctx, XF_E_STATE, "scene already finalized");
+    if (!d->xyz || !d->tets || d->n_nodes < 4 || d->n_tets < 1)
+        return Fail(ctx, XF_E_INVALID, "body needs xyz, tets, >= 4 nodes and >= 1 tet");
+    if (!(d->density > 0.0f)) return Fail(ctx, XF_E_INVALID, "density must be positive");
+    if ((uint64_t)s->N + d->n_nodes >= (1ull << 31) || (uint64_t)s->T + d->n_tets >= (1ull << 31))
+        return Fail(ctx, XF_E_INVALID, "scene too large");
+    if (s->bodies.size() >= (1u << 30)) return Fail(ctx, XF_E_INVALID, "too many bodies");
+    Body b;
+    b.n = d->n_nodes;
+    b.t = d->n_tets;
+    b.density = d->density;
+    b.xyz.assign(d->xyz, d->xyz + 3 * (size_t)b.n);
+    b.tets.assign(d->tets, d->tets + 4 * (size_t)b.t);
+    for (uint32_t v : b.tets)
+        if (v >= b.n) return Fail(ctx, XF_E_INVALID, "tet index %u out of range (n_nodes %u)", v, b.n);
+    for (uint32_t t = 0; t < b.t; t++) {
+        const uint32_t* is = &b.tets[4 * (size_t)t];
+        for (int i = 0; i < 4; i++)
+            for (int j = i + 1; j < 4; j++)
+                if (is[i] == is[j]) return Fail(ctx, XF_E_INVALID, "tet %u repeats node %u", t, is[i]);
+    }
+    for (float v : b.xyz)
+        if (!std::isfinite(v)) return Fail(ctx, XF_E_INVALID, "non-finite node position");
+    b.vel.assign(3 * (size_t)b.n, 0.0f);
+    if (d->velocity) b.vel.assign(d->velocity, d->velocity + 3 * (size_t)b.n);
+    b.pinned.assign(b.n, 0);
+    if (d->pinned)
+        for (uint32_t i = 0; i < b.n; i++) b.pinned[i] = d->pinned[i] ? 1 : 0;
+    if (d->tet_mu) {
+        b.tet_mu.assign(d->tet_mu, d->tet_mu + b.t);
+        for (float m : b.tet_mu)
+            if (!(m > 0.0f) || !std::isfinite(m)) return Fail(ctx, XF_E_INVALID, "tet_mu must be positive");
+    }
+    xf_settings_default(&b.settings);
+    if (!xf::InitElements(b.xyz.data(), b.n, b.tets.data(), b.t, b.density, &b.ec))
+        return Fail(ctx, XF_E_INVALID, "bad tet indices");
+    for (uint32_t t = 0; t < b.t; t++)
+        if (!(b.ec.vol[t] > 0.0f)) return Fail(ctx, XF_E_INVALID, "tet %u has non-positive rest volume", t);
+    for (uint32_t i = 0; i < b.n; i++) {
+        if (!std::isfinite(b.ec.w[i])) return Fail(ctx, XF_E_INVALID, "node %u belongs to no tet", i);
+        if (b.pinned[i]) b.ec.w[i] = 0.0f;
+    }
+    b.node_base = s->N;
+    s->N += b.n;
+    s->T += b.t;
+    s->bodies.push_back(std::move(b));
+    if (body_id) *body_id = (uint32_t)s->bodies.size() - 1;
+    return XF_OK;
+}
+
+XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
+    if (!s) return XF_E_INVALID;
+    xf_ctx* ctx = s->ctx;
+    if (s->finalized) return Fail(ctx, XF_E_STATE, "scene already finalized");
+    if (s->bodies.empty()) return Fail(ctx, XF_E_STATE, "scene has no bodies");
+    if (opts) s->opts = *opts;
+    if (s->opts.block_size == 0) s->opts.block_size = 64;
+    if (s->opts.block_size % 64 || s->opts.block_size > 256)
+        return Fail(ctx, XF_E_INVALID, "block_size must be 64, 128, 192 or 256");
+    if (s->opts.colour_order > XF_COLOUR_MINSTD) return Fail(ctx, XF_E_INVALID, "unknown colour order");
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+
+    // Colour each body; the scene runs max(ncol) phases.
+    s->ncol = 0;
+    for (Body& b : s->bodies) {
+        b.ncol = xf::ColourSweepOrder(b.n, b.tets.data(), b.t, s->opts.colour_order == XF_COLOUR_MINSTD, &b.colour,
+                                      &b.order);
+        if (b.ncol == 0) return Fail(ctx, XF_E_UNSUPPORTED, "mesh needs more than %u colours", xf::kMaxColours);
+        s->ncol = std::max(s->ncol, b.ncol);
+    }
+    // Schedule: colour-major, body-major within a colour, each body's tets
+    // of that colour in its sweep order.
+    s->colour_begin.assign(s->ncol + 1, 0);
+    std::vector<uint4> h_tidx(s->T);
+    std::vector<float4> h_tg0(s->T), h_tg1(s->T), h_tg2(s->T);
+    s->h_tg3.assign(s->T, make_float4(0, 0, 0, 0));
+    s->h_taux.assign(s->T, make_float4(0, 0, 0, 0));
+    std::vector<uint32_t> cursor(s->bodies.size(), 0);
+    uint32_t pos = 0;
+    for (uint32_t c = 0; c < s->ncol; c++) {
+        s->colour_begin[c] = pos;
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            Body& b = s->bodies[bi];
+            if (b.sched_pos.empty()) b.sched_pos.assign(b.t, 0);
+            uint32_t& k = cursor[bi];
+            while (k < b.t && b.colour[b.order[k]] == c) {
+                uint32_t t = b.order[k++];
+                const uint32_t* is = &b.tets[4 * (size_t)t];
+                const float* q = &b.ec.qi[9 * (size_t)t];
+                const float* qq = &b.ec.qq[3 * (size_t)t];
+                const float* qr = &b.ec.qr[3 * (size_t)t];
+                h_tidx[pos] = make_uint4(b.node_base + is[0], b.node_base + is[1], b.node_base + is[2],
+                                         b.node_base + is[3]);
+                h_tg0[pos] = make_float4(q[0], q[1], q[2], q[3]);
+                h_tg1[pos] = make_float4(q[4], q[5], q[6], q[7]);
+                h_tg2[pos] = make_float4(q[8], qq[0], qq[1], qq[2]);
+                s->h_tg3[pos] = make_float4(qr[0], qr[1], qr[2], 0.0f);
+                uint32_t bid = (uint32_t)bi;
+                float bidf;
+                memcpy(&bidf, &bid, 4);
+                s->h_taux[pos] = make_float4(0.0f, 1.0f, bidf, 0.0f);
+                b.sched_pos[t] = pos;
+                ++pos;
+            }
+        }
+    }
+    s->colour_begin[s->ncol] = pos;
+    if (pos != s->T) return Fail(ctx, XF_E_INVALID, "internal: schedule covers %u of %u tets", pos, s->T);
+
+    // Nodes.
+    std::vector<double4> hX(s->N), hO(s->N), hV(s->N);
+    s->h_w.assign(s->N, 0.0);
+    s->h_oflags.assign(s->N, 0.0);
+    for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+        const Body& b = s->bodies[bi];
+        for (uint32_t i = 0; i < b.n; i++) {
+            uint32_t g = b.node_base + i;
+            double flags = (double)(((uint32_t)bi << 1) | (b.pinned[i] ? 1u : 0u));
+            s->h_w[g] = (double)b.ec.w[i];
+            s->h_oflags[g] = flags;
+            // Node data enters as float and is held in double (Geo.cpp:713).
+            hX[g] = make_double4(b.xyz[3 * i], b.xyz[3 * i + 1], b.xyz[3 * i + 2], s->h_w[g]);
+            hO[g] = make_double4(b.xyz[3 * i], b.xyz[3 * i + 1], b.xyz[3 * i + 2], flags);
+            hV[g] = make_double4(b.vel[3 * i], b.vel[3 * i + 1], b.vel[3 * i + 2], 0.0);
+        }
+    }
+    int rc;
+    if ((rc = DevAlloc(ctx, &s->X, s->N, &s->device_bytes)) || (rc = DevAlloc(ctx, &s->O, s->N, &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->V, s->N, &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->tidx, s->T, &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->bp, s->bodies.size(), &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->clock, 1, &s->device_bytes)) || (rc = DevAlloc(ctx, &s->counter, 1, &s->device_bytes)))
+        return rc;
+    for (int k = 0; k < 4; k++)
+        if ((rc = DevAlloc(ctx, &s->tg[k], s->T, &s->device_bytes))) return rc;
+    hipStream_t st = ctx->stream;
+    XF_HIP(ctx, hipMemcpyAsync(s->X, hX.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->O, hO.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->V, hV.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->tidx, h_tidx.data(), sizeof(uint4) * s->T, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->tg[0], h_tg0.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->tg[1], h_tg1.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->tg[2], h_tg2.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
+    float zero = 0.0f;
+    XF_HIP(ctx, hipMemcpyAsync(s->clock, &zero, sizeof(float), hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipStreamSynchronize(st));
+    s->time_host = 0.0f;
+    s->finalized = true;
+    s->material_dirty = true;
+    s->params_dirty = true;
+    return XF_OK;
+}
+
+XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
+    if (!s || !info) return XF_E_INVALID;
+    info->n_bodies = (uint32_t)s->bodies.size();
+    info->n_nodes = s->N;
+    info->n_tets = s->T;
+    info->n_colours = s->ncol;
+    info->device_bytes = s->device_bytes;
+    return XF_OK;
+}
+
+XF_API int xf_scene_set_settings(xf_scene* s, uint32_t body, const xf_settings* st) {
+    if (!s || !st) return XF_E_INVALID;
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    int rc = CheckSettings(s->ctx, st);
+    if (rc) return rc;
+    s->bodies[body].settings = *st;
+    s->material_dirty = true;
+    s->params_dirty = true;
+    return XF_OK;
+}
+
+XF_API int xf_scene_set_tet_mu(xf_scene* s, uint32_t body, const float* tet_mu) {
+    if (!s || !tet_mu) return XF_E_INVALID;
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    Body& b = s->bodies[body];
+    for (uint32_t t = 0; t < b.t; t++)
+        if (!(tet_mu[t] > 0.0f) || !std::isfinite(tet_mu[t]))
+            return Fail(s->ctx, XF_E_INVALID, "tet_mu must be positive and finite");
+    b.tet_mu.assign(tet_mu, tet_mu + b.t);
+    s->material_dirty = true;
+    return XF_OK;
+}
+
+XF_API int xf_scene_step(xf_scene* s, uint32_t substeps, float dt) {
+    if (!s) return XF_E_INVALID;
+    if (substeps == 0) return XF_OK;
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    int energy;
+    bool sim;
+    int rc = Prepare(s, dt, &energy, &sim);
+    if (rc) return rc;
+    if (!s->opts.use_graph) {
+        rc = Enqueue(s, substeps, dt, energy, sim, nullptr);
+    } else {
+        if (!s->graph_exec || s->graph_substeps != substeps || s->graph_dt != dt || s->graph_energy != energy ||
+            s->graph_sim != sim || s->graph_aux != s->aux) {
+            DropGraph(s);
+            XF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            rc = Enqueue(s, substeps, dt, energy, sim, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return Fail(ctx, XF_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            s->graph = g;
+            XF_HIP(ctx, hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+            s->graph_substeps = substeps;
+            s->graph_dt = dt;
+            s->graph_energy = energy;
+            s->graph_sim = sim;
+            s->graph_aux = s->aux;
+        }
+        XF_HIP(ctx, hipGraphLaunch(s->graph_exec, ctx->stream));
+    }
+    if (rc) return rc;
+    for (uint32_t k = 0; k < substeps; k++) s->time_host = s->time_host + dt;
+    return XF_OK;
+}
+
+XF_API int xf_scene_sync(xf_scene* s) {
+    if (!s) return XF_E_INVALID;
+    XF_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return XF_OK;
+}
+
+XF_API int xf_scene_get_state(xf_scene* s, uint32_t body, double* x, double* v) {
+    if (!s) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    const Body& b = s->bodies[body];
+    std::vector<double4> tmp(b.n);
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    if (x) {
+        XF_HIP(ctx, hipMemcpyAsync(tmp.data(), s->X + b.node_base, sizeof(double4) * b.n, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < b.n; i++) {
+            x[3 * i] = tmp[i].x;
+            x[3 * i + 1] = tmp[i].y;
+            x[3 * i + 2] = tmp[i].z;
+        }
+    }
+    if (v) {
+        XF_HIP(ctx, hipMemcpyAsync(tmp.data(), s->V + b.node_base, sizeof(double4) * b.n, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < b.n; i++) {
+            v[3 * i] = tmp[i].x;
+            v[3 * i + 1] = tmp[i].y;
+            v[3 * i + 2] = tmp[i].z;
+        }
+    }
+    return XF_OK;
+}
+
+XF_API int xf_scene_set_state(xf_scene* s, uint32_t body, const double* x, const double* v) {
+    if (!s) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    const Body& b = s->bodies[body];
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<double4> tmp(b.n);
+    if (x) {
+        for (uint32_t i = 0; i < b.n; i++)
+            tmp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], s->h_w[b.node_base + i]);
+        XF_HIP(ctx, hipMemcpyAsync(s->X + b.node_base, tmp.data(), sizeof(double4) * b.n, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < b.n; i++) tmp[i].w = s->h_oflags[b.node_base + i];
+        XF_HIP(ctx, hipMemcpyAsync(s->O + b.node_base, tmp.data(), sizeof(double4) * b.n, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (v) {
+        for (uint32_t i = 0; i < b.n; i++) tmp[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0);
+        XF_HIP(ctx, hipMemcpyAsync(s->V + b.node_base, tmp.data(), sizeof(double4) * b.n, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return XF_OK;
+}
+
+XF_API int xf_scene_get_time(const xf_scene* s, float* t) {
+    if (!s || !t) return XF_E_INVALID;
+    *t = s->time_host;
+    return XF_OK;
+}
+
+XF_API int xf_scene_set_time(xf_scene* s, float t) {
+    if (!s) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    XF_HIP(ctx, hipMemcpyAsync(s->clock, &t, sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->time_host = t;
+    return XF_OK;
+}
+
+XF_API int xf_scene_get_order(const xf_scene* s, uint32_t body, uint32_t* order, uint32_t* colour_of_tet) {
+    if (!s) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    const Body& b = s->bodies[body];
+    if (order) memcpy(order, b.order.data(), sizeof(uint32_t) * b.t);
+    if (colour_of_tet) memcpy(colour_of_tet, b.colour.data(), sizeof(uint32_t) * b.t);
+    return XF_OK;
+}
+
+XF_API int xf_scene_get_elements(const xf_scene* s, uint32_t body, float* qi9, float* qq3, float* qr3, float* vol,
+                                 float* w) {
+    if (!s) return XF_E_INVALID;
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    const Body& b = s->bodies[body];
+    if (qi9) memcpy(qi9, b.ec.qi.data(), sizeof(float) * b.ec.qi.size());
+    if (qq3) memcpy(qq3, b.ec.qq.data(), sizeof(float) * b.ec.qq.size());
+    if (qr3) memcpy(qr3, b.ec.qr.data(), sizeof(float) * b.ec.qr.size());
+    if (vol) memcpy(vol, b.ec.vol.data(), sizeof(float) * b.ec.vol.size());
+    if (w) memcpy(w, b.ec.w.data(), sizeof(float) * b.ec.w.size());
+    return XF_OK;
+}
+
+XF_API int xf_scene_profile(xf_scene* s, uint32_t substeps, float dt, float* ms_project, float* ms_node,
+                            uint32_t* launches_project) {
+    if (!s) return XF_E_INVALID;
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    int energy;
+    bool sim;
+    int rc = Prepare(s, dt, &energy, &sim);
+    if (rc) return rc;
+    LaunchTimer timer;
+    rc = Enqueue(s, substeps, dt, energy, sim, &timer);
+    hipError_t se = hipStreamSynchronize(ctx->stream);
+    float tp = 0.0f, tn = 0.0f;
+    auto sum = [&](std::vector<hipEvent_t>& ev, float* acc) {
+        for (size_t k = 0; k + 1 < ev.size(); k += 2) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) *acc += ms;
+        }
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    };
+    uint32_t nproj = (uint32_t)(timer.ev_proj.size() / 2);
+    sum(timer.ev_proj, &tp);
+    sum(timer.ev_node, &tn);
+    if (rc) return rc;
+    if (se != hipSuccess) return Fail(ctx, XF_E_HIP, "profile sync: %s", hipGetErrorString(se));
+    for (uint32_t k = 0; k < substeps; k++) s->time_host = s->time_host + dt;
+    if (ms_project) *ms_project = tp;
+    if (ms_node) *ms_node = tn;
+    if (launches_project) *launches_project = nproj;
+    return XF_OK;
+}
+
+XF_API int xf_scene_count_nonfinite(xf_scene* s, uint32_t* count) {
+    if (!s || !count) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    uint32_t zero = 0;
+    XF_HIP(ctx, hipMemcpyAsync(s->counter, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    XF_HIP(ctx, xf_launch_count_nonfinite(ctx->stream, s->N, s->X, s->counter));
+    XF_HIP(ctx, hipMemcpyAsync(count, s->counter, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XF_OK;
+}
+
+XF_API int xf_mesh_tet_block(uint32_t W, uint32_t H, uint32_t D, const float scale[3], const float origin[3],
+                             uint32_t mirrored, float* xyz, uint32_t* tets) {
+    if (!W || !H || !D || !scale || !origin || !xyz || !tets) return XF_E_INVALID;
+    if ((uint64_t)W * H * D * 6 >= (1ull << 31)) return XF_E_INVALID;
+    xf::TetBlock(W, H, D, scale, origin, mirrored != 0, xyz, tets);
+    return XF_OK;
+}
+
+XF_API int xf_minstd_order(uint32_t n, uint32_t* order) {
+    if (!order) return XF_E_INVALID;
+    xf::MinstdOrder(n, order);
+    return XF_OK;
+}
+
+XF_API int xf_normal_fill(uint64_t seed, float sigma, uint32_t n, float* out) {
+    if (!out) return XF_E_INVALID;
+    xf::NormalFill(seed, sigma, n, out);
+    return XF_OK;
+}
+
+XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, uint32_t colour_order,
+                          uint32_t* order, uint32_t* colour, uint32_t* n_colours) {
+    if (!tets || !order || !n_colours || colour_order > XF_COLOUR_MINSTD) return XF_E_INVALID;
+    for (uint64_t k = 0; k < 4ull * n_tets; k++)
+        if (tets[k] >= n_nodes) return XF_E_INVALID;
+    std::vector<uint32_t> col, ord;
+    uint32_t nc = xf::ColourSweepOrder(n_nodes, tets, n_tets, colour_order == XF_COLOUR_MINSTD, &col, &ord);
+    if (nc == 0) return XF_E_UNSUPPORTED;
+    memcpy(order, ord.data(), sizeof(uint32_t) * n_tets);
+    if (colour) memcpy(colour, col.data(), sizeof(uint32_t) * n_tets);
+    *n_colours = nc;
+    return XF_OK;
+}
+
+XF_API int xf_ldlt_solve(uint32_t n, const float* A, const float* b, float* x) {
+    if (!A || !b || !x) return XF_E_INVALID;
+    return xf::LdltSolve(n, A, b, x) == 0 ? XF_OK : XF_E_INVALID;
+}
+
+}  // extern "C"

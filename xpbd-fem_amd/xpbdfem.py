@@ -1,0 +1,346 @@
+"""Python binding of libxpbdfem.so (the C ABI declared in include/xpbdfem.h).
+
+Thin ctypes layer used by the tests and the benchmark. All computation runs
+in the HIP library; this module only marshals numpy arrays. There is no CPU
+fallback: loading fails loudly when the library is missing, and creating a
+context fails when no GPU is visible.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxpbdfem.so")
+
+ENERGY_MIXED, ENERGY_MIXED_SEL, ENERGY_YEOH_SKIN, ENERGY_YEOH_SKIN_SEL, ENERGY_YEOH_SKIN_FAST = 3, 4, 5, 6, 7
+COLLIDE_GROUND, COLLIDE_SPHERE = 1, 2
+COLOUR_NATURAL, COLOUR_MINSTD = 0, 1
+SKIN_YEOH = (0.1095, 14.95, 4.595)  # Fem.cpp:526
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+class XfError(RuntimeError):
+    pass
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("energy", C.c_uint32),
+        ("simultaneous", C.c_uint32),
+        ("compliance", C.c_float),
+        ("poissons_ratio", C.c_float),
+        ("gravity", C.c_float * 2),
+        ("drag", C.c_float),
+        ("damping", C.c_float),
+        ("pbd_damping", C.c_float),
+        ("volume_passes", C.c_uint32),
+        ("yeoh", C.c_float * 3),
+        ("collide", C.c_uint32),
+        ("ground_y", C.c_float),
+        ("friction", C.c_float),
+        ("sphere_c0", C.c_float * 3),
+        ("sphere_v", C.c_float * 3),
+        ("sphere_r", C.c_float),
+    ]
+
+
+class BodyDesc(C.Structure):
+    _fields_ = [
+        ("xyz", C.c_void_p),
+        ("n_nodes", C.c_uint32),
+        ("tets", C.c_void_p),
+        ("n_tets", C.c_uint32),
+        ("density", C.c_float),
+        ("pinned", C.c_void_p),
+        ("velocity", C.c_void_p),
+        ("tet_mu", C.c_void_p),
+    ]
+
+
+class SceneOpts(C.Structure):
+    _fields_ = [("colour_order", C.c_uint32), ("block_size", C.c_uint32), ("use_graph", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_bodies", C.c_uint32), ("n_nodes", C.c_uint32), ("n_tets", C.c_uint32),
+                ("n_colours", C.c_uint32), ("device_bytes", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libxpbdfem.so from the package directory (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise XfError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "xf_version": ([], C.c_char_p),
+        "xf_settings_default": ([C.POINTER(Settings)], None),
+        "xf_ctx_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+        "xf_ctx_destroy": ([vp], None),
+        "xf_last_error": ([vp], C.c_char_p),
+        "xf_scene_create": ([vp, C.POINTER(vp)], C.c_int),
+        "xf_scene_destroy": ([vp], None),
+        "xf_scene_add_body": ([vp, C.POINTER(BodyDesc), C.POINTER(C.c_uint32)], C.c_int),
+        "xf_scene_finalize": ([vp, C.POINTER(SceneOpts)], C.c_int),
+        "xf_scene_info_get": ([vp, C.POINTER(SceneInfo)], C.c_int),
+        "xf_scene_set_settings": ([vp, C.c_uint32, C.POINTER(Settings)], C.c_int),
+        "xf_scene_set_tet_mu": ([vp, C.c_uint32, _f32p], C.c_int),
+        "xf_scene_step": ([vp, C.c_uint32, C.c_float], C.c_int),
+        "xf_scene_sync": ([vp], C.c_int),
+        "xf_scene_get_state": ([vp, C.c_uint32, vp, vp], C.c_int),
+        "xf_scene_set_state": ([vp, C.c_uint32, vp, vp], C.c_int),
+        "xf_scene_get_time": ([vp, C.POINTER(C.c_float)], C.c_int),
+        "xf_scene_set_time": ([vp, C.c_float], C.c_int),
+        "xf_scene_get_order": ([vp, C.c_uint32, vp, vp], C.c_int),
+        "xf_scene_get_elements": ([vp, C.c_uint32, vp, vp, vp, vp, vp], C.c_int),
+        "xf_scene_profile": ([vp, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                              C.POINTER(C.c_uint32)], C.c_int),
+        "xf_scene_count_nonfinite": ([vp, C.POINTER(C.c_uint32)], C.c_int),
+        "xf_mesh_tet_block": ([C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, _u32p],
+                              C.c_int),
+        "xf_minstd_order": ([C.c_uint32, _u32p], C.c_int),
+        "xf_normal_fill": ([C.c_uint64, C.c_float, C.c_uint32, _f32p], C.c_int),
+        "xf_ldlt_solve": ([C.c_uint32, _f32p, _f32p, _f32p], C.c_int),
+        "xf_mesh_colour": ([C.c_uint32, _u32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint32)],
+                           C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the binding expects the library to export."""
+    return [
+        "xf_version", "xf_settings_default", "xf_ctx_create", "xf_ctx_destroy", "xf_last_error", "xf_scene_create",
+        "xf_scene_destroy", "xf_scene_add_body", "xf_scene_finalize", "xf_scene_info_get", "xf_scene_set_settings",
+        "xf_scene_set_tet_mu", "xf_scene_step", "xf_scene_sync", "xf_scene_get_state", "xf_scene_set_state",
+        "xf_scene_get_time", "xf_scene_set_time", "xf_scene_get_order", "xf_scene_get_elements", "xf_scene_profile",
+        "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
+        "xf_mesh_colour",
+    ]
+
+
+# -------------------------------------------------------------- utilities
+
+def tet_block(W, H, D, scale, origin, mirrored=False):
+    n = (W + 1) * (H + 1) * (D + 1)
+    xyz = np.zeros(3 * n, np.float32)
+    tets = np.zeros(24 * W * H * D, np.uint32)
+    rc = lib().xf_mesh_tet_block(W, H, D, np.asarray(scale, np.float32), np.asarray(origin, np.float32),
+                                 int(mirrored), xyz, tets)
+    if rc:
+        raise XfError(f"xf_mesh_tet_block failed ({rc})")
+    return xyz, tets
+
+
+def minstd_order(n):
+    o = np.zeros(n, np.uint32)
+    lib().xf_minstd_order(n, o)
+    return o
+
+
+def normal_fill(seed, sigma, n):
+    out = np.zeros(n, np.float32)
+    lib().xf_normal_fill(seed, sigma, n, out)
+    return out
+
+
+def mesh_colour(n_nodes, tets, colour_order=COLOUR_NATURAL):
+    """Host-side greedy colouring: (sweep order, colour per tet, colour count)."""
+    tets = np.ascontiguousarray(tets, np.uint32)
+    t = tets.size // 4
+    order = np.zeros(t, np.uint32)
+    colour = np.zeros(t, np.uint32)
+    nc = C.c_uint32()
+    rc = lib().xf_mesh_colour(n_nodes, tets, t, colour_order, order, colour, C.byref(nc))
+    if rc:
+        raise XfError(f"xf_mesh_colour failed ({rc})")
+    return order, colour, nc.value
+
+
+def ldlt_solve(A_packed, b):
+    A = np.ascontiguousarray(A_packed, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    x = np.zeros_like(b)
+    rc = lib().xf_ldlt_solve(b.size, A, b, x)
+    if rc:
+        raise XfError("xf_ldlt_solve failed")
+    return x
+
+
+def default_settings(**kw):
+    s = Settings()
+    lib().xf_settings_default(C.byref(s))
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        fld = getattr(s, k)
+        if isinstance(fld, C.Array):
+            fld[:] = v
+        else:
+            setattr(s, k, v)
+    return s
+
+
+# ---------------------------------------------------------------- objects
+
+class Context:
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.xf_ctx_create(device, C.byref(h))
+        if rc:
+            raise XfError(f"xf_ctx_create(device={device}) failed ({rc}): no usable HIP device")
+        self.h = h
+
+    def check(self, rc, what):
+        if rc:
+            msg = lib().xf_last_error(self.h).decode()
+            raise XfError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self.h:
+            lib().xf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().xf_scene_create(ctx.h, C.byref(h)), "xf_scene_create")
+        self.h = h
+        self.bodies = []  # (n_nodes, n_tets)
+        self._keep = []
+
+    def add_body(self, xyz, tets, density, pinned=None, velocity=None, tet_mu=None):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+        tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+        d = BodyDesc()
+        d.xyz = xyz.ctypes.data
+        d.n_nodes = xyz.size // 3
+        d.tets = tets.ctypes.data
+        d.n_tets = tets.size // 4
+        d.density = density
+        keep = [xyz, tets]
+        if pinned is not None:
+            p = np.ascontiguousarray(pinned, np.uint8).reshape(-1)
+            d.pinned = p.ctypes.data
+            keep.append(p)
+        if velocity is not None:
+            v = np.ascontiguousarray(velocity, np.float32).reshape(-1)
+            d.velocity = v.ctypes.data
+            keep.append(v)
+        if tet_mu is not None:
+            m = np.ascontiguousarray(tet_mu, np.float32).reshape(-1)
+            d.tet_mu = m.ctypes.data
+            keep.append(m)
+        bid = C.c_uint32()
+        self.ctx.check(lib().xf_scene_add_body(self.h, C.byref(d), C.byref(bid)), "xf_scene_add_body")
+        self.bodies.append((d.n_nodes, d.n_tets))
+        return bid.value
+
+    def finalize(self, colour_order=COLOUR_NATURAL, block_size=64, use_graph=True):
+        o = SceneOpts()
+        o.colour_order = colour_order
+        o.block_size = block_size
+        o.use_graph = 1 if use_graph else 0
+        self.ctx.check(lib().xf_scene_finalize(self.h, C.byref(o)), "xf_scene_finalize")
+
+    def info(self):
+        i = SceneInfo()
+        self.ctx.check(lib().xf_scene_info_get(self.h, C.byref(i)), "xf_scene_info_get")
+        return dict(n_bodies=i.n_bodies, n_nodes=i.n_nodes, n_tets=i.n_tets, n_colours=i.n_colours,
+                    device_bytes=i.device_bytes)
+
+    def set_settings(self, body, settings=None, **kw):
+        s = settings if settings is not None else default_settings(**kw)
+        self.ctx.check(lib().xf_scene_set_settings(self.h, body, C.byref(s)), "xf_scene_set_settings")
+
+    def set_tet_mu(self, body, mu):
+        mu = np.ascontiguousarray(mu, np.float32)
+        self.ctx.check(lib().xf_scene_set_tet_mu(self.h, body, mu), "xf_scene_set_tet_mu")
+
+    def step(self, substeps, dt):
+        self.ctx.check(lib().xf_scene_step(self.h, substeps, dt), "xf_scene_step")
+
+    def sync(self):
+        self.ctx.check(lib().xf_scene_sync(self.h), "xf_scene_sync")
+
+    def get_state(self, body=0):
+        n = self.bodies[body][0]
+        x = np.zeros(3 * n, np.float64)
+        v = np.zeros(3 * n, np.float64)
+        self.ctx.check(lib().xf_scene_get_state(self.h, body, x.ctypes.data, v.ctypes.data), "xf_scene_get_state")
+        return x, v
+
+    def set_state(self, body, x=None, v=None):
+        xp = np.ascontiguousarray(x, np.float64) if x is not None else None
+        vp = np.ascontiguousarray(v, np.float64) if v is not None else None
+        self.ctx.check(lib().xf_scene_set_state(self.h, body, xp.ctypes.data if xp is not None else None,
+                                                vp.ctypes.data if vp is not None else None), "xf_scene_set_state")
+
+    def time(self):
+        t = C.c_float()
+        self.ctx.check(lib().xf_scene_get_time(self.h, C.byref(t)), "xf_scene_get_time")
+        return t.value
+
+    def set_time(self, t):
+        self.ctx.check(lib().xf_scene_set_time(self.h, t), "xf_scene_set_time")
+
+    def order(self, body=0):
+        t = self.bodies[body][1]
+        o = np.zeros(t, np.uint32)
+        c = np.zeros(t, np.uint32)
+        self.ctx.check(lib().xf_scene_get_order(self.h, body, o.ctypes.data, c.ctypes.data), "xf_scene_get_order")
+        return o, c
+
+    def elements(self, body=0):
+        n, t = self.bodies[body]
+        out = dict(qi=np.zeros(9 * t, np.float32), qq=np.zeros(3 * t, np.float32), qr=np.zeros(3 * t, np.float32),
+                   vol=np.zeros(t, np.float32), w=np.zeros(n, np.float32))
+        self.ctx.check(lib().xf_scene_get_elements(self.h, body, out["qi"].ctypes.data, out["qq"].ctypes.data,
+                                                   out["qr"].ctypes.data, out["vol"].ctypes.data,
+                                                   out["w"].ctypes.data), "xf_scene_get_elements")
+        return out
+
+    def profile(self, substeps, dt):
+        mp, mn, nl = C.c_float(), C.c_float(), C.c_uint32()
+        self.ctx.check(lib().xf_scene_profile(self.h, substeps, dt, C.byref(mp), C.byref(mn), C.byref(nl)),
+                       "xf_scene_profile")
+        return mp.value, mn.value, nl.value
+
+    def count_nonfinite(self):
+        c = C.c_uint32()
+        self.ctx.check(lib().xf_scene_count_nonfinite(self.h, C.byref(c)), "xf_scene_count_nonfinite")
+        return c.value
+
+    def close(self):
+        if self.h:
+            lib().xf_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
