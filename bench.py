@@ -69,12 +69,53 @@ def cpu_baseline(wl, budget_s=12.0):
                 sample=f"{what}: {n} substeps of the {t}-tet beam, single thread, {el:.1f} s")
 
 
+def batched_leg(ctx, frames=3):
+    """256 boxes: one batched scene (colour c of all bodies per launch,
+    autotuned workgroup size) against the plain per-body launch loop (one
+    scene per body, stepped one after another)."""
+    b = workloads.boxes_batch()
+    S, dt = b["substeps"], float(b["dt"])
+    T = sum(x["tets"].size // 4 for x in b["bodies"])
+    sc = workloads.make_batch_scene(ctx, b)
+    block, _ = sc.autotune(S, dt, reps=2)
+    sc.step(S, dt)
+    sc.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sc.step(S, dt)
+    sc.sync()
+    el_b = time.perf_counter() - t0
+    finite = sc.count_nonfinite() == 0
+    ncol = sc.info()["n_colours"]
+    sc.close()
+    singles = [workloads.make_batch_scene(ctx, b, bodies=[i]) for i in range(len(b["bodies"]))]
+    for s in singles:
+        s.step(S, dt)
+    for s in singles:
+        s.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        for s in singles:
+            s.step(S, dt)
+    for s in singles:
+        s.sync()
+    el_p = time.perf_counter() - t0
+    for s in singles:
+        s.close()
+    batched = T * S * frames / el_b
+    plain = T * S * frames / el_p
+    return {"workload": f"{b['name']}: {len(b['bodies'])} bodies, {T} tets, {S} substeps/frame, {ncol} colours",
+            "batched_tet_substeps_per_s": batched, "plain_loop_tet_substeps_per_s": plain,
+            "batched_over_plain": batched / plain, "frames": frames, "block_size": block, "finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
@@ -128,6 +169,8 @@ def main():
                      "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
                      "bytes_per_launch": bytes_proj / max(n_launch, 1)},
     }
+    if not args.no_batch:
+        out["batched"] = batched_leg(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out))

@@ -182,6 +182,13 @@ XF_API int xf_scene_get_elements(const xf_scene* scene, uint32_t body, float* qi
  * projection launches. */
 XF_API int xf_scene_profile(xf_scene* scene, uint32_t substeps, float dt, float* ms_project, float* ms_node,
                             uint32_t* launches_project);
+/* Autotuner: times `reps` calls of xf_scene_step(substeps, dt) for each
+ * projection workgroup size (64, 128, 192, 256) and keeps the fastest in the
+ * scene options. The node state and clock are restored afterwards, so tuning
+ * does not advance the simulation. chosen / best_ms (ms per call) nullable.
+ * No reference counterpart (the reference is single-threaded). */
+XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,
+                             float* best_ms);
 /* Counts non-finite node positions (0 = all finite). */
 XF_API int xf_scene_count_nonfinite(xf_scene* scene, uint32_t* count);
 

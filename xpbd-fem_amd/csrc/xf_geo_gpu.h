@@ -1,0 +1,174 @@
+// xf_geo_gpu.h -- header-only C++ adapter with the step interface of the
+// reference's GeoLinear3d (Geo.h:146-164; Geo3d::Substep Geo.cpp:305-356),
+// backed by the gfx950 library through the C ABI of include/xpbdfem.h.
+//
+// The reference calls geos[i]->Substep(settings, manip, dt) once per substep
+// (Demo.cpp:86-88). GeoGpuLinear3d::Substep has the same signature and
+// meaning; Frame() runs several substeps as one captured hipGraph.
+//
+// The settings and manipulator types are template parameters: any struct
+// with the reference's Settings field names (flags, compliance,
+// poissonsRatio, gravity.x/.y, damping, pbdDamping, timeCorrectedDrag,
+// volumePasses; Settings.h:79-102) works, so the adapter needs none of the
+// reference's headers. The manipulator is accepted for signature
+// compatibility; interactive picking is not supported (pickedGeo must not
+// point at this body).
+#pragma once
+
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xpbdfem.h"
+
+namespace xfgpu {
+
+// Flag layout of Settings::flags (Settings.h:9-29).
+constexpr uint32_t kEnergyBit = 6, kEnergyMask = 31, kXpbdSolveBit = 11, kLockLeft = 1u << 26;
+
+class GeoGpuLinear3d {
+   public:
+    GeoGpuLinear3d() = default;
+    GeoGpuLinear3d(const GeoGpuLinear3d&) = delete;
+    GeoGpuLinear3d& operator=(const GeoGpuLinear3d&) = delete;
+    ~GeoGpuLinear3d() {
+        if (scene_) xf_scene_destroy(scene_);
+        if (ctx_) xf_ctx_destroy(ctx_);
+    }
+
+    // GeoLinear3d::Init (Geo.cpp:697): node data xyz, index stream
+    // [4, i0, i1, i2, i3]* (only tets). Nodes whose normalised x is below 2%
+    // get the Left flag (Geo.cpp:724); they are pinned when the first
+    // Substep sees Settings_LockLeft. Returns an xf_status.
+    int Init(float density, const float* nodeData, uint32_t nodeDataCount, const uint32_t* idxData,
+             uint32_t idxDataCount, int device = 0) {
+        if (nodeDataCount % 3) return XF_E_INVALID;
+        nodes_.assign(nodeData, nodeData + nodeDataCount);
+        tets_.clear();
+        for (uint32_t i = 0; i < idxDataCount;) {
+            if (idxData[i] != 4 || i + 4 >= idxDataCount) return XF_E_UNSUPPORTED;  // tets only
+            tets_.insert(tets_.end(), idxData + i + 1, idxData + i + 5);
+            i += 5;
+        }
+        density_ = density;
+        n_ = nodeDataCount / 3;
+        float mn = 1e24f, mx = -1e24f;
+        for (uint32_t i = 0; i < n_; i++) {
+            mn = std::fmin(mn, nodes_[3 * i]);
+            mx = std::fmax(mx, nodes_[3 * i]);
+        }
+        left_.assign(n_, 0);
+        for (uint32_t i = 0; i < n_; i++)
+            if ((nodes_[3 * i] - mn) / (mx - mn) < 0.02f) left_[i] = 1;
+        return xf_ctx_create(device, &ctx_);
+    }
+
+    // Geo3d::Substep: one substep of length dt.
+    template <class SettingsT, class ManipT>
+    int Substep(const SettingsT& settings, const ManipT& /*manip*/, float dt) {
+        return Frame(settings, 1, dt);
+    }
+
+    // `substeps` consecutive Substep calls, enqueued as one hipGraph.
+    template <class SettingsT>
+    int Frame(const SettingsT& settings, uint32_t substeps, float dt) {
+        int rc = EnsureScene(settings.flags);
+        if (rc) return rc;
+        xf_settings s;
+        xf_settings_default(&s);
+        s.energy = (settings.flags >> kEnergyBit) & kEnergyMask;
+        s.simultaneous = (settings.flags >> kXpbdSolveBit) & 1u;
+        s.compliance = settings.compliance;
+        s.poissons_ratio = settings.poissonsRatio;
+        s.gravity[0] = settings.gravity.x;
+        s.gravity[1] = settings.gravity.y;
+        s.drag = settings.timeCorrectedDrag;
+        s.damping = settings.damping;
+        s.pbd_damping = settings.pbdDamping;
+        s.volume_passes = settings.volumePasses;
+        s.collide = collide_;
+        s.ground_y = ground_y_;
+        s.friction = friction_;
+        if ((rc = xf_scene_set_settings(scene_, 0, &s))) return rc;
+        return xf_scene_step(scene_, substeps, dt);
+    }
+
+    // Ground-plane contact (no reference counterpart).
+    void SetGround(float y, float friction) {
+        collide_ |= XF_COLLIDE_GROUND;
+        ground_y_ = y;
+        friction_ = friction;
+    }
+
+    // GeoLinear3d::CalculateVolume (Geo.cpp:827-832): sum of signed tet volumes.
+    float CalculateVolume() {
+        std::vector<double> X(3 * (size_t)n_);
+        if (GetState(X.data(), nullptr)) return NAN;
+        float volume = 0.0f;
+        for (size_t t = 0; t < tets_.size() / 4; t++) {
+            const uint32_t* is = &tets_[4 * t];
+            float m[3][3];
+            for (int c = 0; c < 3; c++)
+                for (int r = 0; r < 3; r++) m[c][r] = (float)(X[3 * is[c] + r] - X[3 * is[3] + r]);
+            float det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                        m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+                        m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+            volume += (1.0f / 6.0f) * det;
+        }
+        return volume;
+    }
+
+    uint32_t VertCount() const { return n_; }
+    uint32_t ElementCount() const { return (uint32_t)(tets_.size() / 4); }
+
+    // FP64 positions / velocities (the reference's dvec3 X, V).
+    int GetState(double* X, double* V) {
+        if (!scene_) {
+            if (X)
+                for (uint32_t i = 0; i < 3 * n_; i++) X[i] = nodes_[i];
+            if (V) memset(V, 0, sizeof(double) * 3 * n_);
+            return XF_OK;
+        }
+        int rc = xf_scene_sync(scene_);
+        return rc ? rc : xf_scene_get_state(scene_, 0, X, V);
+    }
+
+    const char* LastError() const { return ctx_ ? xf_last_error(ctx_) : "no context"; }
+
+   private:
+    int EnsureScene(uint32_t flags) {
+        if (scene_) return XF_OK;
+        if (!ctx_) return XF_E_STATE;
+        int rc = xf_scene_create(ctx_, &scene_);
+        if (rc) return rc;
+        xf_body_desc d;
+        memset(&d, 0, sizeof(d));
+        d.xyz = nodes_.data();
+        d.n_nodes = n_;
+        d.tets = tets_.data();
+        d.n_tets = (uint32_t)(tets_.size() / 4);
+        d.density = density_;
+        d.pinned = (flags & kLockLeft) ? left_.data() : nullptr;
+        if ((rc = xf_scene_add_body(scene_, &d, nullptr))) return rc;
+        xf_scene_opts o;
+        o.colour_order = XF_COLOUR_NATURAL;
+        o.block_size = 64;
+        o.use_graph = 1;
+        return xf_scene_finalize(scene_, &o);
+    }
+
+    xf_ctx* ctx_ = nullptr;
+    xf_scene* scene_ = nullptr;
+    std::vector<float> nodes_;
+    std::vector<uint32_t> tets_;
+    std::vector<uint8_t> left_;
+    float density_ = 0.0f;
+    uint32_t n_ = 0;
+    uint32_t collide_ = 0;
+    float ground_y_ = 0.0f, friction_ = 0.0f;
+};
+
+}  // namespace xfgpu

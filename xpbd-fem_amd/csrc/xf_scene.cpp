@@ -739,6 +739,87 @@ XF_API int xf_scene_profile(xf_scene* s, uint32_t substeps, float dt, float* ms_
     return XF_OK;
 }
 
+XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,
+                             float* best_ms) {
+    if (!s || substeps == 0 || reps == 0) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    int energy;
+    bool sim;
+    int rc = Prepare(s, dt, &energy, &sim);
+    if (rc) return rc;
+    // Snapshot of the node state and clock, restored before every trial and
+    // at the end: tuning does not advance the simulation.
+    double4 *sx = nullptr, *so = nullptr, *sv = nullptr;
+    float* sc = nullptr;
+    uint64_t dummy = 0;
+    if ((rc = DevAlloc(ctx, &sx, s->N, &dummy)) || (rc = DevAlloc(ctx, &so, s->N, &dummy)) ||
+        (rc = DevAlloc(ctx, &sv, s->N, &dummy)) || (rc = DevAlloc(ctx, &sc, 1, &dummy))) {
+        for (void* p : {(void*)sx, (void*)so, (void*)sv, (void*)sc})
+            if (p) (void)hipFree(p);
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    auto copy_all = [&](bool save) -> hipError_t {
+        hipError_t e;
+        if ((e = hipMemcpyAsync(save ? sx : s->X, save ? s->X : sx, sizeof(double4) * s->N, hipMemcpyDeviceToDevice,
+                                st)))
+            return e;
+        if ((e = hipMemcpyAsync(save ? so : s->O, save ? s->O : so, sizeof(double4) * s->N, hipMemcpyDeviceToDevice,
+                                st)))
+            return e;
+        if ((e = hipMemcpyAsync(save ? sv : s->V, save ? s->V : sv, sizeof(double4) * s->N, hipMemcpyDeviceToDevice,
+                                st)))
+            return e;
+        if ((e = hipMemcpyAsync(save ? sc : s->clock, save ? s->clock : sc, sizeof(float), hipMemcpyDeviceToDevice,
+                                st)))
+            return e;
+        return hipStreamSynchronize(st);
+    };
+    const uint32_t candidates[] = {64, 128, 192, 256};
+    xf_scene_opts keep = s->opts;
+    uint32_t best_block = keep.block_size;
+    float best = -1.0f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = copy_all(true);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    float time_keep = s->time_host;
+    for (uint32_t bs : candidates) {
+        if (he != hipSuccess || rc) break;
+        s->opts.block_size = bs;
+        DropGraph(s);
+        if ((he = copy_all(false)) != hipSuccess) break;
+        if ((rc = xf_scene_step(s, substeps, dt))) break;  // warm-up and graph capture
+        if ((he = hipEventRecord(e0, st)) != hipSuccess) break;
+        for (uint32_t r = 0; r < reps && !rc; r++) rc = xf_scene_step(s, substeps, dt);
+        if (rc) break;
+        if ((he = hipEventRecord(e1, st)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess) break;
+        float ms = 0.0f;
+        if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+        ms /= (float)reps;
+        if (best < 0.0f || ms < best) {
+            best = ms;
+            best_block = bs;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    hipError_t he2 = copy_all(false);
+    for (void* p : {(void*)sx, (void*)so, (void*)sv, (void*)sc}) (void)hipFree(p);
+    s->time_host = time_keep;
+    s->opts = keep;
+    s->opts.block_size = best_block;
+    DropGraph(s);
+    if (rc) return rc;
+    if (he != hipSuccess) return Fail(ctx, XF_E_HIP, "autotune: %s", hipGetErrorString(he));
+    if (he2 != hipSuccess) return Fail(ctx, XF_E_HIP, "autotune restore: %s", hipGetErrorString(he2));
+    if (chosen) *chosen = s->opts;
+    if (best_ms) *best_ms = best;
+    return XF_OK;
+}
+
 XF_API int xf_scene_count_nonfinite(xf_scene* s, uint32_t* count) {
     if (!s || !count) return XF_E_INVALID;
     if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");

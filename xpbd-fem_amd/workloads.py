@@ -71,6 +71,87 @@ def beam_fit_16x4x4(mu_split=(2e5, 2e4), x_split=0.8):
         dt=np.float32(1.0) / np.float32(60 * substeps), substeps=substeps, frames=30)
 
 
+class SplitMix64:
+    """Deterministic uniform stream for workload draws (portable across
+    numpy versions)."""
+
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next() >> 11) / 9007199254740992.0)
+
+    def integer(self, lo, hi):
+        """Uniform integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1)) % (hi - lo + 1)
+
+    def normal(self):
+        u1 = ((self.next() >> 11) + 1.0) / 9007199254740992.0
+        u2 = (self.next() >> 11) / 9007199254740992.0
+        return float(np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2))
+
+
+def random_rotation(rng):
+    q = np.array([rng.normal() for _ in range(4)])
+    a, b, c, d = q / np.linalg.norm(q)
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def boxes_batch(n_bodies=256, seed=2, cells=(3, 24), h=0.1, frames=60):
+    """n_bodies independent boxes: per-axis cell counts uniform in
+    [cells[0], cells[1]] (cells of h m, 6 tets each), MixedSel or
+    YeohSkinFast 50/50, mu (or C10) log-uniform over 1e4-1e6 Pa,
+    incompressible, density uniform over 500-2000, a uniform random rotation,
+    dropped from 0.2-1.0 m onto a ground plane y = 0 (friction 0.5) of its
+    own; gravity -9.81 y; 1/60 s frames of 20 substeps. Yeoh bodies use the
+    reference's skin shape constants (Fem.cpp:526) scaled so that the linear
+    coefficient equals C10 (compliance = 0.1095 / C10). Bodies are placed 5 m
+    apart along z (they never interact)."""
+    rng = SplitMix64(seed)
+    bodies = []
+    for b in range(n_bodies):
+        W, H, D = (rng.integer(*cells) for _ in range(3))
+        yeoh = rng.uniform() < 0.5
+        mod = float(10.0 ** rng.uniform(4.0, 6.0))
+        rho = rng.uniform(500.0, 2000.0)
+        drop = rng.uniform(0.2, 1.0)
+        R = random_rotation(rng)
+        xyz, tets = xf.tet_block(W, H, D, (h, h, h), (-0.5 * W * h, -0.5 * H * h, -0.5 * D * h))
+        p = xyz.reshape(-1, 3).astype(np.float64) @ R.T
+        p[:, 1] += drop - p[:, 1].min()
+        p[:, 2] += 5.0 * b
+        settings = dict(energy=xf.ENERGY_YEOH_SKIN_FAST if yeoh else xf.ENERGY_MIXED_SEL, simultaneous=1,
+                        compliance=(0.1095 / mod) if yeoh else 1.0 / mod, poissons_ratio=0.5, gravity=(0.0, -9.81),
+                        yeoh=xf.SKIN_YEOH, collide=xf.COLLIDE_GROUND, ground_y=0.0, friction=0.5)
+        bodies.append(dict(name=f"box{b}_{W}x{H}x{D}", xyz=p.astype(np.float32).reshape(-1), tets=tets, density=rho,
+                           pinned=None, velocity=None, tet_mu=None, settings=settings))
+    substeps = 20
+    return dict(name=f"boxes{n_bodies}", bodies=bodies, dt=np.float32(1.0) / np.float32(60 * substeps),
+                substeps=substeps, frames=frames)
+
+
+def make_batch_scene(ctx, batch, bodies=None, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64):
+    """One scene holding the given bodies of a batch (all by default)."""
+    sc = xf.Scene(ctx)
+    chosen = batch["bodies"] if bodies is None else [batch["bodies"][i] for i in bodies]
+    for wl in chosen:
+        sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"], velocity=wl["velocity"],
+                    tet_mu=wl["tet_mu"])
+    sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size)
+    for i, wl in enumerate(chosen):
+        sc.set_settings(i, **wl["settings"])
+    return sc
+
+
 def make_scene(ctx, wl, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64):
     """Builds and finalizes a one-body scene of a workload."""
     sc = xf.Scene(ctx)

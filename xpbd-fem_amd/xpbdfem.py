@@ -105,6 +105,8 @@ def lib():
         "xf_scene_profile": ([vp, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
                               C.POINTER(C.c_uint32)], C.c_int),
         "xf_scene_count_nonfinite": ([vp, C.POINTER(C.c_uint32)], C.c_int),
+        "xf_scene_autotune": ([vp, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(SceneOpts), C.POINTER(C.c_float)],
+                              C.c_int),
         "xf_mesh_tet_block": ([C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, _u32p],
                               C.c_int),
         "xf_minstd_order": ([C.c_uint32, _u32p], C.c_int),
@@ -129,7 +131,7 @@ def exported_symbols():
         "xf_scene_set_tet_mu", "xf_scene_step", "xf_scene_sync", "xf_scene_get_state", "xf_scene_set_state",
         "xf_scene_get_time", "xf_scene_set_time", "xf_scene_get_order", "xf_scene_get_elements", "xf_scene_profile",
         "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
-        "xf_mesh_colour",
+        "xf_mesh_colour", "xf_scene_autotune",
     ]
 
 
@@ -328,6 +330,14 @@ class Scene:
         self.ctx.check(lib().xf_scene_profile(self.h, substeps, dt, C.byref(mp), C.byref(mn), C.byref(nl)),
                        "xf_scene_profile")
         return mp.value, mn.value, nl.value
+
+    def autotune(self, substeps, dt, reps=3):
+        """Picks the fastest projection workgroup size; returns (block, ms per call)."""
+        o = SceneOpts()
+        ms = C.c_float()
+        self.ctx.check(lib().xf_scene_autotune(self.h, substeps, dt, reps, C.byref(o), C.byref(ms)),
+                       "xf_scene_autotune")
+        return o.block_size, ms.value
 
     def count_nonfinite(self):
         c = C.c_uint32()
