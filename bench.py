@@ -77,7 +77,7 @@ def batched_leg(ctx, frames=3):
     S, dt = b["substeps"], float(b["dt"])
     T = sum(x["tets"].size // 4 for x in b["bodies"])
     sc = workloads.make_batch_scene(ctx, b)
-    block, _ = sc.autotune(S, dt, reps=2)
+    mode, size, _ = sc.autotune(S, dt, reps=2)
     sc.step(S, dt)
     sc.sync()
     t0 = time.perf_counter()
@@ -106,7 +106,8 @@ def batched_leg(ctx, frames=3):
     plain = T * S * frames / el_p
     return {"workload": f"{b['name']}: {len(b['bodies'])} bodies, {T} tets, {S} substeps/frame, {ncol} colours",
             "batched_tet_substeps_per_s": batched, "plain_loop_tet_substeps_per_s": plain,
-            "batched_over_plain": batched / plain, "frames": frames, "block_size": block, "finite": finite}
+            "batched_over_plain": batched / plain, "frames": frames,
+            "launch": {"mode": "persistent" if mode else "per-colour", "size": size}, "finite": finite}
 
 
 def main():
@@ -128,6 +129,7 @@ def main():
     info = sc.info()
     T, N, S = info["n_tets"], info["n_nodes"], wl["substeps"]
     dt = float(wl["dt"])
+    mode, size, _ = sc.autotune(S, dt, reps=2)
 
     for _ in range(args.warmup):
         sc.step(S, dt)
@@ -142,10 +144,11 @@ def main():
         print(f"non-finite positions after the run: {bad}", file=sys.stderr)
         sys.exit(1)
 
-    # Per-launch device time of the projection kernels (HIP events on the
-    # library stream, eager launches of one frame).
+    # Device time of the dominant kernel (HIP events on the library stream
+    # around each launch of one frame): the per-colour projection launches,
+    # or the single persistent launch (which also holds the node phases).
     ms_proj, ms_node, n_launch = sc.profile(S, dt)
-    bytes_proj = BYTES_PER_TET * T * S
+    bytes_proj = BYTES_PER_TET * T * S + (BYTES_PER_NODE * N * S if mode else 0)
     achieved = bytes_proj / (ms_proj * 1e-3) / 1e9
 
     out = {
@@ -162,10 +165,11 @@ def main():
         "dtype": "fp32 element math, fp64 node state",
         "data": "synthetic",
         "config": {"workload": f"{wl['name']}: {T} tets, {N} nodes, {S} substeps/frame, {info['n_colours']} "
-                               f"colours, ground + sphere", "step": "one frame"},
+                               f"colours, ground + sphere", "step": "one frame",
+                   "launch": {"mode": "persistent" if mode else "per-colour", "size": size}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "xf_project_colour", "launches": n_launch,
+                     "kernel": "xf_frame_persistent" if mode else "xf_project_colour", "launches": n_launch,
                      "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
                      "bytes_per_launch": bytes_proj / max(n_launch, 1)},
     }

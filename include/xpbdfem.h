@@ -119,10 +119,18 @@ typedef struct xf_body_desc {
     const float* tet_mu;
 } xf_body_desc;
 
+/* Launch modes of xf_scene_step. */
+enum {
+    XF_LAUNCH_PER_COLOUR = 0, /* one launch per colour phase and node phase (hipGraph-captured if use_graph) */
+    XF_LAUNCH_PERSISTENT = 1  /* one cooperative launch per step call, grid barriers between phases */
+};
+
 typedef struct xf_scene_opts {
-    uint32_t colour_order;   /* XF_COLOUR_* */
-    uint32_t block_size;     /* projection workgroup size, 0 = default (64) */
-    uint32_t use_graph;      /* capture each xf_scene_step call shape in a hipGraph */
+    uint32_t colour_order;      /* XF_COLOUR_* */
+    uint32_t block_size;        /* per-colour projection workgroup size, 0 = default (64) */
+    uint32_t use_graph;         /* capture each per-colour xf_scene_step call shape in a hipGraph */
+    uint32_t launch_mode;       /* XF_LAUNCH_* */
+    uint32_t persistent_blocks; /* grid size of the persistent launch, 0 = one per compute unit */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {
@@ -183,8 +191,9 @@ XF_API int xf_scene_get_elements(const xf_scene* scene, uint32_t body, float* qi
 XF_API int xf_scene_profile(xf_scene* scene, uint32_t substeps, float dt, float* ms_project, float* ms_node,
                             uint32_t* launches_project);
 /* Autotuner: times `reps` calls of xf_scene_step(substeps, dt) for each
- * projection workgroup size (64, 128, 192, 256) and keeps the fastest in the
- * scene options. The node state and clock are restored afterwards, so tuning
+ * launch shape -- per-colour launches with workgroup sizes 64, 128, 256 and
+ * the persistent launch with 64, 128, 256 and 512 blocks (where co-resident)
+ * -- and keeps the fastest in the scene options. The node state and clock are restored afterwards, so tuning
  * does not advance the simulation. chosen / best_ms (ms per call) nullable.
  * No reference counterpart (the reference is single-threaded). */
 XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,

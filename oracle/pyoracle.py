@@ -92,6 +92,14 @@ def lib():
             fn = getattr(L, "xo_substep" + sfx)
             fn.argtypes = [C.POINTER(XoParams), C.c_uint32, sp, sp, sp, _f32p, C.c_void_p, C.c_uint32, _u32p,
                            _u32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+            fn = getattr(L, "xo_adjoint_tet" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), sp, sp, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
+            fn.restype = C.c_double if sfx == "_f64" else C.c_float
+            fn = getattr(L, "xo_rollout_adjoint" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), C.c_uint32, sp, sp, _f32p, C.c_void_p, C.c_uint32, _u32p, _u32p,
+                           _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_uint32, _u32p, sp, C.c_uint32,
+                           _f64p]
+            fn.restype = C.c_double
             fn = getattr(L, "xo_collide" + sfx)
             fn.argtypes = [C.POINTER(XoParams), C.c_float, sp, sp]
             fn = getattr(L, "xo_solve_tet" + sfx)
@@ -301,3 +309,21 @@ class RefBody:
         o = np.zeros(self.t, np.uint32)
         ref().xr_get_order(self.h, o)
         return o
+
+
+def rollout_adjoint(body, params, order, n_sub, loss_at, targets, window=20):
+    """Forward rollout of `body` (an OracleBody; its X, V are advanced) and the
+    adjoint of the mean-squared position loss at substeps `loss_at`.
+    Returns (L, dL/dlog(comp0) per tet)."""
+    fn = getattr(lib(), "xo_rollout_adjoint_" + body.variant)
+    st = body.X.dtype
+    loss_at = np.ascontiguousarray(loss_at, np.uint32)
+    targets = np.ascontiguousarray(np.asarray(targets).reshape(-1), st)
+    glog = np.zeros(body.t, np.float64)
+    pin = body.pinned.ctypes.data if body.pinned is not None else None
+    L = fn(C.byref(params), body.n, body.X, body.V, body.w, pin, body.t, np.ascontiguousarray(order, np.uint32),
+           body.tets, body.qi, body.qq, body.qr, body.c0, body.c1, body.a, n_sub, loss_at.size, loss_at, targets,
+           window, glog)
+    if L < 0:
+        raise ValueError("rollout adjoint rejected its input")
+    return L, glog

@@ -68,8 +68,8 @@ def test_autotune_restores_state_and_keeps_results(ctx, small_batch):
     plain = workloads.make_batch_scene(ctx, b)
     before = [tuned.get_state(i) for i in range(len(b["bodies"]))]
     t0 = tuned.time()
-    block, ms = tuned.autotune(S, dt, reps=2)
-    assert block in (64, 128, 192, 256) and ms > 0
+    mode, size, ms = tuned.autotune(S, dt, reps=2)
+    assert mode in (0, 1) and size in (64, 128, 256, 512) and ms > 0
     assert tuned.time() == t0
     for i in range(len(b["bodies"])):
         x, v = tuned.get_state(i)
@@ -82,3 +82,18 @@ def test_autotune_restores_state_and_keeps_results(ctx, small_batch):
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
     tuned.close()
     plain.close()
+
+
+def test_persistent_batch_equals_per_colour(ctx, small_batch):
+    b = small_batch
+    dt, S = float(b["dt"]), b["substeps"]
+    pc = workloads.make_batch_scene(ctx, b)
+    pe = workloads.make_batch_scene(ctx, b, launch_mode=1)
+    pc.step(S, dt)
+    pe.step(S, dt)
+    for i in range(len(b["bodies"])):
+        xa, va = pc.get_state(i)
+        xb, vb = pe.get_state(i)
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    pc.close()
+    pe.close()

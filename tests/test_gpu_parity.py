@@ -32,8 +32,10 @@ def rel_err(x, ref, xyz0):
     return float(np.abs(x.reshape(-1, 3) - ref.reshape(-1, 3)).max() / bbox_diag(xyz0))
 
 
-def run_gpu(ctx, wl, substeps, use_graph=True, colour_order=xf.COLOUR_NATURAL, chunks=1):
-    sc = workloads.make_scene(ctx, wl, use_graph=use_graph, colour_order=colour_order)
+def run_gpu(ctx, wl, substeps, use_graph=True, colour_order=xf.COLOUR_NATURAL, chunks=1,
+            launch_mode=xf.LAUNCH_PER_COLOUR, persistent_blocks=0):
+    sc = workloads.make_scene(ctx, wl, use_graph=use_graph, colour_order=colour_order, launch_mode=launch_mode,
+                              persistent_blocks=persistent_blocks)
     for _ in range(chunks):
         sc.step(substeps, float(wl["dt"]))
     x, v = sc.get_state(0)
@@ -145,3 +147,20 @@ def test_big_yeoh_beam_one_frame_bit_exact(ctx):
     print(f"\nbig beam: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
     assert np.array_equal(x, ob.X)
     assert np.array_equal(v, ob.V)
+
+
+@pytest.mark.parametrize("energy,sim", [(4, True), (7, True), (4, False), (3, True)])
+@pytest.mark.parametrize("blocks", [0, 64])
+def test_persistent_launch_bit_exact(ctx, energy, sim, blocks):
+    wl = workloads.beam_neo_12x4x4(energy=energy, simultaneous=sim)
+    x, v, order = run_gpu(ctx, wl, wl["substeps"], chunks=2, launch_mode=xf.LAUNCH_PERSISTENT,
+                          persistent_blocks=blocks)
+    ob = run_oracle(wl, order, 2 * wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_persistent_big_beam_bit_exact(ctx):
+    wl = workloads.beam_yeoh_96x24x24()
+    x, v, order = run_gpu(ctx, wl, wl["substeps"], launch_mode=xf.LAUNCH_PERSISTENT)
+    ob = run_oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)

@@ -409,6 +409,152 @@ __global__ __launch_bounds__(256) void xf_finish(uint32_t n, double4* __restrict
     }
 }
 
+// ------------------------------------------------------ persistent frame
+
+namespace {
+
+// Spin bound of one grid-barrier wait (each spin sleeps ~2 x 64 cycles): far
+// beyond any real barrier wait, small enough that a stalled grid gives up
+// within a fraction of a second.
+constexpr uint32_t kBarrierSpinLimit = 1u << 21;
+
+// Grid barrier of a co-resident grid (cooperative launch). `arrive` counts
+// block arrivals since the launch (zeroed by the host before it); barrier k
+// waits for k * gridDim.x arrivals. On a spin timeout the error flag is set
+// and every block returns false from its next barrier, so all blocks leave
+// the kernel.
+__device__ __forceinline__ bool grid_barrier(uint32_t* arrive, uint32_t target, uint32_t* err) {
+    __shared__ uint32_t ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0, good = 1;
+        while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kBarrierSpinLimit) {
+                __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0u;
+}
+
+template <int E, bool SIM, bool AUX>
+__device__ __forceinline__ void project_one(uint32_t i, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+                                            const float4* __restrict__ tg1, const float4* __restrict__ tg2,
+                                            const float4* __restrict__ tg3, const float4* __restrict__ taux,
+                                            double4* __restrict__ X, const XfBodyParams* __restrict__ bodies,
+                                            float comp1_u, float a_u, float dt) {
+    uint4 id = tidx[i];
+    float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i], q3 = tg3[i];
+    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
+    float comp1 = comp1_u, a = a_u;
+    uint32_t body = 0;
+    if constexpr (AUX) {
+        float4 ax = taux[i];
+        comp1 = ax.x;
+        a = ax.y;
+        body = __float_as_uint(ax.z);
+    }
+    const XfBodyParams& bp = bodies[body];
+    float C[3] = {bp.yeoh[0], bp.yeoh[1], bp.yeoh[2]};
+    bool yeohFast = bp.energy == 7;
+    float Qi[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+    float QQ[3] = {q2.y, q2.z, q2.w};
+    float QR[3] = {q3.x, q3.y, q3.z};
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    solve_tet<E, SIM>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt);
+    X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
+    X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
+    X[id.z] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
+    X[id.w] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+}
+
+// End of substep + next prediction for one node (the xf_finish body).
+__device__ __forceinline__ void finish_one(uint32_t i, bool predict_next, double4* __restrict__ X,
+                                           double4* __restrict__ O, double4* __restrict__ V,
+                                           const XfBodyParams* __restrict__ bodies, float t_end) {
+    double4 xi = X[i];
+    double4 oi = O[i];
+    uint32_t flags = (uint32_t)oi.w;
+    const XfBodyParams& p = bodies[flags >> 1];
+    double x[3] = {xi.x, xi.y, xi.z};
+    double o[3] = {oi.x, oi.y, oi.z};
+    if (flags & 1u) {
+        x[0] = o[0];
+        x[1] = o[1];
+        x[2] = o[2];
+    } else if (p.collide) {
+        collide_node(p, t_end, o, x);
+    }
+    double v[3];
+    double invdt = (double)p.invdt;
+#pragma unroll
+    for (int r = 0; r < 3; r++) v[r] = (x[r] - o[r]) * invdt;
+    if (predict_next) {
+        predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
+    } else {
+        X[i] = make_double4(x[0], x[1], x[2], xi.w);
+        V[i] = make_double4(v[0], v[1], v[2], 0.0);
+    }
+}
+
+}  // namespace
+
+// A whole step call (`substeps` substeps) in one cooperative launch: the
+// first prediction, then per substep every colour phase and the fused
+// finish/predict node phase, with a grid barrier after each phase. Work is
+// spread over the grid by global thread index (grid-stride loops). The
+// results equal the per-colour launches bit-for-bit: each tet and node runs
+// the same code on the same inputs in the same phase order.
+template <int E, bool SIM, bool AUX>
+__global__ __launch_bounds__(256) void xf_frame_persistent(
+    uint32_t substeps, uint32_t ncol, const uint32_t* __restrict__ colour_begin, uint32_t n_nodes,
+    const uint4* __restrict__ tidx, const float4* __restrict__ tg0, const float4* __restrict__ tg1,
+    const float4* __restrict__ tg2, const float4* __restrict__ tg3, const float4* __restrict__ taux,
+    double4* __restrict__ X, double4* __restrict__ O, double4* __restrict__ V, const XfBodyParams* __restrict__ bodies,
+    float comp1_u, float a_u, float dt, float* __restrict__ clock, uint32_t* __restrict__ arrive,
+    uint32_t* __restrict__ err) {
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t target = 0;
+    for (uint32_t i = gid; i < n_nodes; i += nthreads) {
+        double4 xi = X[i];
+        double4 oi = O[i];
+        double4 vi = V[i];
+        const XfBodyParams& p = bodies[(uint32_t)oi.w >> 1];
+        double v[3] = {vi.x, vi.y, vi.z};
+        double x[3] = {xi.x, xi.y, xi.z};
+        predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
+    }
+    target += gridDim.x;
+    if (!grid_barrier(arrive, target, err)) return;
+    for (uint32_t s = 0; s < substeps; s++) {
+        for (uint32_t c = 0; c < ncol; c++) {
+            if (c == 0 && gid == 0) clock[0] = clock[0] + dt;
+            uint32_t begin = colour_begin[c], count = colour_begin[c + 1] - begin;
+            for (uint32_t k = gid; k < count; k += nthreads)
+                project_one<E, SIM, AUX>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, X, bodies, comp1_u, a_u, dt);
+            target += gridDim.x;
+            if (!grid_barrier(arrive, target, err)) return;
+        }
+        float t_end = clock[0];
+        bool next = s + 1 < substeps;
+        for (uint32_t i = gid; i < n_nodes; i += nthreads) finish_one(i, next, X, O, V, bodies, t_end);
+        if (next) {
+            target += gridDim.x;
+            if (!grid_barrier(arrive, target, err)) return;
+        }
+    }
+}
+
 // Counts nodes whose position is not finite.
 __global__ __launch_bounds__(256) void xf_count_nonfinite(uint32_t n, const double4* __restrict__ X,
                                                           uint32_t* __restrict__ count) {
@@ -481,4 +627,52 @@ extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(xf_count_nonfinite, dim3((n + 255) / 256), dim3(256), 0, s, n, X, count);
     return hipGetLastError();
+}
+
+typedef void (*XfFrameFn)(uint32_t, uint32_t, const uint32_t*, uint32_t, const uint4*, const float4*, const float4*,
+                          const float4*, const float4*, const float4*, double4*, double4*, double4*,
+                          const XfBodyParams*, float, float, float, float*, uint32_t*, uint32_t*);
+
+namespace {
+template <int E>
+XfFrameFn pick_frame(bool sim, bool aux) {
+    if (sim) return aux ? xf_frame_persistent<E, true, true> : xf_frame_persistent<E, true, false>;
+    return aux ? xf_frame_persistent<E, false, true> : xf_frame_persistent<E, false, false>;
+}
+XfFrameFn frame_fn(int energy, bool sim, bool aux) {
+    switch (energy) {
+        case 0: return pick_frame<0>(sim, aux);
+        case 3: return pick_frame<3>(sim, aux);
+        case 4: return pick_frame<4>(sim, aux);
+        case 5: return pick_frame<5>(sim, aux);
+        case 6: return pick_frame<6>(sim, aux);
+        case 7: return pick_frame<7>(sim, aux);
+        default: return nullptr;
+    }
+}
+}  // namespace
+
+// Largest co-resident grid of the persistent kernel at 256 threads per block.
+extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, int* max_blocks) {
+    XfFrameFn fn = frame_fn(energy, sim, aux);
+    if (!fn) return hipErrorInvalidValue;
+    int dev = 0, per_cu = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, 256, 0);
+    if (e == hipSuccess) *max_blocks = per_cu * cus;
+    return e;
+}
+
+extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, uint32_t blocks,
+                                      uint32_t substeps, uint32_t ncol, const uint32_t* colour_begin,
+                                      uint32_t n_nodes, const uint4* tidx, const float4* tg0, const float4* tg1,
+                                      const float4* tg2, const float4* tg3, const float4* taux, double4* X,
+                                      double4* O, double4* V, const XfBodyParams* bodies, float comp1_u, float a_u,
+                                      float dt, float* clock, uint32_t* arrive, uint32_t* err) {
+    XfFrameFn fn = frame_fn(energy, sim, aux);
+    if (!fn) return hipErrorInvalidValue;
+    void* args[] = {&substeps, &ncol,  &colour_begin, &n_nodes, &tidx,   &tg0,   &tg1, &tg2,   &tg3, &taux,
+                    &X,        &O,     &V,            &bodies,  &comp1_u, &a_u,  &dt,  &clock, &arrive, &err};
+    return hipLaunchCooperativeKernel((const void*)fn, dim3(blocks), dim3(256), args, 0, s);
 }

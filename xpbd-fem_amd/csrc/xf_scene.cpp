@@ -28,6 +28,13 @@ extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, d
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
                                        double4* V, const XfBodyParams* bodies, const float* clock);
 extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const double4* X, uint32_t* count);
+extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, int* max_blocks);
+extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, uint32_t blocks,
+                                      uint32_t substeps, uint32_t ncol, const uint32_t* colour_begin,
+                                      uint32_t n_nodes, const uint4* tidx, const float4* tg0, const float4* tg1,
+                                      const float4* tg2, const float4* tg3, const float4* taux, double4* X,
+                                      double4* O, double4* V, const XfBodyParams* bodies, float comp1_u, float a_u,
+                                      float dt, float* clock, uint32_t* arrive, uint32_t* err);
 
 struct xf_ctx {
     int device = 0;
@@ -99,7 +106,11 @@ struct xf_scene {
     XfBodyParams* bp = nullptr;
     float* clock = nullptr;
     uint32_t* counter = nullptr;
+    uint32_t* colour_begin_d = nullptr;  // ncol + 1 offsets (persistent launch)
+    uint32_t* arrive = nullptr;          // grid-barrier arrivals (persistent launch)
+    uint32_t* barrier_err = nullptr;     // set when a grid barrier timed out
     uint64_t device_bytes = 0;
+    int n_cu = 256;
     // host mirrors
     std::vector<float4> h_tg3, h_taux;
     std::vector<double> h_w;       // per node inverse mass
@@ -250,6 +261,29 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
         }
         return XF_OK;
     };
+    if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT) {
+        int maxb = 0;
+        XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, &maxb));
+        uint32_t blocks = s->opts.persistent_blocks ? s->opts.persistent_blocks : (uint32_t)s->n_cu;
+        if (maxb <= 0) return Fail(ctx, XF_E_UNSUPPORTED, "persistent kernel does not fit on the device");
+        if (blocks > (uint32_t)maxb) blocks = (uint32_t)maxb;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timer) {
+            XF_HIP(ctx, hipEventCreate(&e0));
+            XF_HIP(ctx, hipEventCreate(&e1));
+        }
+        XF_HIP(ctx, hipMemsetAsync(s->arrive, 0, sizeof(uint32_t), st));
+        if (timer) XF_HIP(ctx, hipEventRecord(e0, st));
+        XF_HIP(ctx, xf_launch_frame(st, energy, sim, s->aux, blocks, substeps, s->ncol, s->colour_begin_d, s->N,
+                                    s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O, s->V,
+                                    s->bp, s->comp1_u, s->a_u, dt, s->clock, s->arrive, s->barrier_err));
+        if (timer) {
+            XF_HIP(ctx, hipEventRecord(e1, st));
+            timer->ev_proj.push_back(e0);
+            timer->ev_proj.push_back(e1);
+        }
+        return XF_OK;
+    }
     int rc = node(0, false);
     if (rc) return rc;
     for (uint32_t k = 0; k < substeps; k++) {
@@ -358,6 +392,8 @@ XF_API int xf_scene_create(xf_ctx* ctx, xf_scene** out) {
     s->opts.colour_order = XF_COLOUR_NATURAL;
     s->opts.block_size = 64;
     s->opts.use_graph = 1;
+    s->opts.launch_mode = XF_LAUNCH_PER_COLOUR;
+    s->opts.persistent_blocks = 0;
     *out = s;
     return XF_OK;
 }
@@ -367,8 +403,9 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     DropGraph(s);
-    void* ptrs[] = {s->X, s->O, s->V, s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->bp, s->clock,
-                    s->counter};
+    void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
+                    s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
+                    s->barrier_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -437,6 +474,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     if (s->opts.block_size % 64 || s->opts.block_size > 256)
         return Fail(ctx, XF_E_INVALID, "block_size must be 64, 128, 192 or 256");
     if (s->opts.colour_order > XF_COLOUR_MINSTD) return Fail(ctx, XF_E_INVALID, "unknown colour order");
+    if (s->opts.launch_mode > XF_LAUNCH_PERSISTENT) return Fail(ctx, XF_E_INVALID, "unknown launch mode");
     XF_HIP(ctx, hipSetDevice(ctx->device));
 
     // Colour each body; the scene runs max(ncol) phases.
@@ -512,6 +550,11 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         return rc;
     for (int k = 0; k < 4; k++)
         if ((rc = DevAlloc(ctx, &s->tg[k], s->T, &s->device_bytes))) return rc;
+    if ((rc = DevAlloc(ctx, &s->colour_begin_d, s->ncol + 1, &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->arrive, 1, &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->barrier_err, 1, &s->device_bytes)))
+        return rc;
+    XF_HIP(ctx, hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     hipStream_t st = ctx->stream;
     XF_HIP(ctx, hipMemcpyAsync(s->X, hX.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->O, hO.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
@@ -521,7 +564,11 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     XF_HIP(ctx, hipMemcpyAsync(s->tg[1], h_tg1.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[2], h_tg2.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     float zero = 0.0f;
+    uint32_t uzero = 0;
     XF_HIP(ctx, hipMemcpyAsync(s->clock, &zero, sizeof(float), hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->colour_begin_d, s->colour_begin.data(), sizeof(uint32_t) * (s->ncol + 1),
+                               hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->barrier_err, &uzero, sizeof(uint32_t), hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipStreamSynchronize(st));
     s->time_host = 0.0f;
     s->finalized = true;
@@ -572,7 +619,7 @@ XF_API int xf_scene_step(xf_scene* s, uint32_t substeps, float dt) {
     bool sim;
     int rc = Prepare(s, dt, &energy, &sim);
     if (rc) return rc;
-    if (!s->opts.use_graph) {
+    if (!s->opts.use_graph || s->opts.launch_mode == XF_LAUNCH_PERSISTENT) {
         rc = Enqueue(s, substeps, dt, energy, sim, nullptr);
     } else {
         if (!s->graph_exec || s->graph_substeps != substeps || s->graph_dt != dt || s->graph_energy != energy ||
@@ -602,10 +649,23 @@ XF_API int xf_scene_step(xf_scene* s, uint32_t substeps, float dt) {
     return XF_OK;
 }
 
+namespace {
+// Reports a grid-barrier timeout of an earlier persistent launch (the flag
+// stays set, so the scene keeps failing loudly).
+int CheckBarrier(xf_scene* s) {
+    if (!s->barrier_err) return XF_OK;
+    uint32_t e = 0;
+    XF_HIP(s->ctx, hipMemcpyAsync(&e, s->barrier_err, sizeof(uint32_t), hipMemcpyDeviceToHost, s->ctx->stream));
+    XF_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    if (e) return Fail(s->ctx, XF_E_HIP, "persistent launch: grid barrier timed out (grid not co-resident?)");
+    return XF_OK;
+}
+}  // namespace
+
 XF_API int xf_scene_sync(xf_scene* s) {
     if (!s) return XF_E_INVALID;
     XF_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    return XF_OK;
+    return s->finalized ? CheckBarrier(s) : XF_OK;
 }
 
 XF_API int xf_scene_get_state(xf_scene* s, uint32_t body, double* x, double* v) {
@@ -616,6 +676,8 @@ XF_API int xf_scene_get_state(xf_scene* s, uint32_t body, double* x, double* v) 
     std::vector<double4> tmp(b.n);
     xf_ctx* ctx = s->ctx;
     XF_HIP(ctx, hipSetDevice(ctx->device));
+    int brc = CheckBarrier(s);
+    if (brc) return brc;
     if (x) {
         XF_HIP(ctx, hipMemcpyAsync(tmp.data(), s->X + b.node_base, sizeof(double4) * b.n, hipMemcpyDeviceToHost,
                                    ctx->stream));
@@ -777,18 +839,28 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
             return e;
         return hipStreamSynchronize(st);
     };
-    const uint32_t candidates[] = {64, 128, 192, 256};
+    struct Cand {
+        uint32_t mode, size;
+    };
+    const Cand candidates[] = {{XF_LAUNCH_PER_COLOUR, 64},  {XF_LAUNCH_PER_COLOUR, 128}, {XF_LAUNCH_PER_COLOUR, 256},
+                               {XF_LAUNCH_PERSISTENT, 64},  {XF_LAUNCH_PERSISTENT, 128}, {XF_LAUNCH_PERSISTENT, 256},
+                               {XF_LAUNCH_PERSISTENT, 512}};
     xf_scene_opts keep = s->opts;
-    uint32_t best_block = keep.block_size;
+    Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size};
+    int maxb = 0;
+    XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, &maxb));
     float best = -1.0f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t he = copy_all(true);
     if (he == hipSuccess) he = hipEventCreate(&e0);
     if (he == hipSuccess) he = hipEventCreate(&e1);
     float time_keep = s->time_host;
-    for (uint32_t bs : candidates) {
+    for (const Cand& cand : candidates) {
         if (he != hipSuccess || rc) break;
-        s->opts.block_size = bs;
+        if (cand.mode == XF_LAUNCH_PERSISTENT && (int)cand.size > maxb) continue;
+        s->opts.launch_mode = cand.mode;
+        if (cand.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = cand.size;
+        else s->opts.block_size = cand.size;
         DropGraph(s);
         if ((he = copy_all(false)) != hipSuccess) break;
         if ((rc = xf_scene_step(s, substeps, dt))) break;  // warm-up and graph capture
@@ -801,8 +873,9 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
         ms /= (float)reps;
         if (best < 0.0f || ms < best) {
             best = ms;
-            best_block = bs;
+            best_c = cand;
         }
+        if ((rc = CheckBarrier(s))) break;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -810,7 +883,9 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     for (void* p : {(void*)sx, (void*)so, (void*)sv, (void*)sc}) (void)hipFree(p);
     s->time_host = time_keep;
     s->opts = keep;
-    s->opts.block_size = best_block;
+    s->opts.launch_mode = best_c.mode;
+    if (best_c.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = best_c.size;
+    else s->opts.block_size = best_c.size;
     DropGraph(s);
     if (rc) return rc;
     if (he != hipSuccess) return Fail(ctx, XF_E_HIP, "autotune: %s", hipGetErrorString(he));

@@ -139,24 +139,28 @@ def boxes_batch(n_bodies=256, seed=2, cells=(3, 24), h=0.1, frames=60):
                 substeps=substeps, frames=frames)
 
 
-def make_batch_scene(ctx, batch, bodies=None, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64):
+def make_batch_scene(ctx, batch, bodies=None, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64,
+                     launch_mode=xf.LAUNCH_PER_COLOUR, persistent_blocks=0):
     """One scene holding the given bodies of a batch (all by default)."""
     sc = xf.Scene(ctx)
     chosen = batch["bodies"] if bodies is None else [batch["bodies"][i] for i in bodies]
     for wl in chosen:
         sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"], velocity=wl["velocity"],
                     tet_mu=wl["tet_mu"])
-    sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size)
+    sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size, launch_mode=launch_mode,
+                persistent_blocks=persistent_blocks)
     for i, wl in enumerate(chosen):
         sc.set_settings(i, **wl["settings"])
     return sc
 
 
-def make_scene(ctx, wl, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64):
+def make_scene(ctx, wl, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64,
+               launch_mode=xf.LAUNCH_PER_COLOUR, persistent_blocks=0):
     """Builds and finalizes a one-body scene of a workload."""
     sc = xf.Scene(ctx)
     sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"], velocity=wl["velocity"],
                 tet_mu=wl["tet_mu"])
-    sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size)
+    sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size, launch_mode=launch_mode,
+                persistent_blocks=persistent_blocks)
     sc.set_settings(0, **wl["settings"])
     return sc

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libxpbdfem.so")
 ENERGY_MIXED, ENERGY_MIXED_SEL, ENERGY_YEOH_SKIN, ENERGY_YEOH_SKIN_SEL, ENERGY_YEOH_SKIN_FAST = 3, 4, 5, 6, 7
 COLLIDE_GROUND, COLLIDE_SPHERE = 1, 2
 COLOUR_NATURAL, COLOUR_MINSTD = 0, 1
+LAUNCH_PER_COLOUR, LAUNCH_PERSISTENT = 0, 1
 SKIN_YEOH = (0.1095, 14.95, 4.595)  # Fem.cpp:526
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -61,7 +62,8 @@ class BodyDesc(C.Structure):
 
 
 class SceneOpts(C.Structure):
-    _fields_ = [("colour_order", C.c_uint32), ("block_size", C.c_uint32), ("use_graph", C.c_uint32)]
+    _fields_ = [("colour_order", C.c_uint32), ("block_size", C.c_uint32), ("use_graph", C.c_uint32),
+                ("launch_mode", C.c_uint32), ("persistent_blocks", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -261,11 +263,14 @@ class Scene:
         self.bodies.append((d.n_nodes, d.n_tets))
         return bid.value
 
-    def finalize(self, colour_order=COLOUR_NATURAL, block_size=64, use_graph=True):
+    def finalize(self, colour_order=COLOUR_NATURAL, block_size=64, use_graph=True, launch_mode=LAUNCH_PER_COLOUR,
+                 persistent_blocks=0):
         o = SceneOpts()
         o.colour_order = colour_order
         o.block_size = block_size
         o.use_graph = 1 if use_graph else 0
+        o.launch_mode = launch_mode
+        o.persistent_blocks = persistent_blocks
         self.ctx.check(lib().xf_scene_finalize(self.h, C.byref(o)), "xf_scene_finalize")
 
     def info(self):
@@ -332,12 +337,14 @@ class Scene:
         return mp.value, mn.value, nl.value
 
     def autotune(self, substeps, dt, reps=3):
-        """Picks the fastest projection workgroup size; returns (block, ms per call)."""
+        """Picks the fastest launch shape; returns (launch_mode, size, ms per call),
+        size = workgroup size (per-colour) or grid blocks (persistent)."""
         o = SceneOpts()
         ms = C.c_float()
         self.ctx.check(lib().xf_scene_autotune(self.h, substeps, dt, reps, C.byref(o), C.byref(ms)),
                        "xf_scene_autotune")
-        return o.block_size, ms.value
+        size = o.persistent_blocks if o.launch_mode == LAUNCH_PERSISTENT else o.block_size
+        return o.launch_mode, size, ms.value
 
     def count_nonfinite(self):
         c = C.c_uint32()
