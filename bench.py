@@ -110,6 +110,50 @@ def batched_leg(ctx, frames=3):
             "launch": {"mode": "persistent" if mode else "per-colour", "size": size}, "finite": finite}
 
 
+def adjoint_leg(ctx, reps=3):
+    """Adjoint of the 16x4x4 fitting beam (1,536 tets, per-tet mu, 30 frames of
+    20 substeps, loss at frames 10, 20, 30, checkpoints every 20 substeps):
+    wall time of xf_scene_adjoint (forward + checkpointed reverse) against the
+    plain forward rollout of the same 600 substeps."""
+    truth = workloads.beam_fit_16x4x4()
+    guess = dict(truth)
+    guess["tet_mu"] = np.full(truth["tets"].size // 4, 5e4, np.float32)
+    dt, S = float(truth["dt"]), truth["substeps"]
+    n_sub = S * truth["frames"]
+    loss_at = np.array([10 * S, 20 * S, 30 * S], np.uint32)
+    ref = workloads.make_scene(ctx, truth)
+    targets, done = [], 0
+    for la in loss_at:
+        ref.step(int(la) - done, dt)
+        done = int(la)
+        targets.append(ref.get_state(0)[0])
+    ref.close()
+    targets = np.array(targets)
+    fwd = workloads.make_scene(ctx, guess)
+    adj = workloads.make_scene(ctx, guess)
+    x0, v0 = fwd.get_state(0)
+    fwd.step(n_sub, dt)
+    fwd.sync()
+    adj.adjoint(n_sub, dt, loss_at, targets)
+    t_f, t_a = [], []
+    for _ in range(reps):
+        fwd.set_state(0, x0, v0)
+        t0 = time.perf_counter()
+        fwd.step(n_sub, dt)
+        fwd.sync()
+        t_f.append(time.perf_counter() - t0)
+        adj.set_state(0, x0, v0)
+        t0 = time.perf_counter()
+        L, g = adj.adjoint(n_sub, dt, loss_at, targets)
+        t_a.append(time.perf_counter() - t0)
+    fwd.close()
+    adj.close()
+    tf, ta = min(t_f), min(t_a)
+    return {"workload": f"beam16x4x4_fit: {truth['tets'].size // 4} tets, {n_sub} substeps, loss at frames 10/20/30",
+            "forward_ms": 1e3 * tf, "adjoint_ms": 1e3 * ta, "adjoint_over_forward": ta / tf, "loss": L,
+            "grad_norm": float(np.linalg.norm(g))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +161,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
@@ -175,6 +220,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched_leg(ctx)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out))

@@ -117,6 +117,8 @@ typedef struct xf_body_desc {
     const uint8_t* pinned;
     const float* velocity;
     const float* tet_mu;
+    const uint32_t* tet_colour; /* nullable: caller's colouring (< 256 colours, node-disjoint
+                                   classes, validated); else the scene's greedy colouring */
 } xf_body_desc;
 
 /* Launch modes of xf_scene_step. */
@@ -198,6 +200,21 @@ XF_API int xf_scene_profile(xf_scene* scene, uint32_t substeps, float dt, float*
  * No reference counterpart (the reference is single-threaded). */
 XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,
                              float* best_ms);
+/* Adjoint (no reference counterpart). Rolls every body forward `substeps`
+ * substeps of length dt from the current state (as xf_scene_step would) and
+ * returns the trajectory loss
+ *   L = sum_j sum_b (1/n_b) sum_{i in b} |x_i(after substep loss_at[j]) - target_j,i|^2
+ * and grad_log_mu[t] = dL/d(log mu_t) for every tet of every body (bodies
+ * concatenated, each in its own tet order). loss_at: n_loss ascending
+ * substep counts in [1, substeps]; targets: n_loss x 3 x (total nodes), body
+ * order. The forward states are checkpointed every `window` substeps and each
+ * window is replayed once (recording) before its reverse sweep; per-tet
+ * gradients are summed in FP32 within a window and in FP64 across windows.
+ * Scope: simultaneous MixedSel / YeohSkinFast, pins, ground contact (sphere
+ * contact and serial mode return XF_E_UNSUPPORTED). The scene is left at the
+ * end of the forward rollout. */
+XF_API int xf_scene_adjoint(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
+                            const double* targets, uint32_t window, double* loss, double* grad_log_mu);
 /* Counts non-finite node positions (0 = all finite). */
 XF_API int xf_scene_count_nonfinite(xf_scene* scene, uint32_t* count);
 
@@ -208,6 +225,13 @@ XF_API int xf_scene_count_nonfinite(xf_scene* scene, uint32_t* count);
  * tets[24*W*H*D]. mirrored selects Pattern_Mirrored. */
 XF_API int xf_mesh_tet_block(uint32_t W, uint32_t H, uint32_t D, const float scale[3], const float origin[3],
                              uint32_t mirrored, float* xyz, uint32_t* tets);
+/* The 24-colouring of a uniform-pattern xf_mesh_tet_block mesh (the node
+ * valence lower bound): colour = 6 * ((x + y + z) mod 4) + split index, for
+ * the tet of split index s in cell (x, y, z). Every split tet spans its cell
+ * diagonal in unit steps, so two tets of one split index share a node only
+ * in cells whose offset has coordinate sum 1..3, never 0 mod 4. colour[6WHD].
+ * No reference counterpart. */
+XF_API int xf_mesh_tet_block_colours(uint32_t W, uint32_t H, uint32_t D, uint32_t* colour);
 /* The reference's MINSTD sweep order for n tets (Geo.cpp:759-771). */
 XF_API int xf_minstd_order(uint32_t n, uint32_t* order);
 /* n normal deviates N(0, sigma^2) from SplitMix64(seed) + Box-Muller. */

@@ -1,0 +1,119 @@
+"""GPU adjoint (xf_scene_adjoint) on the 16x4x4 fitting beam.
+
+Bars:
+  * bit-exact loss and dL/dlog(mu) against the oracle's adjoint at the same
+    precision (FP64 state, FP32 element math, FP32 per-window gradient sums);
+  * the scene ends at the forward rollout's end state, bit-exact with plain
+    stepping;
+  * against the FP64 oracle adjoint (itself pinned by finite differences in
+    test_oracle_adjoint.py): relative L2 distance of the gradients reported
+    and bounded by GRAD_REL_FP64.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import workloads
+import xpbdfem as xf
+
+pytestmark = pytest.mark.gpu
+
+GRAD_REL_FP64 = 0.05
+
+
+def targets_for(wl, order, loss_at, variant="ref64"):
+    p = po.params_from_settings(wl["dt"], wl["settings"])
+    b = po.body_from_workload(wl, variant)
+    out, done = [], 0
+    for la in loss_at:
+        b.substep(p, order, count=int(la) - done)
+        done = int(la)
+        out.append(b.X.copy())
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def fit():
+    truth = workloads.beam_fit_16x4x4()
+    guess = dict(truth)
+    guess["tet_mu"] = np.full(truth["tets"].size // 4, 5e4, np.float32)
+    n = truth["xyz"].size // 3
+    order, _, _ = xf.mesh_colour(n, truth["tets"])
+    return truth, guess, order
+
+
+@pytest.mark.parametrize("frames", [3, 30])
+def test_adjoint_bit_exact_vs_oracle(ctx, fit, frames):
+    truth, guess, order = fit
+    S = truth["substeps"]
+    loss_at = np.array([f * S for f in (1, 2, 3)] if frames == 3 else [200, 400, 600], np.uint32)
+    targets = targets_for(truth, order, loss_at)
+    sc = workloads.make_scene(ctx, guess)
+    L, g = sc.adjoint(frames * S, float(truth["dt"]), loss_at, targets, window=20)
+    ob = po.body_from_workload(guess, "ref64")
+    Lo, glog = po.rollout_adjoint(ob, po.params_from_settings(truth["dt"], truth["settings"]), order, frames * S,
+                                  loss_at, targets, window=20)
+    print(f"\nframes {frames}: loss GPU {L:.12e} oracle {Lo:.12e}; |grad| {np.linalg.norm(g):.4e}")
+    assert L == Lo
+    assert np.array_equal(g, -glog)
+    x, v = sc.get_state(0)
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+    sc.close()
+
+
+def test_adjoint_vs_fp64_oracle(ctx, fit):
+    truth, guess, order = fit
+    loss_at = np.array([200, 400, 600], np.uint32)
+    targets = targets_for(truth, order, loss_at, "f64")
+    sc = workloads.make_scene(ctx, guess)
+    L, g = sc.adjoint(600, float(truth["dt"]), loss_at, targets, window=20)
+    ob = po.body_from_workload(guess, "f64")
+    L64, glog = po.rollout_adjoint(ob, po.params_from_settings(truth["dt"], truth["settings"]), order, 600, loss_at,
+                                   targets, window=20)
+    rel = np.linalg.norm(g + glog) / np.linalg.norm(glog)
+    cos = float(g @ -glog / (np.linalg.norm(g) * np.linalg.norm(glog)))
+    print(f"\nloss GPU {L:.6e} FP64 {L64:.6e}; gradient rel L2 distance {rel:.3e}, cosine {cos:.6f}")
+    assert abs(L - L64) <= 1e-2 * L64
+    assert rel <= GRAD_REL_FP64
+    sc.close()
+
+
+def test_adjoint_leaves_forward_end_state(ctx, fit):
+    truth, guess, order = fit
+    S = truth["substeps"]
+    loss_at = np.array([S, 2 * S], np.uint32)
+    targets = targets_for(truth, order, loss_at)
+    a = workloads.make_scene(ctx, guess)
+    b = workloads.make_scene(ctx, guess)
+    a.adjoint(2 * S, float(truth["dt"]), loss_at, targets, window=7)
+    b.step(2 * S, float(truth["dt"]))
+    xa, va = a.get_state(0)
+    xb, vb = b.get_state(0)
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    assert a.time() == b.time()
+    a.close()
+    b.close()
+
+
+def test_batched_adjoint_equals_single(ctx, fit):
+    """Two bodies in one scene: each body's loss term and gradient equal its
+    own single-body adjoint (losses add across bodies)."""
+    truth, guess, order = fit
+    S = truth["substeps"]
+    loss_at = np.array([S, 2 * S], np.uint32)
+    tg = targets_for(truth, order, loss_at)
+    other = dict(guess)
+    other["tet_mu"] = np.full(guess["tets"].size // 4, 8e4, np.float32)
+    batch = dict(bodies=[guess, other], dt=truth["dt"], substeps=S)
+    both = workloads.make_batch_scene(ctx, batch)
+    L2, g2 = both.adjoint(2 * S, float(truth["dt"]), loss_at, np.concatenate([tg, tg], axis=1), window=20)
+    t = guess["tets"].size // 4
+    total = 0.0
+    for i, wl in enumerate((guess, other)):
+        one = workloads.make_scene(ctx, wl)
+        L1, g1 = one.adjoint(2 * S, float(truth["dt"]), loss_at, tg, window=20)
+        assert np.array_equal(g1, g2[i * t:(i + 1) * t])
+        total += L1
+        one.close()
+    assert np.isclose(L2, total, rtol=1e-14)
+    both.close()

@@ -164,3 +164,16 @@ def test_persistent_big_beam_bit_exact(ctx):
     x, v, order = run_gpu(ctx, wl, wl["substeps"], launch_mode=xf.LAUNCH_PERSISTENT)
     ob = run_oracle(wl, order, wl["substeps"])
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_user_colouring_validated_and_bit_exact(ctx):
+    wl = workloads.beam_neo_12x4x4()
+    bad = np.zeros(wl["tets"].size // 4, np.uint32)  # one colour for all: shares nodes
+    sc = xf.Scene(ctx)
+    with pytest.raises(xf.XfError):
+        sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"], tet_colour=bad)
+    sc.close()
+    wl["tet_colour"] = xf.tet_block_colours(12, 4, 4)
+    x, v, order = run_gpu(ctx, wl, wl["substeps"])
+    ob = run_oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)

@@ -208,6 +208,19 @@ def test_colouring_valid_and_product_equals_oracle(order, dims, mirrored):
         assert np.unique(nodes).size == nodes.size, f"colour {c} shares a node"
 
 
+@pytest.mark.parametrize("dims", [(12, 4, 4), (96, 24, 24), (5, 7, 3), (3, 3, 3)])
+def test_block_24_colouring_valid(dims):
+    xyz, tets = xf.tet_block(*dims, (0.1, 0.1, 0.1), (0, 0, 0))
+    colour = xf.tet_block_colours(*dims)
+    assert colour.max() == 23
+    T4 = tets.reshape(-1, 4)
+    for c in range(24):
+        nodes = T4[colour == c].reshape(-1)
+        assert np.unique(nodes).size == nodes.size, f"colour {c} shares a node"
+    cnt = np.bincount(colour)
+    assert cnt.max() - cnt.min() <= max(1, cnt.max() // 10)
+
+
 def test_ldlt_usage_example():
     A = np.array([1, 2, 3, 4, 5, 6], np.float32)
     b = np.array([7, 8, 9], np.float32)

@@ -1,0 +1,486 @@
+// xf_adjoint.hip -- gfx950 kernels of the adjoint (reverse mode) of the
+// substep: gradients of a trajectory loss with respect to per-tet log mu.
+// No reference counterpart (the reference has no adjoint).
+//
+// Precision follows the forward: FP64 node state and adjoint state, FP32
+// element arithmetic; per-tet gradients are summed in FP32 over a checkpoint
+// window and added to an FP64 accumulator at the window boundary. Every
+// forward expression is differentiated as written, clamps included. The
+// operation order matches the CPU oracle's adjoint of the same precision.
+//
+// Scope: simultaneous MixedSel / YeohSkinFast (energy 4, 7 or per-body 0),
+// pins, ground contact. Sphere contact is not differentiated.
+#include <hip/hip_runtime.h>
+
+#include "xf_device.h"
+
+namespace {
+
+__device__ __forceinline__ float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__device__ __forceinline__ void defgrad(const float Qi[3][3], const float P[3][3], float F[3][3]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) F[j][r] = Qi[0][r] * P[0][j] + Qi[1][r] * P[1][j] + Qi[2][r] * P[2][j];
+}
+
+__device__ __forceinline__ float adjdet(const float F[3][3], float adj[3][3]) {
+    float m00 = F[0][0], m01 = F[0][1], m02 = F[0][2];
+    float m10 = F[1][0], m11 = F[1][1], m12 = F[1][2];
+    float m20 = F[2][0], m21 = F[2][1], m22 = F[2][2];
+    adj[0][0] = +(m11 * m22 - m12 * m21);
+    adj[0][1] = -(m01 * m22 - m02 * m21);
+    adj[0][2] = +(m01 * m12 - m02 * m11);
+    adj[1][0] = -(m10 * m22 - m12 * m20);
+    adj[1][1] = +(m00 * m22 - m02 * m20);
+    adj[1][2] = -(m00 * m12 - m02 * m10);
+    adj[2][0] = +(m10 * m21 - m11 * m20);
+    adj[2][1] = -(m00 * m21 - m01 * m20);
+    adj[2][2] = +(m00 * m11 - m01 * m10);
+    return F[0][0] * adj[0][0] + F[0][1] * adj[1][0] + F[0][2] * adj[2][0];
+}
+
+__device__ __forceinline__ void apply_grad(const float G[3][3], const float Qi[3][3], float g[4][3]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) g[j][r] = g[j][r] + (G[0][r] * Qi[j][0] + G[1][r] * Qi[j][1] + G[2][r] * Qi[j][2]);
+}
+
+__device__ __forceinline__ void root_grad(float g[4][3]) {
+#pragma unroll
+    for (int r = 0; r < 3; r++) g[3][r] = ((0.0f - g[0][r]) - g[1][r]) - g[2][r];
+}
+
+__device__ __forceinline__ void neo_prefactored(const float QQ[3], const float QR[3], const float P[3][3], float& U,
+                                                float g[4][3]) {
+    float u = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        u = u + QQ[i] * dot3(P[i], P[i]);
+#pragma unroll
+        for (int r = 0; r < 3; r++) g[i][r] = (2.0f * QQ[i]) * P[i][r];
+    }
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i + 1; j < 3; j++) {
+            u = u + QR[k] * dot3(P[i], P[j]);
+#pragma unroll
+            for (int r = 0; r < 3; r++) g[i][r] = g[i][r] + QR[k] * P[j][r];
+#pragma unroll
+            for (int r = 0; r < 3; r++) g[j][r] = g[j][r] + QR[k] * P[i][r];
+            ++k;
+        }
+    root_grad(g);
+    U = u;
+}
+
+// Adjoint of one simultaneous projection (forward: xf_kernels.hip solve_tet
+// for E = 4 / 7). x: positions before it; xbar: in, adjoint of the positions
+// after it; out, adjoint of the positions before it. Returns
+// d(loss)/d(log comp0) of this projection.
+__device__ __forceinline__ float adjoint_tet(bool yeoh, const float C[3], const double x[4][3], const float w[4],
+                                             const float Qi[3][3], const float QQ[3], const float QR[3], float comp0,
+                                             float comp1, float a, float dt, double xbar[4][3]) {
+    float P[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) P[i][r] = (float)(x[i][r] - x[3][r]);
+    float g0n[4][3], U0n;
+    neo_prefactored(QQ, QR, P, U0n, g0n);
+    float U0 = U0n, IM = 0.0f, cpoly = 0.0f, s = 1.0f, g0[4][3];
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) g0[n][r] = g0n[n][r];
+    if (yeoh) {
+        IM = U0n - 3.0f;
+        cpoly = C[0] * IM + C[1] * IM * IM + C[2] * IM * IM * IM;
+        U0 = fmaxf(0.0001f, cpoly);
+        s = C[0] + 2.0f * C[1] * IM + 3.0f * C[2] * IM * IM;
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) g0[n][r] = g0n[n][r] * s;
+    }
+    float F[3][3], adj[3][3], G[3][3], g1[4][3];
+    defgrad(Qi, P, F);
+    float J = adjdet(F, adj);
+    float d = J - a;
+    float U1 = 0.0f + 1.0f * (d * d);
+    float sv = 2.0f * d;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) G[c][r] = adj[c][r] * sv;
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) g1[n][r] = 0.0f;
+    apply_grad(G, Qi, g1);
+    root_grad(g1);
+    float alpha0 = comp0 / (dt * dt), alpha1 = comp1 / (dt * dt);
+    float A0 = 1.0e-22f, A1 = 1.0e-22f, A2 = 1.0e-22f;
+#pragma unroll
+    for (int n = 0; n < 4; n++) A0 = A0 + w[n] * dot3(g0[n], g0[n]);
+    A0 = A0 + 2.0f * U0 * alpha0;
+    float b0 = -2.0f * U0;
+#pragma unroll
+    for (int n = 0; n < 4; n++) A1 = A1 + w[n] * dot3(g1[n], g0[n]);
+#pragma unroll
+    for (int n = 0; n < 4; n++) A2 = A2 + w[n] * dot3(g1[n], g1[n]);
+    A2 = A2 + 2.0f * U1 * alpha1;
+    float b1 = -2.0f * U1;
+    float u = 1.0f / A0, v = 1.0f / A2;
+    float p = A1 * u, q = A1 * v, be0 = b0 * u, be1 = b1 * v;
+    float Dr = 1.0f - p * q;
+    float invDet = 1.0f / fmaxf(0.00000001f, Dr);
+    float r0 = be0 - p * be1, r1 = be1 - be0 * q;
+    float l0 = invDet * r0, l1 = invDet * r1;
+
+    float g0b[4][3], g1b[4][3];
+    float l0b = 0.0f, l1b = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            float accb = w[n] * (float)xbar[n][r];
+            l0b = l0b + g0[n][r] * accb;
+            l1b = l1b + g1[n][r] * accb;
+            g0b[n][r] = l0 * accb;
+            g1b[n][r] = l1 * accb;
+        }
+    float invDetb = l0b * r0 + l1b * r1;
+    float r0b = l0b * invDet, r1b = l1b * invDet;
+    float be0b = r0b - r1b * q, be1b = r1b - r0b * p;
+    float pb = 0.0f - r0b * be1, qb = 0.0f - r1b * be0;
+    float Drb = Dr > 0.00000001f ? (0.0f - invDetb * invDet * invDet) : 0.0f;
+    pb = pb - Drb * q;
+    qb = qb - Drb * p;
+    float A1b = pb * u + qb * v;
+    float ub = pb * A1 + be0b * b0, vb = qb * A1 + be1b * b1;
+    float b0b = be0b * u, b1b = be1b * v;
+    float A0b = 0.0f - ub * u * u, A2b = 0.0f - vb * v * v;
+    float U0b = -2.0f * b0b + 2.0f * alpha0 * A0b;
+    float U1b = -2.0f * b1b + 2.0f * alpha1 * A2b;
+    float alpha0b = 2.0f * U0 * A0b;
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            g0b[n][r] = g0b[n][r] + 2.0f * A0b * w[n] * g0[n][r] + A1b * w[n] * g1[n][r];
+            g1b[n][r] = g1b[n][r] + A1b * w[n] * g0[n][r] + 2.0f * A2b * w[n] * g1[n][r];
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) g1b[i][r] = g1b[i][r] - g1b[3][r];
+    float adjb[3][3], Fb[3][3], Pb[3][3];
+    float svb = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            float Gb = g1b[0][r] * Qi[0][m] + g1b[1][r] * Qi[1][m] + g1b[2][r] * Qi[2][m];
+            adjb[m][r] = Gb * sv;
+            svb = svb + Gb * adj[m][r];
+        }
+    float db = 2.0f * svb + 2.0f * d * U1b;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) Fb[c][r] = db * adj[r][c];
+    {
+        float m00 = F[0][0], m01 = F[0][1], m02 = F[0][2];
+        float m10 = F[1][0], m11 = F[1][1], m12 = F[1][2];
+        float m20 = F[2][0], m21 = F[2][1], m22 = F[2][2];
+        float t;
+        t = adjb[0][0]; Fb[1][1] += t * m22; Fb[2][2] += t * m11; Fb[1][2] -= t * m21; Fb[2][1] -= t * m12;
+        t = adjb[0][1]; Fb[0][1] -= t * m22; Fb[2][2] -= t * m01; Fb[0][2] += t * m21; Fb[2][1] += t * m02;
+        t = adjb[0][2]; Fb[0][1] += t * m12; Fb[1][2] += t * m01; Fb[0][2] -= t * m11; Fb[1][1] -= t * m02;
+        t = adjb[1][0]; Fb[1][0] -= t * m22; Fb[2][2] -= t * m10; Fb[1][2] += t * m20; Fb[2][0] += t * m12;
+        t = adjb[1][1]; Fb[0][0] += t * m22; Fb[2][2] += t * m00; Fb[0][2] -= t * m20; Fb[2][0] -= t * m02;
+        t = adjb[1][2]; Fb[0][0] -= t * m12; Fb[1][2] -= t * m00; Fb[0][2] += t * m10; Fb[1][0] += t * m02;
+        t = adjb[2][0]; Fb[1][0] += t * m21; Fb[2][1] += t * m10; Fb[1][1] -= t * m20; Fb[2][0] -= t * m11;
+        t = adjb[2][1]; Fb[0][0] -= t * m21; Fb[2][1] -= t * m00; Fb[0][1] += t * m20; Fb[2][0] += t * m01;
+        t = adjb[2][2]; Fb[0][0] += t * m11; Fb[1][1] += t * m00; Fb[0][1] -= t * m10; Fb[1][0] -= t * m01;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Pb[m][j] = Qi[m][0] * Fb[j][0] + Qi[m][1] * Fb[j][1] + Qi[m][2] * Fb[j][2];
+    float g0nb[4][3], U0nb;
+    if (yeoh) {
+        float polyb = cpoly > 0.0001f ? U0b : 0.0f;
+        float sb = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                sb = sb + g0b[n][r] * g0n[n][r];
+                g0nb[n][r] = g0b[n][r] * s;
+            }
+        U0nb = polyb * s + sb * (2.0f * C[1] + 6.0f * C[2] * IM);
+    } else {
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) g0nb[n][r] = g0b[n][r];
+        U0nb = U0b;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) g0nb[i][r] = g0nb[i][r] - g0nb[3][r];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) Pb[i][r] = Pb[i][r] + U0nb * g0n[i][r] + 2.0f * QQ[i] * g0nb[i][r];
+    {
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = i + 1; j < 3; j++) {
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    Pb[j][r] = Pb[j][r] + QR[k] * g0nb[i][r];
+                    Pb[i][r] = Pb[i][r] + QR[k] * g0nb[j][r];
+                }
+                ++k;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            xbar[i][r] = xbar[i][r] + (double)Pb[i][r];
+            xbar[3][r] = xbar[3][r] - (double)Pb[i][r];
+        }
+    return alpha0b * alpha0;
+}
+
+struct TetData {
+    float Qi[3][3], QQ[3], QR[3], comp0, comp1, a;
+    uint32_t body;
+};
+
+template <bool AUX>
+__device__ __forceinline__ TetData load_tet(uint32_t i, const float4* tg0, const float4* tg1, const float4* tg2,
+                                            const float4* tg3, const float4* taux, float comp1_u, float a_u) {
+    TetData t;
+    float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i], q3 = tg3[i];
+    t.Qi[0][0] = q0.x; t.Qi[0][1] = q0.y; t.Qi[0][2] = q0.z;
+    t.Qi[1][0] = q0.w; t.Qi[1][1] = q1.x; t.Qi[1][2] = q1.y;
+    t.Qi[2][0] = q1.z; t.Qi[2][1] = q1.w; t.Qi[2][2] = q2.x;
+    t.QQ[0] = q2.y; t.QQ[1] = q2.z; t.QQ[2] = q2.w;
+    t.QR[0] = q3.x; t.QR[1] = q3.y; t.QR[2] = q3.z;
+    t.comp0 = q3.w;
+    t.comp1 = comp1_u;
+    t.a = a_u;
+    t.body = 0;
+    if constexpr (AUX) {
+        float4 ax = taux[i];
+        t.comp1 = ax.x;
+        t.a = ax.y;
+        t.body = __float_as_uint(ax.z);
+    }
+    return t;
+}
+
+}  // namespace
+
+// Reverse sweep of one colour: tets [begin, begin+count) of the sweep order.
+// tape holds each tet's 4 node positions before its projection (12 doubles
+// per schedule position) for this substep. gacc accumulates d(loss)/d(log
+// comp0) per schedule position.
+template <bool AUX>
+__global__ __launch_bounds__(256) void xf_project_reverse(
+    uint32_t begin, uint32_t count, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+    const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+    const float4* __restrict__ taux, const double* __restrict__ tape, const double4* __restrict__ X,
+    double4* __restrict__ Xb, float* __restrict__ gacc, const XfBodyParams* __restrict__ bodies, float comp1_u,
+    float a_u, float dt) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    uint32_t i = begin + k;
+    uint4 id = tidx[i];
+    TetData t = load_tet<AUX>(i, tg0, tg1, tg2, tg3, taux, comp1_u, a_u);
+    const XfBodyParams& bp = bodies[t.body];
+    float C[3] = {bp.yeoh[0], bp.yeoh[1], bp.yeoh[2]};
+    const double* tp = tape + 12 * (size_t)i;
+    double x[4][3];
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) x[n][r] = tp[3 * n + r];
+    uint32_t nid[4] = {id.x, id.y, id.z, id.w};
+    float w[4];
+    double xb[4][3];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        w[n] = (float)X[nid[n]].w;
+        double4 b = Xb[nid[n]];
+        xb[n][0] = b.x;
+        xb[n][1] = b.y;
+        xb[n][2] = b.z;
+    }
+    float gl = adjoint_tet(bp.energy == 7, C, x, w, t.Qi, t.QQ, t.QR, t.comp0, t.comp1, t.a, dt, xb);
+    gacc[i] = gacc[i] + gl;
+#pragma unroll
+    for (int n = 0; n < 4; n++) Xb[nid[n]] = make_double4(xb[n][0], xb[n][1], xb[n][2], 0.0);
+}
+
+// Reverse of the end of one substep (velocity, pin, ground contact) for one
+// node, with the loss injected first when this substep ends a loss frame:
+// Xb += lw_i * (x_loss - target). xs / o: the node's position before the
+// contact pass and at the substep start (recorded by the forward replay).
+__global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4* __restrict__ O,
+                                                       const double4* __restrict__ xs, const double4* __restrict__ o,
+                                                       double4* __restrict__ Xb, double4* __restrict__ Vb,
+                                                       double4* __restrict__ Ob, const XfBodyParams* __restrict__ bodies,
+                                                       const double4* __restrict__ xloss,
+                                                       const double4* __restrict__ target,
+                                                       const double* __restrict__ lw) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t flags = (uint32_t)O[i].w;
+    const XfBodyParams& p = bodies[flags >> 1];
+    double4 xb4 = Xb[i], vb4 = Vb[i];
+    double xbv[3] = {xb4.x, xb4.y, xb4.z};
+    if (xloss) {
+        double4 xl = xloss[i], tg = target[i];
+        double wl = lw[i];
+        xbv[0] = xbv[0] + wl * (xl.x - tg.x);
+        xbv[1] = xbv[1] + wl * (xl.y - tg.y);
+        xbv[2] = xbv[2] + wl * (xl.z - tg.z);
+    }
+    double vbv[3] = {vb4.x, vb4.y, vb4.z};
+    double invdt = (double)p.invdt;
+    double xcb[3], ob[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        xcb[r] = xbv[r] + vbv[r] * invdt;
+        ob[r] = 0.0 - vbv[r] * invdt;
+    }
+    double out[3] = {xcb[0], xcb[1], xcb[2]};
+    if (flags & 1u) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            ob[r] = ob[r] + xcb[r];
+            out[r] = 0.0;
+        }
+    } else if (p.collide & 1u) {
+        double4 s4 = xs[i], o4 = o[i];
+        float gy = p.ground_y;
+        float xy = (float)s4.y;
+        if (xy < gy) {
+            float mu = p.friction;
+            float depth = gy - xy;
+            float dx = (float)(s4.x - o4.x), dz = (float)(s4.z - o4.z);
+            float t2 = dx * dx + dz * dz;
+            float lim = mu * depth;
+            out[1] = 0.0;
+            if (t2 <= lim * lim) {
+                ob[0] = ob[0] + xcb[0];
+                ob[2] = ob[2] + xcb[2];
+                out[0] = 0.0;
+                out[2] = 0.0;
+            } else {
+                float L = sqrtf(t2);
+                float ux = dx / L, uz = dz / L;
+                float bx = (float)xcb[0], bz = (float)xcb[2];
+                float ubv = ux * bx + uz * bz;
+                float kk = lim / L;
+                float px = kk * (bx - ux * ubv), pz = kk * (bz - uz * ubv);
+                out[0] = xcb[0] - (double)px;
+                out[2] = xcb[2] - (double)pz;
+                ob[0] = ob[0] + (double)px;
+                ob[2] = ob[2] + (double)pz;
+                out[1] = (double)(mu * ubv);
+            }
+        }
+    }
+    Xb[i] = make_double4(out[0], out[1], out[2], 0.0);
+    Ob[i] = make_double4(ob[0], ob[1], ob[2], 0.0);
+}
+
+// Reverse of the prediction (Geo.cpp:307-312) for one node: Xb <- Xb + Ob,
+// Vb <- Xb_pred * dt * (1 - drag).
+__global__ __launch_bounds__(256) void xf_adj_node_start(uint32_t n, const double4* __restrict__ O,
+                                                         double4* __restrict__ Xb, double4* __restrict__ Vb,
+                                                         const double4* __restrict__ Ob,
+                                                         const XfBodyParams* __restrict__ bodies) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const XfBodyParams& p = bodies[(uint32_t)O[i].w >> 1];
+    double4 xb = Xb[i], obv = Ob[i];
+    double dt = (double)p.dt, dragf = (double)p.dragf;
+    Xb[i] = make_double4(xb.x + obv.x, xb.y + obv.y, xb.z + obv.z, 0.0);
+    Vb[i] = make_double4((xb.x * dt) * dragf, (xb.y * dt) * dragf, (xb.z * dt) * dragf, 0.0);
+}
+
+// Window boundary: glog += gacc (FP32 window sum into the FP64 total); gacc = 0.
+__global__ __launch_bounds__(256) void xf_adj_flush(uint32_t t, float* __restrict__ gacc, double* __restrict__ glog) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t) return;
+    glog[i] += (double)gacc[i];
+    gacc[i] = 0.0f;
+}
+
+// Zero-fill of a double4 array (adjoint state reset).
+__global__ __launch_bounds__(256) void xf_zero_d4(uint32_t n, double4* __restrict__ a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+// ------------------------------------------------------ launch wrappers
+
+extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, uint32_t begin, uint32_t count,
+                                                const uint4* tidx, const float4* tg0, const float4* tg1,
+                                                const float4* tg2, const float4* tg3, const float4* taux,
+                                                const double* tape, const double4* X, double4* Xb, float* gacc,
+                                                const XfBodyParams* bodies, float comp1_u, float a_u, float dt) {
+    if (count == 0) return hipSuccess;
+    dim3 grid((count + 63) / 64), block(64);
+    if (aux)
+        hipLaunchKernelGGL(xf_project_reverse<true>, grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2, tg3, taux,
+                           tape, X, Xb, gacc, bodies, comp1_u, a_u, dt);
+    else
+        hipLaunchKernelGGL(xf_project_reverse<false>, grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2, tg3,
+                           taux, tape, X, Xb, gacc, bodies, comp1_u, a_u, dt);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, uint32_t n, const double4* O, const double4* xs,
+                                             const double4* o, double4* Xb, double4* Vb, double4* Ob,
+                                             const XfBodyParams* bodies, const double4* xloss, const double4* target,
+                                             const double* lw) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_adj_node_end, dim3((n + 255) / 256), dim3(256), 0, s, n, O, xs, o, Xb, Vb, Ob, bodies, xloss,
+                       target, lw);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const double4* O, double4* Xb, double4* Vb,
+                                               const double4* Ob, const XfBodyParams* bodies) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_adj_node_start, dim3((n + 255) / 256), dim3(256), 0, s, n, O, Xb, Vb, Ob, bodies);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog) {
+    if (t == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_adj_flush, dim3((t + 255) / 256), dim3(256), 0, s, t, gacc, glog);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_zero_d4, dim3((n + 255) / 256), dim3(256), 0, s, n, a);
+    return hipGetLastError();
+}

@@ -154,9 +154,11 @@ class GeoGpuLinear3d {
         d.pinned = (flags & kLockLeft) ? left_.data() : nullptr;
         if ((rc = xf_scene_add_body(scene_, &d, nullptr))) return rc;
         xf_scene_opts o;
+        memset(&o, 0, sizeof(o));
         o.colour_order = XF_COLOUR_NATURAL;
         o.block_size = 64;
         o.use_graph = 1;
+        o.launch_mode = XF_LAUNCH_PER_COLOUR;
         return xf_scene_finalize(scene_, &o);
     }
 

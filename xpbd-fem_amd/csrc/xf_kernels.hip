@@ -409,6 +409,50 @@ __global__ __launch_bounds__(256) void xf_finish(uint32_t n, double4* __restrict
     }
 }
 
+// Forward projection of one colour that also records each tet's 4 node
+// positions before the projection into tape (12 doubles per schedule
+// position), for the adjoint's replay of a checkpoint window.
+template <int E, bool AUX>
+__global__ __launch_bounds__(256) void xf_project_record(
+    uint32_t begin, uint32_t count, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+    const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+    const float4* __restrict__ taux, double4* __restrict__ X, double* __restrict__ tape,
+    const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock) {
+    if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) clock[0] = clock[0] + dt;
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    uint32_t i = begin + k;
+    uint4 id = tidx[i];
+    float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i], q3 = tg3[i];
+    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
+    float comp1 = comp1_u, a = a_u;
+    uint32_t body = 0;
+    if constexpr (AUX) {
+        float4 ax = taux[i];
+        comp1 = ax.x;
+        a = ax.y;
+        body = __float_as_uint(ax.z);
+    }
+    const XfBodyParams& bp = bodies[body];
+    float C[3] = {bp.yeoh[0], bp.yeoh[1], bp.yeoh[2]};
+    bool yeohFast = bp.energy == 7;
+    float Qi[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+    float QQ[3] = {q2.y, q2.z, q2.w};
+    float QR[3] = {q3.x, q3.y, q3.z};
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    double* tp = tape + 12 * (size_t)i;
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) tp[3 * n + r] = x[n][r];
+    solve_tet<E, true>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt);
+    X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
+    X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
+    X[id.z] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
+    X[id.w] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+}
+
 // ------------------------------------------------------ persistent frame
 
 namespace {
@@ -675,4 +719,23 @@ extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool 
     void* args[] = {&substeps, &ncol,  &colour_begin, &n_nodes, &tidx,   &tg0,   &tg1, &tg2,   &tg3, &taux,
                     &X,        &O,     &V,            &bodies,  &comp1_u, &a_u,  &dt,  &clock, &arrive, &err};
     return hipLaunchCooperativeKernel((const void*)fn, dim3(blocks), dim3(256), args, 0, s);
+}
+
+extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool aux, uint32_t begin, uint32_t count,
+                                               const uint4* tidx, const float4* tg0, const float4* tg1,
+                                               const float4* tg2, const float4* tg3, const float4* taux, double4* X,
+                                               double* tape, const XfBodyParams* bodies, float comp1_u, float a_u,
+                                               float dt, float* clock) {
+    if (count == 0) return hipSuccess;
+    dim3 grid((count + 63) / 64), block(64);
+#define XF_REC(E, A) hipLaunchKernelGGL((xf_project_record<E, A>), grid, block, 0, s, begin, count, tidx, tg0, tg1, \
+                                        tg2, tg3, taux, X, tape, bodies, comp1_u, a_u, dt, clock)
+    switch (energy) {
+        case 0: if (aux) XF_REC(0, true); else XF_REC(0, false); break;
+        case 4: if (aux) XF_REC(4, true); else XF_REC(4, false); break;
+        case 7: if (aux) XF_REC(7, true); else XF_REC(7, false); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef XF_REC
+    return hipGetLastError();
 }

@@ -29,6 +29,24 @@ extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_
                                        double4* V, const XfBodyParams* bodies, const float* clock);
 extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const double4* X, uint32_t* count);
 extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, int* max_blocks);
+extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool aux, uint32_t begin, uint32_t count,
+                                               const uint4* tidx, const float4* tg0, const float4* tg1,
+                                               const float4* tg2, const float4* tg3, const float4* taux, double4* X,
+                                               double* tape, const XfBodyParams* bodies, float comp1_u, float a_u,
+                                               float dt, float* clock);
+extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, uint32_t begin, uint32_t count,
+                                                const uint4* tidx, const float4* tg0, const float4* tg1,
+                                                const float4* tg2, const float4* tg3, const float4* taux,
+                                                const double* tape, const double4* X, double4* Xb, float* gacc,
+                                                const XfBodyParams* bodies, float comp1_u, float a_u, float dt);
+extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, uint32_t n, const double4* O, const double4* xs,
+                                             const double4* o, double4* Xb, double4* Vb, double4* Ob,
+                                             const XfBodyParams* bodies, const double4* xloss, const double4* target,
+                                             const double* lw);
+extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const double4* O, double4* Xb, double4* Vb,
+                                               const double4* Ob, const XfBodyParams* bodies);
+extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog);
+extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a);
 extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, uint32_t blocks,
                                       uint32_t substeps, uint32_t ncol, const uint32_t* colour_begin,
                                       uint32_t n_nodes, const uint4* tidx, const float4* tg0, const float4* tg1,
@@ -66,6 +84,7 @@ struct Body {
     float density = 0.0f;
     std::vector<float> xyz, vel, tet_mu;
     std::vector<uint32_t> tets;
+    std::vector<uint32_t> user_colour;  // caller-supplied colouring (empty: greedy)
     std::vector<uint8_t> pinned;
     xf_settings settings{};
     // finalize products
@@ -120,6 +139,13 @@ struct xf_scene {
     float comp1_u = 0.0f, a_u = 1.0f;
     float params_dt = -1.0f;
     float time_host = 0.0f;
+    // adjoint buffers (grown on demand, freed with the scene)
+    struct AdjBuf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    AdjBuf adj_ck_x, adj_ck_v, adj_tape, adj_xs, adj_o, adj_xb, adj_vb, adj_ob, adj_gacc, adj_glog, adj_xloss,
+        adj_tgt, adj_lw, adj_save_x, adj_save_v;
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -408,6 +434,10 @@ XF_API void xf_scene_destroy(xf_scene* s) {
                     s->barrier_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
+                                &s->adj_vb, &s->adj_ob, &s->adj_gacc, &s->adj_glog, &s->adj_xloss, &s->adj_tgt,
+                                &s->adj_lw, &s->adj_save_x, &s->adj_save_v})
+        if (b->p) (void)hipFree(b->p);
     delete s;
 }
 
@@ -447,6 +477,26 @@ XF_API int xf_scene_add_body(xf_scene* s, const xf_body_desc* d, uint32_t* body_
         for (float m : b.tet_mu)
             if (!(m > 0.0f) || !std::isfinite(m)) return Fail(ctx, XF_E_INVALID, "tet_mu must be positive");
     }
+    if (d->tet_colour) {
+        b.user_colour.assign(d->tet_colour, d->tet_colour + b.t);
+        // each colour class must be node-disjoint
+        std::vector<uint32_t> seen(b.n, ~0u);
+        std::vector<uint32_t> order(b.t);
+        for (uint32_t t = 0; t < b.t; t++) {
+            if (b.user_colour[t] >= xf::kMaxColours)
+                return Fail(ctx, XF_E_INVALID, "tet %u: colour %u >= %u", t, b.user_colour[t], xf::kMaxColours);
+            order[t] = t;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t c) { return b.user_colour[a] < b.user_colour[c]; });
+        for (uint32_t t : order)
+            for (int i = 0; i < 4; i++) {
+                uint32_t v = b.tets[4 * (size_t)t + i];
+                if (seen[v] == b.user_colour[t])
+                    return Fail(ctx, XF_E_INVALID, "colour %u has two tets sharing node %u", b.user_colour[t], v);
+                seen[v] = b.user_colour[t];
+            }
+    }
     xf_settings_default(&b.settings);
     if (!xf::InitElements(b.xyz.data(), b.n, b.tets.data(), b.t, b.density, &b.ec))
         return Fail(ctx, XF_E_INVALID, "bad tet indices");
@@ -480,6 +530,16 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     // Colour each body; the scene runs max(ncol) phases.
     s->ncol = 0;
     for (Body& b : s->bodies) {
+        if (!b.user_colour.empty()) {
+            b.colour = b.user_colour;
+            b.ncol = 1 + *std::max_element(b.colour.begin(), b.colour.end());
+            b.order.resize(b.t);
+            for (uint32_t t = 0; t < b.t; t++) b.order[t] = t;
+            std::stable_sort(b.order.begin(), b.order.end(),
+                             [&](uint32_t a, uint32_t c) { return b.colour[a] < b.colour[c]; });
+            s->ncol = std::max(s->ncol, b.ncol);
+            continue;
+        }
         b.ncol = xf::ColourSweepOrder(b.n, b.tets.data(), b.t, s->opts.colour_order == XF_COLOUR_MINSTD, &b.colour,
                                       &b.order);
         if (b.ncol == 0) return Fail(ctx, XF_E_UNSUPPORTED, "mesh needs more than %u colours", xf::kMaxColours);
@@ -895,6 +955,191 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     return XF_OK;
 }
 
+
+namespace {
+int Reserve(xf_scene* s, xf_scene::AdjBuf* b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b->bytes >= bytes) return XF_OK;
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->bytes = 0;
+    hipError_t e = hipMalloc(&b->p, bytes);
+    if (e != hipSuccess) return Fail(s->ctx, XF_E_NOMEM, "adjoint buffer of %zu bytes: %s", bytes, hipGetErrorString(e));
+    b->bytes = bytes;
+    return XF_OK;
+}
+}  // namespace
+
+XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
+                            const double* targets, uint32_t window, double* loss, double* grad_log_mu) {
+    if (!s) return XF_E_INVALID;
+    xf_ctx* ctx = s->ctx;
+    if (!s->finalized) return Fail(ctx, XF_E_STATE, "scene is not finalized");
+    if (substeps == 0 || window == 0) return Fail(ctx, XF_E_INVALID, "substeps and window must be positive");
+    if (n_loss == 0 || !loss_at || !targets) return Fail(ctx, XF_E_INVALID, "need at least one loss substep");
+    for (uint32_t j = 0; j < n_loss; j++)
+        if (loss_at[j] < 1 || loss_at[j] > substeps || (j && loss_at[j] <= loss_at[j - 1]))
+            return Fail(ctx, XF_E_INVALID, "loss_at must ascend within [1, substeps]");
+    for (const Body& b : s->bodies) {
+        const xf_settings& st = b.settings;
+        if (!st.simultaneous || (st.energy != 4 && st.energy != 7) || (st.collide & XF_COLLIDE_SPHERE))
+            return Fail(ctx, XF_E_UNSUPPORTED,
+                        "adjoint covers simultaneous MixedSel/YeohSkinFast with pins and ground contact");
+    }
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    int energy;
+    bool sim;
+    int rc = Prepare(s, dt, &energy, &sim);
+    if (rc) return rc;
+    const uint32_t N = s->N, T = s->T;
+    const uint32_t nwin = (substeps + window - 1) / window;
+    const uint32_t wlen = std::min(window, substeps);
+    const size_t n4 = sizeof(double4) * N;
+    if ((rc = Reserve(s, &s->adj_ck_x, n4 * nwin)) || (rc = Reserve(s, &s->adj_ck_v, n4 * nwin)) ||
+        (rc = Reserve(s, &s->adj_tape, sizeof(double) * 12 * (size_t)T * wlen)) ||
+        (rc = Reserve(s, &s->adj_xs, n4 * wlen)) || (rc = Reserve(s, &s->adj_o, n4 * wlen)) ||
+        (rc = Reserve(s, &s->adj_xb, n4)) || (rc = Reserve(s, &s->adj_vb, n4)) || (rc = Reserve(s, &s->adj_ob, n4)) ||
+        (rc = Reserve(s, &s->adj_gacc, sizeof(float) * T)) || (rc = Reserve(s, &s->adj_glog, sizeof(double) * T)) ||
+        (rc = Reserve(s, &s->adj_xloss, n4 * n_loss)) || (rc = Reserve(s, &s->adj_tgt, n4 * n_loss)) ||
+        (rc = Reserve(s, &s->adj_lw, sizeof(double) * N)) || (rc = Reserve(s, &s->adj_save_x, n4)) ||
+        (rc = Reserve(s, &s->adj_save_v, n4)))
+        return rc;
+    double4* ckx = (double4*)s->adj_ck_x.p;
+    double4* ckv = (double4*)s->adj_ck_v.p;
+    double* tape = (double*)s->adj_tape.p;
+    double4* xs = (double4*)s->adj_xs.p;
+    double4* oo = (double4*)s->adj_o.p;
+    double4* Xb = (double4*)s->adj_xb.p;
+    double4* Vb = (double4*)s->adj_vb.p;
+    double4* Ob = (double4*)s->adj_ob.p;
+    float* gacc = (float*)s->adj_gacc.p;
+    double* glog = (double*)s->adj_glog.p;
+    double4* xloss = (double4*)s->adj_xloss.p;
+    double4* tgt = (double4*)s->adj_tgt.p;
+    double* lw = (double*)s->adj_lw.p;
+    hipStream_t st = ctx->stream;
+
+    // Targets (body-concatenated xyz) and per-node loss-gradient weights 2/n_b.
+    {
+        std::vector<double4> ht((size_t)N * n_loss);
+        for (size_t k = 0; k < ht.size(); k++)
+            ht[k] = make_double4(targets[3 * k], targets[3 * k + 1], targets[3 * k + 2], 0.0);
+        std::vector<double> hw(N);
+        for (const Body& b : s->bodies)
+            for (uint32_t i = 0; i < b.n; i++) hw[b.node_base + i] = 2.0 / (double)b.n;
+        XF_HIP(ctx, hipMemcpyAsync(tgt, ht.data(), ht.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+        XF_HIP(ctx, hipMemcpyAsync(lw, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        XF_HIP(ctx, hipStreamSynchronize(st));
+    }
+
+    // Forward pass with checkpoints at window starts and states at loss substeps.
+    uint32_t k = 0, j = 0;
+    while (k < substeps) {
+        if (k % window == 0) {
+            XF_HIP(ctx, hipMemcpyAsync(ckx + (size_t)N * (k / window), s->X, n4, hipMemcpyDeviceToDevice, st));
+            XF_HIP(ctx, hipMemcpyAsync(ckv + (size_t)N * (k / window), s->V, n4, hipMemcpyDeviceToDevice, st));
+        }
+        uint32_t next = std::min(substeps, (k / window + 1) * window);
+        if (j < n_loss) next = std::min(next, loss_at[j]);
+        if ((rc = xf_scene_step(s, next - k, dt))) return rc;
+        k = next;
+        if (j < n_loss && loss_at[j] == k) {
+            XF_HIP(ctx, hipMemcpyAsync(xloss + (size_t)N * j, s->X, n4, hipMemcpyDeviceToDevice, st));
+            j++;
+        }
+    }
+    XF_HIP(ctx, hipMemcpyAsync(s->adj_save_x.p, s->X, n4, hipMemcpyDeviceToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->adj_save_v.p, s->V, n4, hipMemcpyDeviceToDevice, st));
+    float clock_end = 0.0f;
+    XF_HIP(ctx, hipMemcpyAsync(&clock_end, s->clock, sizeof(float), hipMemcpyDeviceToHost, st));
+    XF_HIP(ctx, hipStreamSynchronize(st));
+    if ((rc = CheckBarrier(s))) return rc;
+
+    // Loss: sum over loss substeps and bodies of (1/n_b) sum |x - target|^2.
+    {
+        std::vector<double4> hx((size_t)N * n_loss);
+        XF_HIP(ctx, hipMemcpy(hx.data(), xloss, hx.size() * sizeof(double4), hipMemcpyDeviceToHost));
+        double L = 0.0;
+        for (uint32_t jj = 0; jj < n_loss; jj++)
+            for (const Body& b : s->bodies) {
+                double e = 0.0;
+                for (uint32_t i = 0; i < b.n; i++) {
+                    size_t g = (size_t)N * jj + b.node_base + i;
+                    const double* tg = targets + 3 * g;
+                    double d0 = hx[g].x - tg[0], d1 = hx[g].y - tg[1], d2 = hx[g].z - tg[2];
+                    e += d0 * d0;
+                    e += d1 * d1;
+                    e += d2 * d2;
+                }
+                L += e / (double)b.n;
+            }
+        if (loss) *loss = L;
+    }
+
+    // Reverse pass, window by window from the end.
+    XF_HIP(ctx, xf_launch_zero_d4(st, N, Xb));
+    XF_HIP(ctx, xf_launch_zero_d4(st, N, Vb));
+    XF_HIP(ctx, hipMemsetAsync(gacc, 0, sizeof(float) * T, st));
+    XF_HIP(ctx, hipMemsetAsync(glog, 0, sizeof(double) * T, st));
+    const uint32_t block = s->opts.block_size ? s->opts.block_size : 64;
+    uint32_t jl = n_loss;
+    for (uint32_t w = nwin; w-- > 0;) {
+        const uint32_t k0 = w * window, k1 = std::min(substeps, k0 + window);
+        XF_HIP(ctx, hipMemcpyAsync(s->X, ckx + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
+        XF_HIP(ctx, hipMemcpyAsync(s->V, ckv + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
+        // replay of the window, recording the sweep tape and the node states
+        XF_HIP(ctx, xf_launch_predict(st, N, s->X, s->O, s->V, s->bp));
+        for (uint32_t kk = k0; kk < k1; kk++) {
+            double* tp = tape + 12 * (size_t)T * (kk - k0);
+            for (uint32_t c = 0; c < s->ncol; c++) {
+                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                XF_HIP(ctx, xf_launch_project_record(st, energy, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1],
+                                                     s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp, s->comp1_u, s->a_u,
+                                                     dt, nullptr));
+            }
+            XF_HIP(ctx, hipMemcpyAsync(xs + (size_t)N * (kk - k0), s->X, n4, hipMemcpyDeviceToDevice, st));
+            XF_HIP(ctx, hipMemcpyAsync(oo + (size_t)N * (kk - k0), s->O, n4, hipMemcpyDeviceToDevice, st));
+            XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, N, s->X, s->O, s->V, s->bp, s->clock));
+        }
+        (void)block;
+        for (uint32_t kk = k1; kk-- > k0;) {
+            const double4* xl = nullptr;
+            const double4* tg = nullptr;
+            if (jl > 0 && loss_at[jl - 1] == kk + 1) {
+                jl--;
+                xl = xloss + (size_t)N * jl;
+                tg = tgt + (size_t)N * jl;
+            }
+            XF_HIP(ctx, xf_launch_adj_node_end(st, N, s->O, xs + (size_t)N * (kk - k0), oo + (size_t)N * (kk - k0), Xb,
+                                               Vb, Ob, s->bp, xl, tg, lw));
+            const double* tp = tape + 12 * (size_t)T * (kk - k0);
+            for (uint32_t c = s->ncol; c-- > 0;) {
+                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                XF_HIP(ctx, xf_launch_project_reverse(st, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                                                      s->tg[3], s->taux, tp, s->X, Xb, gacc, s->bp, s->comp1_u,
+                                                      s->a_u, dt));
+            }
+            XF_HIP(ctx, xf_launch_adj_node_start(st, N, s->O, Xb, Vb, Ob, s->bp));
+        }
+        XF_HIP(ctx, xf_launch_adj_flush(st, T, gacc, glog));
+    }
+    // Leave the scene at the end of the forward rollout.
+    XF_HIP(ctx, hipMemcpyAsync(s->X, s->adj_save_x.p, n4, hipMemcpyDeviceToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->V, s->adj_save_v.p, n4, hipMemcpyDeviceToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->clock, &clock_end, sizeof(float), hipMemcpyHostToDevice, st));
+    std::vector<double> hg(T);
+    XF_HIP(ctx, hipMemcpyAsync(hg.data(), glog, sizeof(double) * T, hipMemcpyDeviceToHost, st));
+    XF_HIP(ctx, hipStreamSynchronize(st));
+    if (grad_log_mu) {
+        size_t off = 0;
+        for (const Body& b : s->bodies) {
+            for (uint32_t t = 0; t < b.t; t++) grad_log_mu[off + t] = -hg[b.sched_pos[t]];
+            off += b.t;
+        }
+    }
+    return XF_OK;
+}
+
 XF_API int xf_scene_count_nonfinite(xf_scene* s, uint32_t* count) {
     if (!s || !count) return XF_E_INVALID;
     if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
@@ -913,6 +1158,17 @@ XF_API int xf_mesh_tet_block(uint32_t W, uint32_t H, uint32_t D, const float sca
     if (!W || !H || !D || !scale || !origin || !xyz || !tets) return XF_E_INVALID;
     if ((uint64_t)W * H * D * 6 >= (1ull << 31)) return XF_E_INVALID;
     xf::TetBlock(W, H, D, scale, origin, mirrored != 0, xyz, tets);
+    return XF_OK;
+}
+
+XF_API int xf_mesh_tet_block_colours(uint32_t W, uint32_t H, uint32_t D, uint32_t* colour) {
+    if (!W || !H || !D || !colour) return XF_E_INVALID;
+    if ((uint64_t)W * H * D * 6 >= (1ull << 31)) return XF_E_INVALID;
+    size_t t = 0;
+    for (uint32_t z = 0; z < D; z++)
+        for (uint32_t y = 0; y < H; y++)
+            for (uint32_t x = 0; x < W; x++)
+                for (uint32_t k = 0; k < 6; k++) colour[t++] = 6 * ((x + y + z) % 4) + k;
     return XF_OK;
 }
 

@@ -32,7 +32,7 @@ def beam_neo_12x4x4(energy=xf.ENERGY_MIXED_SEL, simultaneous=True, collide=True)
         dt=np.float32(1.0) / np.float32(60 * substeps), substeps=substeps, frames=120)
 
 
-def beam_yeoh_96x24x24(seed=1, h=0.01):
+def beam_yeoh_96x24x24(seed=1, h=0.01, structured=True):
     """96x24x24 cells of h = 0.01 m (331,776 tets, 60,625 nodes), Yeoh
     C10 = 5e4, C20 = 5e3, C30 = 5e2 Pa (compliance 1/C10, C = (1, 0.1, 0.01),
     YeohSkinFast form), incompressible, density 1000, initial velocity
@@ -48,7 +48,7 @@ def beam_yeoh_96x24x24(seed=1, h=0.01):
     c10, c20, c30 = 5e4, 5e3, 5e2
     return dict(
         name="beam96x24x24_yeoh", xyz=xyz, tets=tets, density=1000.0, pinned=_pins_x0(xyz), velocity=vel,
-        tet_mu=None,
+        tet_mu=None, tet_colour=xf.tet_block_colours(W, H, D) if structured else None,
         settings=dict(energy=xf.ENERGY_YEOH_SKIN_FAST, simultaneous=1, compliance=1.0 / c10, poissons_ratio=0.5,
                       gravity=(0.0, -9.81), yeoh=(1.0, c20 / c10, c30 / c10),
                       collide=xf.COLLIDE_GROUND | xf.COLLIDE_SPHERE, ground_y=-0.5, friction=0.5,
@@ -133,7 +133,8 @@ def boxes_batch(n_bodies=256, seed=2, cells=(3, 24), h=0.1, frames=60):
                         compliance=(0.1095 / mod) if yeoh else 1.0 / mod, poissons_ratio=0.5, gravity=(0.0, -9.81),
                         yeoh=xf.SKIN_YEOH, collide=xf.COLLIDE_GROUND, ground_y=0.0, friction=0.5)
         bodies.append(dict(name=f"box{b}_{W}x{H}x{D}", xyz=p.astype(np.float32).reshape(-1), tets=tets, density=rho,
-                           pinned=None, velocity=None, tet_mu=None, settings=settings))
+                           pinned=None, velocity=None, tet_mu=None, tet_colour=xf.tet_block_colours(W, H, D),
+                           settings=settings))
     substeps = 20
     return dict(name=f"boxes{n_bodies}", bodies=bodies, dt=np.float32(1.0) / np.float32(60 * substeps),
                 substeps=substeps, frames=frames)
@@ -146,7 +147,7 @@ def make_batch_scene(ctx, batch, bodies=None, colour_order=xf.COLOUR_NATURAL, us
     chosen = batch["bodies"] if bodies is None else [batch["bodies"][i] for i in bodies]
     for wl in chosen:
         sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"], velocity=wl["velocity"],
-                    tet_mu=wl["tet_mu"])
+                    tet_mu=wl["tet_mu"], tet_colour=wl.get("tet_colour"))
     sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size, launch_mode=launch_mode,
                 persistent_blocks=persistent_blocks)
     for i, wl in enumerate(chosen):
@@ -159,7 +160,7 @@ def make_scene(ctx, wl, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_si
     """Builds and finalizes a one-body scene of a workload."""
     sc = xf.Scene(ctx)
     sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"], velocity=wl["velocity"],
-                tet_mu=wl["tet_mu"])
+                tet_mu=wl["tet_mu"], tet_colour=wl.get("tet_colour"))
     sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size, launch_mode=launch_mode,
                 persistent_blocks=persistent_blocks)
     sc.set_settings(0, **wl["settings"])

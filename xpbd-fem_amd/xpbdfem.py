@@ -58,6 +58,7 @@ class BodyDesc(C.Structure):
         ("pinned", C.c_void_p),
         ("velocity", C.c_void_p),
         ("tet_mu", C.c_void_p),
+        ("tet_colour", C.c_void_p),
     ]
 
 
@@ -107,11 +108,14 @@ def lib():
         "xf_scene_profile": ([vp, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
                               C.POINTER(C.c_uint32)], C.c_int),
         "xf_scene_count_nonfinite": ([vp, C.POINTER(C.c_uint32)], C.c_int),
+        "xf_scene_adjoint": ([vp, C.c_uint32, C.c_float, C.c_uint32, _u32p, vp, C.c_uint32, C.POINTER(C.c_double),
+                              vp], C.c_int),
         "xf_scene_autotune": ([vp, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(SceneOpts), C.POINTER(C.c_float)],
                               C.c_int),
         "xf_mesh_tet_block": ([C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, _u32p],
                               C.c_int),
         "xf_minstd_order": ([C.c_uint32, _u32p], C.c_int),
+        "xf_mesh_tet_block_colours": ([C.c_uint32, C.c_uint32, C.c_uint32, _u32p], C.c_int),
         "xf_normal_fill": ([C.c_uint64, C.c_float, C.c_uint32, _f32p], C.c_int),
         "xf_ldlt_solve": ([C.c_uint32, _f32p, _f32p, _f32p], C.c_int),
         "xf_mesh_colour": ([C.c_uint32, _u32p, C.c_uint32, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_uint32)],
@@ -133,7 +137,7 @@ def exported_symbols():
         "xf_scene_set_tet_mu", "xf_scene_step", "xf_scene_sync", "xf_scene_get_state", "xf_scene_set_state",
         "xf_scene_get_time", "xf_scene_set_time", "xf_scene_get_order", "xf_scene_get_elements", "xf_scene_profile",
         "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
-        "xf_mesh_colour", "xf_scene_autotune",
+        "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
     ]
 
 
@@ -148,6 +152,14 @@ def tet_block(W, H, D, scale, origin, mirrored=False):
     if rc:
         raise XfError(f"xf_mesh_tet_block failed ({rc})")
     return xyz, tets
+
+
+def tet_block_colours(W, H, D):
+    """The 24-colouring of a uniform-pattern tet block."""
+    out = np.zeros(6 * W * H * D, np.uint32)
+    if lib().xf_mesh_tet_block_colours(W, H, D, out):
+        raise XfError("xf_mesh_tet_block_colours failed")
+    return out
 
 
 def minstd_order(n):
@@ -236,7 +248,7 @@ class Scene:
         self.bodies = []  # (n_nodes, n_tets)
         self._keep = []
 
-    def add_body(self, xyz, tets, density, pinned=None, velocity=None, tet_mu=None):
+    def add_body(self, xyz, tets, density, pinned=None, velocity=None, tet_mu=None, tet_colour=None):
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
         tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
         d = BodyDesc()
@@ -258,6 +270,10 @@ class Scene:
             m = np.ascontiguousarray(tet_mu, np.float32).reshape(-1)
             d.tet_mu = m.ctypes.data
             keep.append(m)
+        if tet_colour is not None:
+            c = np.ascontiguousarray(tet_colour, np.uint32).reshape(-1)
+            d.tet_colour = c.ctypes.data
+            keep.append(c)
         bid = C.c_uint32()
         self.ctx.check(lib().xf_scene_add_body(self.h, C.byref(d), C.byref(bid)), "xf_scene_add_body")
         self.bodies.append((d.n_nodes, d.n_tets))
@@ -345,6 +361,18 @@ class Scene:
                        "xf_scene_autotune")
         size = o.persistent_blocks if o.launch_mode == LAUNCH_PERSISTENT else o.block_size
         return o.launch_mode, size, ms.value
+
+    def adjoint(self, substeps, dt, loss_at, targets, window=20):
+        """Forward rollout + adjoint: returns (loss, dL/dlog mu per tet, all
+        bodies concatenated). targets: [n_loss, total nodes * 3] float64."""
+        loss_at = np.ascontiguousarray(loss_at, np.uint32)
+        targets = np.ascontiguousarray(targets, np.float64)
+        n_tets = sum(t for _, t in self.bodies)
+        grad = np.zeros(n_tets, np.float64)
+        L = C.c_double()
+        self.ctx.check(lib().xf_scene_adjoint(self.h, substeps, dt, loss_at.size, loss_at, targets.ctypes.data,
+                                              window, C.byref(L), grad.ctypes.data), "xf_scene_adjoint")
+        return L.value, grad
 
     def count_nonfinite(self):
         c = C.c_uint32()

@@ -1,0 +1,32 @@
+"""Thin torch.autograd.Function over xf_scene_adjoint, for gradient checks.
+
+RolloutLoss.apply(log_mu, scene, x0, v0, substeps, dt, loss_at, targets,
+window) sets every body's per-tet mu = exp(log_mu) (bodies concatenated),
+resets the bodies to (x0, v0), rolls the scene forward and returns the
+trajectory loss as a 0-d float64 tensor; backward returns
+grad_output * dL/dlog(mu) from the hand-written adjoint. All work happens in
+the HIP library; torch only carries the values.
+"""
+import numpy as np
+import torch
+
+import xpbdfem as xf  # noqa: F401  (the scene objects come from this binding)
+
+
+class RolloutLoss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, log_mu, scene, x0, v0, substeps, dt, loss_at, targets, window=20):
+        mu = torch.exp(log_mu.detach().to("cpu", torch.float64)).numpy().astype(np.float32)
+        off = 0
+        for b, (n, t) in enumerate(scene.bodies):
+            scene.set_tet_mu(b, mu[off:off + t])
+            scene.set_state(b, x0[b], v0[b])
+            off += t
+        L, g = scene.adjoint(substeps, dt, loss_at, targets, window)
+        ctx.save_for_backward(torch.from_numpy(g).to(log_mu.device))
+        return torch.tensor(L, dtype=torch.float64, device=log_mu.device)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        (g,) = ctx.saved_tensors
+        return (grad_output * g, None, None, None, None, None, None, None, None)
