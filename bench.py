@@ -20,6 +20,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "xpbd-fem_amd"), os.path.join(ROOT, "oracle")]
 
+import fitting  # noqa: E402
 import workloads  # noqa: E402
 import xpbdfem as xf  # noqa: E402
 
@@ -34,19 +35,57 @@ BYTES_PER_TET = 16 + 64 + 112 + 96
 BYTES_PER_NODE = 96
 
 
-def cpu_baseline(wl, budget_s=12.0):
+def rel_pos_err(x, x_ref, x0):
+    """max_i |x_i - x_ref,i|_2 / bounding-box diagonal of the rest positions."""
+    d = np.linalg.norm((x - x_ref).reshape(-1, 3), axis=1).max()
+    p0 = x0.reshape(-1, 3).astype(np.float64)
+    return float(d / np.linalg.norm(p0.max(axis=0) - p0.min(axis=0)))
+
+
+def cpu_baseline(ctx, wl, budget_s=12.0):
     """The reference's own Geo3d::Substep (oracle/_ref, compiled from the
     reference sources) on the same mesh, pins and velocities, 1 thread,
-    YeohSkinFast; falls back to the oracle's reference-precision port."""
+    YeohSkinFast, its own MINSTD sweep order; falls back to the oracle's
+    reference-precision restatement when the compiled reference is absent.
+
+    Also checks one frame of the GPU path: against the compiled reference
+    (the GPU's colour order as tOrder, the reference's fixed Yeoh constants,
+    no contacts: the reference has neither custom constants nor contacts)
+    and against the FP64 oracle with the workload's full settings."""
     import pyoracle as po
 
     xyz, tets = wl["xyz"], wl["tets"]
     t = tets.size // 4
     dt = np.float32(wl["dt"])
+    S = wl["substeps"]
     s = wl["settings"]
+    x0 = xyz.astype(np.float64)
+    check = {"frames": 1}
+    plain = dict(wl, settings=dict(s, collide=0, yeoh=xf.SKIN_YEOH))
+    g = workloads.make_scene(ctx, plain)
+    g.step(S, float(dt))
+    xg, _ = g.get_state(0)
+    order = g.order(0)[0]
+    g.close()
+    if po.have_ref():
+        rb = po.RefBody(xyz, tets, wl["density"], wl["pinned"], order=order)
+        rb.set_state(X=x0, V=wl["velocity"].astype(np.float64))
+        rb.substeps(rb.settings(s["compliance"], nu=s["poissons_ratio"], energy=7, simultaneous=True), dt, S)
+        check["rel_pos_err_vs_reference"] = rel_pos_err(xg, rb.state()[0], xyz)
+        del rb
+    g = workloads.make_scene(ctx, wl)
+    g.step(S, float(dt))
+    xg, _ = g.get_state(0)
+    order = g.order(0)[0]
+    g.close()
+    ob = po.body_from_workload(wl, variant="f64")
+    ob.substep(po.params_from_settings(dt, s), order, count=S)
+    check["rel_pos_err_vs_fp64_oracle"] = rel_pos_err(xg, ob.X, xyz)
+    del ob
+
     if po.have_ref():
         body = po.RefBody(xyz, tets, wl["density"], wl["pinned"])
-        body.set_state(X=xyz.astype(np.float64), V=wl["velocity"].astype(np.float64))
+        body.set_state(X=x0, V=wl["velocity"].astype(np.float64))
         st = body.settings(s["compliance"], nu=s["poissons_ratio"], energy=7, simultaneous=True)
         run = lambda k: body.substeps(st, dt, k)  # noqa: E731
         kind, what = "reference", "reference Geo3d::Substep (no collision pass: the reference has none)"
@@ -66,7 +105,7 @@ def cpu_baseline(wl, budget_s=12.0):
         if el >= budget_s:
             break
     return dict(value=t * n / el, unit="tet-substeps/s", cores=1, kind=kind,
-                sample=f"{what}: {n} substeps of the {t}-tet beam, single thread, {el:.1f} s")
+                sample=f"{what}: {n} substeps of the {t}-tet beam, single thread, {el:.1f} s", check=check)
 
 
 def batched_leg(ctx, frames=3):
@@ -150,8 +189,57 @@ def adjoint_leg(ctx, reps=3):
     adj.close()
     tf, ta = min(t_f), min(t_a)
     return {"workload": f"beam16x4x4_fit: {truth['tets'].size // 4} tets, {n_sub} substeps, loss at frames 10/20/30",
-            "forward_ms": 1e3 * tf, "adjoint_ms": 1e3 * ta, "adjoint_over_forward": ta / tf, "loss": L,
-            "grad_norm": float(np.linalg.norm(g))}
+            "forward_ms": 1e3 * tf, "adjoint_ms": 1e3 * ta, "adjoint_over_forward": ta / tf,
+            "reverse_over_forward": ta / tf - 1.0, "loss": L, "grad_norm": float(np.linalg.norm(g))}
+
+
+def fit_leg(ctx, iters=2):
+    """Batched fitting: 64 Yeoh beams (cells 4-16 per axis, hidden per-tet
+    log-mu fields), 30 frames of 40 substeps, loss at frames 10/20/30. Times
+    the forward rollout of all bodies and `iters` Adam iterations, each one
+    xf_scene_adjoint over the whole batch (one gradient per body)."""
+    fit = fitting.BatchFit(ctx, workloads.fit_batch())
+    fit.forward()
+    t0 = time.perf_counter()
+    fit.forward()
+    tf = time.perf_counter() - t0
+    fit.step()  # warm-up: sizes the checkpoint and tape buffers
+    losses = []
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        losses.append(fit.step())
+    ta = (time.perf_counter() - t0) / iters
+    nb = len(fit.tets_per_body)
+    out = {"workload": f"fit{nb}: {nb} beams, {sum(fit.tets_per_body)} tets, {fit.n_sub} substeps, loss at frames "
+                       f"10/20/30, window {fit.window}",
+           "forward_ms": 1e3 * tf, "adjoint_ms": 1e3 * ta, "adjoint_over_forward": ta / tf,
+           "reverse_over_forward": ta / tf - 1.0, "body_gradients_per_s": nb / ta, "adam_iterations": iters + 1,
+           "loss": losses[-1]}
+    fit.close()
+    return out
+
+
+def copy_probe(nbytes=1 << 28, reps=4):
+    """Plain device-to-device copies of a known size (the HIP runtime's copy
+    kernel): the reference reading for scaling the FETCH_SIZE / WRITE_SIZE
+    counters of a rocprofv3 --pmc pass to bytes."""
+    import ctypes as C
+
+    xf.lib()  # binds the HIP runtime the library uses
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": reps}))
 
 
 def main():
@@ -161,8 +249,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs (single beam and batched fit)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="per-colour launches of 64 threads instead of the autotuned shape (short profiler passes)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 4 device copies of 256 MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
               file=sys.stderr)
@@ -174,7 +269,7 @@ def main():
     info = sc.info()
     T, N, S = info["n_tets"], info["n_nodes"], wl["substeps"]
     dt = float(wl["dt"])
-    mode, size, _ = sc.autotune(S, dt, reps=2)
+    mode, size = (0, 64) if args.no_autotune else sc.autotune(S, dt, reps=2)[:2]
 
     for _ in range(args.warmup):
         sc.step(S, dt)
@@ -218,12 +313,20 @@ def main():
                      "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
                      "bytes_per_launch": bytes_proj / max(n_launch, 1)},
     }
+    legs = []
     if not args.no_batch:
-        out["batched"] = batched_leg(ctx)
+        legs.append(("batched", batched_leg))
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_leg(ctx)
+        legs += [("adjoint", adjoint_leg), ("fit", fit_leg)]
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl)
+        legs.append(("cpu_baseline", lambda c: cpu_baseline(c, wl)))
+    for name, leg in legs:
+        try:
+            out[name] = leg(ctx)
+        except (xf.XfError, FloatingPointError) as e:  # a side leg must not hide the main line
+            out[name] = {"error": str(e)}
+    sc.close()
+    ctx.close()
     print(json.dumps(out))
 
 

@@ -50,6 +50,8 @@ enum {
 /* Collision flags (no reference counterpart: the reference has no
  * collision code). */
 enum { XF_COLLIDE_GROUND = 1, XF_COLLIDE_SPHERE = 2 };
+/* Rayleigh damping types (Settings.h:69-72). */
+enum { XF_RAYLEIGH_PAPER = 0, XF_RAYLEIGH_LIMIT = 1, XF_RAYLEIGH_POST = 2, XF_RAYLEIGH_POST_AMORTIZED = 3 };
 
 /* Colouring visit orders for the greedy node-disjoint tet colouring. */
 enum {
@@ -72,9 +74,29 @@ typedef struct xf_scene xf_scene;
  *   yeoh[3]               Yeoh C used by energies 5/6/7; the reference
  *                         hard-codes {0.1095, 14.95, 4.595} (Fem.cpp:471,526).
  *                         A Yeoh material C10, C20, C30 maps to
- *                         compliance = 1/C10, yeoh = {1, C20/C10, C30/C10}.
- * damping and pbd_damping must be 0 (Rayleigh / PBD damping passes,
- * Geo.cpp:790-811, are not implemented). volume_passes must be 0. */
+ *                         compliance = k/C10, yeoh = k*{1, C20/C10, C30/C10}
+ *                         for any k > 0; k = 0.1095 (the reference's skin
+ *                         scale) keeps the simultaneous solve stable where
+ *                         k = 1 does not (same energy, different XPBD
+ *                         conditioning).
+ *   volume_passes         <- Settings::volumePasses (at most 64): extra
+ *                            SolveElementVolume sweeps per substep with
+ *                            compliance * tet volume (Fem.cpp:840-867),
+ *                            using this compliance even for per-tet mu
+ *   damping               <- Settings::damping (Rayleigh gamma)
+ *   rayleigh_type         <- flag bits 20-21: XF_RAYLEIGH_PAPER and _LIMIT
+ *                            damp inside the projection (Xpbd.h:87-214);
+ *                            _POST and _POST_AMORTIZED run RayleighDamp on the
+ *                            velocities after the velocity update
+ *                            (GeoLinear3d::Damp, Geo.cpp:790-811), the
+ *                            amortized form on 1/8 of the sweep order per
+ *                            substep (tick % 8) with damping x 8
+ *   pbd_damping           <- Settings::volumeAndTimeCorrectedPbdDamping, the
+ *                            per-volume factor of the PBD velocity damping
+ *                            (Xpbd.h:306-348, min(1, pbd_damping / area));
+ *                            the pass runs when it is > 0
+ *   pbd_damping_amortized <- amortizedVolumeAndTimeCorrectedPbdDamping, used
+ *                            instead under XF_RAYLEIGH_POST_AMORTIZED */
 typedef struct xf_settings {
     uint32_t energy;
     uint32_t simultaneous;
@@ -92,6 +114,8 @@ typedef struct xf_settings {
     float sphere_c0[3];    /* sphere centre at simulation time 0 */
     float sphere_v[3];     /* sphere velocity (kinematic) */
     float sphere_r;
+    uint32_t rayleigh_type;       /* XF_RAYLEIGH_* (Settings flag bits 20-21) */
+    float pbd_damping_amortized;  /* amortizedVolumeAndTimeCorrectedPbdDamping */
 } xf_settings;
 
 /* Fills defaults: MixedSel, simultaneous, compliance 0.01, nu 0.45,
@@ -145,6 +169,8 @@ typedef struct xf_scene_info {
 
 /* ------------------------------------------------------------ context */
 XF_API int xf_ctx_create(int device, xf_ctx** out);
+/* Destroys the context and every scene still open on it; handles of those
+ * scenes become invalid and must not be passed to xf_scene_destroy again. */
 XF_API void xf_ctx_destroy(xf_ctx* ctx);
 XF_API const char* xf_last_error(const xf_ctx* ctx);
 XF_API const char* xf_version(void);
@@ -176,6 +202,11 @@ XF_API int xf_scene_set_state(xf_scene* scene, uint32_t body, const double* x, c
 /* Simulation clock (seconds) that drives kinematic colliders. */
 XF_API int xf_scene_get_time(const xf_scene* scene, float* t);
 XF_API int xf_scene_set_time(xf_scene* scene, float t);
+/* Settings::tickId of the next substep (the amortized damping picks its
+ * 1/8 of the sweep order by tick % 8). Advances by one per substep; starts
+ * at 0. Sim::Update keeps one counter for all bodies (Demo.cpp:81-89). */
+XF_API int xf_scene_get_tick(const xf_scene* scene, uint32_t* tick);
+XF_API int xf_scene_set_tick(xf_scene* scene, uint32_t tick);
 
 /* The sweep order of one body: its tets (body-local ids) in colour order.
  * Sweeping this order serially reproduces the GPU result. */
@@ -212,9 +243,12 @@ XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint3
  * gradients are summed in FP32 within a window and in FP64 across windows.
  * Scope: simultaneous MixedSel / YeohSkinFast, pins, ground contact (sphere
  * contact and serial mode return XF_E_UNSUPPORTED). The scene is left at the
- * end of the forward rollout. */
+ * end of the forward rollout. body_loss (nullable) receives each body's
+ * term of L (one double per body), so a caller can tell which bodies of a
+ * batch went non-finite. */
 XF_API int xf_scene_adjoint(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
-                            const double* targets, uint32_t window, double* loss, double* grad_log_mu);
+                            const double* targets, uint32_t window, double* loss, double* grad_log_mu,
+                            double* body_loss);
 /* Counts non-finite node positions (0 = all finite). */
 XF_API int xf_scene_count_nonfinite(xf_scene* scene, uint32_t* count);
 

@@ -37,6 +37,13 @@ class XoParams(C.Structure):
         ("sphere_c0", C.c_float * 3),
         ("sphere_v", C.c_float * 3),
         ("sphere_r", C.c_float),
+        ("volume_passes", C.c_int),
+        ("compliance", C.c_float),
+        ("damping", C.c_float),
+        ("rayleigh_type", C.c_int),
+        ("pbd", C.c_float),
+        ("pbd_amortized", C.c_float),
+        ("tick", C.c_uint32),
     ]
 
 
@@ -48,6 +55,13 @@ class XrSettings(C.Structure):
         ("drag", C.c_float),
         ("energy", C.c_uint32),
         ("simultaneous", C.c_uint32),
+        ("volume_passes", C.c_uint32),
+        ("damping", C.c_float),
+        ("rayleigh_type", C.c_uint32),
+        ("pbd_damping", C.c_float),
+        ("volume_pbd", C.c_float),
+        ("amortized_volume_pbd", C.c_float),
+        ("tick_id", C.c_uint32),
     ]
 
 
@@ -55,7 +69,8 @@ SKIN_YEOH = (0.1095, 14.95, 4.595)  # Fem.cpp:526
 
 
 def make_params(dt, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True, yeoh=SKIN_YEOH, collide=0,
-                ground_y=0.0, friction=0.0, sphere_c0=(0, 0, 0), sphere_v=(0, 0, 0), sphere_r=0.0):
+                ground_y=0.0, friction=0.0, sphere_c0=(0, 0, 0), sphere_v=(0, 0, 0), sphere_r=0.0, volume_passes=0,
+                compliance=0.0, damping=0.0, rayleigh_type=0, pbd=0.0, pbd_amortized=0.0):
     p = XoParams()
     p.dt = dt
     p.gravity[:] = gravity
@@ -69,6 +84,12 @@ def make_params(dt, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True,
     p.sphere_c0[:] = sphere_c0
     p.sphere_v[:] = sphere_v
     p.sphere_r = sphere_r
+    p.volume_passes = volume_passes
+    p.compliance = compliance
+    p.damping = damping
+    p.rayleigh_type = rayleigh_type
+    p.pbd = pbd
+    p.pbd_amortized = pbd_amortized
     return p
 
 
@@ -91,7 +112,14 @@ def lib():
         for sfx, sp in (("_ref64", _f64p), ("_f32", _f32p), ("_f64", _f64p)):
             fn = getattr(L, "xo_substep" + sfx)
             fn.argtypes = [C.POINTER(XoParams), C.c_uint32, sp, sp, sp, _f32p, C.c_void_p, C.c_uint32, _u32p,
-                           _u32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+                           _u32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+            fn = getattr(L, "xo_solve_volume" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), sp, _f32p, _f32p, C.c_float]
+            fn = getattr(L, "xo_pbd_damp" + sfx)
+            fn.argtypes = [sp, sp, _f32p, C.c_float]
+            fn = getattr(L, "xo_rayleigh_tet" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), sp, sp, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float,
+                           C.c_float]
             fn = getattr(L, "xo_adjoint_tet" + sfx)
             fn.argtypes = [C.POINTER(XoParams), sp, sp, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
             fn.restype = C.c_double if sfx == "_f64" else C.c_float
@@ -105,6 +133,7 @@ def lib():
             fn = getattr(L, "xo_solve_tet" + sfx)
             fn.argtypes = [C.POINTER(XoParams), sp, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
         L.xo_ldlt_solve.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
+        L.xo_surface_area.argtypes = [_f32p, _u32p, C.c_uint32, _f32p]
         _lib = L
     return _lib
 
@@ -134,6 +163,7 @@ def ref():
         L.xr_set_pins.argtypes = [C.c_void_p, _u8p]
         L.xr_get_elements.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, _f32p, _f32p]
         L.xr_get_tets.argtypes = [C.c_void_p, _u32p]
+        L.xr_get_area.argtypes = [C.c_void_p, _f32p]
         L.xr_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.xr_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.xr_substeps.argtypes = [C.c_void_p, C.POINTER(XrSettings), C.c_float, C.c_uint32]
@@ -221,6 +251,9 @@ class OracleBody:
         if velocity is not None:
             self.V[:] = np.asarray(velocity, np.float32).reshape(-1)
         self.clock = np.float32(0.0)
+        self.tick = 0  # Settings::tickId of the next substep
+        self.area = np.zeros(self.t, np.float32)
+        lib().xo_surface_area(np.ascontiguousarray(xyz, np.float32), self.tets, self.t, self.area)
 
     def substep(self, params, order, count=1):
         """`count` substeps; the collider clock advances by dt in float32 per
@@ -231,8 +264,10 @@ class OracleBody:
         dt = np.float32(params.dt)
         for _ in range(count):
             self.clock = np.float32(self.clock + dt)
+            params.tick = self.tick
             fn(C.byref(params), self.n, self.X, self.O, self.V, self.w, pin, self.t, order, self.tets, self.qi,
-               self.qq, self.qr, self.c0, self.c1, self.a, float(self.clock))
+               self.qq, self.qr, self.c0, self.c1, self.a, self.vol, self.area, float(self.clock))
+            self.tick += 1
 
 
 def params_from_settings(dt, settings):
@@ -242,7 +277,10 @@ def params_from_settings(dt, settings):
                        energy=s.get("energy", 4), simultaneous=bool(s.get("simultaneous", 1)),
                        yeoh=s.get("yeoh", SKIN_YEOH), collide=s.get("collide", 0), ground_y=s.get("ground_y", 0.0),
                        friction=s.get("friction", 0.0), sphere_c0=s.get("sphere_c0", (0, 0, 0)),
-                       sphere_v=s.get("sphere_v", (0, 0, 0)), sphere_r=s.get("sphere_r", 0.0))
+                       sphere_v=s.get("sphere_v", (0, 0, 0)), sphere_r=s.get("sphere_r", 0.0),
+                       volume_passes=s.get("volume_passes", 0), compliance=s.get("compliance", 0.0) or 0.0,
+                       damping=s.get("damping", 0.0), rayleigh_type=s.get("rayleigh_type", 0),
+                       pbd=s.get("pbd_damping", 0.0), pbd_amortized=s.get("pbd_damping_amortized", 0.0))
 
 
 def body_from_workload(wl, variant="f32"):
@@ -271,8 +309,17 @@ class RefBody:
             ref().xr_destroy(self.h)
             self.h = None
 
-    def settings(self, compliance, nu=0.5, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True):
+    def settings(self, compliance, nu=0.5, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True,
+                 volume_passes=0, damping=0.0, rayleigh_type=0, pbd_damping=0.0, volume_pbd=0.0,
+                 amortized_volume_pbd=0.0, tick_id=0):
         s = XrSettings()
+        s.volume_passes = volume_passes
+        s.damping = damping
+        s.rayleigh_type = rayleigh_type
+        s.pbd_damping = pbd_damping
+        s.volume_pbd = volume_pbd
+        s.amortized_volume_pbd = amortized_volume_pbd
+        s.tick_id = tick_id
         s.compliance = compliance
         s.poissons_ratio = nu
         s.gravity[:] = gravity
@@ -304,6 +351,11 @@ class RefBody:
         w = np.zeros(self.n, np.float32)
         ref().xr_get_elements(self.h, qi, qq, qr, vol, w)
         return dict(qi=qi, qq=qq, qr=qr, vol=vol, w=w)
+
+    def area(self):
+        a = np.zeros(self.t, np.float32)
+        ref().xr_get_area(self.h, a)
+        return a
 
     def order(self):
         o = np.zeros(self.t, np.uint32)

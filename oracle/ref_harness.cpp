@@ -35,6 +35,13 @@ struct xr_settings {
     float drag;
     uint32_t energy;
     uint32_t simultaneous;
+    uint32_t volume_passes;       // Settings::volumePasses
+    float damping;                // Settings::damping (Rayleigh gamma)
+    uint32_t rayleigh_type;       // Settings flag bits 20-21 (Rayleigh_Paper .. Rayleigh_PostAmortized)
+    float pbd_damping;            // Settings::pbdDamping (only its sign gates the pass, Geo.cpp:807)
+    float volume_pbd;             // Settings::volumeAndTimeCorrectedPbdDamping
+    float amortized_volume_pbd;   // Settings::amortizedVolumeAndTimeCorrectedPbdDamping
+    uint32_t tick_id;             // Settings::tickId of the first substep; +1 per substep
 };
 
 Settings MakeSettings(const xr_settings* s) {
@@ -42,13 +49,16 @@ Settings MakeSettings(const xr_settings* s) {
     st.compliance = s->compliance;
     st.poissonsRatio = s->poissons_ratio;
     st.gravity = vec2(s->gravity[0], s->gravity[1]);
-    st.damping = 0.0f;
-    st.pbdDamping = 0.0f;
+    st.damping = s->damping;
+    st.pbdDamping = s->pbd_damping;
+    st.volumeAndTimeCorrectedPbdDamping = s->volume_pbd;
+    st.amortizedVolumeAndTimeCorrectedPbdDamping = s->amortized_volume_pbd;
     st.drag = 0.0f;
     st.timeCorrectedDrag = s->drag;
-    st.volumePasses = 0;
+    st.volumePasses = s->volume_passes;
+    st.tickId = s->tick_id;
     st.flags = (s->energy << Settings_EnergyBit) | (s->simultaneous ? (1u << Settings_XpbdSolveBit) : 0u) |
-               Settings_LockLeft;
+               ((s->rayleigh_type & Settings_RayleighTypeMask) << Settings_RayleighTypeBit) | Settings_LockLeft;
     return st;
 }
 }  // namespace
@@ -111,6 +121,12 @@ void xr_get_elements(void* h, float* qi9, float* qq3, float* qr3, float* vol, fl
     for (uint32_t i = 0; i < g->vertCount; i++) w[i] = g->w[i];
 }
 
+// ElementParamsLinearTet::surfaceArea per tet (used by the PBD damping pass).
+void xr_get_area(void* h, float* area) {
+    GeoLinear3d* g = ((RefBody*)h)->geo;
+    for (uint32_t t = 0; t < g->con.tetCount; t++) area[t] = g->t[t].ep.surfaceArea;
+}
+
 void xr_get_tets(void* h, uint32_t* tets) {
     GeoLinear3d* g = ((RefBody*)h)->geo;
     for (uint32_t t = 0; t < g->con.tetCount; t++)
@@ -141,7 +157,10 @@ void xr_substeps(void* h, const xr_settings* s, float dt, uint32_t count) {
     Settings st = MakeSettings(s);
     Manipulator manip;
     manip.pickedGeo = nullptr;
-    for (uint32_t k = 0; k < count; k++) g->Substep(st, manip, dt);
+    for (uint32_t k = 0; k < count; k++) {
+        g->Substep(st, manip, dt);
+        ++st.tickId;  // Sim::Update advances tickId once per substep (Demo.cpp:81-89)
+    }
 }
 
 // One SolveElement call (Fem.cpp:892) on a 4-node local state.

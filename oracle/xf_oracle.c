@@ -25,6 +25,21 @@
  *                        simultaneous (committed golden fixtures + live)
  *   xo_substep_f32    == GPU path bit-exact; ~= ref64 within tolerance
  *   xo_substep_f64    error-reporting oracle; ~= ref64 within tolerance
+ *   volume passes     == reference SolveElementVolume sweeps inside
+ *                        Geo3d::Substep, bit-exact (live, energies 3/4/7,
+ *                        both solve modes); xo_solve_volume: with zero
+ *                        compliance one projection is the PBD step of
+ *                        C = J - 1, so |J - 1| drops to second order
+ *   damping           in-projection Rayleigh (Paper, Limit), Rayleigh post
+ *                        (Post, PostAmortized) and PBD damping: == reference
+ *                        Geo3d::Substep bit-exact (live: energies 3/4/7,
+ *                        both solve modes, every type, amortized ticks);
+ *                        xo_pbd_damp conserves mass-weighted linear and
+ *                        angular momentum and leaves a rigid velocity field
+ *                        at factor 1; xo_rayleigh_tet conserves momentum
+ *                        and scales the constraint velocity g.V by
+ *                        2 U b / (2 U b + w g.g), b = compliance / (dt damp)
+ *   surface area      == reference ElementParamsLinearTet::surfaceArea
  *   collision pass    no reference counterpart: closed-form tests only
  *                        (ground/sphere non-penetration, static and kinetic
  *                        friction cases)
@@ -48,6 +63,13 @@ typedef struct xo_params {
     float sphere_c0[3];
     float sphere_v[3];
     float sphere_r;
+    int volume_passes;  /* Settings::volumePasses (Geo.cpp:779-782) */
+    float compliance;   /* Settings::compliance, used by the volume passes (Fem.cpp:845) */
+    float damping;      /* Settings::damping (Rayleigh gamma, Fem.cpp:450) */
+    int rayleigh_type;  /* Settings flag bits 20-21: Paper 0, Limit 1, Post 2, PostAmortized 3 */
+    float pbd;          /* Settings::volumeAndTimeCorrectedPbdDamping (pass on when > 0) */
+    float pbd_amortized;/* Settings::amortizedVolumeAndTimeCorrectedPbdDamping */
+    uint32_t tick;      /* Settings::tickId of the substep */
 } xo_params;
 
 /* ---------------------------------------------------------------- mesh */
@@ -185,6 +207,31 @@ void xo_init_elements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, 
         for (int n = 0; n < 4; n++) w[is[n]] += 0.25f * density * v;
     }
     for (uint32_t i = 0; i < n_nodes; i++) w[i] = 1.0f / w[i];
+}
+
+/* ElementParamsLinearTet::surfaceArea (Fem.cpp:205-215): P_n = X_n - X_3 in
+ * float, sum of the 4 face areas 0.5 |(P_i0 - P_i2) x (P_i1 - P_i2)| over the
+ * faces (0,1,2), (1,3,2), (0,2,3), (3,1,0). */
+void xo_surface_area(const float* xyz, const uint32_t* tets, uint32_t n_tets, float* area) {
+    static const int faces[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {3, 1, 0}};
+    for (uint32_t t = 0; t < n_tets; t++) {
+        const uint32_t* is = tets + 4 * t;
+        float P[4][3];
+        for (int n = 0; n < 4; n++)
+            for (int r = 0; r < 3; r++) P[n][r] = (float)((double)xyz[3 * is[n] + r] - (double)xyz[3 * is[3] + r]);
+        float sum = 0.0f;
+        for (int f = 0; f < 4; f++) {
+            const float* a = P[faces[f][0]];
+            const float* b = P[faces[f][1]];
+            const float* c = P[faces[f][2]];
+            float x[3] = {a[0] - c[0], a[1] - c[1], a[2] - c[2]};
+            float y[3] = {b[0] - c[0], b[1] - c[1], b[2] - c[2]};
+            float cr[3] = {x[1] * y[2] - y[1] * x[2], x[2] * y[0] - y[2] * x[0], x[0] * y[1] - y[0] * x[1]};
+            float fa = 0.5f * sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            sum = f == 0 ? fa : sum + fa;
+        }
+        area[t] = sum;
+    }
 }
 
 /* Material constants of SolveElementMixed (Fem.cpp:445-449):

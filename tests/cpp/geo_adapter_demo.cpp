@@ -23,7 +23,10 @@ struct DemoSettings {
     float pbdDamping = 0.0f;
     float poissonsRatio = 0.5f;
     uint32_t flags = 0;
+    float volumeAndTimeCorrectedPbdDamping = 0.0f;
+    float amortizedVolumeAndTimeCorrectedPbdDamping = 0.0f;
     float timeCorrectedDrag = 0.0f;
+    uint32_t tickId = 0;
 };
 struct DemoManipulator {
     void* pickedGeo = nullptr;
@@ -32,7 +35,7 @@ struct DemoManipulator {
 
 int main(int argc, char** argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s OUT FRAMES SUBSTEPS ENERGY\n", argv[0]);
+        fprintf(stderr, "usage: %s OUT FRAMES SUBSTEPS ENERGY [RAYLEIGH_TYPE DAMPING PBD PBD_AMORTIZED]\n", argv[0]);
         return 2;
     }
     const int frames = atoi(argv[2]), substeps = atoi(argv[3]);
@@ -56,6 +59,13 @@ int main(int argc, char** argv) {
     geo.SetGround(-0.5f, 0.5f);
     DemoSettings settings;
     settings.flags = (energy << xfgpu::kEnergyBit) | (1u << xfgpu::kXpbdSolveBit) | xfgpu::kLockLeft;
+    if (argc >= 9) {  // damping: Settings flag bits 20-21 and the derived PBD factors
+        settings.flags |= ((uint32_t)atoi(argv[5]) & xfgpu::kRayleighMask) << xfgpu::kRayleighBit;
+        settings.damping = (float)atof(argv[6]);
+        settings.volumeAndTimeCorrectedPbdDamping = (float)atof(argv[7]);
+        settings.amortizedVolumeAndTimeCorrectedPbdDamping = (float)atof(argv[8]);
+        settings.pbdDamping = settings.volumeAndTimeCorrectedPbdDamping > 0.0f ? 1.0f : 0.0f;
+    }
     DemoManipulator manip;
     const float dt = 1.0f / settings.substepsPerSecond;
     float v0 = geo.CalculateVolume();
@@ -64,6 +74,8 @@ int main(int argc, char** argv) {
             if ((rc = geo.Substep(settings, manip, dt))) {
                 fprintf(stderr, "Substep failed (%d): %s\n", rc, geo.LastError());
                 return 1;
+            } else {
+                ++settings.tickId;  // Sim::Update advances tickId once per substep (Demo.cpp:81-89)
             }
     std::vector<double> X(3 * geo.VertCount()), V(3 * geo.VertCount());
     if ((rc = geo.GetState(X.data(), V.data()))) return 1;

@@ -166,6 +166,19 @@ def test_persistent_big_beam_bit_exact(ctx):
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
 
 
+@pytest.mark.parametrize("energy,sim", [(4, True), (7, True), (4, False), (3, True), (7, False)])
+@pytest.mark.parametrize("passes", [1, 2])
+@pytest.mark.parametrize("mode", [xf.LAUNCH_PER_COLOUR, xf.LAUNCH_PERSISTENT])
+def test_volume_passes_bit_exact(ctx, energy, sim, passes, mode):
+    """Settings::volumePasses sweeps after the main colour sweep, with ground
+    contact on: GPU == oracle (itself == the compiled reference)."""
+    wl = workloads.beam_neo_12x4x4(energy=energy, simultaneous=sim)
+    wl["settings"] = dict(wl["settings"], volume_passes=passes)
+    x, v, order = run_gpu(ctx, wl, wl["substeps"], chunks=2, launch_mode=mode)
+    ob = run_oracle(wl, order, 2 * wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
 def test_user_colouring_validated_and_bit_exact(ctx):
     wl = workloads.beam_neo_12x4x4()
     bad = np.zeros(wl["tets"].size // 4, np.uint32)  # one colour for all: shares nodes

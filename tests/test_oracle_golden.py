@@ -121,6 +121,30 @@ def test_live_substeps_all_energies(energy, sim, nu):
 
 
 @needs_ref
+@pytest.mark.parametrize("energy", [3, 4, 7])
+@pytest.mark.parametrize("sim", [True, False])
+@pytest.mark.parametrize("passes", [1, 3])
+def test_live_volume_passes(energy, sim, passes):
+    """Settings::volumePasses: SolveElementVolume sweeps after the main sweep
+    (Geo.cpp:779-782), oracle == compiled reference."""
+    xyz, tets = po.tet_block(6, 3, 3, (0.1, 0.1, 0.1), (0.0, -0.15, -0.15))
+    pinned = (xyz[0::3] == 0.0).astype(np.uint8)
+    order = po.minstd_order(tets.size // 4)
+    rb = po.RefBody(xyz, tets, 1000.0, pinned, order)
+    st = rb.settings(2e-5, energy=energy, simultaneous=sim, volume_passes=passes)
+    ob = po.OracleBody(xyz, tets, 1000.0, pinned, compliance=2e-5, variant="ref64")
+    p = po.make_params(float(np.float32(1) / np.float32(1200)), energy=energy, simultaneous=sim,
+                       volume_passes=passes, compliance=2e-5)
+    rb.substeps(st, p.dt, 40)
+    ob.substep(p, order, count=40)
+    X, V = rb.state()
+    assert np.array_equal(X, ob.X) and np.array_equal(V, ob.V)
+    plain = po.OracleBody(xyz, tets, 1000.0, pinned, compliance=2e-5, variant="ref64")
+    plain.substep(po.make_params(p.dt, energy=energy, simultaneous=sim), order, count=40)
+    assert not np.array_equal(plain.X, ob.X)  # the passes do something
+
+
+@needs_ref
 def test_live_ldlt_usage_example():
     A = np.array([1, 2, 3, 4, 5, 6], np.float32)
     b = np.array([7, 8, 9], np.float32)

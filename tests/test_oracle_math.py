@@ -221,6 +221,61 @@ def test_block_24_colouring_valid(dims):
     assert cnt.max() - cnt.min() <= max(1, cnt.max() // 10)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_volume_pass_is_pbd_step_of_j_minus_1(seed):
+    """SolveVolumeOnly with zero compliance: one projection of U = (J-1)^2 is
+    the PBD step of C = J - 1 (lambda = -2U / sum w |dU|^2 gives
+    dx = -C w dC / sum w |dC|^2), so J returns to 1 up to second order; with
+    compliance the step shrinks by sum w|dU|^2 / (sum w|dU|^2 + 2 U alpha)."""
+    rng = np.random.default_rng(100 + seed)
+    rest = rest_tet(rng)
+    el = po.init_elements(rest.reshape(-1), np.arange(4, dtype=np.uint32), 1000.0)
+    w = rng.uniform(0.5, 2.0, 4).astype(np.float32)
+    X = rest.astype(np.float64) + rng.normal(0, 0.004, (4, 3))
+    Dmi = np.linalg.inv((rest[:3] - rest[3]).T.astype(np.float64))
+
+    def J(Y):
+        return np.linalg.det((Y[:3] - Y[3]).T @ Dmi)
+
+    p = po.make_params(DT)
+
+    def residual(scale):
+        Xs = rest.astype(np.float64) + scale * (X - rest)
+        Ys = Xs.reshape(-1).copy()
+        po.lib().xo_solve_volume_f64(p, Ys, w, el["qi"], 0.0)
+        return abs(J(Xs) - 1.0), abs(J(Ys.reshape(4, 3)) - 1.0)
+
+    # second order: halving the deformation quarters the remaining volume error
+    # (scales keep the error far above the ~1e-8 left by the float32 Qi)
+    e0, e1 = residual(0.4)
+    h0, h1 = residual(0.2)
+    assert e1 < 0.05 * e0
+    assert 0.15 < h1 / e1 < 0.35, (e1, h1)
+    Y = X.reshape(-1).copy()
+    po.lib().xo_solve_volume_f64(p, Y, w, el["qi"], 0.0)
+    e0 = abs(J(X) - 1.0)
+    # planted mistake: the PBD step with the wrong sign doubles the error
+    Z = X.reshape(-1).copy()
+    po.lib().xo_solve_volume_f64(p, Z, w, el["qi"], 0.0)
+    Zbad = X + (X - Z.reshape(4, 3))
+    assert abs(J(Zbad) - 1.0) > 1.5 * e0
+    # compliance: the move shrinks by wgg / (wgg + 2 U alpha), from the textbook gradient
+    comp = 3e-9
+    Yc = X.reshape(-1).copy()
+    po.lib().xo_solve_volume_f64(p, Yc, w, el["qi"], comp)
+    F = (X[:3] - X[3]).T @ Dmi
+    dJ = J(X) * np.linalg.inv(F).T @ Dmi.T
+    g = np.zeros((4, 3))
+    g[:3] = (2.0 * (J(X) - 1.0) * dJ).T
+    g[3] = -g[:3].sum(0)
+    wgg = np.sum(w[:, None] * g * g)
+    U = (J(X) - 1.0) ** 2
+    shrink = wgg / (wgg + 2.0 * U * comp / DT ** 2)
+    move0 = np.linalg.norm(Y - X.reshape(-1))
+    movec = np.linalg.norm(Yc - X.reshape(-1))
+    assert np.isclose(movec / move0, shrink, rtol=1e-9)
+
+
 def test_ldlt_usage_example():
     A = np.array([1, 2, 3, 4, 5, 6], np.float32)
     b = np.array([7, 8, 9], np.float32)

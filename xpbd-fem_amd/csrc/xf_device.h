@@ -1,16 +1,20 @@
 // xf_device.h -- layouts shared by the host code and the gfx950 kernels.
 //
 // Device memory of a scene (all bodies concatenated):
-//   nodes  X[N]  float4 (x, y, z, w)        w = inverse mass, 0 when pinned
-//          O[N]  float4 (x, y, z, flags)    flags bits: 0 pinned, 1.. body id
-//          V[N]  float4 (vx, vy, vz, 0)     written at the end of each step call
+//   nodes  X[N]  double4 (x, y, z, w)        w = inverse mass, 0 when pinned
+//          O[N]  double4 (x, y, z, flags)    flags bits: 0 pinned, 1.. body id
+//          V[N]  double4 (vx, vy, vz, 0)     written at the end of each step
+//                                            call (every substep with damping)
 //   tets in sweep order (colour-major, body-major inside a colour):
 //          tidx[T] uint4 (global node ids; node 3 is the root node)
 //          tg0..tg3[T] float4: Qi (column-major) | QQ | QR | comp0
 //            tg0 = Qi00 Qi01 Qi02 Qi10   tg1 = Qi11 Qi12 Qi20 Qi21
 //            tg2 = Qi22 QQ0  QQ1  QQ2    tg3 = QR0  QR1  QR2  comp0
-//          taux[T] float4 (comp1, a, body id bits, 0) -- read only when the
-//            scene is compressible or holds more than one body
+//          taux[T] float4 (comp1, a, body id bits, compliance * volume) --
+//            read by the projection only when the scene is compressible or
+//            holds more than one body; always by the volume and damping passes
+//          tdmp[T] float2 (surface area, position in the body's sweep order
+//            as uint bits) -- damping passes only
 //   body table  XfBodyParams[B]
 #pragma once
 #include <stdint.h>
@@ -28,8 +32,20 @@ struct XfBodyParams {
     float sr;
     uint32_t energy;     // 4 or 7 (per-body dispatch in multi-body scenes)
     float yeoh[3];
-    float pad;
+    uint32_t volume_passes;  // Settings::volumePasses (Geo.cpp:779-782)
+    // Damping (GeoLinear3d::Damp, Geo.cpp:790-811; Xpbd.h:87-348)
+    float damping;           // Settings::damping (Rayleigh gamma)
+    uint32_t rtype;          // Rayleigh_Paper 0, Limit 1, Post 2, PostAmortized 3
+    float pbd;               // volumeAndTimeCorrectedPbdDamping (pass on when > 0)
+    float pbd_amortized;     // amortizedVolumeAndTimeCorrectedPbdDamping
+    uint32_t n_tets;         // tets of the body (amortized range)
+    uint32_t simultaneous;   // Settings_XpbdSolveBit (RayleighDamp form)
 };
+
+// The scene clock buffer: [0] simulation time (float), [1] tick (uint32,
+// Settings::tickId of the next substep). The first colour launch of a
+// substep advances both; kernels later in the same substep use tick - 1.
+constexpr int kClockWords = 2;
 
 // Kernel selector for the projection launch.
 struct XfProjectArgs {

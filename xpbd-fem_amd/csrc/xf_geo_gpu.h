@@ -8,9 +8,11 @@
 //
 // The settings and manipulator types are template parameters: any struct
 // with the reference's Settings field names (flags, compliance,
-// poissonsRatio, gravity.x/.y, damping, pbdDamping, timeCorrectedDrag,
-// volumePasses; Settings.h:79-102) works, so the adapter needs none of the
-// reference's headers. The manipulator is accepted for signature
+// poissonsRatio, gravity.x/.y, damping, pbdDamping,
+// volumeAndTimeCorrectedPbdDamping,
+// amortizedVolumeAndTimeCorrectedPbdDamping, timeCorrectedDrag,
+// volumePasses, tickId; Settings.h:79-102) works, so the adapter needs none
+// of the reference's headers. The manipulator is accepted for signature
 // compatibility; interactive picking is not supported (pickedGeo must not
 // point at this body).
 #pragma once
@@ -27,7 +29,8 @@
 namespace xfgpu {
 
 // Flag layout of Settings::flags (Settings.h:9-29).
-constexpr uint32_t kEnergyBit = 6, kEnergyMask = 31, kXpbdSolveBit = 11, kLockLeft = 1u << 26;
+constexpr uint32_t kEnergyBit = 6, kEnergyMask = 31, kXpbdSolveBit = 11, kRayleighBit = 20, kRayleighMask = 3,
+                   kLockLeft = 1u << 26;
 
 class GeoGpuLinear3d {
    public:
@@ -87,12 +90,19 @@ class GeoGpuLinear3d {
         s.gravity[1] = settings.gravity.y;
         s.drag = settings.timeCorrectedDrag;
         s.damping = settings.damping;
-        s.pbd_damping = settings.pbdDamping;
+        s.rayleigh_type = (settings.flags >> kRayleighBit) & kRayleighMask;
+        // the PBD pass runs when pbdDamping > 0 (Geo.cpp:806) with the
+        // volume- and time-corrected factors derived from it (Demo.cpp:53-61)
+        s.pbd_damping = settings.pbdDamping > 0.0f ? settings.volumeAndTimeCorrectedPbdDamping : 0.0f;
+        s.pbd_damping_amortized = settings.amortizedVolumeAndTimeCorrectedPbdDamping;
         s.volume_passes = settings.volumePasses;
         s.collide = collide_;
         s.ground_y = ground_y_;
         s.friction = friction_;
         if ((rc = xf_scene_set_settings(scene_, 0, &s))) return rc;
+        uint32_t tick = 0;
+        if ((rc = xf_scene_get_tick(scene_, &tick))) return rc;
+        if (tick != settings.tickId && (rc = xf_scene_set_tick(scene_, settings.tickId))) return rc;
         return xf_scene_step(scene_, substeps, dt);
     }
 

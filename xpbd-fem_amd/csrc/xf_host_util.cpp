@@ -114,6 +114,7 @@ bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint
     out->qq.assign((size_t)3 * n_tets, 0.0f);
     out->qr.assign((size_t)3 * n_tets, 0.0f);
     out->vol.assign(n_tets, 0.0f);
+    out->area.assign(n_tets, 0.0f);
     out->w.assign(n_nodes, 0.0f);
     const float kVolMult = 1.0f / 6.0f;  // VolumeMult<Tet> (Fem.cpp:80)
     const float kQuadW = 0.25f;          // QuadratureWeights<Tet, 4> (Fem.cpp:11)
@@ -155,6 +156,28 @@ bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint
             out->qr[3 * (size_t)t + i] = QR[i];
         }
         out->vol[t] = v;
+        // surfaceArea (Fem.cpp:211-214): half cross-product norms of the faces
+        // (0,1,2), (1,3,2), (0,2,3), (3,1,0), with P3 = 0.
+        {
+            static const int kFaces[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {3, 1, 0}};
+            float Q[4][3];
+            for (int nd = 0; nd < 3; nd++)
+                for (int r = 0; r < 3; r++) Q[nd][r] = P[nd][r];
+            for (int r = 0; r < 3; r++) Q[3][r] = (float)((double)xyz[3 * (size_t)is[3] + r] - (double)xyz[3 * (size_t)is[3] + r]);
+            float sum = 0.0f;
+            for (int f = 0; f < 4; f++) {
+                const float* a = Q[kFaces[f][0]];
+                const float* b = Q[kFaces[f][1]];
+                const float* c = Q[kFaces[f][2]];
+                float e0[3] = {a[0] - c[0], a[1] - c[1], a[2] - c[2]};
+                float e1[3] = {b[0] - c[0], b[1] - c[1], b[2] - c[2]};
+                float n[3] = {e0[1] * e1[2] - e1[1] * e0[2], e0[2] * e1[0] - e1[2] * e0[0],
+                              e0[0] * e1[1] - e1[0] * e0[1]};
+                float fa = 0.5f * std::sqrt(Dot3(n, n));
+                sum = f == 0 ? fa : sum + fa;
+            }
+            out->area[t] = sum;
+        }
         // Lumped mass, LumpsTet4 = 1/4 each (Fem.cpp:220-223).
         for (int c = 0; c < 4; c++) out->w[is[c]] += 0.25f * density * v;
     }

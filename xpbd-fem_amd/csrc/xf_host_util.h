@@ -23,8 +23,9 @@ struct ElementConstants {
     std::vector<float> qi;   // 9 per tet, column-major inverse rest map
     std::vector<float> qq;   // 3 per tet
     std::vector<float> qr;   // 3 per tet
-    std::vector<float> vol;  // rest volume per tet
-    std::vector<float> w;    // inverse lumped mass per node
+    std::vector<float> vol;   // rest volume per tet
+    std::vector<float> area;  // rest surface area per tet (PBD damping)
+    std::vector<float> w;     // inverse lumped mass per node
 };
 
 // InitFiniteElement for T4 over a whole mesh (Fem.cpp:197-224, 269;

@@ -179,13 +179,124 @@ __device__ __forceinline__ void constrain2(double x[4][3], const float w[4], con
     }
 }
 
+// EnergyXpbdConstrain with in-projection Rayleigh damping (Xpbd.h:87-120):
+// rtype Rayleigh_Paper (0) or Rayleigh_Limit (1) damp along the substep
+// displacement x - o; without damping this is constrain1.
+__device__ __forceinline__ void constrain1_damped(double x[4][3], const double o[4][3], const float w[4], float U,
+                                                  const float g[4][3], float dt, float comp, float damp, uint32_t rtype) {
+    if (!(damp > 0.0f && rtype < 2u)) {
+        constrain1(x, w, U, g, dt, comp);
+        return;
+    }
+    float alpha = comp / (dt * dt);
+    float gamma = damp / dt;
+    float wgg = 1.0e-22f, gV = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float v[3] = {(float)(x[n][0] - o[n][0]), (float)(x[n][1] - o[n][1]), (float)(x[n][2] - o[n][2])};
+        wgg = wgg + w[n] * dot3(g[n], g[n]);
+        gV = gV + dot3(v, g[n]);
+    }
+    float lp;
+    if (rtype == 0u) {
+        float A = wgg * (1.0f + gamma) + (2.0f * U * alpha);
+        lp = (-2.0f * U - gamma * gV) / A;
+    } else {
+        lp = (-2.0f * U) / (wgg + (2.0f * U * alpha));
+        float invBeta = comp / (dt * damp);
+        float A = wgg + 2.0f * U * invBeta;
+        float b = -gV - wgg * lp;
+        b = b * (A / fmaxf(A, 4.0f * wgg));
+        lp = lp + b / A;
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float s = w[n] * lp;
+#pragma unroll
+        for (int r = 0; r < 3; r++) x[n][r] = x[n][r] + (double)(s * g[n][r]);
+    }
+}
+
+// EnergyXpbdConstrainSimultaneous<2> with in-projection Rayleigh damping,
+// gamma = damp / dt on both constraints (Xpbd.h:123-214).
+__device__ __forceinline__ void constrain2_damped(double x[4][3], const double o[4][3], const float w[4],
+                                                  const float U[2], const float g0[4][3], const float g1[4][3],
+                                                  float dt, float comp0, float comp1, float damp, uint32_t rtype) {
+    if (!(damp > 0.0f && rtype < 2u)) {
+        constrain2(x, w, U, g0, g1, dt, comp0, comp1);
+        return;
+    }
+    float comp[2] = {comp0, comp1};
+    float alpha[2] = {comp0 / (dt * dt), comp1 / (dt * dt)};
+    float gamma = damp / dt;
+    float gV[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float v[3] = {(float)(x[n][0] - o[n][0]), (float)(x[n][1] - o[n][1]), (float)(x[n][2] - o[n][2])};
+        gV[0] = gV[0] + dot3(g0[n], v);
+        gV[1] = gV[1] + dot3(g1[n], v);
+    }
+    float wgg[3] = {1.0e-22f, 1.0e-22f, 1.0e-22f};
+#pragma unroll
+    for (int n = 0; n < 4; n++) wgg[0] = wgg[0] + w[n] * dot3(g0[n], g0[n]);
+#pragma unroll
+    for (int n = 0; n < 4; n++) wgg[1] = wgg[1] + w[n] * dot3(g1[n], g0[n]);
+#pragma unroll
+    for (int n = 0; n < 4; n++) wgg[2] = wgg[2] + w[n] * dot3(g1[n], g1[n]);
+    float A[3] = {wgg[0], wgg[1], wgg[2]}, b[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        int d = 2 * i;
+        if (rtype == 0u && gamma > 0.0f) {
+            A[d] = A[d] + (2.0f * U[i] * alpha[i]) / (1.0f + gamma);
+            b[i] = (-2.0f * U[i] - gamma * gV[i]) / (1.0f + gamma);
+        } else {
+            A[d] = A[d] + 2.0f * U[i] * alpha[i];
+            b[i] = -2.0f * U[i];
+        }
+    }
+    float invA00 = 1.0f / A[0];
+    float invA11 = 1.0f / A[2];
+    float invDet = 1.0f / fmaxf(0.00000001f, 1.0f - (A[1] * invA00) * (A[1] * invA11));
+    float l0 = invDet * ((b[0] * invA00) - (A[1] * invA00) * (b[1] * invA11));
+    float l1 = invDet * ((b[1] * invA11) - (b[0] * invA00) * (A[1] * invA11));
+    if (rtype == 1u && gamma > 0.0f) {
+        float lam[2] = {l0, l1};
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            int d = 2 * i;
+            float invBeta = comp[i] / (dt * damp);
+            A[d] = wgg[d] + 2.0f * U[i] * invBeta;
+            b[i] = b[i] * (-1.0f - alpha[i] * lam[i]);
+            b[i] = b[i] + -gV[i];
+            b[i] = b[i] * (A[d] / fmaxf(A[d], 8.0f * wgg[d]));
+        }
+        invA00 = 1.0f / A[0];
+        invA11 = 1.0f / A[2];
+        invDet = 1.0f / fmaxf(0.00000001f, 1.0f - (A[1] * invA00) * (A[1] * invA11));
+        l0 = l0 + invDet * ((b[0] * invA00) - (A[1] * invA00) * (b[1] * invA11));
+        l1 = l1 + invDet * ((b[1] * invA11) - (b[0] * invA00) * (A[1] * invA11));
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            float acc = 0.0f + l0 * g0[n][r];
+            acc = acc + l1 * g1[n][r];
+            x[n][r] = x[n][r] + (double)(w[n] * acc);
+        }
+    }
+}
+
 // SolveElementMixed for a linear tet (Fem.cpp:437-564), damping off.
 // E: 3 Mixed, 4 MixedSel, 5 YeohSkin, 6 YeohSkinSel, 7 YeohSkinFast,
 //    0 = MixedSel or YeohSkinFast chosen per tet by `yeohFast`.
-template <int E, bool SIM>
+// DAMP: in-projection Rayleigh damping of the body (damp, rtype) along x - o.
+template <int E, bool SIM, bool DAMP = false>
 __device__ __forceinline__ void solve_tet(bool yeohFast, const float C[3], double x[4][3], const float w[4],
                                           const float Qi[3][3], const float QQ[3], const float QR[3], float comp0,
-                                          float comp1, float a, float dt) {
+                                          float comp1, float a, float dt, const double (*o)[3] = nullptr,
+                                          float damp = 0.0f, uint32_t rtype = 0u) {
     float P[3][3];
     gather_p(x, P);
     float U[2] = {0.0f, 0.0f};
@@ -203,14 +314,17 @@ __device__ __forceinline__ void solve_tet(bool yeohFast, const float C[3], doubl
             if constexpr (E == 6) volume_term(Qi, P, a, U[1], g1);
             root_grad(g0);
             root_grad(g1);
-            constrain2(x, w, U, g0, g1, dt, comp0, comp1);
+            if constexpr (DAMP) constrain2_damped(x, o, w, U, g0, g1, dt, comp0, comp1, damp, rtype);
+            else constrain2(x, w, U, g0, g1, dt, comp0, comp1);
         } else {
             root_grad(g0);
-            constrain1(x, w, U[0], g0, dt, comp0);
+            if constexpr (DAMP) constrain1_damped(x, o, w, U[0], g0, dt, comp0, damp, rtype);
+            else constrain1(x, w, U[0], g0, dt, comp0);
             gather_p(x, P);
             volume_term(Qi, P, a, U[1], g1);
             root_grad(g1);
-            constrain1(x, w, U[1], g1, dt, comp1);
+            if constexpr (DAMP) constrain1_damped(x, o, w, U[1], g1, dt, comp1, damp, rtype);
+            else constrain1(x, w, U[1], g1, dt, comp1);
         }
     } else {
         neo_prefactored(QQ, QR, P, U[0], g0);
@@ -225,32 +339,367 @@ __device__ __forceinline__ void solve_tet(bool yeohFast, const float C[3], doubl
                 for (int r = 0; r < 3; r++) g0[n][r] = g0[n][r] * gScale;
         }
         if constexpr (!SIM) {
-            constrain1(x, w, U[0], g0, dt, comp0);
+            if constexpr (DAMP) constrain1_damped(x, o, w, U[0], g0, dt, comp0, damp, rtype);
+            else constrain1(x, w, U[0], g0, dt, comp0);
             gather_p(x, P);
         }
         volume_term(Qi, P, a, U[1], g1);
         root_grad(g1);
-        if constexpr (!SIM) constrain1(x, w, U[1], g1, dt, comp1);
-        else constrain2(x, w, U, g0, g1, dt, comp0, comp1);
+        if constexpr (!SIM) {
+            if constexpr (DAMP) constrain1_damped(x, o, w, U[1], g1, dt, comp1, damp, rtype);
+            else constrain1(x, w, U[1], g1, dt, comp1);
+        } else {
+            if constexpr (DAMP) constrain2_damped(x, o, w, U, g0, g1, dt, comp0, comp1, damp, rtype);
+            else constrain2(x, w, U, g0, g1, dt, comp0, comp1);
+        }
     }
+}
+
+// SolveVolumeOnly for tet i of the sweep order (Fem.cpp:840-867): the
+// (J - 1)^2 term alone with compliance = settings.compliance * volume (taux.w)
+// and a scalar XPBD update. Bodies with fewer than pass + 1 volume passes
+// skip the tet.
+__device__ __forceinline__ void volume_one(uint32_t i, uint32_t pass, const uint4* __restrict__ tidx,
+                                           const float4* __restrict__ tg0, const float4* __restrict__ tg1,
+                                           const float4* __restrict__ tg2, const float4* __restrict__ taux,
+                                           double4* __restrict__ X, const XfBodyParams* __restrict__ bodies,
+                                           float dt) {
+    float4 ax = taux[i];
+    if (pass >= bodies[__float_as_uint(ax.z)].volume_passes) return;
+    uint4 id = tidx[i];
+    float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i];
+    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
+    float Qi[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    float P[3][3];
+    gather_p(x, P);
+    float U = 0.0f;
+    float g[4][3];
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) g[n][r] = 0.0f;
+    volume_term(Qi, P, 1.0f, U, g);
+    root_grad(g);
+    constrain1(x, w, U, g, dt, ax.w);
+    X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
+    X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
+    X[id.z] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
+    X[id.w] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+}
+
+// U and root-completed gradients of SolveElementMixed in DampingMode::On
+// (Fem.cpp:437-564 with the solve forced to Simultaneous), no projection.
+template <int E>
+__device__ __forceinline__ void element_ug(bool yeohFast, const float C[3], const double x[4][3], const float Qi[3][3],
+                                           const float QQ[3], const float QR[3], float a, float U[2], float g0[4][3],
+                                           float g1[4][3]) {
+    float P[3][3];
+    gather_p(x, P);
+    U[0] = 0.0f;
+    U[1] = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) { g0[n][r] = 0.0f; g1[n][r] = 0.0f; }
+    if constexpr (E == 3 || E == 5 || E == 6) {
+        i1_quad_term(E != 3, C, Qi, P, U[0], g0);
+        if constexpr (E != 6) volume_term(Qi, P, a, U[1], g1);
+        U[0] = fmaxf(0.0001f, U[0]);
+        if constexpr (E == 6) volume_term(Qi, P, a, U[1], g1);
+        root_grad(g0);
+        root_grad(g1);
+    } else {
+        neo_prefactored(QQ, QR, P, U[0], g0);
+        if ((E == 7) || (E == 0 && yeohFast)) {
+            float IM = U[0] - 3.0f;
+            U[0] = fmaxf(0.0001f, C[0] * IM + C[1] * IM * IM + C[2] * IM * IM * IM);
+            float gScale = C[0] + 2.0f * C[1] * IM + 3.0f * C[2] * IM * IM;
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) g0[n][r] = g0[n][r] * gScale;
+        }
+        volume_term(Qi, P, a, U[1], g1);
+        root_grad(g1);
+    }
+}
+
+// RayleighDamp, one constraint (Xpbd.h:216-230).
+__device__ __forceinline__ void rayleigh1(double v[4][3], const float w[4], float U, const float g[4][3], float dt,
+                                          float comp, float damp) {
+    float invBeta = comp / (dt * damp);
+    float wgg = 1.0e-22f, gV = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float vf[3] = {(float)v[n][0], (float)v[n][1], (float)v[n][2]};
+        wgg = wgg + w[n] * dot3(g[n], g[n]);
+        gV = gV + dot3(g[n], vf);
+    }
+    float lambda = -gV / (2.0f * U * invBeta + wgg);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float s = w[n] * lambda;
+#pragma unroll
+        for (int r = 0; r < 3; r++) v[n][r] = v[n][r] + (double)(s * g[n][r]);
+    }
+}
+
+// RayleighDamp, two coupled constraints (Xpbd.h:232-263).
+__device__ __forceinline__ void rayleigh2(double v[4][3], const float w[4], const float U[2], const float g0[4][3],
+                                          const float g1[4][3], float dt, float comp0, float comp1, float damp) {
+    float invBeta0 = comp0 / (dt * damp), invBeta1 = comp1 / (dt * damp);
+    float A0 = 1.0e-22f, A1 = 1.0e-22f, A2 = 1.0e-22f, b0 = 0.0f, b1 = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        A0 = A0 + w[n] * dot3(g0[n], g0[n]);
+        A1 = A1 + w[n] * dot3(g0[n], g1[n]);
+        A2 = A2 + w[n] * dot3(g1[n], g1[n]);
+        float vf[3] = {(float)v[n][0], (float)v[n][1], (float)v[n][2]};
+        b0 = b0 - dot3(g0[n], vf);
+        b1 = b1 - dot3(g1[n], vf);
+    }
+    A0 = A0 + 2.0f * U[0] * invBeta0;
+    A2 = A2 + 2.0f * U[1] * invBeta1;
+    float invA00 = 1.0f / A0;
+    float invA11 = 1.0f / A2;
+    float invDet = 1.0f / fmaxf(0.00000001f, 1.0f - (A1 * invA00) * (A1 * invA11));
+    float l0 = invDet * ((b0 * invA00) - (A1 * invA00) * (b1 * invA11));
+    float l1 = invDet * ((b1 * invA11) - (b0 * invA00) * (A1 * invA11));
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) v[n][r] = v[n][r] + (double)(w[n] * (l0 * g0[n][r] + l1 * g1[n][r]));
+}
+
+// cross(x, y) of vectormath.h:364.
+__device__ __forceinline__ void cross3(const float x[3], const float y[3], float o[3]) {
+    o[0] = x[1] * y[2] - y[1] * x[2];
+    o[1] = x[2] * y[0] - y[2] * x[0];
+    o[2] = x[0] * y[1] - y[0] * x[1];
+}
+
+// PbdDamp<4> for dvec3 (Xpbd.h:306-348): pull the node velocities toward the
+// tet's mass-weighted rigid motion by `damping`; inertia inverted as
+// inverse(mat3) does (vectormath.cpp:41-60: adjugate in FP64, det and the
+// 1/det scale in FP32).
+__device__ __forceinline__ void pbd_damp(const double x[4][3], double v[4][3], const float W[4], float damping) {
+    float X[4][3], V[4][3], M[4];
+    float Xcm[3] = {0.0f, 0.0f, 0.0f}, Vcm[3] = {0.0f, 0.0f, 0.0f};
+    float Msum = 0.0f;
+    float Wsum = 1.0e-8f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) Wsum = Wsum + W[i];
+    float Waverage = Wsum * (1.0f / 4.0f);
+    float Wmin = 0.0001f * Waverage;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            X[i][r] = (float)(x[i][r] - x[3][r]);
+            V[i][r] = (float)v[i][r];
+        }
+        M[i] = Waverage / fmaxf(Wmin, W[i]);
+#pragma unroll
+        for (int r = 0; r < 3; r++) Xcm[r] = Xcm[r] + X[i][r] * M[i];
+#pragma unroll
+        for (int r = 0; r < 3; r++) Vcm[r] = Vcm[r] + V[i][r] * M[i];
+        Msum = Msum + M[i];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        Xcm[r] = Xcm[r] / Msum;
+        Vcm[r] = Vcm[r] / Msum;
+    }
+    float rv[4][3], L[3] = {0.0f, 0.0f, 0.0f};
+    float I[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) rv[i][r] = X[i][r] - Xcm[r];
+        float c[3];
+        cross3(rv[i], V[i], c);
+#pragma unroll
+        for (int r = 0; r < 3; r++) L[r] = L[r] + M[i] * c[r];
+        const float* q = rv[i];
+        float rr[3] = {q[0] * q[0], q[1] * q[1], q[2] * q[2]};
+        float rp[3] = {q[0] * q[1], q[1] * q[2], q[2] * q[0]};
+        float K[3][3] = {{rr[2] + rr[1], -rp[0], -rp[2]}, {-rp[0], rr[2] + rr[0], -rp[1]}, {-rp[2], -rp[1], rr[1] + rr[0]}};
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) I[cc][r] = I[cc][r] + K[cc][r] * M[i];
+    }
+    double m00 = I[0][0], m01 = I[0][1], m02 = I[0][2];
+    double m10 = I[1][0], m11 = I[1][1], m12 = I[1][2];
+    double m20 = I[2][0], m21 = I[2][1], m22 = I[2][2];
+    float adj[3][3];
+    adj[0][0] = (float)(+(m11 * m22 - m12 * m21));
+    adj[0][1] = (float)(-(m01 * m22 - m02 * m21));
+    adj[0][2] = (float)(+(m01 * m12 - m02 * m11));
+    adj[1][0] = (float)(-(m10 * m22 - m12 * m20));
+    adj[1][1] = (float)(+(m00 * m22 - m02 * m20));
+    adj[1][2] = (float)(-(m00 * m12 - m02 * m10));
+    adj[2][0] = (float)(+(m10 * m21 - m11 * m20));
+    adj[2][1] = (float)(-(m00 * m21 - m01 * m20));
+    adj[2][2] = (float)(+(m00 * m11 - m01 * m10));
+    float det = I[0][0] * adj[0][0] + I[0][1] * adj[1][0] + I[0][2] * adj[2][0];
+    float inv = 1.0f / det;
+    float om[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) om[r] = (adj[0][r] * inv) * L[0] + (adj[1][r] * inv) * L[1] + (adj[2][r] * inv) * L[2];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float c[3];
+        cross3(om, rv[i], c);
+#pragma unroll
+        for (int r = 0; r < 3; r++) v[i][r] = v[i][r] + (double)(damping * ((Vcm[r] + c[r]) - V[i][r]));
+    }
+}
+
+// Amortized damping (Rayleigh_PostAmortized) covers sweep positions
+// [T k / 8, T (k + 1) / 8) of the body, k = tickId % 8 (Geo.cpp:794-798).
+__device__ __forceinline__ bool in_damp_range(const XfBodyParams& bp, float2 dm, uint32_t tick) {
+    if (bp.rtype != 3u) return true;
+    uint32_t k = tick % 8u;
+    uint32_t pos = __float_as_uint(dm.y);
+    return pos >= bp.n_tets * k / 8u && pos < bp.n_tets * (k + 1u) / 8u;
+}
+
+// Rayleigh post damping of tet i (DampElement, Fem.cpp:910-929).
+template <int E, bool AUX>
+__device__ __forceinline__ void damp_one(uint32_t i, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+                                         const float4* __restrict__ tg1, const float4* __restrict__ tg2,
+                                         const float4* __restrict__ tg3, const float4* __restrict__ taux,
+                                         const float2* __restrict__ tdmp, const double4* __restrict__ X,
+                                         double4* __restrict__ V, const XfBodyParams* __restrict__ bodies,
+                                         float comp1_u, float a_u, float dt, uint32_t tick) {
+    float comp1 = comp1_u, a = a_u;
+    uint32_t body = 0;
+    if constexpr (AUX) {
+        float4 ax = taux[i];
+        comp1 = ax.x;
+        a = ax.y;
+        body = __float_as_uint(ax.z);
+    }
+    const XfBodyParams& bp = bodies[body];
+    if (!(bp.rtype >= 2u && bp.damping > 0.0f) || !in_damp_range(bp, tdmp[i], tick)) return;
+    float damp = bp.rtype == 3u ? bp.damping * 8.0f : bp.damping;
+    uint4 id = tidx[i];
+    float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i], q3 = tg3[i];
+    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
+    double4 va = V[id.x], vb = V[id.y], vc = V[id.z], vd = V[id.w];
+    float C[3] = {bp.yeoh[0], bp.yeoh[1], bp.yeoh[2]};
+    float Qi[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+    float QQ[3] = {q2.y, q2.z, q2.w};
+    float QR[3] = {q3.x, q3.y, q3.z};
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    double v[4][3] = {{va.x, va.y, va.z}, {vb.x, vb.y, vb.z}, {vc.x, vc.y, vc.z}, {vd.x, vd.y, vd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    float U[2], g0[4][3], g1[4][3];
+    element_ug<E>(bp.energy == 7u, C, x, Qi, QQ, QR, a, U, g0, g1);
+    if (bp.simultaneous) {
+        rayleigh2(v, w, U, g0, g1, dt, q3.w, comp1, damp);
+    } else {
+        rayleigh1(v, w, U[0], g0, dt, q3.w, damp);
+        rayleigh1(v, w, U[1], g1, dt, comp1, damp);
+    }
+    V[id.x] = make_double4(v[0][0], v[0][1], v[0][2], 0.0);
+    V[id.y] = make_double4(v[1][0], v[1][1], v[1][2], 0.0);
+    V[id.z] = make_double4(v[2][0], v[2][1], v[2][2], 0.0);
+    V[id.w] = make_double4(v[3][0], v[3][1], v[3][2], 0.0);
+}
+
+// PBD damping of tet i (Geo.cpp:806-809), factor min(1, pbd / surfaceArea).
+template <bool AUX>
+__device__ __forceinline__ void pbd_one(uint32_t i, const uint4* __restrict__ tidx, const float4* __restrict__ taux,
+                                        const float2* __restrict__ tdmp, const double4* __restrict__ X,
+                                        double4* __restrict__ V, const XfBodyParams* __restrict__ bodies,
+                                        uint32_t tick) {
+    uint32_t body = 0;
+    if constexpr (AUX) body = __float_as_uint(taux[i].z);
+    const XfBodyParams& bp = bodies[body];
+    float2 dm = tdmp[i];
+    if (!(bp.pbd > 0.0f) || !in_damp_range(bp, dm, tick)) return;
+    float f = (bp.rtype == 3u ? bp.pbd_amortized : bp.pbd) / dm.x;
+    f = f < 1.0f ? f : 1.0f;
+    uint4 id = tidx[i];
+    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
+    double4 va = V[id.x], vb = V[id.y], vc = V[id.z], vd = V[id.w];
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    double v[4][3] = {{va.x, va.y, va.z}, {vb.x, vb.y, vb.z}, {vc.x, vc.y, vc.z}, {vd.x, vd.y, vd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    pbd_damp(x, v, w, f);
+    V[id.x] = make_double4(v[0][0], v[0][1], v[0][2], 0.0);
+    V[id.y] = make_double4(v[1][0], v[1][1], v[1][2], 0.0);
+    V[id.z] = make_double4(v[2][0], v[2][1], v[2][2], 0.0);
+    V[id.w] = make_double4(v[3][0], v[3][1], v[3][2], 0.0);
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- kernels
 
+// Volume pass `pass` over one colour: tets [begin, begin+count).
+__global__ __launch_bounds__(256) void xf_volume_colour(uint32_t begin, uint32_t count, uint32_t pass,
+                                                        const uint4* __restrict__ tidx,
+                                                        const float4* __restrict__ tg0,
+                                                        const float4* __restrict__ tg1,
+                                                        const float4* __restrict__ tg2,
+                                                        const float4* __restrict__ taux, double4* __restrict__ X,
+                                                        const XfBodyParams* __restrict__ bodies, float dt) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    volume_one(begin + k, pass, tidx, tg0, tg1, tg2, taux, X, bodies, dt);
+}
+
 // Projects one colour: tets [begin, begin+count) of the sweep order.
 // AUX: per-tet (comp1, a, body) from taux; otherwise comp1/a are uniform and
 // the body is 0. The first colour launch of a substep also advances the
 // simulation clock (clock != nullptr); no other kernel of the colour sweep
 // reads it.
-template <int E, bool SIM, bool AUX>
+// Rayleigh post damping (Geo.cpp:797-802) of one colour.
+template <int E, bool AUX>
+__global__ __launch_bounds__(256) void xf_damp_colour(uint32_t begin, uint32_t count, const uint4* __restrict__ tidx,
+                                                      const float4* __restrict__ tg0, const float4* __restrict__ tg1,
+                                                      const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+                                                      const float4* __restrict__ taux, const float2* __restrict__ tdmp,
+                                                      const double4* __restrict__ X, double4* __restrict__ V,
+                                                      const XfBodyParams* __restrict__ bodies, float comp1_u,
+                                                      float a_u, float dt, const float* __restrict__ clock) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    uint32_t tick = reinterpret_cast<const uint32_t*>(clock)[1] - 1u;
+    damp_one<E, AUX>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, tdmp, X, V, bodies, comp1_u, a_u, dt, tick);
+}
+
+// PBD damping (Geo.cpp:806-809) of one colour.
+template <bool AUX>
+__global__ __launch_bounds__(256) void xf_pbd_colour(uint32_t begin, uint32_t count, const uint4* __restrict__ tidx,
+                                                     const float4* __restrict__ taux, const float2* __restrict__ tdmp,
+                                                     const double4* __restrict__ X, double4* __restrict__ V,
+                                                     const XfBodyParams* __restrict__ bodies,
+                                                     const float* __restrict__ clock) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    uint32_t tick = reinterpret_cast<const uint32_t*>(clock)[1] - 1u;
+    pbd_one<AUX>(begin + k, tidx, taux, tdmp, X, V, bodies, tick);
+}
+
+// DAMP: bodies may damp inside the projection (Rayleigh_Paper / _Limit);
+// reads O of the 4 nodes.
+template <int E, bool SIM, bool AUX, bool DAMP>
 __global__ __launch_bounds__(256) void xf_project_colour(
     uint32_t begin, uint32_t count, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
     const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
-    const float4* __restrict__ taux, double4* __restrict__ X, const XfBodyParams* __restrict__ bodies, float comp1_u,
-    float a_u, float dt, float* __restrict__ clock) {
-    if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) clock[0] = clock[0] + dt;
+    const float4* __restrict__ taux, double4* __restrict__ X, const double4* __restrict__ O,
+    const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock) {
+    if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+        clock[0] = clock[0] + dt;
+        uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
+        tick[0] = tick[0] + 1u;
+    }
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     uint32_t i = begin + k;
@@ -277,7 +726,13 @@ __global__ __launch_bounds__(256) void xf_project_colour(
     double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
     float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
 
-    solve_tet<E, SIM>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt);
+    if constexpr (DAMP) {
+        double4 oa = O[id.x], ob = O[id.y], oc = O[id.z], od = O[id.w];
+        double o[4][3] = {{oa.x, oa.y, oa.z}, {ob.x, ob.y, ob.z}, {oc.x, oc.y, oc.z}, {od.x, od.y, od.z}};
+        solve_tet<E, SIM, true>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt, o, bp.damping, bp.rtype);
+    } else {
+        solve_tet<E, SIM>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt);
+    }
 
     X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
     X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
@@ -418,7 +873,11 @@ __global__ __launch_bounds__(256) void xf_project_record(
     const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
     const float4* __restrict__ taux, double4* __restrict__ X, double* __restrict__ tape,
     const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock) {
-    if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) clock[0] = clock[0] + dt;
+    if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+        clock[0] = clock[0] + dt;
+        uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
+        tick[0] = tick[0] + 1u;
+    }
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     uint32_t i = begin + k;
@@ -489,12 +948,13 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* arrive, uint32_t target, 
     return ok != 0u;
 }
 
-template <int E, bool SIM, bool AUX>
+template <int E, bool SIM, bool AUX, bool DAMP>
 __device__ __forceinline__ void project_one(uint32_t i, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
                                             const float4* __restrict__ tg1, const float4* __restrict__ tg2,
                                             const float4* __restrict__ tg3, const float4* __restrict__ taux,
-                                            double4* __restrict__ X, const XfBodyParams* __restrict__ bodies,
-                                            float comp1_u, float a_u, float dt) {
+                                            double4* __restrict__ X, const double4* __restrict__ O,
+                                            const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u,
+                                            float dt) {
     uint4 id = tidx[i];
     float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i], q3 = tg3[i];
     double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
@@ -514,7 +974,13 @@ __device__ __forceinline__ void project_one(uint32_t i, const uint4* __restrict_
     float QR[3] = {q3.x, q3.y, q3.z};
     double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
     float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
-    solve_tet<E, SIM>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt);
+    if constexpr (DAMP) {
+        double4 oa = O[id.x], ob = O[id.y], oc = O[id.z], od = O[id.w];
+        double o[4][3] = {{oa.x, oa.y, oa.z}, {ob.x, ob.y, ob.z}, {oc.x, oc.y, oc.z}, {od.x, od.y, od.z}};
+        solve_tet<E, SIM, true>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt, o, bp.damping, bp.rtype);
+    } else {
+        solve_tet<E, SIM>(yeohFast, C, x, w, Qi, QQ, QR, q3.w, comp1, a, dt);
+    }
     X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
     X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
     X[id.z] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
@@ -550,49 +1016,96 @@ __device__ __forceinline__ void finish_one(uint32_t i, bool predict_next, double
     }
 }
 
+// Prediction of one node from its stored velocity (the xf_predict body).
+__device__ __forceinline__ void predict_one(uint32_t i, double4* __restrict__ X, double4* __restrict__ O,
+                                            const double4* __restrict__ V, const XfBodyParams* __restrict__ bodies) {
+    double4 xi = X[i];
+    double4 oi = O[i];
+    double4 vi = V[i];
+    const XfBodyParams& p = bodies[(uint32_t)oi.w >> 1];
+    double v[3] = {vi.x, vi.y, vi.z};
+    double x[3] = {xi.x, xi.y, xi.z};
+    predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
+}
+
 }  // namespace
 
 // A whole step call (`substeps` substeps) in one cooperative launch: the
-// first prediction, then per substep every colour phase and the fused
-// finish/predict node phase, with a grid barrier after each phase. Work is
+// first prediction, then per substep every colour phase, `vpasses` rounds of
+// volume colour phases and the fused finish/predict node phase, with a grid
+// barrier after each phase. With post damping (post bit 0: Rayleigh post
+// pass, bit 1: PBD pass) the node phase stores the velocity instead, the
+// damping colour phases follow, and then a separate prediction phase. Work is
 // spread over the grid by global thread index (grid-stride loops). The
 // results equal the per-colour launches bit-for-bit: each tet and node runs
 // the same code on the same inputs in the same phase order.
-template <int E, bool SIM, bool AUX>
+template <int E, bool SIM, bool AUX, bool DAMP>
 __global__ __launch_bounds__(256) void xf_frame_persistent(
-    uint32_t substeps, uint32_t ncol, const uint32_t* __restrict__ colour_begin, uint32_t n_nodes,
-    const uint4* __restrict__ tidx, const float4* __restrict__ tg0, const float4* __restrict__ tg1,
+    uint32_t substeps, uint32_t ncol, uint32_t vpasses, uint32_t post, const uint32_t* __restrict__ colour_begin,
+    uint32_t n_nodes, const uint4* __restrict__ tidx, const float4* __restrict__ tg0, const float4* __restrict__ tg1,
     const float4* __restrict__ tg2, const float4* __restrict__ tg3, const float4* __restrict__ taux,
-    double4* __restrict__ X, double4* __restrict__ O, double4* __restrict__ V, const XfBodyParams* __restrict__ bodies,
-    float comp1_u, float a_u, float dt, float* __restrict__ clock, uint32_t* __restrict__ arrive,
-    uint32_t* __restrict__ err) {
+    const float2* __restrict__ tdmp, double4* __restrict__ X, double4* __restrict__ O, double4* __restrict__ V,
+    const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock,
+    uint32_t* __restrict__ arrive, uint32_t* __restrict__ err) {
     const uint32_t nthreads = gridDim.x * blockDim.x;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t* tickw = reinterpret_cast<uint32_t*>(clock) + 1;
     uint32_t target = 0;
-    for (uint32_t i = gid; i < n_nodes; i += nthreads) {
-        double4 xi = X[i];
-        double4 oi = O[i];
-        double4 vi = V[i];
-        const XfBodyParams& p = bodies[(uint32_t)oi.w >> 1];
-        double v[3] = {vi.x, vi.y, vi.z};
-        double x[3] = {xi.x, xi.y, xi.z};
-        predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
-    }
+    for (uint32_t i = gid; i < n_nodes; i += nthreads) predict_one(i, X, O, V, bodies);
     target += gridDim.x;
     if (!grid_barrier(arrive, target, err)) return;
     for (uint32_t s = 0; s < substeps; s++) {
         for (uint32_t c = 0; c < ncol; c++) {
-            if (c == 0 && gid == 0) clock[0] = clock[0] + dt;
+            if (c == 0 && gid == 0) {
+                clock[0] = clock[0] + dt;
+                tickw[0] = tickw[0] + 1u;
+            }
             uint32_t begin = colour_begin[c], count = colour_begin[c + 1] - begin;
             for (uint32_t k = gid; k < count; k += nthreads)
-                project_one<E, SIM, AUX>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, X, bodies, comp1_u, a_u, dt);
+                project_one<E, SIM, AUX, DAMP>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, X, O, bodies, comp1_u, a_u,
+                                               dt);
             target += gridDim.x;
             if (!grid_barrier(arrive, target, err)) return;
         }
+        for (uint32_t pass = 0; pass < vpasses; pass++)
+            for (uint32_t c = 0; c < ncol; c++) {
+                uint32_t begin = colour_begin[c], count = colour_begin[c + 1] - begin;
+                for (uint32_t k = gid; k < count; k += nthreads)
+                    volume_one(begin + k, pass, tidx, tg0, tg1, tg2, taux, X, bodies, dt);
+                target += gridDim.x;
+                if (!grid_barrier(arrive, target, err)) return;
+            }
         float t_end = clock[0];
         bool next = s + 1 < substeps;
-        for (uint32_t i = gid; i < n_nodes; i += nthreads) finish_one(i, next, X, O, V, bodies, t_end);
+        if (post == 0u) {
+            for (uint32_t i = gid; i < n_nodes; i += nthreads) finish_one(i, next, X, O, V, bodies, t_end);
+            if (next) {
+                target += gridDim.x;
+                if (!grid_barrier(arrive, target, err)) return;
+            }
+            continue;
+        }
+        for (uint32_t i = gid; i < n_nodes; i += nthreads) finish_one(i, false, X, O, V, bodies, t_end);
+        target += gridDim.x;
+        if (!grid_barrier(arrive, target, err)) return;
+        uint32_t tick = tickw[0] - 1u;
+        for (uint32_t kind = 0; kind < 2; kind++) {
+            if (!(post & (1u << kind))) continue;
+            for (uint32_t c = 0; c < ncol; c++) {
+                uint32_t begin = colour_begin[c], count = colour_begin[c + 1] - begin;
+                for (uint32_t k = gid; k < count; k += nthreads) {
+                    if (kind == 0)
+                        damp_one<E, AUX>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, tdmp, X, V, bodies, comp1_u, a_u,
+                                         dt, tick);
+                    else
+                        pbd_one<AUX>(begin + k, tidx, taux, tdmp, X, V, bodies, tick);
+                }
+                target += gridDim.x;
+                if (!grid_barrier(arrive, target, err)) return;
+            }
+        }
         if (next) {
+            for (uint32_t i = gid; i < n_nodes; i += nthreads) predict_one(i, X, O, V, bodies);
             target += gridDim.x;
             if (!grid_barrier(arrive, target, err)) return;
         }
@@ -615,37 +1128,99 @@ __global__ __launch_bounds__(256) void xf_count_nonfinite(uint32_t n, const doub
 // ------------------------------------------------------ launch wrappers
 
 typedef void (*XfProjectFn)(uint32_t, uint32_t, const uint4*, const float4*, const float4*, const float4*,
-                            const float4*, const float4*, double4*, const XfBodyParams*, float, float, float, float*);
+                            const float4*, const float4*, double4*, const double4*, const XfBodyParams*, float, float,
+                            float, float*);
 
 namespace {
+template <int E, bool SIM>
+XfProjectFn pick_aux_damp(bool aux, bool damp) {
+    if (aux) return damp ? xf_project_colour<E, SIM, true, true> : xf_project_colour<E, SIM, true, false>;
+    return damp ? xf_project_colour<E, SIM, false, true> : xf_project_colour<E, SIM, false, false>;
+}
 template <int E>
-XfProjectFn pick_aux(bool sim, bool aux) {
-    if (sim) return aux ? xf_project_colour<E, true, true> : xf_project_colour<E, true, false>;
-    return aux ? xf_project_colour<E, false, true> : xf_project_colour<E, false, false>;
+XfProjectFn pick_aux(bool sim, bool aux, bool damp) {
+    return sim ? pick_aux_damp<E, true>(aux, damp) : pick_aux_damp<E, false>(aux, damp);
 }
 }  // namespace
 
 // Host-callable: launch one projection kernel. energy 3..7, or 0 for
-// per-body MixedSel/YeohSkinFast dispatch.
-extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, uint32_t block,
+// per-body MixedSel/YeohSkinFast dispatch. damp: some body damps inside
+// the projection (O is then read).
+extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t block,
                                         uint32_t begin, uint32_t count, const uint4* tidx, const float4* tg0,
                                         const float4* tg1, const float4* tg2, const float4* tg3, const float4* taux,
-                                        double4* X, const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
-                                        float* clock) {
+                                        double4* X, const double4* O, const XfBodyParams* bodies, float comp1_u,
+                                        float a_u, float dt, float* clock) {
     XfProjectFn fn = nullptr;
     switch (energy) {
-        case 0: fn = pick_aux<0>(sim, aux); break;
-        case 3: fn = pick_aux<3>(sim, aux); break;
-        case 4: fn = pick_aux<4>(sim, aux); break;
-        case 5: fn = pick_aux<5>(sim, aux); break;
-        case 6: fn = pick_aux<6>(sim, aux); break;
-        case 7: fn = pick_aux<7>(sim, aux); break;
+        case 0: fn = pick_aux<0>(sim, aux, damp); break;
+        case 3: fn = pick_aux<3>(sim, aux, damp); break;
+        case 4: fn = pick_aux<4>(sim, aux, damp); break;
+        case 5: fn = pick_aux<5>(sim, aux, damp); break;
+        case 6: fn = pick_aux<6>(sim, aux, damp); break;
+        case 7: fn = pick_aux<7>(sim, aux, damp); break;
         default: return hipErrorInvalidValue;
     }
     if (count == 0) return hipSuccess;
     uint32_t grid = (count + block - 1) / block;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, s, begin, count, tidx, tg0, tg1, tg2, tg3, taux, X, bodies,
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, s, begin, count, tidx, tg0, tg1, tg2, tg3, taux, X, O, bodies,
                        comp1_u, a_u, dt, clock);
+    return hipGetLastError();
+}
+
+typedef void (*XfDampFn)(uint32_t, uint32_t, const uint4*, const float4*, const float4*, const float4*,
+                         const float4*, const float4*, const float2*, const double4*, double4*, const XfBodyParams*,
+                         float, float, float, const float*);
+
+namespace {
+template <int E>
+XfDampFn pick_damp(bool aux) {
+    return aux ? xf_damp_colour<E, true> : xf_damp_colour<E, false>;
+}
+}  // namespace
+
+// Rayleigh post damping of one colour (bodies with type Post/PostAmortized).
+extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32_t block, uint32_t begin,
+                                     uint32_t count, const uint4* tidx, const float4* tg0, const float4* tg1,
+                                     const float4* tg2, const float4* tg3, const float4* taux, const float2* tdmp,
+                                     const double4* X, double4* V, const XfBodyParams* bodies, float comp1_u,
+                                     float a_u, float dt, const float* clock) {
+    XfDampFn fn = nullptr;
+    switch (energy) {
+        case 0: fn = pick_damp<0>(aux); break;
+        case 3: fn = pick_damp<3>(aux); break;
+        case 4: fn = pick_damp<4>(aux); break;
+        case 5: fn = pick_damp<5>(aux); break;
+        case 6: fn = pick_damp<6>(aux); break;
+        case 7: fn = pick_damp<7>(aux); break;
+        default: return hipErrorInvalidValue;
+    }
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(fn, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count, tidx, tg0, tg1, tg2,
+                       tg3, taux, tdmp, X, V, bodies, comp1_u, a_u, dt, clock);
+    return hipGetLastError();
+}
+
+// PBD damping of one colour.
+extern "C" hipError_t xf_launch_pbd(hipStream_t s, bool aux, uint32_t block, uint32_t begin, uint32_t count,
+                                    const uint4* tidx, const float4* taux, const float2* tdmp, const double4* X,
+                                    double4* V, const XfBodyParams* bodies, const float* clock) {
+    if (count == 0) return hipSuccess;
+    if (aux)
+        hipLaunchKernelGGL(xf_pbd_colour<true>, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count,
+                           tidx, taux, tdmp, X, V, bodies, clock);
+    else
+        hipLaunchKernelGGL(xf_pbd_colour<false>, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count,
+                           tidx, taux, tdmp, X, V, bodies, clock);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t begin, uint32_t count, uint32_t pass,
+                                       const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
+                                       const float4* taux, double4* X, const XfBodyParams* bodies, float dt) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_volume_colour, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count, pass, tidx,
+                       tg0, tg1, tg2, taux, X, bodies, dt);
     return hipGetLastError();
 }
 
@@ -673,32 +1248,37 @@ extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const
     return hipGetLastError();
 }
 
-typedef void (*XfFrameFn)(uint32_t, uint32_t, const uint32_t*, uint32_t, const uint4*, const float4*, const float4*,
-                          const float4*, const float4*, const float4*, double4*, double4*, double4*,
-                          const XfBodyParams*, float, float, float, float*, uint32_t*, uint32_t*);
+typedef void (*XfFrameFn)(uint32_t, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, const uint4*,
+                          const float4*, const float4*, const float4*, const float4*, const float4*, const float2*,
+                          double4*, double4*, double4*, const XfBodyParams*, float, float, float, float*, uint32_t*,
+                          uint32_t*);
 
 namespace {
-template <int E>
-XfFrameFn pick_frame(bool sim, bool aux) {
-    if (sim) return aux ? xf_frame_persistent<E, true, true> : xf_frame_persistent<E, true, false>;
-    return aux ? xf_frame_persistent<E, false, true> : xf_frame_persistent<E, false, false>;
+template <int E, bool SIM>
+XfFrameFn pick_frame_ad(bool aux, bool damp) {
+    if (aux) return damp ? xf_frame_persistent<E, SIM, true, true> : xf_frame_persistent<E, SIM, true, false>;
+    return damp ? xf_frame_persistent<E, SIM, false, true> : xf_frame_persistent<E, SIM, false, false>;
 }
-XfFrameFn frame_fn(int energy, bool sim, bool aux) {
+template <int E>
+XfFrameFn pick_frame(bool sim, bool aux, bool damp) {
+    return sim ? pick_frame_ad<E, true>(aux, damp) : pick_frame_ad<E, false>(aux, damp);
+}
+XfFrameFn frame_fn(int energy, bool sim, bool aux, bool damp) {
     switch (energy) {
-        case 0: return pick_frame<0>(sim, aux);
-        case 3: return pick_frame<3>(sim, aux);
-        case 4: return pick_frame<4>(sim, aux);
-        case 5: return pick_frame<5>(sim, aux);
-        case 6: return pick_frame<6>(sim, aux);
-        case 7: return pick_frame<7>(sim, aux);
+        case 0: return pick_frame<0>(sim, aux, damp);
+        case 3: return pick_frame<3>(sim, aux, damp);
+        case 4: return pick_frame<4>(sim, aux, damp);
+        case 5: return pick_frame<5>(sim, aux, damp);
+        case 6: return pick_frame<6>(sim, aux, damp);
+        case 7: return pick_frame<7>(sim, aux, damp);
         default: return nullptr;
     }
 }
 }  // namespace
 
 // Largest co-resident grid of the persistent kernel at 256 threads per block.
-extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, int* max_blocks) {
-    XfFrameFn fn = frame_fn(energy, sim, aux);
+extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, bool damp, int* max_blocks) {
+    XfFrameFn fn = frame_fn(energy, sim, aux, damp);
     if (!fn) return hipErrorInvalidValue;
     int dev = 0, per_cu = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -708,16 +1288,19 @@ extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, int* m
     return e;
 }
 
-extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, uint32_t blocks,
-                                      uint32_t substeps, uint32_t ncol, const uint32_t* colour_begin,
+extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t blocks,
+                                      uint32_t substeps, uint32_t ncol, uint32_t vpasses, uint32_t post,
+                                      const float2* tdmp, const uint32_t* colour_begin,
                                       uint32_t n_nodes, const uint4* tidx, const float4* tg0, const float4* tg1,
                                       const float4* tg2, const float4* tg3, const float4* taux, double4* X,
                                       double4* O, double4* V, const XfBodyParams* bodies, float comp1_u, float a_u,
                                       float dt, float* clock, uint32_t* arrive, uint32_t* err) {
-    XfFrameFn fn = frame_fn(energy, sim, aux);
+    XfFrameFn fn = frame_fn(energy, sim, aux, damp);
     if (!fn) return hipErrorInvalidValue;
-    void* args[] = {&substeps, &ncol,  &colour_begin, &n_nodes, &tidx,   &tg0,   &tg1, &tg2,   &tg3, &taux,
-                    &X,        &O,     &V,            &bodies,  &comp1_u, &a_u,  &dt,  &clock, &arrive, &err};
+    // same order as the xf_frame_persistent parameters
+    void* args[] = {&substeps, &ncol, &vpasses, &post, &colour_begin, &n_nodes, &tidx, &tg0,     &tg1, &tg2, &tg3,
+                    &taux,     &tdmp, &X,       &O,    &V,            &bodies,  &comp1_u, &a_u, &dt,  &clock,
+                    &arrive,   &err};
     return hipLaunchCooperativeKernel((const void*)fn, dim3(blocks), dim3(256), args, 0, s);
 }
 

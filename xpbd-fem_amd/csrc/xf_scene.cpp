@@ -18,17 +18,28 @@
 #include "xf_device.h"
 #include "xf_host_util.h"
 
-extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, uint32_t block,
+extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t block,
                                         uint32_t begin, uint32_t count, const uint4* tidx, const float4* tg0,
                                         const float4* tg1, const float4* tg2, const float4* tg3, const float4* taux,
-                                        double4* X, const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
-                                        float* clock);
+                                        double4* X, const double4* O, const XfBodyParams* bodies, float comp1_u,
+                                        float a_u, float dt, float* clock);
+extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32_t block, uint32_t begin,
+                                     uint32_t count, const uint4* tidx, const float4* tg0, const float4* tg1,
+                                     const float4* tg2, const float4* tg3, const float4* taux, const float2* tdmp,
+                                     const double4* X, double4* V, const XfBodyParams* bodies, float comp1_u,
+                                     float a_u, float dt, const float* clock);
+extern "C" hipError_t xf_launch_pbd(hipStream_t s, bool aux, uint32_t block, uint32_t begin, uint32_t count,
+                                    const uint4* tidx, const float4* taux, const float2* tdmp, const double4* X,
+                                    double4* V, const XfBodyParams* bodies, const float* clock);
+extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t begin, uint32_t count, uint32_t pass,
+                                       const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
+                                       const float4* taux, double4* X, const XfBodyParams* bodies, float dt);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
                                        double4* V, const XfBodyParams* bodies, const float* clock);
 extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const double4* X, uint32_t* count);
-extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, int* max_blocks);
+extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, bool damp, int* max_blocks);
 extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool aux, uint32_t begin, uint32_t count,
                                                const uint4* tidx, const float4* tg0, const float4* tg1,
                                                const float4* tg2, const float4* tg3, const float4* taux, double4* X,
@@ -47,8 +58,9 @@ extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const 
                                                const double4* Ob, const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog);
 extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a);
-extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, uint32_t blocks,
-                                      uint32_t substeps, uint32_t ncol, const uint32_t* colour_begin,
+extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t blocks,
+                                      uint32_t substeps, uint32_t ncol, uint32_t vpasses, uint32_t post,
+                                      const float2* tdmp, const uint32_t* colour_begin,
                                       uint32_t n_nodes, const uint4* tidx, const float4* tg0, const float4* tg1,
                                       const float4* tg2, const float4* tg3, const float4* taux, double4* X,
                                       double4* O, double4* V, const XfBodyParams* bodies, float comp1_u, float a_u,
@@ -58,6 +70,7 @@ struct xf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    std::vector<xf_scene*> scenes;  // open scenes, destroyed with the context
 };
 
 namespace {
@@ -108,6 +121,9 @@ int DevAlloc(xf_ctx* ctx, T** p, size_t count, uint64_t* total) {
 
 bool EnergySupported(uint32_t e) { return e >= 3 && e <= 7; }
 
+// Volume passes per substep accepted by xf_scene_set_settings.
+constexpr uint32_t kMaxVolumePasses = 64;
+
 }  // namespace
 
 struct xf_scene {
@@ -122,8 +138,9 @@ struct xf_scene {
     uint4* tidx = nullptr;
     float4* tg[4] = {nullptr, nullptr, nullptr, nullptr};
     float4* taux = nullptr;
+    float2* tdmp = nullptr;  // per tet (surface area, position in the body's sweep order)
     XfBodyParams* bp = nullptr;
-    float* clock = nullptr;
+    float* clock = nullptr;  // kClockWords: time, tick
     uint32_t* counter = nullptr;
     uint32_t* colour_begin_d = nullptr;  // ncol + 1 offsets (persistent launch)
     uint32_t* arrive = nullptr;          // grid-barrier arrivals (persistent launch)
@@ -136,9 +153,13 @@ struct xf_scene {
     std::vector<double> h_oflags;  // per node O.w: (body << 1) | pinned
     bool material_dirty = true, params_dirty = true;
     bool aux = false;
+    uint32_t vpasses = 0;  // max volume passes over bodies (taux is then always present)
+    bool dproj = false;    // some body damps inside the projection (Rayleigh Paper / Limit)
+    uint32_t post = 0;     // damping passes after the velocity update: bit 0 Rayleigh post, bit 1 PBD
     float comp1_u = 0.0f, a_u = 1.0f;
     float params_dt = -1.0f;
     float time_host = 0.0f;
+    uint32_t tick_host = 0;
     // adjoint buffers (grown on demand, freed with the scene)
     struct AdjBuf {
         void* p = nullptr;
@@ -152,7 +173,8 @@ struct xf_scene {
     uint32_t graph_substeps = 0;
     float graph_dt = 0.0f;
     int graph_energy = -1;
-    bool graph_sim = false, graph_aux = false;
+    bool graph_sim = false, graph_aux = false, graph_dproj = false;
+    uint32_t graph_vpasses = 0, graph_post = 0;
 };
 
 namespace {
@@ -206,10 +228,21 @@ void ComputeMaterial(xf_scene* s) {
             s->h_tg3[pos].w = comp0;
             s->h_taux[pos].x = comp1;
             s->h_taux[pos].y = a;
+            s->h_taux[pos].w = b.settings.compliance * b.ec.vol[t];  // SolveVolumeOnly (Fem.cpp:845)
             if (comp1 != 0.0f || a != 1.0f) compressible = true;
         }
     }
     s->aux = compressible || s->bodies.size() > 1;
+    s->vpasses = 0;
+    s->dproj = false;
+    s->post = 0;
+    for (const Body& b : s->bodies) {
+        const xf_settings& st = b.settings;
+        s->vpasses = std::max(s->vpasses, st.volume_passes);
+        if (st.damping > 0.0f && st.rayleigh_type < XF_RAYLEIGH_POST) s->dproj = true;
+        if (st.damping > 0.0f && st.rayleigh_type >= XF_RAYLEIGH_POST) s->post |= 1u;
+        if (st.pbd_damping > 0.0f) s->post |= 2u;
+    }
 }
 
 int UploadMaterial(xf_scene* s) {
@@ -217,12 +250,13 @@ int UploadMaterial(xf_scene* s) {
     bool had_aux = s->aux;
     ComputeMaterial(s);
     if (s->aux != had_aux) DropGraph(s);
-    if (s->aux && !s->taux) {
+    bool need_taux = s->aux || s->vpasses > 0;
+    if (need_taux && !s->taux) {
         int rc = DevAlloc(ctx, &s->taux, s->T, &s->device_bytes);
         if (rc) return rc;
     }
     XF_HIP(ctx, hipMemcpyAsync(s->tg[3], s->h_tg3.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, ctx->stream));
-    if (s->aux)
+    if (need_taux)
         XF_HIP(ctx, hipMemcpyAsync(s->taux, s->h_taux.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice,
                                    ctx->stream));
     XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -252,6 +286,13 @@ int UploadParams(xf_scene* s, float dt) {
         }
         p.sr = st.sphere_r;
         p.energy = st.energy;
+        p.volume_passes = st.volume_passes;
+        p.damping = st.damping;
+        p.rtype = st.rayleigh_type;
+        p.pbd = st.pbd_damping;
+        p.pbd_amortized = st.pbd_damping_amortized;
+        p.n_tets = s->bodies[i].t;
+        p.simultaneous = st.simultaneous ? 1u : 0u;
     }
     XF_HIP(ctx, hipMemcpyAsync(s->bp, hp.data(), sizeof(XfBodyParams) * hp.size(), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -289,7 +330,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
     };
     if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT) {
         int maxb = 0;
-        XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, &maxb));
+        XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
         uint32_t blocks = s->opts.persistent_blocks ? s->opts.persistent_blocks : (uint32_t)s->n_cu;
         if (maxb <= 0) return Fail(ctx, XF_E_UNSUPPORTED, "persistent kernel does not fit on the device");
         if (blocks > (uint32_t)maxb) blocks = (uint32_t)maxb;
@@ -300,9 +341,10 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
         }
         XF_HIP(ctx, hipMemsetAsync(s->arrive, 0, sizeof(uint32_t), st));
         if (timer) XF_HIP(ctx, hipEventRecord(e0, st));
-        XF_HIP(ctx, xf_launch_frame(st, energy, sim, s->aux, blocks, substeps, s->ncol, s->colour_begin_d, s->N,
-                                    s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O, s->V,
-                                    s->bp, s->comp1_u, s->a_u, dt, s->clock, s->arrive, s->barrier_err));
+        XF_HIP(ctx, xf_launch_frame(st, energy, sim, s->aux, s->dproj, blocks, substeps, s->ncol, s->vpasses, s->post,
+                                    s->tdmp, s->colour_begin_d, s->N, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                                    s->tg[3], s->taux, s->X, s->O, s->V, s->bp, s->comp1_u, s->a_u, dt, s->clock,
+                                    s->arrive, s->barrier_err));
         if (timer) {
             XF_HIP(ctx, hipEventRecord(e1, st));
             timer->ev_proj.push_back(e0);
@@ -321,17 +363,48 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                 XF_HIP(ctx, hipEventCreate(&e1));
                 XF_HIP(ctx, hipEventRecord(e0, st));
             }
-            XF_HIP(ctx, xf_launch_project(st, energy, sim, s->aux, block, begin, count, s->tidx, s->tg[0], s->tg[1],
-                                          s->tg[2], s->tg[3], s->taux, s->X, s->bp, s->comp1_u, s->a_u, dt,
-                                          c == 0 ? s->clock : nullptr));
+            XF_HIP(ctx, xf_launch_project(st, energy, sim, s->aux, s->dproj, block, begin, count, s->tidx, s->tg[0],
+                                          s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O, s->bp, s->comp1_u,
+                                          s->a_u, dt, c == 0 ? s->clock : nullptr));
             if (timer) {
                 XF_HIP(ctx, hipEventRecord(e1, st));
                 timer->ev_proj.push_back(e0);
                 timer->ev_proj.push_back(e1);
             }
         }
-        rc = node(1, k + 1 < substeps);
+        // volumePasses x SolveElementVolume over the same colour order (Geo.cpp:779-782)
+        for (uint32_t pass = 0; pass < s->vpasses; pass++)
+            for (uint32_t c = 0; c < s->ncol; c++) {
+                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                XF_HIP(ctx, xf_launch_volume(st, block, begin, count, pass, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                                             s->taux, s->X, s->bp, dt));
+            }
+        if (s->post == 0) {
+            rc = node(1, k + 1 < substeps);
+            if (rc) return rc;
+            continue;
+        }
+        // GeoLinear3d::Damp after the velocity update (Geo.cpp:346-355, 790-811):
+        // velocities are stored, damped colour by colour, then the next prediction.
+        rc = node(1, false);
         if (rc) return rc;
+        if (s->post & 1u)
+            for (uint32_t c = 0; c < s->ncol; c++) {
+                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                XF_HIP(ctx, xf_launch_damp(st, energy, s->aux, block, begin, count, s->tidx, s->tg[0], s->tg[1],
+                                           s->tg[2], s->tg[3], s->taux, s->tdmp, s->X, s->V, s->bp, s->comp1_u,
+                                           s->a_u, dt, s->clock));
+            }
+        if (s->post & 2u)
+            for (uint32_t c = 0; c < s->ncol; c++) {
+                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                XF_HIP(ctx, xf_launch_pbd(st, s->aux, block, begin, count, s->tidx, s->taux, s->tdmp, s->X, s->V,
+                                          s->bp, s->clock));
+            }
+        if (k + 1 < substeps) {
+            rc = node(0, false);
+            if (rc) return rc;
+        }
     }
     return XF_OK;
 }
@@ -355,9 +428,14 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
 int CheckSettings(xf_ctx* ctx, const xf_settings* st) {
     if (!EnergySupported(st->energy))
         return Fail(ctx, XF_E_UNSUPPORTED, "energy %u not implemented (3..7 are)", st->energy);
-    if (st->damping != 0.0f || st->pbd_damping != 0.0f)
-        return Fail(ctx, XF_E_UNSUPPORTED, "Rayleigh/PBD damping passes are not implemented (must be 0)");
-    if (st->volume_passes != 0) return Fail(ctx, XF_E_UNSUPPORTED, "volume passes are not implemented (must be 0)");
+    if (!(st->damping >= 0.0f) || !std::isfinite(st->damping) || !(st->pbd_damping >= 0.0f) ||
+        !std::isfinite(st->pbd_damping) || !(st->pbd_damping_amortized >= 0.0f) ||
+        !std::isfinite(st->pbd_damping_amortized))
+        return Fail(ctx, XF_E_INVALID, "damping, pbd_damping and pbd_damping_amortized must be finite and >= 0");
+    if (st->rayleigh_type > XF_RAYLEIGH_POST_AMORTIZED)
+        return Fail(ctx, XF_E_INVALID, "rayleigh_type %u unknown (0..3)", st->rayleigh_type);
+    if (st->volume_passes > kMaxVolumePasses)
+        return Fail(ctx, XF_E_INVALID, "volume_passes %u above the limit %u", st->volume_passes, kMaxVolumePasses);
     if (!(st->compliance > 0.0f)) return Fail(ctx, XF_E_INVALID, "compliance must be positive");
     if (!(st->poissons_ratio > 0.0f && st->poissons_ratio <= 0.5f))
         return Fail(ctx, XF_E_INVALID, "poissons_ratio must be in (0, 0.5]");
@@ -405,6 +483,7 @@ XF_API int xf_ctx_create(int device, xf_ctx** out) {
 
 XF_API void xf_ctx_destroy(xf_ctx* ctx) {
     if (!ctx) return;
+    while (!ctx->scenes.empty()) xf_scene_destroy(ctx->scenes.back());
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -415,6 +494,7 @@ XF_API int xf_scene_create(xf_ctx* ctx, xf_scene** out) {
     if (!ctx || !out) return XF_E_INVALID;
     xf_scene* s = new xf_scene();
     s->ctx = ctx;
+    ctx->scenes.push_back(s);
     s->opts.colour_order = XF_COLOUR_NATURAL;
     s->opts.block_size = 64;
     s->opts.use_graph = 1;
@@ -426,12 +506,14 @@ XF_API int xf_scene_create(xf_ctx* ctx, xf_scene** out) {
 
 XF_API void xf_scene_destroy(xf_scene* s) {
     if (!s) return;
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    xf_ctx* ctx = s->ctx;
+    ctx->scenes.erase(std::remove(ctx->scenes.begin(), ctx->scenes.end(), s), ctx->scenes.end());
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     DropGraph(s);
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
-                    s->barrier_err};
+                    s->barrier_err, s->tdmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -552,6 +634,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     std::vector<float4> h_tg0(s->T), h_tg1(s->T), h_tg2(s->T);
     s->h_tg3.assign(s->T, make_float4(0, 0, 0, 0));
     s->h_taux.assign(s->T, make_float4(0, 0, 0, 0));
+    std::vector<float2> h_tdmp(s->T);
     std::vector<uint32_t> cursor(s->bodies.size(), 0);
     uint32_t pos = 0;
     for (uint32_t c = 0; c < s->ncol; c++) {
@@ -576,6 +659,10 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
                 float bidf;
                 memcpy(&bidf, &bid, 4);
                 s->h_taux[pos] = make_float4(0.0f, 1.0f, bidf, 0.0f);
+                uint32_t local = k - 1;  // position in the body's sweep order
+                float localf;
+                memcpy(&localf, &local, 4);
+                h_tdmp[pos] = make_float2(b.ec.area[t], localf);
                 b.sched_pos[t] = pos;
                 ++pos;
             }
@@ -606,7 +693,8 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         (rc = DevAlloc(ctx, &s->V, s->N, &s->device_bytes)) ||
         (rc = DevAlloc(ctx, &s->tidx, s->T, &s->device_bytes)) ||
         (rc = DevAlloc(ctx, &s->bp, s->bodies.size(), &s->device_bytes)) ||
-        (rc = DevAlloc(ctx, &s->clock, 1, &s->device_bytes)) || (rc = DevAlloc(ctx, &s->counter, 1, &s->device_bytes)))
+        (rc = DevAlloc(ctx, &s->clock, kClockWords, &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->counter, 1, &s->device_bytes)) || (rc = DevAlloc(ctx, &s->tdmp, s->T, &s->device_bytes)))
         return rc;
     for (int k = 0; k < 4; k++)
         if ((rc = DevAlloc(ctx, &s->tg[k], s->T, &s->device_bytes))) return rc;
@@ -623,14 +711,16 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     XF_HIP(ctx, hipMemcpyAsync(s->tg[0], h_tg0.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[1], h_tg1.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[2], h_tg2.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
-    float zero = 0.0f;
+    XF_HIP(ctx, hipMemcpyAsync(s->tdmp, h_tdmp.data(), sizeof(float2) * s->T, hipMemcpyHostToDevice, st));
     uint32_t uzero = 0;
-    XF_HIP(ctx, hipMemcpyAsync(s->clock, &zero, sizeof(float), hipMemcpyHostToDevice, st));
+    static const uint32_t kClockZero[kClockWords] = {0u, 0u};  // time 0.0f, tick 0
+    XF_HIP(ctx, hipMemcpyAsync(s->clock, kClockZero, sizeof(kClockZero), hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->colour_begin_d, s->colour_begin.data(), sizeof(uint32_t) * (s->ncol + 1),
                                hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->barrier_err, &uzero, sizeof(uint32_t), hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipStreamSynchronize(st));
     s->time_host = 0.0f;
+    s->tick_host = 0;
     s->finalized = true;
     s->material_dirty = true;
     s->params_dirty = true;
@@ -683,7 +773,8 @@ XF_API int xf_scene_step(xf_scene* s, uint32_t substeps, float dt) {
         rc = Enqueue(s, substeps, dt, energy, sim, nullptr);
     } else {
         if (!s->graph_exec || s->graph_substeps != substeps || s->graph_dt != dt || s->graph_energy != energy ||
-            s->graph_sim != sim || s->graph_aux != s->aux) {
+            s->graph_sim != sim || s->graph_aux != s->aux || s->graph_vpasses != s->vpasses ||
+            s->graph_dproj != s->dproj || s->graph_post != s->post) {
             DropGraph(s);
             XF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             rc = Enqueue(s, substeps, dt, energy, sim, nullptr);
@@ -701,11 +792,15 @@ XF_API int xf_scene_step(xf_scene* s, uint32_t substeps, float dt) {
             s->graph_energy = energy;
             s->graph_sim = sim;
             s->graph_aux = s->aux;
+            s->graph_vpasses = s->vpasses;
+            s->graph_dproj = s->dproj;
+            s->graph_post = s->post;
         }
         XF_HIP(ctx, hipGraphLaunch(s->graph_exec, ctx->stream));
     }
     if (rc) return rc;
     for (uint32_t k = 0; k < substeps; k++) s->time_host = s->time_host + dt;
+    s->tick_host += substeps;
     return XF_OK;
 }
 
@@ -806,6 +901,23 @@ XF_API int xf_scene_set_time(xf_scene* s, float t) {
     return XF_OK;
 }
 
+XF_API int xf_scene_get_tick(const xf_scene* s, uint32_t* tick) {
+    if (!s || !tick) return XF_E_INVALID;
+    *tick = s->tick_host;
+    return XF_OK;
+}
+
+XF_API int xf_scene_set_tick(xf_scene* s, uint32_t tick) {
+    if (!s) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    XF_HIP(ctx, hipMemcpyAsync(s->clock + 1, &tick, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->tick_host = tick;
+    return XF_OK;
+}
+
 XF_API int xf_scene_get_order(const xf_scene* s, uint32_t body, uint32_t* order, uint32_t* colour_of_tet) {
     if (!s) return XF_E_INVALID;
     if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
@@ -855,6 +967,7 @@ XF_API int xf_scene_profile(xf_scene* s, uint32_t substeps, float dt, float* ms_
     if (rc) return rc;
     if (se != hipSuccess) return Fail(ctx, XF_E_HIP, "profile sync: %s", hipGetErrorString(se));
     for (uint32_t k = 0; k < substeps; k++) s->time_host = s->time_host + dt;
+    s->tick_host += substeps;
     if (ms_project) *ms_project = tp;
     if (ms_node) *ms_node = tn;
     if (launches_project) *launches_project = nproj;
@@ -877,7 +990,7 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     float* sc = nullptr;
     uint64_t dummy = 0;
     if ((rc = DevAlloc(ctx, &sx, s->N, &dummy)) || (rc = DevAlloc(ctx, &so, s->N, &dummy)) ||
-        (rc = DevAlloc(ctx, &sv, s->N, &dummy)) || (rc = DevAlloc(ctx, &sc, 1, &dummy))) {
+        (rc = DevAlloc(ctx, &sv, s->N, &dummy)) || (rc = DevAlloc(ctx, &sc, kClockWords, &dummy))) {
         for (void* p : {(void*)sx, (void*)so, (void*)sv, (void*)sc})
             if (p) (void)hipFree(p);
         return rc;
@@ -894,8 +1007,8 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
         if ((e = hipMemcpyAsync(save ? sv : s->V, save ? s->V : sv, sizeof(double4) * s->N, hipMemcpyDeviceToDevice,
                                 st)))
             return e;
-        if ((e = hipMemcpyAsync(save ? sc : s->clock, save ? s->clock : sc, sizeof(float), hipMemcpyDeviceToDevice,
-                                st)))
+        if ((e = hipMemcpyAsync(save ? sc : s->clock, save ? s->clock : sc, sizeof(float) * kClockWords,
+                                hipMemcpyDeviceToDevice, st)))
             return e;
         return hipStreamSynchronize(st);
     };
@@ -908,13 +1021,14 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     xf_scene_opts keep = s->opts;
     Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size};
     int maxb = 0;
-    XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, &maxb));
+    XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
     float best = -1.0f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t he = copy_all(true);
     if (he == hipSuccess) he = hipEventCreate(&e0);
     if (he == hipSuccess) he = hipEventCreate(&e1);
     float time_keep = s->time_host;
+    uint32_t tick_keep = s->tick_host;
     for (const Cand& cand : candidates) {
         if (he != hipSuccess || rc) break;
         if (cand.mode == XF_LAUNCH_PERSISTENT && (int)cand.size > maxb) continue;
@@ -942,6 +1056,7 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     hipError_t he2 = copy_all(false);
     for (void* p : {(void*)sx, (void*)so, (void*)sv, (void*)sc}) (void)hipFree(p);
     s->time_host = time_keep;
+    s->tick_host = tick_keep;
     s->opts = keep;
     s->opts.launch_mode = best_c.mode;
     if (best_c.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = best_c.size;
@@ -971,7 +1086,8 @@ int Reserve(xf_scene* s, xf_scene::AdjBuf* b, size_t bytes) {
 }  // namespace
 
 XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
-                            const double* targets, uint32_t window, double* loss, double* grad_log_mu) {
+                            const double* targets, uint32_t window, double* loss, double* grad_log_mu,
+                            double* body_loss) {
     if (!s) return XF_E_INVALID;
     xf_ctx* ctx = s->ctx;
     if (!s->finalized) return Fail(ctx, XF_E_STATE, "scene is not finalized");
@@ -982,9 +1098,11 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
             return Fail(ctx, XF_E_INVALID, "loss_at must ascend within [1, substeps]");
     for (const Body& b : s->bodies) {
         const xf_settings& st = b.settings;
-        if (!st.simultaneous || (st.energy != 4 && st.energy != 7) || (st.collide & XF_COLLIDE_SPHERE))
+        if (!st.simultaneous || (st.energy != 4 && st.energy != 7) || (st.collide & XF_COLLIDE_SPHERE) ||
+            st.volume_passes != 0 || st.damping > 0.0f || st.pbd_damping > 0.0f)
             return Fail(ctx, XF_E_UNSUPPORTED,
-                        "adjoint covers simultaneous MixedSel/YeohSkinFast with pins and ground contact");
+                        "adjoint covers simultaneous MixedSel/YeohSkinFast with pins and ground contact "
+                        "(no volume passes, no damping)");
     }
     XF_HIP(ctx, hipSetDevice(ctx->device));
     int energy;
@@ -1060,8 +1178,11 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
         std::vector<double4> hx((size_t)N * n_loss);
         XF_HIP(ctx, hipMemcpy(hx.data(), xloss, hx.size() * sizeof(double4), hipMemcpyDeviceToHost));
         double L = 0.0;
+        if (body_loss)
+            for (size_t bi = 0; bi < s->bodies.size(); bi++) body_loss[bi] = 0.0;
         for (uint32_t jj = 0; jj < n_loss; jj++)
-            for (const Body& b : s->bodies) {
+            for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+                const Body& b = s->bodies[bi];
                 double e = 0.0;
                 for (uint32_t i = 0; i < b.n; i++) {
                     size_t g = (size_t)N * jj + b.node_base + i;
@@ -1072,6 +1193,7 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
                     e += d2 * d2;
                 }
                 L += e / (double)b.n;
+                if (body_loss) body_loss[bi] += e / (double)b.n;
             }
         if (loss) *loss = L;
     }
@@ -1081,7 +1203,6 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
     XF_HIP(ctx, xf_launch_zero_d4(st, N, Vb));
     XF_HIP(ctx, hipMemsetAsync(gacc, 0, sizeof(float) * T, st));
     XF_HIP(ctx, hipMemsetAsync(glog, 0, sizeof(double) * T, st));
-    const uint32_t block = s->opts.block_size ? s->opts.block_size : 64;
     uint32_t jl = n_loss;
     for (uint32_t w = nwin; w-- > 0;) {
         const uint32_t k0 = w * window, k1 = std::min(substeps, k0 + window);
@@ -1101,7 +1222,6 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
             XF_HIP(ctx, hipMemcpyAsync(oo + (size_t)N * (kk - k0), s->O, n4, hipMemcpyDeviceToDevice, st));
             XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, N, s->X, s->O, s->V, s->bp, s->clock));
         }
-        (void)block;
         for (uint32_t kk = k1; kk-- > k0;) {
             const double4* xl = nullptr;
             const double4* tg = nullptr;

@@ -7,6 +7,7 @@ context fails when no GPU is visible.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -15,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libxpbdfem.so")
 
 ENERGY_MIXED, ENERGY_MIXED_SEL, ENERGY_YEOH_SKIN, ENERGY_YEOH_SKIN_SEL, ENERGY_YEOH_SKIN_FAST = 3, 4, 5, 6, 7
 COLLIDE_GROUND, COLLIDE_SPHERE = 1, 2
+RAYLEIGH_PAPER, RAYLEIGH_LIMIT, RAYLEIGH_POST, RAYLEIGH_POST_AMORTIZED = 0, 1, 2, 3
 COLOUR_NATURAL, COLOUR_MINSTD = 0, 1
 LAUNCH_PER_COLOUR, LAUNCH_PERSISTENT = 0, 1
 SKIN_YEOH = (0.1095, 14.95, 4.595)  # Fem.cpp:526
@@ -45,6 +47,8 @@ class Settings(C.Structure):
         ("sphere_c0", C.c_float * 3),
         ("sphere_v", C.c_float * 3),
         ("sphere_r", C.c_float),
+        ("rayleigh_type", C.c_uint32),
+        ("pbd_damping_amortized", C.c_float),
     ]
 
 
@@ -75,6 +79,24 @@ class SceneInfo(C.Structure):
 _lib = None
 
 
+def loaded_hip_runtimes():
+    """Distinct libamdhip64 files mapped into this process.
+
+    PyTorch wheels ship their own HIP runtime. It is shared with this library
+    only when torch is imported before libxpbdfem.so loads; the other order
+    maps two runtimes, which must not both drive the device."""
+    found = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                path = line.split()[-1]
+                if "libamdhip64" in os.path.basename(path):
+                    found.add(os.path.realpath(path))
+    except OSError:
+        pass
+    return sorted(found)
+
+
 def lib():
     """Loads libxpbdfem.so from the package directory (raises if absent)."""
     global _lib
@@ -103,13 +125,15 @@ def lib():
         "xf_scene_set_state": ([vp, C.c_uint32, vp, vp], C.c_int),
         "xf_scene_get_time": ([vp, C.POINTER(C.c_float)], C.c_int),
         "xf_scene_set_time": ([vp, C.c_float], C.c_int),
+        "xf_scene_get_tick": ([vp, C.POINTER(C.c_uint32)], C.c_int),
+        "xf_scene_set_tick": ([vp, C.c_uint32], C.c_int),
         "xf_scene_get_order": ([vp, C.c_uint32, vp, vp], C.c_int),
         "xf_scene_get_elements": ([vp, C.c_uint32, vp, vp, vp, vp, vp], C.c_int),
         "xf_scene_profile": ([vp, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
                               C.POINTER(C.c_uint32)], C.c_int),
         "xf_scene_count_nonfinite": ([vp, C.POINTER(C.c_uint32)], C.c_int),
         "xf_scene_adjoint": ([vp, C.c_uint32, C.c_float, C.c_uint32, _u32p, vp, C.c_uint32, C.POINTER(C.c_double),
-                              vp], C.c_int),
+                              vp, vp], C.c_int),
         "xf_scene_autotune": ([vp, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(SceneOpts), C.POINTER(C.c_float)],
                               C.c_int),
         "xf_mesh_tet_block": ([C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, _u32p],
@@ -138,6 +162,7 @@ def exported_symbols():
         "xf_scene_get_time", "xf_scene_set_time", "xf_scene_get_order", "xf_scene_get_elements", "xf_scene_profile",
         "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
         "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
+        "xf_scene_get_tick", "xf_scene_set_tick",
     ]
 
 
@@ -221,6 +246,7 @@ class Context:
         if rc:
             raise XfError(f"xf_ctx_create(device={device}) failed ({rc}): no usable HIP device")
         self.h = h
+        self.scenes = weakref.WeakSet()
 
     def check(self, rc, what):
         if rc:
@@ -229,6 +255,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for sc in list(self.scenes):
+                sc.close()
             lib().xf_ctx_destroy(self.h)
             self.h = None
 
@@ -245,6 +273,7 @@ class Scene:
         h = C.c_void_p()
         ctx.check(lib().xf_scene_create(ctx.h, C.byref(h)), "xf_scene_create")
         self.h = h
+        ctx.scenes.add(self)
         self.bodies = []  # (n_nodes, n_tets)
         self._keep = []
 
@@ -330,6 +359,14 @@ class Scene:
     def set_time(self, t):
         self.ctx.check(lib().xf_scene_set_time(self.h, t), "xf_scene_set_time")
 
+    def tick(self):
+        k = C.c_uint32()
+        self.ctx.check(lib().xf_scene_get_tick(self.h, C.byref(k)), "xf_scene_get_tick")
+        return k.value
+
+    def set_tick(self, k):
+        self.ctx.check(lib().xf_scene_set_tick(self.h, k), "xf_scene_set_tick")
+
     def order(self, body=0):
         t = self.bodies[body][1]
         o = np.zeros(t, np.uint32)
@@ -362,17 +399,20 @@ class Scene:
         size = o.persistent_blocks if o.launch_mode == LAUNCH_PERSISTENT else o.block_size
         return o.launch_mode, size, ms.value
 
-    def adjoint(self, substeps, dt, loss_at, targets, window=20):
+    def adjoint(self, substeps, dt, loss_at, targets, window=20, per_body=False):
         """Forward rollout + adjoint: returns (loss, dL/dlog mu per tet, all
-        bodies concatenated). targets: [n_loss, total nodes * 3] float64."""
+        bodies concatenated), plus each body's loss term if per_body.
+        targets: [n_loss, total nodes * 3] float64."""
         loss_at = np.ascontiguousarray(loss_at, np.uint32)
         targets = np.ascontiguousarray(targets, np.float64)
         n_tets = sum(t for _, t in self.bodies)
         grad = np.zeros(n_tets, np.float64)
+        body_loss = np.zeros(len(self.bodies), np.float64)
         L = C.c_double()
         self.ctx.check(lib().xf_scene_adjoint(self.h, substeps, dt, loss_at.size, loss_at, targets.ctypes.data,
-                                              window, C.byref(L), grad.ctypes.data), "xf_scene_adjoint")
-        return L.value, grad
+                                              window, C.byref(L), grad.ctypes.data, body_loss.ctypes.data),
+                       "xf_scene_adjoint")
+        return (L.value, grad, body_loss) if per_body else (L.value, grad)
 
     def count_nonfinite(self):
         c = C.c_uint32()
@@ -381,7 +421,8 @@ class Scene:
 
     def close(self):
         if self.h:
-            lib().xf_scene_destroy(self.h)
+            if self.ctx.h:  # xf_ctx_destroy already freed the scene otherwise
+                lib().xf_scene_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -10,7 +10,12 @@ the HIP library; torch only carries the values.
 import numpy as np
 import torch
 
-import xpbdfem as xf  # noqa: F401  (the scene objects come from this binding)
+import xpbdfem as xf
+
+xf.lib()
+if len(xf.loaded_hip_runtimes()) > 1:
+    raise ImportError("two HIP runtimes are loaded (" + ", ".join(xf.loaded_hip_runtimes()) + "): import torch "
+                      "before xpbdfem so that libxpbdfem.so binds to torch's runtime")
 
 
 class RolloutLoss(torch.autograd.Function):
