@@ -276,6 +276,21 @@ XF_API int xf_normal_fill(uint64_t seed, float sigma, uint32_t n, float* out);
  * the reference sweeps its MINSTD order serially (Geo.cpp:759-777). */
 XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, uint32_t colour_order,
                           uint32_t* order, uint32_t* colour, uint32_t* n_colours);
+/* Tet mesh ingestion (no reference counterpart: the reference compiles its
+ * tetgen-made Armadillo volume mesh in, MeshGen.cpp:23-80). Reads a tetgen
+ * .node / .ele pair: call with xyz = tets = NULL for *n_nodes / *n_tets,
+ * then again with xyz[3 * n_nodes] and tets[4 * n_tets]. Numbering may
+ * start at 0 or 1; only 4-node tets; '#' comments. orient != 0 swaps the
+ * first two nodes of negatively oriented tets so every rest volume is
+ * positive. Errors: XF_E_INVALID with the text in xf_mesh_error(). */
+XF_API int xf_mesh_read_tetgen(const char* node_path, const char* ele_path, uint32_t orient, uint32_t* n_nodes,
+                               uint32_t* n_tets, float* xyz, uint32_t* tets);
+/* The element index stream GeoLinear3d::Init takes (Geo.cpp:697),
+ * [4, i0, i1, i2, i3]*, to 4 indices per tet: tets NULL for *n_tets only.
+ * XF_E_UNSUPPORTED for streams holding other elements. */
+XF_API int xf_mesh_tets_from_stream(const uint32_t* idx, uint32_t count, uint32_t* n_tets, uint32_t* tets);
+/* Message of the last failed xf_mesh_* call on this thread. */
+XF_API const char* xf_mesh_error(void);
 /* LDLTDecomposeAndSolve (Ldlt.h:88-95): packed lower-triangular A, n <= 25. */
 XF_API int xf_ldlt_solve(uint32_t n, const float* A, const float* b, float* x);
 

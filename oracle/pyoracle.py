@@ -164,6 +164,8 @@ def ref():
         L.xr_get_elements.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, _f32p, _f32p]
         L.xr_get_tets.argtypes = [C.c_void_p, _u32p]
         L.xr_get_area.argtypes = [C.c_void_p, _f32p]
+        L.xr_armadillo_t4.argtypes = [C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint32)]
         L.xr_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.xr_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.xr_substeps.argtypes = [C.c_void_p, C.POINTER(XrSettings), C.c_float, C.c_uint32]
@@ -289,6 +291,15 @@ def body_from_workload(wl, variant="f32"):
     return OracleBody(wl["xyz"], wl["tets"], wl["density"], wl["pinned"], compliance=s.get("compliance"),
                       tet_mu=wl["tet_mu"], nu=s.get("poissons_ratio", 0.5), variant=variant,
                       velocity=wl["velocity"])
+
+
+def ref_armadillo_t4():
+    """The compiled reference's Armadillo tet mesh: (xyz float32, index
+    stream uint32), copied out of the reference build (test use only)."""
+    nodes, idx = C.POINTER(C.c_float)(), C.POINTER(C.c_uint32)()
+    nn, ni = C.c_uint32(), C.c_uint32()
+    ref().xr_armadillo_t4(C.byref(nodes), C.byref(nn), C.byref(idx), C.byref(ni))
+    return np.ctypeslib.as_array(nodes, (nn.value,)).copy(), np.ctypeslib.as_array(idx, (ni.value,)).copy()
 
 
 class RefBody:

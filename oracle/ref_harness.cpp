@@ -121,6 +121,12 @@ void xr_get_elements(void* h, float* qi9, float* qq3, float* qr3, float* vol, fl
     for (uint32_t i = 0; i < g->vertCount; i++) w[i] = g->w[i];
 }
 
+// The reference's compiled-in Armadillo tet mesh (GenerateArmadillo,
+// MeshGen.cpp:54-81): xyz node data and the [4, i0..i3]* index stream.
+void xr_armadillo_t4(const float** nodes, uint32_t* node_count, const uint32_t** idx, uint32_t* idx_count) {
+    GenerateArmadillo(Element_T4, nodes, node_count, idx, idx_count);
+}
+
 // ElementParamsLinearTet::surfaceArea per tet (used by the PBD damping pass).
 void xr_get_area(void* h, float* area) {
     GeoLinear3d* g = ((RefBody*)h)->geo;

@@ -50,12 +50,11 @@ class GeoGpuLinear3d {
              uint32_t idxDataCount, int device = 0) {
         if (nodeDataCount % 3) return XF_E_INVALID;
         nodes_.assign(nodeData, nodeData + nodeDataCount);
-        tets_.clear();
-        for (uint32_t i = 0; i < idxDataCount;) {
-            if (idxData[i] != 4 || i + 4 >= idxDataCount) return XF_E_UNSUPPORTED;  // tets only
-            tets_.insert(tets_.end(), idxData + i + 1, idxData + i + 5);
-            i += 5;
-        }
+        uint32_t nt = 0;
+        int rc = xf_mesh_tets_from_stream(idxData, idxDataCount, &nt, nullptr);  // tets only
+        if (rc) return rc;
+        tets_.assign(4 * (size_t)nt, 0u);
+        if ((rc = xf_mesh_tets_from_stream(idxData, idxDataCount, &nt, tets_.data()))) return rc;
         density_ = density;
         n_ = nodeDataCount / 3;
         float mn = 1e24f, mx = -1e24f;

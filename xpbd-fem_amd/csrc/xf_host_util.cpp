@@ -2,7 +2,11 @@
 #include "xf_host_util.h"
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
+#include <utility>
 
 namespace xf {
 
@@ -257,6 +261,132 @@ int LdltSolve(uint32_t n, const float* A, const float* b, float* x) {
     for (uint32_t i = 0; i < n; i++)
         for (uint32_t j = 0; j < i; j++) x[n - 1 - i] -= L[rowL[n - 1 - j] + n - 1 - i] * x[n - 1 - j];
     return 0;
+}
+
+namespace {
+
+// Numeric rows of a tetgen file, comments ('#' to end of line) dropped.
+bool ReadRows(const char* path, std::vector<std::vector<double>>* rows, std::string* err) {
+    FILE* f = fopen(path, "r");
+    if (!f) {
+        *err = std::string("cannot open ") + path;
+        return false;
+    }
+    std::string line;
+    int ch;
+    bool ok = true;
+    auto flush = [&]() {
+        size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        std::vector<double> row;
+        const char* p = line.c_str();
+        while (*p && ok) {
+            if (*p == ' ' || *p == '\t' || *p == '\r' || *p == ',') {
+                ++p;
+                continue;
+            }
+            char* end = nullptr;
+            double v = strtod(p, &end);
+            if (end == p) {
+                *err = std::string("unparsable text in ") + path + ": " + line;
+                ok = false;
+                break;
+            }
+            row.push_back(v);
+            p = end;
+        }
+        if (ok && !row.empty()) rows->push_back(row);
+        line.clear();
+    };
+    while ((ch = fgetc(f)) != EOF && ok) {
+        if (ch == '\n') flush();
+        else line.push_back((char)ch);
+    }
+    if (ok) flush();
+    fclose(f);
+    return ok;
+}
+
+}  // namespace
+
+bool ReadTetgen(const char* node_path, const char* ele_path, bool orient, std::vector<float>* xyz,
+                std::vector<uint32_t>* tets, std::string* err) {
+    std::vector<std::vector<double>> nr, er;
+    if (!ReadRows(node_path, &nr, err) || !ReadRows(ele_path, &er, err)) return false;
+    if (nr.empty() || nr[0].size() < 2 || nr[0][1] != 3.0) {
+        *err = "node file: header must read <count> 3 [attributes] [markers]";
+        return false;
+    }
+    size_t n = (size_t)nr[0][0];
+    if (nr.size() < n + 1) {
+        *err = "node file: fewer node lines than the header count";
+        return false;
+    }
+    double base = n ? nr[1][0] : 0.0;
+    if (base != 0.0 && base != 1.0) {
+        *err = "node file: numbering must start at 0 or 1";
+        return false;
+    }
+    xyz->assign(3 * n, 0.0f);
+    for (size_t i = 0; i < n; i++) {
+        const std::vector<double>& r = nr[i + 1];
+        if (r.size() < 4 || r[0] != base + (double)i) {
+            *err = "node file: node lines must be <index> <x> <y> <z> in order";
+            return false;
+        }
+        for (int c = 0; c < 3; c++) (*xyz)[3 * i + c] = (float)r[1 + c];
+    }
+    if (er.empty() || er[0].size() < 2) {
+        *err = "ele file: header must read <count> <nodes per tet> [attributes]";
+        return false;
+    }
+    if (er[0][1] != 4.0) {
+        *err = "ele file: only linear tets (4 nodes per tet) are supported";
+        return false;
+    }
+    size_t t = (size_t)er[0][0];
+    if (er.size() < t + 1) {
+        *err = "ele file: fewer tet lines than the header count";
+        return false;
+    }
+    tets->assign(4 * t, 0u);
+    for (size_t k = 0; k < t; k++) {
+        const std::vector<double>& r = er[k + 1];
+        if (r.size() < 5) {
+            *err = "ele file: tet lines must be <index> <n0> <n1> <n2> <n3>";
+            return false;
+        }
+        for (int c = 0; c < 4; c++) {
+            double v = r[1 + c] - base;
+            if (v < 0.0 || v >= (double)n || v != (double)(uint32_t)v) {
+                *err = "ele file: node index out of range";
+                return false;
+            }
+            (*tets)[4 * k + c] = (uint32_t)v;
+        }
+        if (orient) {
+            uint32_t* is = &(*tets)[4 * k];
+            double P[3][3];
+            for (int a = 0; a < 3; a++)
+                for (int r2 = 0; r2 < 3; r2++)
+                    P[a][r2] = (double)(*xyz)[3 * is[a] + r2] - (double)(*xyz)[3 * is[3] + r2];
+            double det = P[0][0] * (P[1][1] * P[2][2] - P[1][2] * P[2][1]) -
+                         P[0][1] * (P[1][0] * P[2][2] - P[1][2] * P[2][0]) +
+                         P[0][2] * (P[1][0] * P[2][1] - P[1][1] * P[2][0]);
+            if (det < 0.0) std::swap(is[0], is[1]);
+        }
+    }
+    return true;
+}
+
+bool TetsFromStream(const uint32_t* idx, uint32_t count, std::vector<uint32_t>* tets) {
+    tets->clear();
+    for (uint32_t i = 0; i < count;) {
+        if (idx[i] != 4 || i + 4 >= count) return false;
+        tets->insert(tets->end(), idx + i + 1, idx + i + 5);
+        i += 5;
+    }
+    return true;
 }
 
 }  // namespace xf

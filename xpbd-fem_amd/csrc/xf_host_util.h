@@ -3,6 +3,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 namespace xf {
@@ -45,5 +46,18 @@ uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
 
 // LDLTDecomposeAndSolve (Ldlt.h:88-95). Returns 0 on success.
 int LdltSolve(uint32_t n, const float* A, const float* b, float* x);
+
+// Tet mesh from a tetgen .node / .ele pair: xyz (3 per node) and tets (4 per
+// tet, 0-based; tetgen numbering may start at 0 or 1). '#' starts a comment.
+// Only 4-node tets are accepted. If orient, tets of negative volume get their
+// first two nodes swapped (positive rest volume, as generated blocks have).
+// Returns false with a message in *err.
+bool ReadTetgen(const char* node_path, const char* ele_path, bool orient, std::vector<float>* xyz,
+                std::vector<uint32_t>* tets, std::string* err);
+
+// The element index stream of GeoLinear3d::Init (Geo.cpp:697) -- [4, i0, i1,
+// i2, i3]* for tets -- to 4 indices per tet. Returns false on any element
+// that is not a tet.
+bool TetsFromStream(const uint32_t* idx, uint32_t count, std::vector<uint32_t>* tets);
 
 }  // namespace xf

@@ -1318,6 +1318,59 @@ XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
     return XF_OK;
 }
 
+namespace {
+thread_local std::string g_mesh_error;
+}
+
+XF_API const char* xf_mesh_error(void) { return g_mesh_error.c_str(); }
+
+XF_API int xf_mesh_read_tetgen(const char* node_path, const char* ele_path, uint32_t orient, uint32_t* n_nodes,
+                               uint32_t* n_tets, float* xyz, uint32_t* tets) {
+    if (!node_path || !ele_path || !n_nodes || !n_tets) {
+        g_mesh_error = "null argument";
+        return XF_E_INVALID;
+    }
+    std::vector<float> x;
+    std::vector<uint32_t> t;
+    if (!xf::ReadTetgen(node_path, ele_path, orient != 0, &x, &t, &g_mesh_error)) return XF_E_INVALID;
+    if (x.size() / 3 > 0xFFFFFFFFull || t.size() / 4 > 0xFFFFFFFFull) {
+        g_mesh_error = "mesh too large";
+        return XF_E_INVALID;
+    }
+    if (xyz || tets) {
+        if (!xyz || !tets || *n_nodes != x.size() / 3 || *n_tets != t.size() / 4) {
+            g_mesh_error = "output arrays must match the counts of the first call";
+            return XF_E_INVALID;
+        }
+        memcpy(xyz, x.data(), sizeof(float) * x.size());
+        memcpy(tets, t.data(), sizeof(uint32_t) * t.size());
+    }
+    *n_nodes = (uint32_t)(x.size() / 3);
+    *n_tets = (uint32_t)(t.size() / 4);
+    return XF_OK;
+}
+
+XF_API int xf_mesh_tets_from_stream(const uint32_t* idx, uint32_t count, uint32_t* n_tets, uint32_t* tets) {
+    if ((!idx && count) || !n_tets) {
+        g_mesh_error = "null argument";
+        return XF_E_INVALID;
+    }
+    std::vector<uint32_t> t;
+    if (!xf::TetsFromStream(idx, count, &t)) {
+        g_mesh_error = "index stream holds an element other than a 4-node tet";
+        return XF_E_UNSUPPORTED;
+    }
+    if (tets) {
+        if (*n_tets != t.size() / 4) {
+            g_mesh_error = "output array must match the count of the first call";
+            return XF_E_INVALID;
+        }
+        memcpy(tets, t.data(), sizeof(uint32_t) * t.size());
+    }
+    *n_tets = (uint32_t)(t.size() / 4);
+    return XF_OK;
+}
+
 XF_API int xf_ldlt_solve(uint32_t n, const float* A, const float* b, float* x) {
     if (!A || !b || !x) return XF_E_INVALID;
     return xf::LdltSolve(n, A, b, x) == 0 ? XF_OK : XF_E_INVALID;

@@ -126,6 +126,10 @@ def lib():
         "xf_scene_get_time": ([vp, C.POINTER(C.c_float)], C.c_int),
         "xf_scene_set_time": ([vp, C.c_float], C.c_int),
         "xf_scene_get_tick": ([vp, C.POINTER(C.c_uint32)], C.c_int),
+        "xf_mesh_read_tetgen": ([C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 vp, vp], C.c_int),
+        "xf_mesh_tets_from_stream": ([vp, C.c_uint32, C.POINTER(C.c_uint32), vp], C.c_int),
+        "xf_mesh_error": ([], C.c_char_p),
         "xf_scene_set_tick": ([vp, C.c_uint32], C.c_int),
         "xf_scene_get_order": ([vp, C.c_uint32, vp, vp], C.c_int),
         "xf_scene_get_elements": ([vp, C.c_uint32, vp, vp, vp, vp, vp], C.c_int),
@@ -162,7 +166,7 @@ def exported_symbols():
         "xf_scene_get_time", "xf_scene_set_time", "xf_scene_get_order", "xf_scene_get_elements", "xf_scene_profile",
         "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
         "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
-        "xf_scene_get_tick", "xf_scene_set_tick",
+        "xf_scene_get_tick", "xf_scene_set_tick", "xf_mesh_read_tetgen", "xf_mesh_tets_from_stream", "xf_mesh_error",
     ]
 
 
@@ -210,6 +214,53 @@ def mesh_colour(n_nodes, tets, colour_order=COLOUR_NATURAL):
     if rc:
         raise XfError(f"xf_mesh_colour failed ({rc})")
     return order, colour, nc.value
+
+
+def read_tetgen(node_path, ele_path, orient=True):
+    """(xyz float32[3n], tets uint32[4t]) from a tetgen .node / .ele pair."""
+    n, t = C.c_uint32(), C.c_uint32()
+    L = lib()
+    rc = L.xf_mesh_read_tetgen(str(node_path).encode(), str(ele_path).encode(), int(orient), C.byref(n), C.byref(t),
+                               None, None)
+    if rc:
+        raise XfError(f"xf_mesh_read_tetgen failed ({rc}): {L.xf_mesh_error().decode()}")
+    xyz = np.zeros(3 * n.value, np.float32)
+    tets = np.zeros(4 * t.value, np.uint32)
+    rc = L.xf_mesh_read_tetgen(str(node_path).encode(), str(ele_path).encode(), int(orient), C.byref(n), C.byref(t),
+                               xyz.ctypes.data, tets.ctypes.data)
+    if rc:
+        raise XfError(f"xf_mesh_read_tetgen failed ({rc}): {L.xf_mesh_error().decode()}")
+    return xyz, tets
+
+
+def write_tetgen(prefix, xyz, tets, base=0):
+    """Writes prefix.node / prefix.ele in tetgen's format (numbering from base)."""
+    xyz = np.asarray(xyz, np.float64).reshape(-1, 3)
+    tets = np.asarray(tets, np.int64).reshape(-1, 4)
+    with open(f"{prefix}.node", "w") as f:
+        f.write(f"{len(xyz)} 3 0 0\n")
+        for i, p in enumerate(xyz):
+            f.write(f"{i + base} {float(p[0]):.17g} {float(p[1]):.17g} {float(p[2]):.17g}\n")
+    with open(f"{prefix}.ele", "w") as f:
+        f.write(f"{len(tets)} 4 0\n")
+        for k, t in enumerate(tets):
+            f.write(f"{k + base} {t[0] + base} {t[1] + base} {t[2] + base} {t[3] + base}\n")
+    return f"{prefix}.node", f"{prefix}.ele"
+
+
+def tets_from_stream(idx):
+    """The [4, i0, i1, i2, i3]* index stream of GeoLinear3d::Init to tets."""
+    idx = np.ascontiguousarray(idx, np.uint32)
+    t = C.c_uint32()
+    L = lib()
+    rc = L.xf_mesh_tets_from_stream(idx.ctypes.data, idx.size, C.byref(t), None)
+    if rc:
+        raise XfError(f"xf_mesh_tets_from_stream failed ({rc}): {L.xf_mesh_error().decode()}")
+    tets = np.zeros(4 * t.value, np.uint32)
+    rc = L.xf_mesh_tets_from_stream(idx.ctypes.data, idx.size, C.byref(t), tets.ctypes.data)
+    if rc:
+        raise XfError(f"xf_mesh_tets_from_stream failed ({rc}): {L.xf_mesh_error().decode()}")
+    return tets
 
 
 def ldlt_solve(A_packed, b):
