@@ -291,6 +291,17 @@ def main():
     bytes_proj = BYTES_PER_TET * T * S + (BYTES_PER_NODE * N * S if mode else 0)
     achieved = bytes_proj / (ms_proj * 1e-3) / 1e9
 
+    # HBM traffic per projection launch: calibrated FETCH_SIZE + WRITE_SIZE of the
+    # rocprofv3 --pmc passes recorded for this workload (profiles/), when the
+    # launch shape matches.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_beam96x24x24.json")
+    if os.path.exists(tpath) and not mode:
+        with open(tpath) as f:
+            rec = json.load(f)
+        if rec.get("tets_per_dispatch") == T // max(info["n_colours"], 1):
+            traffic = float(rec["traffic_bytes_per_dispatch"])
+
     out = {
         "metric": METRIC,
         "value": T * S * args.steps / el,
@@ -308,7 +319,7 @@ def main():
                                f"colours, ground + sphere", "step": "one frame",
                    "launch": {"mode": "persistent" if mode else "per-colour", "size": size}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "xf_frame_persistent" if mode else "xf_project_colour", "launches": n_launch,
                      "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
                      "bytes_per_launch": bytes_proj / max(n_launch, 1)},

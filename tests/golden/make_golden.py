@@ -15,6 +15,13 @@ Fixtures (numpy .npz, no pickles):
                    serial.
   ref_tet_kats.npz 64 random deformed tets: inputs (X, w, element constants,
                    energy, solve mode) and the reference SolveElement output.
+  ref_passes.npz   6x3x3 block (0.1 m cells, x = 0 face pinned, MINSTD
+                   tOrder): node positions/velocities after 20 substeps of
+                   GeoLinear3d::Substep with volume passes and with each
+                   Rayleigh damping type plus PBD damping (tickId from 5),
+                   for the cases listed in its `cases` array
+                   (energy, simultaneous, volume passes, rayleigh type,
+                   damping, pbd factor, amortized pbd factor).
 """
 import os
 import sys
@@ -104,10 +111,42 @@ def tet_kats(count=64, seed=7):
     np.savez_compressed(os.path.join(HERE, "ref_tet_kats.npz"), **out)
 
 
+PASS_CASES = (
+    # energy, simultaneous, volume passes, rayleigh type, damping, pbd, pbd amortized
+    (4, 1, 2, 0, 0.0, 0.0, 0.0),
+    (7, 0, 1, 0, 0.0, 0.0, 0.0),
+    (4, 1, 0, 0, 0.02, 0.0, 0.0),
+    (4, 0, 0, 1, 0.02, 0.0, 0.0),
+    (7, 1, 0, 1, 0.02, 0.0, 0.0),
+    (4, 1, 0, 2, 0.02, 0.003, 0.012),
+    (3, 1, 0, 3, 0.02, 0.003, 0.012),
+    (7, 0, 0, 3, 0.02, 0.003, 0.012),
+    (4, 1, 1, 3, 0.05, 0.003, 0.012),
+)
+
+
+def passes():
+    xyz, tets = po.tet_block(6, 3, 3, (0.1, 0.1, 0.1), (0.0, -0.15, -0.15))
+    pinned = (xyz[0::3] == 0.0).astype(np.uint8)
+    order = po.minstd_order(tets.size // 4)
+    out = dict(xyz=xyz, tets=tets, pinned=pinned, order=order, dt=np.array([DT], np.float32),
+               compliance=np.array([2e-5], np.float32), tick0=np.array([5], np.uint32),
+               cases=np.array(PASS_CASES, np.float64))
+    for k, (energy, sim, vp, rtype, damp, pbd, pbda) in enumerate(PASS_CASES):
+        rb = po.RefBody(xyz, tets, 1000.0, pinned, order)
+        st = rb.settings(2e-5, energy=int(energy), simultaneous=bool(sim), volume_passes=int(vp), damping=damp,
+                         rayleigh_type=int(rtype), pbd_damping=1.0 if pbd else 0.0, volume_pbd=pbd,
+                         amortized_volume_pbd=pbda, tick_id=5)
+        rb.substeps(st, DT, 20)
+        out[f"X{k}"], out[f"V{k}"] = rb.state()
+    np.savez_compressed(os.path.join(HERE, "ref_passes.npz"), **out)
+
+
 if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref/libxfref.so is missing: build the oracle with the reference present")
     beam12()
     tet_kats()
-    for f in ("ref_beam12.npz", "ref_tet_kats.npz"):
+    passes()
+    for f in ("ref_beam12.npz", "ref_tet_kats.npz", "ref_passes.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -80,6 +80,22 @@ def test_solve_element_kats(kats):
 
 # ------------------------------------------------------------ live reference
 
+def test_volume_and_damping_passes_match_golden():
+    """ref_passes.npz (compiled reference): volume passes, every Rayleigh
+    damping type and PBD damping over 20 substeps from tickId 5."""
+    g = np.load(os.path.join(GOLDEN, "ref_passes.npz"))
+    dt = float(g["dt"][0])
+    comp = float(g["compliance"][0])
+    for k, (energy, sim, vp, rtype, damp, pbd, pbda) in enumerate(g["cases"]):
+        ob = po.OracleBody(g["xyz"], g["tets"], 1000.0, g["pinned"], compliance=comp, variant="ref64")
+        ob.tick = int(g["tick0"][0])
+        p = po.make_params(dt, energy=int(energy), simultaneous=bool(sim), volume_passes=int(vp), compliance=comp,
+                           damping=float(np.float32(damp)), rayleigh_type=int(rtype), pbd=float(np.float32(pbd)),
+                           pbd_amortized=float(np.float32(pbda)))
+        ob.substep(p, g["order"], count=20)
+        assert np.array_equal(ob.X, g[f"X{k}"]) and np.array_equal(ob.V, g[f"V{k}"]), k
+
+
 needs_ref = pytest.mark.skipif(not po.have_ref(), reason="reference build (oracle/_ref) not present")
 
 
