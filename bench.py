@@ -108,14 +108,14 @@ def cpu_baseline(ctx, wl, budget_s=12.0):
                 sample=f"{what}: {n} substeps of the {t}-tet beam, single thread, {el:.1f} s", check=check)
 
 
-def batched_leg(ctx, frames=3):
+def batched_leg(ctx, frames=3, xcd_slabs=False):
     """256 boxes: one batched scene (colour c of all bodies per launch,
     autotuned workgroup size) against the plain per-body launch loop (one
     scene per body, stepped one after another)."""
     b = workloads.boxes_batch()
     S, dt = b["substeps"], float(b["dt"])
     T = sum(x["tets"].size // 4 for x in b["bodies"])
-    sc = workloads.make_batch_scene(ctx, b)
+    sc = workloads.make_batch_scene(ctx, b, xcd_slabs=xcd_slabs)
     mode, size, _ = sc.autotune(S, dt, reps=2)
     sc.step(S, dt)
     sc.sync()
@@ -146,7 +146,8 @@ def batched_leg(ctx, frames=3):
     return {"workload": f"{b['name']}: {len(b['bodies'])} bodies, {T} tets, {S} substeps/frame, {ncol} colours",
             "batched_tet_substeps_per_s": batched, "plain_loop_tet_substeps_per_s": plain,
             "batched_over_plain": batched / plain, "frames": frames,
-            "launch": {"mode": "persistent" if mode else "per-colour", "size": size}, "finite": finite}
+            "launch": {"mode": "persistent" if mode else "per-colour", "size": size, "xcd_slabs": xcd_slabs},
+            "finite": finite}
 
 
 def adjoint_leg(ctx, reps=3):
@@ -242,6 +243,22 @@ def copy_probe(nbytes=1 << 28, reps=4):
     print(json.dumps({"copy_probe_bytes": nbytes, "copies": reps}))
 
 
+def batch_profile(frames=1, xcd_slabs=False):
+    """One batched frame of the 256-box scene with fixed per-colour launches of
+    256 threads (short runs under rocprofv3 counter passes)."""
+    ctx = xf.Context(0)
+    b = workloads.boxes_batch()
+    sc = workloads.make_batch_scene(ctx, b, block_size=256, xcd_slabs=xcd_slabs)
+    for _ in range(frames):
+        sc.step(b["substeps"], float(b["dt"]))
+    sc.sync()
+    info = sc.info()
+    sc.close()
+    ctx.close()
+    print(json.dumps({"batch_profile": b["name"], "tets": info["n_tets"], "colours": info["n_colours"],
+                      "frames": frames, "xcd_slabs": xcd_slabs}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,9 +271,17 @@ def main():
                     help="per-colour launches of 64 threads instead of the autotuned shape (short profiler passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 4 device copies of 256 MiB (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="only run one frame of the 256-box batched scene (rocprofv3 counter passes)")
+    ap.add_argument("--no-xcd-slabs", action="store_true",
+                    help="plain blockIdx-linear schedule instead of the XCD-slab one (colour eighths per XCD)")
     args = ap.parse_args()
+    args.xcd_slabs = not args.no_xcd_slabs
     if args.copy_probe:
         copy_probe()
+        return
+    if args.batch_profile:
+        batch_profile(xcd_slabs=args.xcd_slabs)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
@@ -265,7 +290,7 @@ def main():
 
     wl = workloads.beam_yeoh_96x24x24()
     ctx = xf.Context(0)
-    sc = workloads.make_scene(ctx, wl, use_graph=True)
+    sc = workloads.make_scene(ctx, wl, use_graph=True, xcd_slabs=args.xcd_slabs)
     info = sc.info()
     T, N, S = info["n_tets"], info["n_nodes"], wl["substeps"]
     dt = float(wl["dt"])
@@ -300,7 +325,8 @@ def main():
         with open(tpath) as f:
             rec = json.load(f)
         if rec.get("tets_per_dispatch") == T // max(info["n_colours"], 1):
-            traffic = float(rec["traffic_bytes_per_dispatch"])
+            traffic = float(rec["traffic_bytes_per_dispatch"] if args.xcd_slabs else
+                            rec["plain_schedule"]["traffic_bytes_per_dispatch"])
 
     out = {
         "metric": METRIC,
@@ -317,7 +343,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{wl['name']}: {T} tets, {N} nodes, {S} substeps/frame, {info['n_colours']} "
                                f"colours, ground + sphere", "step": "one frame",
-                   "launch": {"mode": "persistent" if mode else "per-colour", "size": size}},
+                   "launch": {"mode": "persistent" if mode else "per-colour", "size": size,
+                              "xcd_slabs": args.xcd_slabs}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "xf_frame_persistent" if mode else "xf_project_colour", "launches": n_launch,
@@ -326,7 +353,7 @@ def main():
     }
     legs = []
     if not args.no_batch:
-        legs.append(("batched", batched_leg))
+        legs.append(("batched", lambda c: batched_leg(c, xcd_slabs=args.xcd_slabs)))
     if not args.no_adjoint:
         legs += [("adjoint", adjoint_leg), ("fit", fit_leg)]
     if not args.no_cpu:

@@ -157,6 +157,12 @@ typedef struct xf_scene_opts {
     uint32_t use_graph;         /* capture each per-colour xf_scene_step call shape in a hipGraph */
     uint32_t launch_mode;       /* XF_LAUNCH_* */
     uint32_t persistent_blocks; /* grid size of the persistent launch, 0 = one per compute unit */
+    uint32_t xcd_slabs;         /* 1: inside each colour, order tets by (body, centroid x) and let
+                                   workgroup b work on eighth (b mod 8) of the colour, so each of
+                                   the 8 XCDs (workgroups are dispatched round-robin over them)
+                                   sweeps one spatial slab and node lines are fetched by one L2.
+                                   Changes the sweep order inside colours (a valid serial order
+                                   still; xf_scene_get_order reports it). */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {

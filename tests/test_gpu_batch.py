@@ -97,3 +97,20 @@ def test_persistent_batch_equals_per_colour(ctx, small_batch):
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
     pc.close()
     pe.close()
+
+
+def test_xcd_slabs_batch_equals_oracle(ctx, small_batch):
+    """Batched per-colour launches with the XCD-slab mapping: every body equals
+    the oracle's sweep in its reported (slab) order."""
+    b = small_batch
+    dt, S = float(b["dt"]), b["substeps"]
+    sc = workloads.make_batch_scene(ctx, b, xcd_slabs=True, block_size=128)
+    sc.step(S, dt)
+    for i, wl in enumerate(b["bodies"]):
+        order, _ = sc.order(i)
+        ob = po.body_from_workload(wl, "ref64")
+        ob.substep(po.params_from_settings(b["dt"], wl["settings"]), order, count=S)
+        x, v = sc.get_state(i)
+        assert np.array_equal(x, ob.X), wl["name"]
+        assert np.array_equal(v, ob.V), wl["name"]
+    sc.close()

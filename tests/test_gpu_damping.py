@@ -51,6 +51,20 @@ def test_damping_bit_exact(ctx, rtype, energy, sim, mode):
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
 
 
+@pytest.mark.parametrize("rtype", [0, 2, 3])
+def test_damping_xcd_slabs_bit_exact(ctx, rtype):
+    """Post-damping colour kernels under the XCD-slab block mapping."""
+    wl = damped(7, True, rtype)
+    sc = workloads.make_scene(ctx, wl, xcd_slabs=True)
+    S = wl["substeps"]
+    sc.step(S, float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, _ = sc.order(0)
+    sc.close()
+    ob = oracle_run(wl, order, S)
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
 @pytest.mark.parametrize("kind", ["rayleigh", "pbd"])
 @pytest.mark.parametrize("rtype", [2, 3])
 def test_single_damping_pass_bit_exact(ctx, kind, rtype):

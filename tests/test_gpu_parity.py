@@ -33,9 +33,9 @@ def rel_err(x, ref, xyz0):
 
 
 def run_gpu(ctx, wl, substeps, use_graph=True, colour_order=xf.COLOUR_NATURAL, chunks=1,
-            launch_mode=xf.LAUNCH_PER_COLOUR, persistent_blocks=0):
+            launch_mode=xf.LAUNCH_PER_COLOUR, persistent_blocks=0, **opts):
     sc = workloads.make_scene(ctx, wl, use_graph=use_graph, colour_order=colour_order, launch_mode=launch_mode,
-                              persistent_blocks=persistent_blocks)
+                              persistent_blocks=persistent_blocks, **opts)
     for _ in range(chunks):
         sc.step(substeps, float(wl["dt"]))
     x, v = sc.get_state(0)
@@ -190,3 +190,58 @@ def test_user_colouring_validated_and_bit_exact(ctx):
     x, v, order = run_gpu(ctx, wl, wl["substeps"])
     ob = run_oracle(wl, order, wl["substeps"])
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def check_slab_order(sc, wl):
+    """XCD-slab schedule: still colour-major, and inside each colour the tets
+    are ordered by centroid x."""
+    order, colour = sc.order(0)
+    t = wl["tets"].reshape(-1, 4).astype(np.int64)
+    x = wl["xyz"].reshape(-1, 3)[:, 0].astype(np.float32)[t]
+    cx = ((x[:, 0] + x[:, 1]) + x[:, 2]) + x[:, 3]
+    co = colour[order]
+    assert np.all(np.diff(co.astype(np.int64)) >= 0)
+    same = co[1:] == co[:-1]
+    assert np.all(np.diff(cx[order])[same] >= 0)
+
+
+@pytest.mark.parametrize("energy,sim", [(4, True), (7, True), (4, False), (3, True)])
+@pytest.mark.parametrize("block", [64, 256])
+@pytest.mark.parametrize("passes", [0, 1])
+def test_xcd_slabs_bit_exact(ctx, energy, sim, block, passes):
+    """Per-colour launches with the XCD-slab block mapping (workgroup b on
+    eighth b mod 8 of the colour; colours of 40 tets leave eighths of 5, with
+    256-thread blocks almost all lanes idle): same results as the oracle's
+    sweep in the slab schedule's order."""
+    wl = workloads.beam_neo_12x4x4(energy=energy, simultaneous=sim)
+    wl["settings"] = dict(wl["settings"], volume_passes=passes)
+    sc = workloads.make_scene(ctx, wl, xcd_slabs=True, block_size=block)
+    check_slab_order(sc, wl)
+    sc.step(wl["substeps"], float(wl["dt"]))
+    sc.step(wl["substeps"], float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, _ = sc.order(0)
+    sc.close()
+    ob = run_oracle(wl, order, 2 * wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_xcd_slabs_big_beam_bit_exact(ctx):
+    wl = workloads.beam_yeoh_96x24x24()
+    sc = workloads.make_scene(ctx, wl, xcd_slabs=True)
+    check_slab_order(sc, wl)
+    sc.step(wl["substeps"], float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, _ = sc.order(0)
+    sc.close()
+    ob = run_oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_xcd_slabs_flag_validated(ctx):
+    wl = workloads.beam_neo_12x4x4()
+    sc = xf.Scene(ctx)
+    sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"])
+    with pytest.raises(xf.XfError):
+        sc.finalize(xcd_slabs=2)
+    sc.close()

@@ -18,6 +18,24 @@
 
 namespace {
 
+// Schedule position of this lane's tet in a colour [begin, begin + count), or
+// ~0u when the lane has none. slabs != 0: workgroup b works on eighth (b mod 8)
+// of the colour, chunk b / 8 (grid 8 * ceil(ceil(count / 8) / blockDim.x)), so
+// under round-robin dispatch each XCD sweeps the same eighth in every colour.
+__device__ __forceinline__ uint32_t colour_slot(uint32_t begin, uint32_t count, uint32_t slabs) {
+    uint32_t k;
+    if (slabs) {
+        uint32_t r = blockIdx.x & 7u, j = blockIdx.x >> 3;
+        uint32_t lo = (uint32_t)(((uint64_t)count * r) >> 3), hi = (uint32_t)(((uint64_t)count * (r + 1u)) >> 3);
+        k = lo + j * blockDim.x + threadIdx.x;
+        if (k >= hi) return 0xFFFFFFFFu;
+    } else {
+        k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= count) return 0xFFFFFFFFu;
+    }
+    return begin + k;
+}
+
 __device__ __forceinline__ float dot3(const float* a, const float* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
@@ -642,16 +660,16 @@ __device__ __forceinline__ void pbd_one(uint32_t i, const uint4* __restrict__ ti
 // ---------------------------------------------------------------- kernels
 
 // Volume pass `pass` over one colour: tets [begin, begin+count).
-__global__ __launch_bounds__(256) void xf_volume_colour(uint32_t begin, uint32_t count, uint32_t pass,
+__global__ __launch_bounds__(256) void xf_volume_colour(uint32_t begin, uint32_t count, uint32_t slabs, uint32_t pass,
                                                         const uint4* __restrict__ tidx,
                                                         const float4* __restrict__ tg0,
                                                         const float4* __restrict__ tg1,
                                                         const float4* __restrict__ tg2,
                                                         const float4* __restrict__ taux, double4* __restrict__ X,
                                                         const XfBodyParams* __restrict__ bodies, float dt) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    volume_one(begin + k, pass, tidx, tg0, tg1, tg2, taux, X, bodies, dt);
+    uint32_t i = colour_slot(begin, count, slabs);
+    if (i == 0xFFFFFFFFu) return;
+    volume_one(i, pass, tidx, tg0, tg1, tg2, taux, X, bodies, dt);
 }
 
 // Projects one colour: tets [begin, begin+count) of the sweep order.
@@ -661,37 +679,39 @@ __global__ __launch_bounds__(256) void xf_volume_colour(uint32_t begin, uint32_t
 // reads it.
 // Rayleigh post damping (Geo.cpp:797-802) of one colour.
 template <int E, bool AUX>
-__global__ __launch_bounds__(256) void xf_damp_colour(uint32_t begin, uint32_t count, const uint4* __restrict__ tidx,
+__global__ __launch_bounds__(256) void xf_damp_colour(uint32_t begin, uint32_t count, uint32_t slabs,
+                                                      const uint4* __restrict__ tidx,
                                                       const float4* __restrict__ tg0, const float4* __restrict__ tg1,
                                                       const float4* __restrict__ tg2, const float4* __restrict__ tg3,
                                                       const float4* __restrict__ taux, const float2* __restrict__ tdmp,
                                                       const double4* __restrict__ X, double4* __restrict__ V,
                                                       const XfBodyParams* __restrict__ bodies, float comp1_u,
                                                       float a_u, float dt, const float* __restrict__ clock) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
+    uint32_t i = colour_slot(begin, count, slabs);
+    if (i == 0xFFFFFFFFu) return;
     uint32_t tick = reinterpret_cast<const uint32_t*>(clock)[1] - 1u;
-    damp_one<E, AUX>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, tdmp, X, V, bodies, comp1_u, a_u, dt, tick);
+    damp_one<E, AUX>(i, tidx, tg0, tg1, tg2, tg3, taux, tdmp, X, V, bodies, comp1_u, a_u, dt, tick);
 }
 
 // PBD damping (Geo.cpp:806-809) of one colour.
 template <bool AUX>
-__global__ __launch_bounds__(256) void xf_pbd_colour(uint32_t begin, uint32_t count, const uint4* __restrict__ tidx,
+__global__ __launch_bounds__(256) void xf_pbd_colour(uint32_t begin, uint32_t count, uint32_t slabs,
+                                                     const uint4* __restrict__ tidx,
                                                      const float4* __restrict__ taux, const float2* __restrict__ tdmp,
                                                      const double4* __restrict__ X, double4* __restrict__ V,
                                                      const XfBodyParams* __restrict__ bodies,
                                                      const float* __restrict__ clock) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
+    uint32_t i = colour_slot(begin, count, slabs);
+    if (i == 0xFFFFFFFFu) return;
     uint32_t tick = reinterpret_cast<const uint32_t*>(clock)[1] - 1u;
-    pbd_one<AUX>(begin + k, tidx, taux, tdmp, X, V, bodies, tick);
+    pbd_one<AUX>(i, tidx, taux, tdmp, X, V, bodies, tick);
 }
 
 // DAMP: bodies may damp inside the projection (Rayleigh_Paper / _Limit);
 // reads O of the 4 nodes.
 template <int E, bool SIM, bool AUX, bool DAMP>
 __global__ __launch_bounds__(256) void xf_project_colour(
-    uint32_t begin, uint32_t count, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+    uint32_t begin, uint32_t count, uint32_t slabs, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
     const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
     const float4* __restrict__ taux, double4* __restrict__ X, const double4* __restrict__ O,
     const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock) {
@@ -700,9 +720,8 @@ __global__ __launch_bounds__(256) void xf_project_colour(
         uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
         tick[0] = tick[0] + 1u;
     }
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    uint32_t i = begin + k;
+    uint32_t i = colour_slot(begin, count, slabs);
+    if (i == 0xFFFFFFFFu) return;
 
     uint4 id = tidx[i];
     float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i], q3 = tg3[i];
@@ -1127,9 +1146,18 @@ __global__ __launch_bounds__(256) void xf_count_nonfinite(uint32_t n, const doub
 
 // ------------------------------------------------------ launch wrappers
 
-typedef void (*XfProjectFn)(uint32_t, uint32_t, const uint4*, const float4*, const float4*, const float4*,
-                            const float4*, const float4*, double4*, const double4*, const XfBodyParams*, float, float,
-                            float, float*);
+typedef void (*XfProjectFn)(uint32_t, uint32_t, uint32_t, const uint4*, const float4*, const float4*, const float4*,
+                            const float4*, const float4*, double4*, const double4*, const XfBodyParams*, float, float, float,
+                            float*);
+
+namespace {
+// Grid of a per-colour launch (see colour_slot).
+uint32_t colour_grid(uint32_t count, uint32_t block, uint32_t slabs) {
+    if (!slabs) return (count + block - 1) / block;
+    uint32_t eighth = (count + 7) / 8;
+    return 8 * ((eighth + block - 1) / block);
+}
+}  // namespace
 
 namespace {
 template <int E, bool SIM>
@@ -1147,7 +1175,8 @@ XfProjectFn pick_aux(bool sim, bool aux, bool damp) {
 // per-body MixedSel/YeohSkinFast dispatch. damp: some body damps inside
 // the projection (O is then read).
 extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t block,
-                                        uint32_t begin, uint32_t count, const uint4* tidx, const float4* tg0,
+                                        uint32_t slabs, uint32_t begin, uint32_t count, const uint4* tidx,
+                                        const float4* tg0,
                                         const float4* tg1, const float4* tg2, const float4* tg3, const float4* taux,
                                         double4* X, const double4* O, const XfBodyParams* bodies, float comp1_u,
                                         float a_u, float dt, float* clock) {
@@ -1162,15 +1191,14 @@ extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, boo
         default: return hipErrorInvalidValue;
     }
     if (count == 0) return hipSuccess;
-    uint32_t grid = (count + block - 1) / block;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, s, begin, count, tidx, tg0, tg1, tg2, tg3, taux, X, O, bodies,
-                       comp1_u, a_u, dt, clock);
+    hipLaunchKernelGGL(fn, dim3(colour_grid(count, block, slabs)), dim3(block), 0, s, begin, count, slabs, tidx, tg0,
+                       tg1, tg2, tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock);
     return hipGetLastError();
 }
 
-typedef void (*XfDampFn)(uint32_t, uint32_t, const uint4*, const float4*, const float4*, const float4*,
-                         const float4*, const float4*, const float2*, const double4*, double4*, const XfBodyParams*,
-                         float, float, float, const float*);
+typedef void (*XfDampFn)(uint32_t, uint32_t, uint32_t, const uint4*, const float4*, const float4*, const float4*,
+                         const float4*, const float4*, const float2*, const double4*, double4*, const XfBodyParams*, float, float,
+                         float, const float*);
 
 namespace {
 template <int E>
@@ -1180,8 +1208,8 @@ XfDampFn pick_damp(bool aux) {
 }  // namespace
 
 // Rayleigh post damping of one colour (bodies with type Post/PostAmortized).
-extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32_t block, uint32_t begin,
-                                     uint32_t count, const uint4* tidx, const float4* tg0, const float4* tg1,
+extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32_t block, uint32_t slabs,
+                                     uint32_t begin, uint32_t count, const uint4* tidx, const float4* tg0, const float4* tg1,
                                      const float4* tg2, const float4* tg3, const float4* taux, const float2* tdmp,
                                      const double4* X, double4* V, const XfBodyParams* bodies, float comp1_u,
                                      float a_u, float dt, const float* clock) {
@@ -1196,31 +1224,34 @@ extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32
         default: return hipErrorInvalidValue;
     }
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(fn, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count, tidx, tg0, tg1, tg2,
-                       tg3, taux, tdmp, X, V, bodies, comp1_u, a_u, dt, clock);
+    hipLaunchKernelGGL(fn, dim3(colour_grid(count, block, slabs)), dim3(block), 0, s, begin, count, slabs, tidx, tg0,
+                       tg1, tg2, tg3, taux, tdmp, X, V, bodies, comp1_u, a_u, dt, clock);
     return hipGetLastError();
 }
 
 // PBD damping of one colour.
-extern "C" hipError_t xf_launch_pbd(hipStream_t s, bool aux, uint32_t block, uint32_t begin, uint32_t count,
+extern "C" hipError_t xf_launch_pbd(hipStream_t s, bool aux, uint32_t block, uint32_t slabs, uint32_t begin,
+                                    uint32_t count,
                                     const uint4* tidx, const float4* taux, const float2* tdmp, const double4* X,
                                     double4* V, const XfBodyParams* bodies, const float* clock) {
     if (count == 0) return hipSuccess;
+    uint32_t grid = colour_grid(count, block, slabs);
     if (aux)
-        hipLaunchKernelGGL(xf_pbd_colour<true>, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count,
-                           tidx, taux, tdmp, X, V, bodies, clock);
+        hipLaunchKernelGGL(xf_pbd_colour<true>, dim3(grid), dim3(block), 0, s, begin, count, slabs, tidx, taux, tdmp,
+                           X, V, bodies, clock);
     else
-        hipLaunchKernelGGL(xf_pbd_colour<false>, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count,
-                           tidx, taux, tdmp, X, V, bodies, clock);
+        hipLaunchKernelGGL(xf_pbd_colour<false>, dim3(grid), dim3(block), 0, s, begin, count, slabs, tidx, taux, tdmp,
+                           X, V, bodies, clock);
     return hipGetLastError();
 }
 
-extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t begin, uint32_t count, uint32_t pass,
+extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t slabs, uint32_t begin, uint32_t count,
+                                       uint32_t pass,
                                        const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
                                        const float4* taux, double4* X, const XfBodyParams* bodies, float dt) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(xf_volume_colour, dim3((count + block - 1) / block), dim3(block), 0, s, begin, count, pass, tidx,
-                       tg0, tg1, tg2, taux, X, bodies, dt);
+    hipLaunchKernelGGL(xf_volume_colour, dim3(colour_grid(count, block, slabs)), dim3(block), 0, s, begin, count,
+                       slabs, pass, tidx, tg0, tg1, tg2, taux, X, bodies, dt);
     return hipGetLastError();
 }
 

@@ -19,19 +19,22 @@
 #include "xf_host_util.h"
 
 extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t block,
-                                        uint32_t begin, uint32_t count, const uint4* tidx, const float4* tg0,
+                                        uint32_t slabs, uint32_t begin, uint32_t count, const uint4* tidx,
+                                        const float4* tg0,
                                         const float4* tg1, const float4* tg2, const float4* tg3, const float4* taux,
                                         double4* X, const double4* O, const XfBodyParams* bodies, float comp1_u,
                                         float a_u, float dt, float* clock);
-extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32_t block, uint32_t begin,
-                                     uint32_t count, const uint4* tidx, const float4* tg0, const float4* tg1,
+extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32_t block, uint32_t slabs,
+                                     uint32_t begin, uint32_t count, const uint4* tidx, const float4* tg0, const float4* tg1,
                                      const float4* tg2, const float4* tg3, const float4* taux, const float2* tdmp,
                                      const double4* X, double4* V, const XfBodyParams* bodies, float comp1_u,
                                      float a_u, float dt, const float* clock);
-extern "C" hipError_t xf_launch_pbd(hipStream_t s, bool aux, uint32_t block, uint32_t begin, uint32_t count,
+extern "C" hipError_t xf_launch_pbd(hipStream_t s, bool aux, uint32_t block, uint32_t slabs, uint32_t begin,
+                                    uint32_t count,
                                     const uint4* tidx, const float4* taux, const float2* tdmp, const double4* X,
                                     double4* V, const XfBodyParams* bodies, const float* clock);
-extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t begin, uint32_t count, uint32_t pass,
+extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t slabs, uint32_t begin, uint32_t count,
+                                       uint32_t pass,
                                        const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
                                        const float4* taux, double4* X, const XfBodyParams* bodies, float dt);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
@@ -312,6 +315,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
     xf_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     uint32_t block = s->opts.block_size ? s->opts.block_size : 64;
+    uint32_t slabs = s->opts.xcd_slabs;
     auto node = [&](int kind, bool predict_next) -> int {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timer) {
@@ -363,7 +367,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                 XF_HIP(ctx, hipEventCreate(&e1));
                 XF_HIP(ctx, hipEventRecord(e0, st));
             }
-            XF_HIP(ctx, xf_launch_project(st, energy, sim, s->aux, s->dproj, block, begin, count, s->tidx, s->tg[0],
+            XF_HIP(ctx, xf_launch_project(st, energy, sim, s->aux, s->dproj, block, slabs, begin, count, s->tidx, s->tg[0],
                                           s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O, s->bp, s->comp1_u,
                                           s->a_u, dt, c == 0 ? s->clock : nullptr));
             if (timer) {
@@ -376,7 +380,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
         for (uint32_t pass = 0; pass < s->vpasses; pass++)
             for (uint32_t c = 0; c < s->ncol; c++) {
                 uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_volume(st, block, begin, count, pass, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                XF_HIP(ctx, xf_launch_volume(st, block, slabs, begin, count, pass, s->tidx, s->tg[0], s->tg[1], s->tg[2],
                                              s->taux, s->X, s->bp, dt));
             }
         if (s->post == 0) {
@@ -391,14 +395,14 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
         if (s->post & 1u)
             for (uint32_t c = 0; c < s->ncol; c++) {
                 uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_damp(st, energy, s->aux, block, begin, count, s->tidx, s->tg[0], s->tg[1],
+                XF_HIP(ctx, xf_launch_damp(st, energy, s->aux, block, slabs, begin, count, s->tidx, s->tg[0], s->tg[1],
                                            s->tg[2], s->tg[3], s->taux, s->tdmp, s->X, s->V, s->bp, s->comp1_u,
                                            s->a_u, dt, s->clock));
             }
         if (s->post & 2u)
             for (uint32_t c = 0; c < s->ncol; c++) {
                 uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_pbd(st, s->aux, block, begin, count, s->tidx, s->taux, s->tdmp, s->X, s->V,
+                XF_HIP(ctx, xf_launch_pbd(st, s->aux, block, slabs, begin, count, s->tidx, s->taux, s->tdmp, s->X, s->V,
                                           s->bp, s->clock));
             }
         if (k + 1 < substeps) {
@@ -607,6 +611,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         return Fail(ctx, XF_E_INVALID, "block_size must be 64, 128, 192 or 256");
     if (s->opts.colour_order > XF_COLOUR_MINSTD) return Fail(ctx, XF_E_INVALID, "unknown colour order");
     if (s->opts.launch_mode > XF_LAUNCH_PERSISTENT) return Fail(ctx, XF_E_INVALID, "unknown launch mode");
+    if (s->opts.xcd_slabs > 1) return Fail(ctx, XF_E_INVALID, "xcd_slabs must be 0 or 1");
     XF_HIP(ctx, hipSetDevice(ctx->device));
 
     // Colour each body; the scene runs max(ncol) phases.
@@ -627,6 +632,19 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         if (b.ncol == 0) return Fail(ctx, XF_E_UNSUPPORTED, "mesh needs more than %u colours", xf::kMaxColours);
         s->ncol = std::max(s->ncol, b.ncol);
     }
+    // XCD slabs: inside each colour, tets by centroid x (ties keep the order), so
+    // that eighths of a colour's schedule are spatial slabs.
+    if (s->opts.xcd_slabs)
+        for (Body& b : s->bodies) {
+            std::vector<float> cx(b.t);
+            for (uint32_t t = 0; t < b.t; t++) {
+                const uint32_t* is = &b.tets[4 * (size_t)t];
+                cx[t] = b.xyz[3 * is[0]] + b.xyz[3 * is[1]] + b.xyz[3 * is[2]] + b.xyz[3 * is[3]];
+            }
+            std::stable_sort(b.order.begin(), b.order.end(), [&](uint32_t a, uint32_t c) {
+                return b.colour[a] != b.colour[c] ? b.colour[a] < b.colour[c] : cx[a] < cx[c];
+            });
+        }
     // Schedule: colour-major, body-major within a colour, each body's tets
     // of that colour in its sweep order.
     s->colour_begin.assign(s->ncol + 1, 0);
