@@ -44,6 +44,18 @@ class XoParams(C.Structure):
         ("pbd", C.c_float),
         ("pbd_amortized", C.c_float),
         ("tick", C.c_uint32),
+        ("poisson", C.c_float),
+    ]
+
+
+class XoRings(C.Structure):
+    _fields_ = [
+        ("n_rings", C.c_uint32),
+        ("node_off", C.c_void_p),
+        ("nodes", C.c_void_p),
+        ("tet_off", C.c_void_p),
+        ("tets", C.c_void_p),
+        ("idx", C.c_void_p),
     ]
 
 
@@ -70,7 +82,7 @@ SKIN_YEOH = (0.1095, 14.95, 4.595)  # Fem.cpp:526
 
 def make_params(dt, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True, yeoh=SKIN_YEOH, collide=0,
                 ground_y=0.0, friction=0.0, sphere_c0=(0, 0, 0), sphere_v=(0, 0, 0), sphere_r=0.0, volume_passes=0,
-                compliance=0.0, damping=0.0, rayleigh_type=0, pbd=0.0, pbd_amortized=0.0):
+                compliance=0.0, damping=0.0, rayleigh_type=0, pbd=0.0, pbd_amortized=0.0, poisson=0.5):
     p = XoParams()
     p.dt = dt
     p.gravity[:] = gravity
@@ -90,6 +102,7 @@ def make_params(dt, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True,
     p.rayleigh_type = rayleigh_type
     p.pbd = pbd
     p.pbd_amortized = pbd_amortized
+    p.poisson = poisson
     return p
 
 
@@ -104,6 +117,7 @@ def lib():
         L = C.CDLL(ORACLE_SO)
         L.xo_tet_block.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, C.c_int, _f32p, _u32p]
         L.xo_minstd_order.argtypes = [C.c_uint32, _u32p]
+        L.xo_minstd_ring_order.argtypes = [C.c_uint32, C.c_uint32, _u32p]
         L.xo_init_elements.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float, _f32p, _f32p, _f32p, _f32p,
                                        _f32p]
         L.xo_material.argtypes = [C.c_uint32, _f32p, C.c_float, C.c_void_p, C.c_float, _f32p, _f32p, _f32p]
@@ -132,6 +146,14 @@ def lib():
             fn.argtypes = [C.POINTER(XoParams), C.c_float, sp, sp]
             fn = getattr(L, "xo_solve_tet" + sfx)
             fn.argtypes = [C.POINTER(XoParams), sp, _f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
+            fn = getattr(L, "xo_substep_rings" + sfx)
+            fn.argtypes = [C.POINTER(XoParams), C.c_uint32, sp, sp, sp, _f32p, C.c_void_p, C.POINTER(XoRings), _u32p,
+                           C.c_uint32, _u32p, _u32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+            fn = getattr(L, "xo_ring_ug" + sfx)
+            fn.argtypes = [sp, C.POINTER(XoRings), C.c_uint32, _f32p, _f32p, _f32p, _f64p, _f64p, _f64p]
+        L.xo_build_rings.argtypes = [C.c_uint32, _u32p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.xo_build_rings.restype = C.c_uint32
         L.xo_ldlt_solve.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
         L.xo_surface_area.argtypes = [_f32p, _u32p, C.c_uint32, _f32p]
         _lib = L
@@ -176,6 +198,11 @@ def ref():
         L.xr_ldlt.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
         L.xr_volume.argtypes = [C.c_void_p]
         L.xr_volume.restype = C.c_float
+        L.xr_ring_count.argtypes = [C.c_void_p]
+        L.xr_ring_count.restype = C.c_uint32
+        L.xr_get_vorder.argtypes = [C.c_void_p, _u32p]
+        L.xr_set_vorder.argtypes = [C.c_void_p, _u32p]
+        L.xr_get_ring.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _ref = L
     return _ref
 
@@ -194,6 +221,13 @@ def tet_block(W, H, D, scale, origin, mirrored=False):
 def minstd_order(n):
     o = np.zeros(n, np.uint32)
     lib().xo_minstd_order(n, o)
+    return o
+
+
+def minstd_ring_order(n_tets, n_rings):
+    """The reference's vOrder for a mesh of n_tets tets and n_rings vertices."""
+    o = np.zeros(n_rings, np.uint32)
+    lib().xo_minstd_ring_order(n_tets, n_rings, o)
     return o
 
 
@@ -229,6 +263,35 @@ def colour_greedy(n_nodes, tets, visit):
     return nc, col
 
 
+class Rings:
+    """Vertex rings of a tet mesh (xo_build_rings: the reference's OneRing
+    construction) in CSR arrays, plus the XoRings view the oracle takes."""
+
+    def __init__(self, tets):
+        tets = np.ascontiguousarray(tets, np.uint32)
+        t = tets.size // 4
+        nr, tot = C.c_uint32(), C.c_uint32()
+        if lib().xo_build_rings(t, tets, C.byref(nr), C.byref(tot), None, None, None, None, None) == 0:
+            raise ValueError("a vertex ring has more than 256 nodes")
+        self.n_rings = nr.value
+        self.node_off = np.zeros(self.n_rings + 1, np.uint32)
+        self.nodes = np.zeros(max(tot.value, 1), np.uint32)
+        self.tet_off = np.zeros(self.n_rings + 1, np.uint32)
+        self.tets = np.zeros(max(4 * t, 1), np.uint32)
+        self.idx = np.zeros(max(16 * t, 1), np.uint8)
+        self.largest = lib().xo_build_rings(t, tets, C.byref(nr), C.byref(tot), self.node_off.ctypes.data,
+                                            self.nodes.ctypes.data, self.tet_off.ctypes.data, self.tets.ctypes.data,
+                                            self.idx.ctypes.data)
+        self.view = XoRings(self.n_rings, self.node_off.ctypes.data, self.nodes.ctypes.data,
+                            self.tet_off.ctypes.data, self.tets.ctypes.data, self.idx.ctypes.data)
+
+    def ring(self, v):
+        """(nodes, tets, idx[k, 4]) of ring v."""
+        a, b = self.node_off[v], self.node_off[v + 1]
+        c, d = self.tet_off[v], self.tet_off[v + 1]
+        return self.nodes[a:b].copy(), self.tets[c:d].copy(), self.idx[4 * c:4 * d].reshape(-1, 4).copy()
+
+
 class OracleBody:
     """Node state + element constants of one body for the oracle substep."""
 
@@ -257,18 +320,33 @@ class OracleBody:
         self.area = np.zeros(self.t, np.float32)
         lib().xo_surface_area(np.ascontiguousarray(xyz, np.float32), self.tets, self.t, self.area)
 
-    def substep(self, params, order, count=1):
+    def substep(self, params, order, count=1, ring_order=None):
         """`count` substeps; the collider clock advances by dt in float32 per
-        substep (as the device clock does) and the substep sees its end time."""
-        fn = getattr(lib(), "xo_substep_" + self.variant)
+        substep (as the device clock does) and the substep sees its end time.
+        Vertex-ring energies (params.energy 8, 9, 10) sweep rings in
+        ring_order; `order` is then the tet order of the PBD damping pass."""
+        rings = params.energy >= 8
+        if rings:
+            if ring_order is None:
+                raise ValueError("vertex-ring energies need a ring order")
+            if getattr(self, "rings", None) is None:
+                self.rings = Rings(self.tets)
+            fn = getattr(lib(), "xo_substep_rings_" + self.variant)
+            ring_order = np.ascontiguousarray(ring_order, np.uint32)
+        else:
+            fn = getattr(lib(), "xo_substep_" + self.variant)
         pin = self.pinned.ctypes.data if self.pinned is not None else None
         order = np.ascontiguousarray(order, np.uint32)
         dt = np.float32(params.dt)
         for _ in range(count):
             self.clock = np.float32(self.clock + dt)
             params.tick = self.tick
-            fn(C.byref(params), self.n, self.X, self.O, self.V, self.w, pin, self.t, order, self.tets, self.qi,
-               self.qq, self.qr, self.c0, self.c1, self.a, self.vol, self.area, float(self.clock))
+            if rings:
+                fn(C.byref(params), self.n, self.X, self.O, self.V, self.w, pin, C.byref(self.rings.view),
+                   ring_order, self.t, order, self.tets, self.qq, self.qr, self.vol, self.area, float(self.clock))
+            else:
+                fn(C.byref(params), self.n, self.X, self.O, self.V, self.w, pin, self.t, order, self.tets, self.qi,
+                   self.qq, self.qr, self.c0, self.c1, self.a, self.vol, self.area, float(self.clock))
             self.tick += 1
 
 
@@ -282,7 +360,8 @@ def params_from_settings(dt, settings):
                        sphere_v=s.get("sphere_v", (0, 0, 0)), sphere_r=s.get("sphere_r", 0.0),
                        volume_passes=s.get("volume_passes", 0), compliance=s.get("compliance", 0.0) or 0.0,
                        damping=s.get("damping", 0.0), rayleigh_type=s.get("rayleigh_type", 0),
-                       pbd=s.get("pbd_damping", 0.0), pbd_amortized=s.get("pbd_damping_amortized", 0.0))
+                       pbd=s.get("pbd_damping", 0.0), pbd_amortized=s.get("pbd_damping_amortized", 0.0),
+                       poisson=s.get("poissons_ratio", 0.5))
 
 
 def body_from_workload(wl, variant="f32"):
@@ -372,6 +451,25 @@ class RefBody:
         o = np.zeros(self.t, np.uint32)
         ref().xr_get_order(self.h, o)
         return o
+
+    def ring_order(self):
+        """GeoLinear3d::vOrder (one entry per ring)."""
+        o = np.zeros(ref().xr_ring_count(self.h), np.uint32)
+        ref().xr_get_vorder(self.h, o)
+        return o
+
+    def set_ring_order(self, order):
+        ref().xr_set_vorder(self.h, np.ascontiguousarray(order, np.uint32))
+
+    def ring(self, v):
+        """(nodes, tets, idx[k, 4]) of the reference's OneRing of vertex v."""
+        nn, nt = C.c_uint32(), C.c_uint32()
+        ref().xr_get_ring(self.h, v, C.byref(nn), C.byref(nt), None, None, None)
+        nodes = np.zeros(nn.value, np.uint32)
+        tets = np.zeros(nt.value, np.uint32)
+        idx = np.zeros(4 * nt.value, np.uint16)
+        ref().xr_get_ring(self.h, v, C.byref(nn), C.byref(nt), nodes.ctypes.data, tets.ctypes.data, idx.ctypes.data)
+        return nodes, tets, idx.reshape(-1, 4)
 
 
 def rollout_adjoint(body, params, order, n_sub, loss_at, targets, window=20):

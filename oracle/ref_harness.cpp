@@ -6,7 +6,7 @@
 // bench.py drive the reference's own GeoLinear3d on the same mesh and the
 // same sweep order as the GPU path. It calls only the reference's public
 // functions and writes only the public members the reference exposes
-// (tOrder, flags, w, X, V; Geo.h:61-66, 161).
+// (tOrder, vOrder, flags, w, X, V; Geo.h:61-66, 161-162).
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -219,5 +219,31 @@ void xr_ldlt(uint32_t n, const float* A, const float* b, float* x) {
 
 // GeoLinear3d::CalculateVolume (Geo.cpp:827-832)
 float xr_volume(void* h) { return ((RefBody*)h)->geo->CalculateVolume(); }
+
+// Vertex rings (GeoLinear3d::rings and vOrder, Geo.h:158-162; one per
+// Connectivity3d vertex).
+uint32_t xr_ring_count(void* h) { return ((RefBody*)h)->geo->con.vertCount; }
+
+void xr_get_vorder(void* h, uint32_t* order) {
+    GeoLinear3d* g = ((RefBody*)h)->geo;
+    memcpy(order, g->vOrder, sizeof(uint32_t) * g->con.vertCount);
+}
+
+void xr_set_vorder(void* h, const uint32_t* order) {
+    GeoLinear3d* g = ((RefBody*)h)->geo;
+    memcpy(g->vOrder, order, sizeof(uint32_t) * g->con.vertCount);
+}
+
+// OneRing of vertex v: counts always; nodes (Verts), tets (Polys) and the
+// ring-local node indices (Idxs, 4 per tet) where the pointers are set.
+void xr_get_ring(void* h, uint32_t v, uint32_t* n_nodes, uint32_t* n_tets, uint32_t* nodes, uint32_t* tets,
+                 uint16_t* idx) {
+    const OneRing* r = ((RefBody*)h)->geo->rings[v];
+    *n_nodes = r->vertCount;
+    *n_tets = r->polyCount;
+    if (nodes) memcpy(nodes, r->Verts(), sizeof(uint32_t) * r->vertCount);
+    if (tets) memcpy(tets, r->Polys(), sizeof(uint32_t) * r->polyCount);
+    if (idx) memcpy(idx, r->Idxs(), sizeof(uint16_t) * r->idxCount);
+}
 
 }  // extern "C"

@@ -15,6 +15,7 @@
  *   xo_tet_block      == reference GenerateTetBlock, bit-exact (test_oracle_ref)
  *                        + closed-form counts and volumes (test_oracle)
  *   xo_minstd_order   == reference GeoLinear3d::Init tOrder, exact
+ *   xo_minstd_ring_order == reference GeoLinear3d::Init vOrder, exact
  *   xo_init_elements  == reference InitFiniteElement Qi/QQ/QR/volume and
  *                        w = 1/m, bit-exact; Qi*Ds^T = I and
  *                        sum(m) = rho*V closed forms
@@ -44,6 +45,18 @@
  *                        (ground/sphere non-penetration, static and kinetic
  *                        friction cases)
  *   xo_ldlt_solve     == Ldlt.h usage example answer x = (-3, 3, 1)
+ *   xo_build_rings    == reference OneRing arrays (Verts, Polys, Idxs) of
+ *                        every vertex, exact (golden fixture; live on block,
+ *                        mirrored block and the Armadillo mesh)
+ *   xo_substep_rings_ref64
+ *                     == reference Geo3d::Substep with ContinuousPixar /
+ *                        ContinuousMixed / ContinuousSkin, bit-exact (golden
+ *                        fixture over every Rayleigh type and PBD; live with
+ *                        a caller-set vOrder)
+ *   xo_ring_ug        FP64: at rest I1 = 3, J = 1; under x -> s x, 3 s^2 and
+ *                        s^3; rotation invariant; gradients == central
+ *                        differences; gradient sum and torque vanish; a free
+ *                        body keeps zero linear momentum
  */
 #include <math.h>
 #include <stdint.h>
@@ -70,7 +83,21 @@ typedef struct xo_params {
     float pbd;          /* Settings::volumeAndTimeCorrectedPbdDamping (pass on when > 0) */
     float pbd_amortized;/* Settings::amortizedVolumeAndTimeCorrectedPbdDamping */
     uint32_t tick;      /* Settings::tickId of the substep */
+    float poisson;      /* Settings::poissonsRatio, used by the vertex-ring energies (Geo.cpp:21-42) */
 } xo_params;
+
+/* One-ring connectivity of every vertex (OneRing, Connectivity.h:106-155),
+ * in CSR form: ring v has nodes nodes[node_off[v] .. node_off[v+1]) (ring
+ * v's own vertex first) and tets tets[tet_off[v] .. tet_off[v+1]); idx holds
+ * four ring-local node indices per ring tet. */
+typedef struct xo_rings {
+    uint32_t n_rings;
+    const uint32_t* node_off;
+    const uint32_t* nodes;
+    const uint32_t* tet_off;
+    const uint32_t* tets;
+    const uint8_t* idx;
+} xo_rings;
 
 /* ---------------------------------------------------------------- mesh */
 
@@ -121,6 +148,22 @@ void xo_minstd_order(uint32_t n, uint32_t* order) {
     for (uint32_t i = 0; i < n; i++) {
         s = (uint32_t)(((uint64_t)s * 48271u) % 0x7fffffffu);
         uint32_t j = s % n;
+        uint32_t c = order[i];
+        order[i] = order[j];
+        order[j] = c;
+    }
+}
+
+/* GeoLinear3d::Init's vOrder (Geo.cpp:759-771): the same MINSTD state
+ * continues from the tOrder shuffle (n_tets draws; there are no hexes), then
+ * shuffles the n_verts vertex ids. */
+void xo_minstd_ring_order(uint32_t n_tets, uint32_t n_verts, uint32_t* order) {
+    uint32_t s = 1;
+    for (uint32_t i = 0; i < n_tets; i++) s = (uint32_t)(((uint64_t)s * 48271u) % 0x7fffffffu);
+    for (uint32_t i = 0; i < n_verts; i++) order[i] = i;
+    for (uint32_t i = 0; i < n_verts; i++) {
+        s = (uint32_t)(((uint64_t)s * 48271u) % 0x7fffffffu);
+        uint32_t j = s % n_verts;
         uint32_t c = order[i];
         order[i] = order[j];
         order[j] = c;
@@ -278,6 +321,68 @@ uint32_t xo_colour_greedy(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
     }
     free(mask);
     return ncol;
+}
+
+/* ---------------------------------------------------------------- rings */
+
+/* Vertex rings of a linear-tet mesh as GeoLinear3d::Init builds them
+ * (Geo.cpp:745-757). Connectivity3d::Init lists the tets of each vertex in
+ * increasing tet index (Connectivity.cpp:300-312, 436-440); vertices run
+ * 0 .. max index (Connectivity.cpp:270). OneRingConstructor::AddPoly
+ * (Connectivity.h:120-134) appends each tet's nodes, in tet order, the first
+ * time they appear, starting from the ring's own vertex.
+ * Call with node_off == NULL for the sizes: *n_rings and *node_total (ring
+ * tets always total 4 * n_tets). Returns the largest ring's node count, or 0
+ * when a ring would exceed 256 nodes (kMaxRingNodes, Geo.cpp:10). */
+uint32_t xo_build_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_rings, uint32_t* node_total,
+                        uint32_t* node_off, uint32_t* nodes, uint32_t* tet_off, uint32_t* ring_tets, uint8_t* idx) {
+    uint32_t nv = 0;
+    for (uint32_t i = 0; i < 4 * n_tets; i++)
+        if (tets[i] + 1 > nv) nv = tets[i] + 1;
+    uint32_t* cnt = (uint32_t*)calloc((size_t)nv + 1, sizeof(uint32_t));
+    for (uint32_t i = 0; i < 4 * n_tets; i++) cnt[tets[i] + 1]++;
+    for (uint32_t v = 0; v < nv; v++) cnt[v + 1] += cnt[v];
+    uint32_t* list = (uint32_t*)malloc(sizeof(uint32_t) * (4 * (size_t)n_tets + 1));
+    uint32_t* fill = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)nv + 1));
+    memcpy(fill, cnt, sizeof(uint32_t) * ((size_t)nv + 1));
+    for (uint32_t t = 0; t < n_tets; t++)
+        for (int j = 0; j < 4; j++) list[fill[tets[4 * t + j]]++] = t;
+    uint32_t verts[256], total = 0, largest = 0;
+    for (uint32_t v = 0; v < nv; v++) {
+        uint32_t nn = 1;
+        verts[0] = v;
+        if (node_off) {
+            node_off[v] = total;
+            tet_off[v] = cnt[v];
+        }
+        for (uint32_t k = cnt[v]; k < cnt[v + 1]; k++) {
+            uint32_t t = list[k];
+            if (node_off) ring_tets[k] = t;
+            for (int j = 0; j < 4; j++) {
+                uint32_t node = tets[4 * t + j], li = nn;
+                for (uint32_t m = 0; m < nn; m++)
+                    if (verts[m] == node) { li = m; break; }
+                if (li == nn) {
+                    if (nn == 256) { free(cnt); free(list); free(fill); return 0; }
+                    verts[nn++] = node;
+                }
+                if (node_off) idx[4 * (size_t)k + j] = (uint8_t)li;
+            }
+        }
+        if (node_off) memcpy(nodes + total, verts, sizeof(uint32_t) * nn);
+        total += nn;
+        if (nn > largest) largest = nn;
+    }
+    if (node_off) {
+        node_off[nv] = total;
+        tet_off[nv] = cnt[nv];
+    }
+    *n_rings = nv;
+    *node_total = total;
+    free(cnt);
+    free(list);
+    free(fill);
+    return largest;
 }
 
 /* --------------------------------------------------------------- substeps */

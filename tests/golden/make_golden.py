@@ -22,6 +22,14 @@ Fixtures (numpy .npz, no pickles):
                    for the cases listed in its `cases` array
                    (energy, simultaneous, volume passes, rayleigh type,
                    damping, pbd factor, amortized pbd factor).
+  ref_rings.npz    the same 6x3x3 block with the vertex-ring energies
+                   (ContinuousPixar 8, ContinuousMixed 9, ContinuousSkin 10):
+                   the reference's OneRing arrays (CSR: ring_node_off,
+                   ring_nodes, ring_tet_off, ring_tets, ring_idx), its MINSTD
+                   vOrder and tOrder, and node positions/velocities after 20
+                   substeps (tickId from 5) for the cases of its `cases` array
+                   (energy, Poisson ratio, compliance, rayleigh type, damping,
+                   pbd factor, amortized pbd factor).
 """
 import os
 import sys
@@ -142,11 +150,54 @@ def passes():
     np.savez_compressed(os.path.join(HERE, "ref_passes.npz"), **out)
 
 
+RING_CASES = (
+    # energy, nu, compliance, rayleigh type, damping, pbd, pbd amortized
+    (8, 0.45, 1e-6, 0, 0.0, 0.0, 0.0),
+    (8, 0.3, 1e-6, 0, 0.02, 0.0, 0.0),
+    (8, 0.45, 1e-6, 3, 0.02, 0.003, 0.012),
+    (9, 0.45, 2e-5, 0, 0.0, 0.0, 0.0),
+    (9, 0.5, 2e-5, 1, 0.02, 0.0, 0.0),
+    (9, 0.45, 2e-5, 2, 0.02, 0.003, 0.012),
+    (10, 0.45, 2e-5, 0, 0.0, 0.0, 0.0),
+    (10, 0.45, 2e-5, 0, 0.02, 0.0, 0.0),
+    (10, 0.4, 2e-5, 2, 0.05, 0.0, 0.0),
+    (10, 0.45, 2e-5, 3, 0.02, 0.003, 0.012),
+)
+
+
+def rings():
+    xyz, tets = po.tet_block(6, 3, 3, (0.1, 0.1, 0.1), (0.0, -0.15, -0.15))
+    pinned = (xyz[0::3] == 0.0).astype(np.uint8)
+    rb = po.RefBody(xyz, tets, 1000.0, pinned)
+    n_rings = rb.ring_order().size
+    node_off, nodes, tet_off, rtets, idx = [0], [], [0], [], []
+    for v in range(n_rings):
+        a, b, c = rb.ring(v)
+        nodes.append(a)
+        rtets.append(b)
+        idx.append(c.reshape(-1))
+        node_off.append(node_off[-1] + a.size)
+        tet_off.append(tet_off[-1] + b.size)
+    out = dict(xyz=xyz, tets=tets, pinned=pinned, ring_order=rb.ring_order(), order=rb.order(),
+               dt=np.array([DT], np.float32), tick0=np.array([5], np.uint32), cases=np.array(RING_CASES, np.float64),
+               ring_node_off=np.array(node_off, np.uint32), ring_nodes=np.concatenate(nodes).astype(np.uint32),
+               ring_tet_off=np.array(tet_off, np.uint32), ring_tets=np.concatenate(rtets).astype(np.uint32),
+               ring_idx=np.concatenate(idx).astype(np.uint8))
+    for k, (energy, nu, comp, rtype, damp, pbd, pbda) in enumerate(RING_CASES):
+        rb = po.RefBody(xyz, tets, 1000.0, pinned)
+        st = rb.settings(comp, nu=nu, energy=int(energy), damping=damp, rayleigh_type=int(rtype),
+                         pbd_damping=1.0 if pbd else 0.0, volume_pbd=pbd, amortized_volume_pbd=pbda, tick_id=5)
+        rb.substeps(st, DT, 20)
+        out[f"X{k}"], out[f"V{k}"] = rb.state()
+    np.savez_compressed(os.path.join(HERE, "ref_rings.npz"), **out)
+
+
 if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref/libxfref.so is missing: build the oracle with the reference present")
     beam12()
     tet_kats()
     passes()
-    for f in ("ref_beam12.npz", "ref_tet_kats.npz", "ref_passes.npz"):
+    rings()
+    for f in ("ref_beam12.npz", "ref_tet_kats.npz", "ref_passes.npz", "ref_rings.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
