@@ -44,7 +44,16 @@ enum {
     XF_ENERGY_MIXED_SEL = 4,
     XF_ENERGY_YEOH_SKIN = 5,
     XF_ENERGY_YEOH_SKIN_SEL = 6,
-    XF_ENERGY_YEOH_SKIN_FAST = 7
+    XF_ENERGY_YEOH_SKIN_FAST = 7,
+    /* Vertex-ring ("continuous") energies: the sweep runs over vertex rings
+     * instead of tets (GeoLinear3d::Constrain, Geo.cpp:783-786; SolveRing and
+     * SolveSimplex, Geo.cpp:13-64, 148-226). They take settings.compliance
+     * and poissons_ratio (no per-tet mu), ignore volume_passes and
+     * simultaneous as the reference does, and run with per-colour launches.
+     * All bodies of a scene use ring energies or none does. */
+    XF_ENERGY_CONTINUOUS_PIXAR = 8,
+    XF_ENERGY_CONTINUOUS_MIXED = 9,
+    XF_ENERGY_CONTINUOUS_SKIN = 10
 };
 
 /* Collision flags (no reference counterpart: the reference has no
@@ -171,6 +180,8 @@ typedef struct xf_scene_info {
     uint32_t n_tets;
     uint32_t n_colours;      /* colour phases per substep (max over bodies) */
     uint64_t device_bytes;   /* device memory held by the scene */
+    uint32_t n_rings;        /* vertex rings (0 until a ring energy or xf_scene_get_ring_order builds them) */
+    uint32_t n_ring_colours; /* ring colour phases per substep (max over bodies) */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */
@@ -217,6 +228,16 @@ XF_API int xf_scene_set_tick(xf_scene* scene, uint32_t tick);
 /* The sweep order of one body: its tets (body-local ids) in colour order.
  * Sweeping this order serially reproduces the GPU result. */
 XF_API int xf_scene_get_order(const xf_scene* scene, uint32_t body, uint32_t* order, uint32_t* colour_of_tet);
+
+/* The vertex-ring sweep order of one body (used by the ring energies 8..10):
+ * *n_rings = rings of the body (vertices 0 .. largest tet index); order
+ * (nullable) gets them in sweep order, colour_of_ring (nullable) each
+ * ring's colour. Sweeping this order serially reproduces the GPU result.
+ * Replaces GeoLinear3d::vOrder (Geo.h:162, Geo.cpp:759-771), which the
+ * reference shuffles with MINSTD; XF_COLOUR_MINSTD visits rings in that
+ * order when colouring. Builds the rings on first use. */
+XF_API int xf_scene_get_ring_order(xf_scene* scene, uint32_t body, uint32_t* n_rings, uint32_t* order,
+                                   uint32_t* colour_of_ring);
 /* Element constants of one body as computed by InitFiniteElement
  * (Fem.cpp:269): qi9 column-major inverse rest map, qq3, qr3, vol per tet,
  * w per node (0 where pinned). Any pointer may be null. */
@@ -282,6 +303,17 @@ XF_API int xf_normal_fill(uint64_t seed, float sigma, uint32_t n, float* out);
  * the reference sweeps its MINSTD order serially (Geo.cpp:759-777). */
 XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, uint32_t colour_order,
                           uint32_t* order, uint32_t* colour, uint32_t* n_colours);
+/* The vertex rings the ring energies sweep, on the host: the reference's
+ * OneRing of every vertex 0 .. largest index (Connectivity.h:106-155, built by
+ * GeoLinear3d::Init, Geo.cpp:745-757) in CSR form. *n_rings, *node_total and
+ * *largest (nullable; nodes of the largest ring) always; where set,
+ * node_off[n_rings + 1], nodes[node_total] (each ring's own vertex first),
+ * tet_off[n_rings + 1], ring_tets[4 n_tets] and idx[16 n_tets] (four
+ * ring-local node indices per ring tet). XF_E_UNSUPPORTED if a ring would
+ * exceed 256 nodes (kMaxRingNodes, Geo.cpp:10). */
+XF_API int xf_mesh_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_rings, uint32_t* node_total,
+                         uint32_t* node_off, uint32_t* nodes, uint32_t* tet_off, uint32_t* ring_tets, uint8_t* idx,
+                         uint32_t* largest);
 /* Tet mesh ingestion (no reference counterpart: the reference compiles its
  * tetgen-made Armadillo volume mesh in, MeshGen.cpp:23-80). Reads a tetgen
  * .node / .ele pair: call with xyz = tets = NULL for *n_nodes / *n_tets,

@@ -15,6 +15,13 @@
 //            holds more than one body; always by the volume and damping passes
 //          tdmp[T] float2 (surface area, position in the body's sweep order
 //            as uint bits) -- damping passes only
+//   vertex rings in sweep order (colour-major, body-major inside a colour),
+//   built when a body first uses a vertex-ring energy:
+//          rmeta[R] uint4 (node offset, node count, ring-tet offset, tet count)
+//          rinfo[R] uint2 (body id, position in the body's ring sweep order)
+//          rnodes[]  uint32 global node ids, the ring's own vertex first
+//          rtet[2 * ring tets] float4 pairs (QQ0 QQ1 QQ2 QR0 | QR1 QR2 V idx)
+//            idx = four ring-local node indices, one byte each, as float bits
 //   body table  XfBodyParams[B]
 #pragma once
 #include <stdint.h>
@@ -40,6 +47,10 @@ struct XfBodyParams {
     float pbd_amortized;     // amortizedVolumeAndTimeCorrectedPbdDamping
     uint32_t n_tets;         // tets of the body (amortized range)
     uint32_t simultaneous;   // Settings_XpbdSolveBit (RayleighDamp form)
+    // Vertex-ring energies (SolveSimplex, Geo.cpp:13-64)
+    uint32_t n_rings;        // rings of the body (amortized range of the ring damping pass)
+    float compliance;        // Settings::compliance
+    float poisson;           // Settings::poissonsRatio
 };
 
 // The scene clock buffer: [0] simulation time (float), [1] tick (uint32,

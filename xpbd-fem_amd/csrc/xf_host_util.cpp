@@ -1,6 +1,8 @@
 // xf_host_util.cpp -- host-side mesh, element-constant and colouring code.
 #include "xf_host_util.h"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -210,6 +212,90 @@ uint32_t GreedyColour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, c
         (*colour)[t] = c;
         for (int i = 0; i < 4; i++) mask[(size_t)is[i] * kWords + c / 64] |= 1ull << (c % 64);
         if (c + 1 > ncol) ncol = c + 1;
+    }
+    return ncol;
+}
+
+bool BuildRings(const uint32_t* tets, uint32_t n_tets, Rings* out) {
+    uint32_t nv = 0;
+    for (size_t i = 0; i < 4 * (size_t)n_tets; i++) nv = std::max(nv, tets[i] + 1);
+    // tets of each vertex in increasing tet index (Connectivity.cpp:300-312, 436-440)
+    std::vector<uint32_t> first(nv + 1, 0), list(4 * (size_t)n_tets);
+    for (size_t i = 0; i < 4 * (size_t)n_tets; i++) first[tets[i] + 1]++;
+    for (uint32_t v = 0; v < nv; v++) first[v + 1] += first[v];
+    std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+    for (uint32_t t = 0; t < n_tets; t++)
+        for (int j = 0; j < 4; j++) list[fill[tets[4 * (size_t)t + j]]++] = t;
+    Rings& r = *out;
+    r.n = nv;
+    r.largest = 0;
+    r.node_off.assign(nv + 1, 0);
+    r.tet_off = first;
+    r.tets = list;
+    r.idx.assign(16 * (size_t)n_tets, 0);  // 4 ring-local indices per ring tet, 4 ring tets per tet
+    r.nodes.clear();
+    std::vector<uint32_t> verts;
+    for (uint32_t v = 0; v < nv; v++) {
+        verts.assign(1, v);
+        for (uint32_t k = first[v]; k < first[v + 1]; k++) {
+            const uint32_t* is = tets + 4 * (size_t)list[k];
+            for (int j = 0; j < 4; j++) {
+                uint32_t li = (uint32_t)(std::find(verts.begin(), verts.end(), is[j]) - verts.begin());
+                if (li == verts.size()) {
+                    if (verts.size() == kMaxRingNodes) return false;
+                    verts.push_back(is[j]);
+                }
+                r.idx[4 * (size_t)k + j] = (uint8_t)li;
+            }
+        }
+        r.node_off[v] = (uint32_t)r.nodes.size();
+        r.nodes.insert(r.nodes.end(), verts.begin(), verts.end());
+        r.largest = std::max(r.largest, (uint32_t)verts.size());
+    }
+    r.node_off[nv] = (uint32_t)r.nodes.size();
+    return true;
+}
+
+void MinstdRingOrder(uint32_t n_tets, uint32_t n_rings, uint32_t* order) {
+    uint32_t state = 1;
+    for (uint32_t i = 0; i < n_tets; i++) state = (uint32_t)(((uint64_t)state * 48271u) % 0x7fffffffu);
+    for (uint32_t i = 0; i < n_rings; i++) order[i] = i;
+    for (uint32_t i = 0; i < n_rings; i++) {
+        state = (uint32_t)(((uint64_t)state * 48271u) % 0x7fffffffu);
+        uint32_t j = state % n_rings;
+        std::swap(order[i], order[j]);
+    }
+}
+
+uint32_t RingColourSweepOrder(const Rings& r, uint32_t n_nodes, const uint32_t* visit,
+                              std::vector<uint32_t>* colour, std::vector<uint32_t>* order) {
+    // Each ring, in visit order, takes the lowest colour that no ring sharing
+    // one of its nodes has taken; node masks are kWords x 64 bits.
+    const int kWords = 4;
+    std::vector<uint64_t> mask((size_t)n_nodes * kWords, 0);
+    colour->assign(r.n, 0);
+    uint32_t ncol = 0;
+    for (uint32_t k = 0; k < r.n; k++) {
+        uint32_t v = visit[k];
+        uint32_t c = kMaxColours;
+        for (int word = 0; word < kWords && c == kMaxColours; word++) {
+            uint64_t used = 0;
+            for (uint32_t q = r.node_off[v]; q < r.node_off[v + 1]; q++) used |= mask[(size_t)r.nodes[q] * kWords + word];
+            if (used != ~0ull) c = 64 * word + (uint32_t)__builtin_ctzll(~used);
+        }
+        if (c == kMaxColours) return 0;
+        (*colour)[v] = c;
+        for (uint32_t q = r.node_off[v]; q < r.node_off[v + 1]; q++)
+            mask[(size_t)r.nodes[q] * kWords + c / 64] |= 1ull << (c % 64);
+        ncol = std::max(ncol, c + 1);
+    }
+    std::vector<uint32_t> start(ncol + 1, 0);
+    for (uint32_t v = 0; v < r.n; v++) start[(*colour)[v] + 1]++;
+    for (uint32_t c = 0; c < ncol; c++) start[c + 1] += start[c];
+    order->assign(r.n, 0);
+    for (uint32_t k = 0; k < r.n; k++) {
+        uint32_t v = visit[k];
+        (*order)[start[(*colour)[v]]++] = v;
     }
     return ncol;
 }

@@ -44,6 +44,31 @@ uint32_t GreedyColour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, c
 uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, bool minstd_visit,
                           std::vector<uint32_t>* colour, std::vector<uint32_t>* order);
 
+// Vertex rings of a linear-tet mesh as GeoLinear3d::Init builds them
+// (OneRing, Connectivity.h:106-155; Geo.cpp:745-757), in CSR form: ring v
+// has nodes nodes[node_off[v] .. node_off[v+1]) (v first, then every node of
+// its tets in first appearance; the tets of v in increasing index) and tets
+// tets[tet_off[v] .. tet_off[v+1]); idx holds four ring-local node indices
+// per ring tet. Rings exist for vertices 0 .. max index in the tets.
+constexpr uint32_t kMaxRingNodes = 256;  // Geo.cpp:10
+struct Rings {
+    uint32_t n = 0, largest = 0;
+    std::vector<uint32_t> node_off, nodes, tet_off, tets;
+    std::vector<uint8_t> idx;
+};
+// Returns false if a ring would exceed kMaxRingNodes nodes.
+bool BuildRings(const uint32_t* tets, uint32_t n_tets, Rings* out);
+
+// The reference's vOrder (Geo.cpp:759-771): the MINSTD state continues
+// from the n_tets draws of the tet shuffle, then shuffles n_rings vertices.
+void MinstdRingOrder(uint32_t n_tets, uint32_t n_rings, uint32_t* order);
+
+// Greedy colouring of rings in visit order (rings sharing a node get
+// different colours) and the colour-major ring order, stable in visit order.
+// Returns the colour count (0 if more than kMaxColours would be needed).
+uint32_t RingColourSweepOrder(const Rings& r, uint32_t n_nodes, const uint32_t* visit,
+                              std::vector<uint32_t>* colour, std::vector<uint32_t>* order);
+
 // LDLTDecomposeAndSolve (Ldlt.h:88-95). Returns 0 on success.
 int LdltSolve(uint32_t n, const float* A, const float* b, float* x);
 

@@ -15,26 +15,12 @@
 #include <hip/hip_runtime.h>
 
 #include "xf_device.h"
+#include "xf_sched.h"
+
+using xf::colour_grid;
+using xf::colour_slot;
 
 namespace {
-
-// Schedule position of this lane's tet in a colour [begin, begin + count), or
-// ~0u when the lane has none. slabs != 0: workgroup b works on eighth (b mod 8)
-// of the colour, chunk b / 8 (grid 8 * ceil(ceil(count / 8) / blockDim.x)), so
-// under round-robin dispatch each XCD sweeps the same eighth in every colour.
-__device__ __forceinline__ uint32_t colour_slot(uint32_t begin, uint32_t count, uint32_t slabs) {
-    uint32_t k;
-    if (slabs) {
-        uint32_t r = blockIdx.x & 7u, j = blockIdx.x >> 3;
-        uint32_t lo = (uint32_t)(((uint64_t)count * r) >> 3), hi = (uint32_t)(((uint64_t)count * (r + 1u)) >> 3);
-        k = lo + j * blockDim.x + threadIdx.x;
-        if (k >= hi) return 0xFFFFFFFFu;
-    } else {
-        k = blockIdx.x * blockDim.x + threadIdx.x;
-        if (k >= count) return 0xFFFFFFFFu;
-    }
-    return begin + k;
-}
 
 __device__ __forceinline__ float dot3(const float* a, const float* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
@@ -668,7 +654,7 @@ __global__ __launch_bounds__(256) void xf_volume_colour(uint32_t begin, uint32_t
                                                         const float4* __restrict__ taux, double4* __restrict__ X,
                                                         const XfBodyParams* __restrict__ bodies, float dt) {
     uint32_t i = colour_slot(begin, count, slabs);
-    if (i == 0xFFFFFFFFu) return;
+    if (i == xf::kNoSlot) return;
     volume_one(i, pass, tidx, tg0, tg1, tg2, taux, X, bodies, dt);
 }
 
@@ -688,7 +674,7 @@ __global__ __launch_bounds__(256) void xf_damp_colour(uint32_t begin, uint32_t c
                                                       const XfBodyParams* __restrict__ bodies, float comp1_u,
                                                       float a_u, float dt, const float* __restrict__ clock) {
     uint32_t i = colour_slot(begin, count, slabs);
-    if (i == 0xFFFFFFFFu) return;
+    if (i == xf::kNoSlot) return;
     uint32_t tick = reinterpret_cast<const uint32_t*>(clock)[1] - 1u;
     damp_one<E, AUX>(i, tidx, tg0, tg1, tg2, tg3, taux, tdmp, X, V, bodies, comp1_u, a_u, dt, tick);
 }
@@ -702,7 +688,7 @@ __global__ __launch_bounds__(256) void xf_pbd_colour(uint32_t begin, uint32_t co
                                                      const XfBodyParams* __restrict__ bodies,
                                                      const float* __restrict__ clock) {
     uint32_t i = colour_slot(begin, count, slabs);
-    if (i == 0xFFFFFFFFu) return;
+    if (i == xf::kNoSlot) return;
     uint32_t tick = reinterpret_cast<const uint32_t*>(clock)[1] - 1u;
     pbd_one<AUX>(i, tidx, taux, tdmp, X, V, bodies, tick);
 }
@@ -721,7 +707,7 @@ __global__ __launch_bounds__(256) void xf_project_colour(
         tick[0] = tick[0] + 1u;
     }
     uint32_t i = colour_slot(begin, count, slabs);
-    if (i == 0xFFFFFFFFu) return;
+    if (i == xf::kNoSlot) return;
 
     uint4 id = tidx[i];
     float4 q0 = tg0[i], q1 = tg1[i], q2 = tg2[i], q3 = tg3[i];
@@ -1150,14 +1136,6 @@ typedef void (*XfProjectFn)(uint32_t, uint32_t, uint32_t, const uint4*, const fl
                             const float4*, const float4*, double4*, const double4*, const XfBodyParams*, float, float, float,
                             float*);
 
-namespace {
-// Grid of a per-colour launch (see colour_slot).
-uint32_t colour_grid(uint32_t count, uint32_t block, uint32_t slabs) {
-    if (!slabs) return (count + block - 1) / block;
-    uint32_t eighth = (count + 7) / 8;
-    return 8 * ((eighth + block - 1) / block);
-}
-}  // namespace
 
 namespace {
 template <int E, bool SIM>

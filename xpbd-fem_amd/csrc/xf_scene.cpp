@@ -37,6 +37,11 @@ extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t s
                                        uint32_t pass,
                                        const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
                                        const float4* taux, double4* X, const XfBodyParams* bodies, float dt);
+extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t block, uint32_t slabs, uint32_t begin,
+                                      uint32_t count, uint32_t nmax, const uint4* rmeta, const uint2* rinfo,
+                                      const uint32_t* rnodes, const float4* rtet, double4* X, const double4* O,
+                                      double4* V, const XfBodyParams* bodies, float dt, float* clock);
+extern "C" size_t xf_ring_lds_bytes(uint32_t nmax, uint32_t block);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
@@ -110,6 +115,10 @@ struct Body {
     std::vector<uint32_t> sched_pos;  // scene schedule position of each local tet
     uint32_t ncol = 0;
     uint32_t node_base = 0;
+    // vertex rings (built on first use of a vertex-ring energy)
+    xf::Rings rings;
+    std::vector<uint32_t> ring_colour;  // per local ring
+    std::vector<uint32_t> ring_order;   // local rings in sweep order
 };
 
 template <typename T>
@@ -122,7 +131,17 @@ int DevAlloc(xf_ctx* ctx, T** p, size_t count, uint64_t* total) {
     return XF_OK;
 }
 
-bool EnergySupported(uint32_t e) { return e >= 3 && e <= 7; }
+bool EnergySupported(uint32_t e) { return e >= 3 && e <= 10; }
+
+// Energy_ContinuousPixar / ContinuousMixed / ContinuousSkin sweep vertex rings
+// instead of tets (GeoLinear3d::Constrain, Geo.cpp:774-788).
+bool RingEnergy(uint32_t e) { return e >= XF_ENERGY_CONTINUOUS_PIXAR && e <= XF_ENERGY_CONTINUOUS_SKIN; }
+
+// Scene-wide mode value of ResolveMode when every body uses a ring energy.
+constexpr int kRingMode = XF_ENERGY_CONTINUOUS_PIXAR;
+
+// LDS budget of one ring workgroup.
+constexpr size_t kRingLdsLimit = 65536;
 
 // Volume passes per substep accepted by xf_scene_set_settings.
 constexpr uint32_t kMaxVolumePasses = 64;
@@ -148,6 +167,14 @@ struct xf_scene {
     uint32_t* colour_begin_d = nullptr;  // ncol + 1 offsets (persistent launch)
     uint32_t* arrive = nullptr;          // grid-barrier arrivals (persistent launch)
     uint32_t* barrier_err = nullptr;     // set when a grid barrier timed out
+    // vertex rings in sweep order (colour-major, body-major inside a colour)
+    bool rings_host = false, rings_dev = false;
+    uint32_t R = 0, rcol = 0, rnmax = 0, ring_block = 0;
+    std::vector<uint32_t> ring_colour_begin;
+    uint4* rmeta = nullptr;
+    uint2* rinfo = nullptr;
+    uint32_t* rnodes = nullptr;
+    float4* rtet = nullptr;
     uint64_t device_bytes = 0;
     int n_cu = 256;
     // host mirrors
@@ -193,6 +220,16 @@ void DropGraph(xf_scene* s) {
 // Scene-wide projection variant: one energy for all bodies, or per-body
 // MixedSel / YeohSkinFast (energy 0). Returns XF_OK or an error.
 int ResolveMode(xf_scene* s, int* energy, bool* sim) {
+    size_t n_ring = 0;
+    for (const Body& b : s->bodies) n_ring += RingEnergy(b.settings.energy) ? 1 : 0;
+    if (n_ring == s->bodies.size()) {
+        *energy = kRingMode;
+        *sim = false;
+        return XF_OK;
+    }
+    if (n_ring > 0)
+        return Fail(s->ctx, XF_E_UNSUPPORTED,
+                    "bodies of one scene must all use vertex-ring energies (8..10) or all tet energies (3..7)");
     uint32_t e0 = s->bodies[0].settings.energy;
     uint32_t s0 = s->bodies[0].settings.simultaneous ? 1u : 0u;
     bool same = true, all47 = true;
@@ -296,12 +333,110 @@ int UploadParams(xf_scene* s, float dt) {
         p.pbd_amortized = st.pbd_damping_amortized;
         p.n_tets = s->bodies[i].t;
         p.simultaneous = st.simultaneous ? 1u : 0u;
+        p.n_rings = s->bodies[i].rings.n;
+        p.compliance = st.compliance;
+        p.poisson = st.poissons_ratio;
     }
     XF_HIP(ctx, hipMemcpyAsync(s->bp, hp.data(), sizeof(XfBodyParams) * hp.size(), hipMemcpyHostToDevice,
                                ctx->stream));
     XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->params_dt = dt;
     s->params_dirty = false;
+    return XF_OK;
+}
+
+// Host side of the vertex rings: per body the reference's OneRings, a greedy
+// colouring of the ring conflict graph visited in vertex order (or the
+// reference's vOrder with XF_COLOUR_MINSTD), and with xcd_slabs the rings of
+// each colour ordered by vertex x.
+int BuildRingsHost(xf_scene* s) {
+    if (s->rings_host) return XF_OK;
+    xf_ctx* ctx = s->ctx;
+    s->rcol = 0;
+    s->rnmax = 0;
+    s->R = 0;
+    for (Body& b : s->bodies) {
+        if (!xf::BuildRings(b.tets.data(), b.t, &b.rings))
+            return Fail(ctx, XF_E_UNSUPPORTED, "a vertex ring has more than %u nodes", xf::kMaxRingNodes);
+        std::vector<uint32_t> visit(b.rings.n);
+        if (s->opts.colour_order == XF_COLOUR_MINSTD) xf::MinstdRingOrder(b.t, b.rings.n, visit.data());
+        else
+            for (uint32_t v = 0; v < b.rings.n; v++) visit[v] = v;
+        uint32_t nc = xf::RingColourSweepOrder(b.rings, b.n, visit.data(), &b.ring_colour, &b.ring_order);
+        if (nc == 0) return Fail(ctx, XF_E_UNSUPPORTED, "mesh needs more than %u ring colours", xf::kMaxColours);
+        if (s->opts.xcd_slabs)
+            std::stable_sort(b.ring_order.begin(), b.ring_order.end(), [&](uint32_t a, uint32_t c) {
+                return b.ring_colour[a] != b.ring_colour[c] ? b.ring_colour[a] < b.ring_colour[c]
+                                                            : b.xyz[3 * (size_t)a] < b.xyz[3 * (size_t)c];
+            });
+        s->rcol = std::max(s->rcol, nc);
+        s->rnmax = std::max(s->rnmax, b.rings.largest);
+        s->R += b.rings.n;
+    }
+    uint32_t block = s->opts.block_size ? s->opts.block_size : 64;
+    while (block > 1 && xf_ring_lds_bytes(s->rnmax, block) > kRingLdsLimit) block /= 2;
+    s->ring_block = block;
+    s->rings_host = true;
+    return XF_OK;
+}
+
+// Device side: ring schedule, node lists and per-ring-tet constants.
+int UploadRings(xf_scene* s) {
+    if (s->rings_dev) return XF_OK;
+    int rc = BuildRingsHost(s);
+    if (rc) return rc;
+    xf_ctx* ctx = s->ctx;
+    std::vector<uint4> meta(s->R);
+    std::vector<uint2> info(s->R);
+    std::vector<uint32_t> nodes;
+    std::vector<float4> rtet;
+    s->ring_colour_begin.assign(s->rcol + 1, 0);
+    std::vector<uint32_t> cursor(s->bodies.size(), 0);
+    uint32_t pos = 0;
+    for (uint32_t c = 0; c < s->rcol; c++) {
+        s->ring_colour_begin[c] = pos;
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            const Body& b = s->bodies[bi];
+            const xf::Rings& r = b.rings;
+            uint32_t& k = cursor[bi];
+            while (k < r.n && b.ring_colour[b.ring_order[k]] == c) {
+                uint32_t v = b.ring_order[k++];
+                uint32_t n0 = r.node_off[v], n1 = r.node_off[v + 1], t0 = r.tet_off[v], t1 = r.tet_off[v + 1];
+                meta[pos] = make_uint4((uint32_t)nodes.size(), n1 - n0, (uint32_t)(rtet.size() / 2), t1 - t0);
+                info[pos] = make_uint2((uint32_t)bi, k - 1);
+                for (uint32_t q = n0; q < n1; q++) nodes.push_back(b.node_base + r.nodes[q]);
+                for (uint32_t q = t0; q < t1; q++) {
+                    uint32_t t = r.tets[q];
+                    const float* qq = &b.ec.qq[3 * (size_t)t];
+                    const float* qr = &b.ec.qr[3 * (size_t)t];
+                    const uint8_t* li = &r.idx[4 * (size_t)q];
+                    uint32_t bits = (uint32_t)li[0] | ((uint32_t)li[1] << 8) | ((uint32_t)li[2] << 16) |
+                                    ((uint32_t)li[3] << 24);
+                    float bitsf;
+                    memcpy(&bitsf, &bits, 4);
+                    rtet.push_back(make_float4(qq[0], qq[1], qq[2], qr[0]));
+                    rtet.push_back(make_float4(qr[1], qr[2], b.ec.vol[t], bitsf));
+                }
+                ++pos;
+            }
+        }
+    }
+    s->ring_colour_begin[s->rcol] = pos;
+    if (pos != s->R) return Fail(ctx, XF_E_INVALID, "internal: ring schedule covers %u of %u rings", pos, s->R);
+    rc = DevAlloc(ctx, &s->rmeta, s->R, &s->device_bytes);
+    if (!rc) rc = DevAlloc(ctx, &s->rinfo, s->R, &s->device_bytes);
+    if (!rc) rc = DevAlloc(ctx, &s->rnodes, nodes.size(), &s->device_bytes);
+    if (!rc) rc = DevAlloc(ctx, &s->rtet, rtet.size(), &s->device_bytes);
+    if (rc) return rc;
+    XF_HIP(ctx, hipMemcpyAsync(s->rmeta, meta.data(), sizeof(uint4) * s->R, hipMemcpyHostToDevice, ctx->stream));
+    XF_HIP(ctx, hipMemcpyAsync(s->rinfo, info.data(), sizeof(uint2) * s->R, hipMemcpyHostToDevice, ctx->stream));
+    XF_HIP(ctx, hipMemcpyAsync(s->rnodes, nodes.data(), sizeof(uint32_t) * nodes.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    XF_HIP(ctx, hipMemcpyAsync(s->rtet, rtet.data(), sizeof(float4) * rtet.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->rings_dev = true;
+    s->params_dirty = true;  // n_rings per body
     return XF_OK;
 }
 
@@ -332,6 +467,57 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
         }
         return XF_OK;
     };
+    if (energy == kRingMode) {
+        // Geo3d::Substep with a vertex-ring energy: ring colours (the first
+        // advances the clock), velocity, then GeoLinear3d::Damp's ring pass
+        // and PBD tet pass (Geo.cpp:790-811), then the next prediction.
+        uint32_t rb = s->ring_block;
+        int rc = node(0, false);
+        if (rc) return rc;
+        for (uint32_t k = 0; k < substeps; k++) {
+            for (uint32_t c = 0; c < s->rcol; c++) {
+                uint32_t begin = s->ring_colour_begin[c], count = s->ring_colour_begin[c + 1] - begin;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (timer) {
+                    XF_HIP(ctx, hipEventCreate(&e0));
+                    XF_HIP(ctx, hipEventCreate(&e1));
+                    XF_HIP(ctx, hipEventRecord(e0, st));
+                }
+                XF_HIP(ctx, xf_launch_rings(st, false, rb, slabs, begin, count, s->rnmax, s->rmeta, s->rinfo,
+                                            s->rnodes, s->rtet, s->X, s->O, s->V, s->bp, dt,
+                                            c == 0 ? s->clock : nullptr));
+                if (timer) {
+                    XF_HIP(ctx, hipEventRecord(e1, st));
+                    timer->ev_proj.push_back(e0);
+                    timer->ev_proj.push_back(e1);
+                }
+            }
+            if (s->post == 0) {
+                rc = node(1, k + 1 < substeps);
+                if (rc) return rc;
+                continue;
+            }
+            rc = node(1, false);
+            if (rc) return rc;
+            if (s->post & 1u)
+                for (uint32_t c = 0; c < s->rcol; c++) {
+                    uint32_t begin = s->ring_colour_begin[c], count = s->ring_colour_begin[c + 1] - begin;
+                    XF_HIP(ctx, xf_launch_rings(st, true, rb, slabs, begin, count, s->rnmax, s->rmeta, s->rinfo,
+                                                s->rnodes, s->rtet, s->X, s->O, s->V, s->bp, dt, s->clock));
+                }
+            if (s->post & 2u)
+                for (uint32_t c = 0; c < s->ncol; c++) {
+                    uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                    XF_HIP(ctx, xf_launch_pbd(st, s->aux, block, slabs, begin, count, s->tidx, s->taux, s->tdmp,
+                                              s->X, s->V, s->bp, s->clock));
+                }
+            if (k + 1 < substeps) {
+                rc = node(0, false);
+                if (rc) return rc;
+            }
+        }
+        return XF_OK;
+    }
     if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT) {
         int maxb = 0;
         XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
@@ -418,6 +604,16 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) return Fail(s->ctx, XF_E_INVALID, "dt must be positive and finite");
     int rc = ResolveMode(s, energy, sim);
     if (rc) return rc;
+    if (*energy == kRingMode) {
+        if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT)
+            return Fail(s->ctx, XF_E_UNSUPPORTED, "vertex-ring energies run with per-colour launches only");
+        for (const Body& b : s->bodies)
+            if (!b.tet_mu.empty())
+                return Fail(s->ctx, XF_E_UNSUPPORTED,
+                            "vertex-ring energies take settings.compliance; a body has per-tet mu");
+        rc = UploadRings(s);
+        if (rc) return rc;
+    }
     if (s->material_dirty) {
         rc = UploadMaterial(s);
         if (rc) return rc;
@@ -431,7 +627,7 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
 
 int CheckSettings(xf_ctx* ctx, const xf_settings* st) {
     if (!EnergySupported(st->energy))
-        return Fail(ctx, XF_E_UNSUPPORTED, "energy %u not implemented (3..7 are)", st->energy);
+        return Fail(ctx, XF_E_UNSUPPORTED, "energy %u not implemented (3..10 are)", st->energy);
     if (!(st->damping >= 0.0f) || !std::isfinite(st->damping) || !(st->pbd_damping >= 0.0f) ||
         !std::isfinite(st->pbd_damping) || !(st->pbd_damping_amortized >= 0.0f) ||
         !std::isfinite(st->pbd_damping_amortized))
@@ -517,7 +713,7 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     DropGraph(s);
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
-                    s->barrier_err, s->tdmp};
+                    s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rtet};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -752,6 +948,8 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     info->n_tets = s->T;
     info->n_colours = s->ncol;
     info->device_bytes = s->device_bytes;
+    info->n_rings = s->rings_host ? s->R : 0;
+    info->n_ring_colours = s->rings_host ? s->rcol : 0;
     return XF_OK;
 }
 
@@ -943,6 +1141,20 @@ XF_API int xf_scene_get_order(const xf_scene* s, uint32_t body, uint32_t* order,
     const Body& b = s->bodies[body];
     if (order) memcpy(order, b.order.data(), sizeof(uint32_t) * b.t);
     if (colour_of_tet) memcpy(colour_of_tet, b.colour.data(), sizeof(uint32_t) * b.t);
+    return XF_OK;
+}
+
+XF_API int xf_scene_get_ring_order(xf_scene* s, uint32_t body, uint32_t* n_rings, uint32_t* order,
+                                   uint32_t* colour_of_ring) {
+    if (!s || !n_rings) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    int rc = BuildRingsHost(s);
+    if (rc) return rc;
+    const Body& b = s->bodies[body];
+    *n_rings = b.rings.n;
+    if (order) memcpy(order, b.ring_order.data(), sizeof(uint32_t) * b.rings.n);
+    if (colour_of_ring) memcpy(colour_of_ring, b.ring_colour.data(), sizeof(uint32_t) * b.rings.n);
     return XF_OK;
 }
 
@@ -1333,6 +1545,23 @@ XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
     memcpy(order, ord.data(), sizeof(uint32_t) * n_tets);
     if (colour) memcpy(colour, col.data(), sizeof(uint32_t) * n_tets);
     *n_colours = nc;
+    return XF_OK;
+}
+
+XF_API int xf_mesh_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_rings, uint32_t* node_total,
+                         uint32_t* node_off, uint32_t* nodes, uint32_t* tet_off, uint32_t* ring_tets, uint8_t* idx,
+                         uint32_t* largest) {
+    if (!tets || !n_rings || !node_total) return XF_E_INVALID;
+    xf::Rings r;
+    if (!xf::BuildRings(tets, n_tets, &r)) return XF_E_UNSUPPORTED;
+    *n_rings = r.n;
+    *node_total = (uint32_t)r.nodes.size();
+    if (largest) *largest = r.largest;
+    if (node_off) memcpy(node_off, r.node_off.data(), sizeof(uint32_t) * r.node_off.size());
+    if (nodes) memcpy(nodes, r.nodes.data(), sizeof(uint32_t) * r.nodes.size());
+    if (tet_off) memcpy(tet_off, r.tet_off.data(), sizeof(uint32_t) * r.tet_off.size());
+    if (ring_tets) memcpy(ring_tets, r.tets.data(), sizeof(uint32_t) * r.tets.size());
+    if (idx) memcpy(idx, r.idx.data(), r.idx.size());
     return XF_OK;
 }
 

@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxpbdfem.so")
 
 ENERGY_MIXED, ENERGY_MIXED_SEL, ENERGY_YEOH_SKIN, ENERGY_YEOH_SKIN_SEL, ENERGY_YEOH_SKIN_FAST = 3, 4, 5, 6, 7
+# vertex-ring ("continuous") energies: the sweep runs over vertex rings
+ENERGY_CONTINUOUS_PIXAR, ENERGY_CONTINUOUS_MIXED, ENERGY_CONTINUOUS_SKIN = 8, 9, 10
 COLLIDE_GROUND, COLLIDE_SPHERE = 1, 2
 RAYLEIGH_PAPER, RAYLEIGH_LIMIT, RAYLEIGH_POST, RAYLEIGH_POST_AMORTIZED = 0, 1, 2, 3
 COLOUR_NATURAL, COLOUR_MINSTD = 0, 1
@@ -73,7 +75,8 @@ class SceneOpts(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("n_bodies", C.c_uint32), ("n_nodes", C.c_uint32), ("n_tets", C.c_uint32),
-                ("n_colours", C.c_uint32), ("device_bytes", C.c_uint64)]
+                ("n_colours", C.c_uint32), ("device_bytes", C.c_uint64), ("n_rings", C.c_uint32),
+                ("n_ring_colours", C.c_uint32)]
 
 
 _lib = None
@@ -132,6 +135,8 @@ def lib():
         "xf_mesh_error": ([], C.c_char_p),
         "xf_scene_set_tick": ([vp, C.c_uint32], C.c_int),
         "xf_scene_get_order": ([vp, C.c_uint32, vp, vp], C.c_int),
+        "xf_scene_get_ring_order": ([vp, C.c_uint32, vp, vp, vp], C.c_int),
+        "xf_mesh_rings": ([C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "xf_scene_get_elements": ([vp, C.c_uint32, vp, vp, vp, vp, vp], C.c_int),
         "xf_scene_profile": ([vp, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
                               C.POINTER(C.c_uint32)], C.c_int),
@@ -163,10 +168,11 @@ def exported_symbols():
         "xf_version", "xf_settings_default", "xf_ctx_create", "xf_ctx_destroy", "xf_last_error", "xf_scene_create",
         "xf_scene_destroy", "xf_scene_add_body", "xf_scene_finalize", "xf_scene_info_get", "xf_scene_set_settings",
         "xf_scene_set_tet_mu", "xf_scene_step", "xf_scene_sync", "xf_scene_get_state", "xf_scene_set_state",
-        "xf_scene_get_time", "xf_scene_set_time", "xf_scene_get_order", "xf_scene_get_elements", "xf_scene_profile",
-        "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
+        "xf_scene_get_time", "xf_scene_set_time", "xf_scene_get_order", "xf_scene_get_ring_order",
+        "xf_scene_get_elements", "xf_scene_profile", "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
         "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
         "xf_scene_get_tick", "xf_scene_set_tick", "xf_mesh_read_tetgen", "xf_mesh_tets_from_stream", "xf_mesh_error",
+        "xf_mesh_rings",
     ]
 
 
@@ -188,6 +194,26 @@ def tet_block_colours(W, H, D):
     out = np.zeros(6 * W * H * D, np.uint32)
     if lib().xf_mesh_tet_block_colours(W, H, D, out):
         raise XfError("xf_mesh_tet_block_colours failed")
+    return out
+
+
+def mesh_rings(tets):
+    """Vertex rings of a tet mesh (the reference's OneRings) as a dict of CSR
+    arrays: node_off, nodes, tet_off, tets, idx (4 per ring tet), largest."""
+    tets = np.ascontiguousarray(tets, np.uint32)
+    t = tets.size // 4
+    nr, tot, big = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    rc = lib().xf_mesh_rings(t, tets.ctypes.data, C.byref(nr), C.byref(tot), None, None, None, None, None,
+                             C.byref(big))
+    if rc:
+        raise XfError(f"xf_mesh_rings failed ({rc})")
+    out = dict(node_off=np.zeros(nr.value + 1, np.uint32), nodes=np.zeros(tot.value, np.uint32),
+               tet_off=np.zeros(nr.value + 1, np.uint32), tets=np.zeros(4 * t, np.uint32),
+               idx=np.zeros(16 * t, np.uint8))
+    lib().xf_mesh_rings(t, tets.ctypes.data, C.byref(nr), C.byref(tot), out["node_off"].ctypes.data,
+                        out["nodes"].ctypes.data, out["tet_off"].ctypes.data, out["tets"].ctypes.data,
+                        out["idx"].ctypes.data, C.byref(big))
+    out["largest"] = big.value
     return out
 
 
@@ -374,7 +400,7 @@ class Scene:
         i = SceneInfo()
         self.ctx.check(lib().xf_scene_info_get(self.h, C.byref(i)), "xf_scene_info_get")
         return dict(n_bodies=i.n_bodies, n_nodes=i.n_nodes, n_tets=i.n_tets, n_colours=i.n_colours,
-                    device_bytes=i.device_bytes)
+                    device_bytes=i.device_bytes, n_rings=i.n_rings, n_ring_colours=i.n_ring_colours)
 
     def set_settings(self, body, settings=None, **kw):
         s = settings if settings is not None else default_settings(**kw)
@@ -424,6 +450,16 @@ class Scene:
         o = np.zeros(t, np.uint32)
         c = np.zeros(t, np.uint32)
         self.ctx.check(lib().xf_scene_get_order(self.h, body, o.ctypes.data, c.ctypes.data), "xf_scene_get_order")
+        return o, c
+
+    def ring_order(self, body=0):
+        """(vertex rings of the body in sweep order, colour of each ring)."""
+        n = C.c_uint32()
+        self.ctx.check(lib().xf_scene_get_ring_order(self.h, body, C.byref(n), None, None), "xf_scene_get_ring_order")
+        o = np.zeros(n.value, np.uint32)
+        c = np.zeros(n.value, np.uint32)
+        self.ctx.check(lib().xf_scene_get_ring_order(self.h, body, C.byref(n), o.ctypes.data, c.ctypes.data),
+                       "xf_scene_get_ring_order")
         return o, c
 
     def elements(self, body=0):
