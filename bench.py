@@ -150,6 +150,85 @@ def batched_leg(ctx, frames=3, xcd_slabs=False):
             "finite": finite}
 
 
+def rings_leg(ctx, frames=3, budget_s=8.0, xcd_slabs=True):
+    """The 96x24x24 beam with the vertex-ring energy ContinuousSkin (10): the
+    sweep runs over its 60,625 vertex rings (each gathers the ~24 tets around
+    a vertex and solves two constraints over its ~15 nodes).
+
+    Reports ring-substeps/s and tet-substeps/s, the ring kernel's achieved
+    bandwidth over algorithmic bytes (per ring: node position and inverse
+    mass 32 B, position write 24 B and node id 4 B per ring node, 32 B of
+    constants per ring tet, 24 B of ring metadata), the compiled reference's
+    own ring sweep timed on a bounded host sample, and one frame of the GPU
+    against the compiled reference (the GPU's ring and tet orders as vOrder
+    and tOrder, the reference's fixed Skin constants, no contacts)."""
+    import pyoracle as po
+
+    wl = workloads.beam_yeoh_96x24x24()
+    s = dict(wl["settings"], energy=xf.ENERGY_CONTINUOUS_SKIN)
+    wl = dict(wl, settings=s)
+    S, dt = wl["substeps"], float(wl["dt"])
+    T = wl["tets"].size // 4
+    sc = workloads.make_scene(ctx, wl, xcd_slabs=xcd_slabs)
+    mode, size, _ = sc.autotune(S, dt, reps=2)
+    sc.step(S, dt)
+    sc.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sc.step(S, dt)
+    sc.sync()
+    el = time.perf_counter() - t0
+    finite = sc.count_nonfinite() == 0
+    info = sc.info()
+    R = info["n_rings"]
+    ms_ring, _, n_launch = sc.profile(S, dt)
+    sc.close()
+    node_total = int(xf.mesh_rings(wl["tets"])["nodes"].size)
+    bytes_substep = 60 * node_total + 32 * 4 * T + 24 * R
+    achieved = bytes_substep * S / (ms_ring * 1e-3) / 1e9
+    out = {"workload": f"{wl['name']} with ContinuousSkin: {R} rings ({node_total} ring nodes, {4 * T} ring tets), "
+                       f"{info['n_ring_colours']} ring colours, {S} substeps/frame",
+           "ring_substeps_per_s": R * S * frames / el, "tet_substeps_per_s": T * S * frames / el,
+           "ms_per_frame": 1e3 * el / frames, "finite": finite,
+           "launch": {"mode": "per-colour", "size": size, "xcd_slabs": xcd_slabs},
+           "roofline": {"bound": "hbm", "kernel": "xf_ring_colour", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "launches": n_launch,
+                        "avg_launch_us": 1e3 * ms_ring / max(n_launch, 1),
+                        "bytes_per_launch": bytes_substep * S / max(n_launch, 1)}}
+    if not po.have_ref():
+        return out
+    plain = dict(wl, settings=dict(s, collide=0, yeoh=xf.SKIN_YEOH))
+    g = workloads.make_scene(ctx, plain, xcd_slabs=xcd_slabs)
+    g.step(S, dt)
+    xg, _ = g.get_state(0)
+    torder, rorder = g.order(0)[0], g.ring_order(0)[0]
+    g.close()
+    x0 = wl["xyz"].astype(np.float64)
+    v0 = wl["velocity"].astype(np.float64)
+    rb = po.RefBody(wl["xyz"], wl["tets"], wl["density"], wl["pinned"], order=torder)
+    rb.set_ring_order(rorder)
+    rb.set_state(X=x0, V=v0)
+    st = rb.settings(s["compliance"], nu=s["poissons_ratio"], energy=xf.ENERGY_CONTINUOUS_SKIN)
+    rb.substeps(st, np.float32(dt), S)
+    out["check"] = {"frames": 1, "rel_pos_err_vs_reference": rel_pos_err(xg, rb.state()[0], wl["xyz"])}
+    rb.set_state(X=x0, V=v0)
+    rb.substeps(st, np.float32(dt), 1)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        rb.substeps(st, np.float32(dt), 1)
+        n += 1
+        cel = time.perf_counter() - t0
+        if cel >= budget_s:
+            break
+    del rb
+    out["cpu_baseline"] = {"value": R * n / cel, "unit": "ring-substeps/s", "cores": 1, "kind": "reference",
+                           "sample": f"reference Geo3d::Substep with ContinuousSkin: {n} substeps of the "
+                                     f"{R}-ring beam, single thread, {cel:.1f} s"}
+    out["gpu_over_cpu"] = out["ring_substeps_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def adjoint_leg(ctx, reps=3):
     """Adjoint of the 16x4x4 fitting beam (1,536 tets, per-tet mu, 30 frames of
     20 substeps, loss at frames 10, 20, 30, checkpoints every 20 substeps):
@@ -259,6 +338,23 @@ def batch_profile(frames=1, xcd_slabs=False):
                       "frames": frames, "xcd_slabs": xcd_slabs}))
 
 
+def rings_profile(frames=1, xcd_slabs=True):
+    """One frame of the 96x24x24 beam with ContinuousSkin, per-colour launches
+    of 64 threads (short runs under rocprofv3 counter passes)."""
+    ctx = xf.Context(0)
+    wl = workloads.beam_yeoh_96x24x24()
+    wl = dict(wl, settings=dict(wl["settings"], energy=xf.ENERGY_CONTINUOUS_SKIN))
+    sc = workloads.make_scene(ctx, wl, xcd_slabs=xcd_slabs)
+    for _ in range(frames):
+        sc.step(wl["substeps"], float(wl["dt"]))
+    sc.sync()
+    info = sc.info()
+    sc.close()
+    ctx.close()
+    print(json.dumps({"rings_profile": wl["name"], "rings": info["n_rings"], "ring_colours": info["n_ring_colours"],
+                      "frames": frames, "xcd_slabs": xcd_slabs}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,12 +363,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs (single beam and batched fit)")
+    ap.add_argument("--no-rings", action="store_true", help="skip the vertex-ring energy leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="per-colour launches of 64 threads instead of the autotuned shape (short profiler passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 4 device copies of 256 MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only run one frame of the 256-box batched scene (rocprofv3 counter passes)")
+    ap.add_argument("--rings-profile", action="store_true",
+                    help="only run one frame of the beam with the ContinuousSkin ring energy (rocprofv3 counter passes)")
     ap.add_argument("--no-xcd-slabs", action="store_true",
                     help="plain blockIdx-linear schedule instead of the XCD-slab one (colour eighths per XCD)")
     args = ap.parse_args()
@@ -282,6 +381,9 @@ def main():
         return
     if args.batch_profile:
         batch_profile(xcd_slabs=args.xcd_slabs)
+        return
+    if args.rings_profile:
+        rings_profile(xcd_slabs=args.xcd_slabs)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
@@ -354,6 +456,8 @@ def main():
     legs = []
     if not args.no_batch:
         legs.append(("batched", lambda c: batched_leg(c, xcd_slabs=args.xcd_slabs)))
+    if not args.no_rings:
+        legs.append(("rings", lambda c: rings_leg(c, xcd_slabs=args.xcd_slabs)))
     if not args.no_adjoint:
         legs += [("adjoint", adjoint_leg), ("fit", fit_leg)]
     if not args.no_cpu:

@@ -146,3 +146,19 @@ def test_ring_mode_rejections(ctx):
     with pytest.raises(xf.XfError):
         sc.adjoint(S, dt, np.array([S], np.uint32), np.zeros((1, wl["xyz"].size)))
     sc.close()
+
+
+def test_ring_autotune_keeps_results(ctx):
+    """The autotuner picks a per-colour shape in ring mode, restores the state,
+    and the tuned scene still equals the oracle."""
+    wl = ring_beam(10, "amortized")
+    sc = workloads.make_scene(ctx, wl)
+    mode, size, ms = sc.autotune(wl["substeps"], float(wl["dt"]), reps=1)
+    assert mode == xf.LAUNCH_PER_COLOUR and size in (64, 128, 256) and ms > 0.0
+    assert sc.tick() == 0
+    for _ in range(2):
+        sc.step(wl["substeps"], float(wl["dt"]))
+    x, v = sc.get_state(0)
+    ob = oracle(wl, sc.order(0)[0], sc.ring_order(0)[0], 2 * wl["substeps"])
+    sc.close()
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)

@@ -37,11 +37,13 @@ extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t s
                                        uint32_t pass,
                                        const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
                                        const float4* taux, double4* X, const XfBodyParams* bodies, float dt);
-extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t block, uint32_t slabs, uint32_t begin,
-                                      uint32_t count, uint32_t nmax, const uint4* rmeta, const uint2* rinfo,
-                                      const uint32_t* rnodes, const float4* rtet, double4* X, const double4* O,
+extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t teams, uint32_t slabs, uint32_t begin,
+                                      uint32_t count, uint32_t nmax, uint32_t tmax, const uint4* rmeta,
+                                      const uint2* rinfo, const uint32_t* rnodes, const uint16_t* rincb,
+                                      const uint16_t* rinc, const float4* rtet, double4* X, const double4* O,
                                       double4* V, const XfBodyParams* bodies, float dt, float* clock);
-extern "C" size_t xf_ring_lds_bytes(uint32_t nmax, uint32_t block);
+extern "C" uint32_t xf_ring_lanes(void);
+extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
@@ -169,11 +171,13 @@ struct xf_scene {
     uint32_t* barrier_err = nullptr;     // set when a grid barrier timed out
     // vertex rings in sweep order (colour-major, body-major inside a colour)
     bool rings_host = false, rings_dev = false;
-    uint32_t R = 0, rcol = 0, rnmax = 0, ring_block = 0;
+    uint32_t R = 0, rcol = 0, rnmax = 0, rtmax = 0;
     std::vector<uint32_t> ring_colour_begin;
     uint4* rmeta = nullptr;
     uint2* rinfo = nullptr;
     uint32_t* rnodes = nullptr;
+    uint16_t* rincb = nullptr;  // per ring node: first entry of its incidence list (ring-local)
+    uint16_t* rinc = nullptr;   // per ring: (ring tet << 2 | corner) of each node, node-major, tets ascending
     float4* rtet = nullptr;
     uint64_t device_bytes = 0;
     int n_cu = 256;
@@ -354,6 +358,7 @@ int BuildRingsHost(xf_scene* s) {
     xf_ctx* ctx = s->ctx;
     s->rcol = 0;
     s->rnmax = 0;
+    s->rtmax = 0;
     s->R = 0;
     for (Body& b : s->bodies) {
         if (!xf::BuildRings(b.tets.data(), b.t, &b.rings))
@@ -371,11 +376,9 @@ int BuildRingsHost(xf_scene* s) {
             });
         s->rcol = std::max(s->rcol, nc);
         s->rnmax = std::max(s->rnmax, b.rings.largest);
+        for (uint32_t v = 0; v < b.rings.n; v++) s->rtmax = std::max(s->rtmax, b.rings.tet_off[v + 1] - b.rings.tet_off[v]);
         s->R += b.rings.n;
     }
-    uint32_t block = s->opts.block_size ? s->opts.block_size : 64;
-    while (block > 1 && xf_ring_lds_bytes(s->rnmax, block) > kRingLdsLimit) block /= 2;
-    s->ring_block = block;
     s->rings_host = true;
     return XF_OK;
 }
@@ -389,6 +392,7 @@ int UploadRings(xf_scene* s) {
     std::vector<uint4> meta(s->R);
     std::vector<uint2> info(s->R);
     std::vector<uint32_t> nodes;
+    std::vector<uint16_t> incb, inc;
     std::vector<float4> rtet;
     s->ring_colour_begin.assign(s->rcol + 1, 0);
     std::vector<uint32_t> cursor(s->bodies.size(), 0);
@@ -405,6 +409,14 @@ int UploadRings(xf_scene* s) {
                 meta[pos] = make_uint4((uint32_t)nodes.size(), n1 - n0, (uint32_t)(rtet.size() / 2), t1 - t0);
                 info[pos] = make_uint2((uint32_t)bi, k - 1);
                 for (uint32_t q = n0; q < n1; q++) nodes.push_back(b.node_base + r.nodes[q]);
+                // incidence lists: for each ring node, its (tet, corner) slots in ring-tet order
+                std::vector<uint16_t> cnt(n1 - n0 + 1, 0);
+                for (uint32_t q = 4 * t0; q < 4 * t1; q++) cnt[r.idx[q] + 1]++;
+                for (uint32_t n = 0; n < n1 - n0; n++) cnt[n + 1] += cnt[n];
+                incb.insert(incb.end(), cnt.begin(), cnt.end() - 1);
+                size_t base = inc.size();
+                inc.resize(base + 4 * (size_t)(t1 - t0));
+                for (uint32_t q = 4 * t0; q < 4 * t1; q++) inc[base + cnt[r.idx[q]]++] = (uint16_t)(q - 4 * t0);
                 for (uint32_t q = t0; q < t1; q++) {
                     uint32_t t = r.tets[q];
                     const float* qq = &b.ec.qq[3 * (size_t)t];
@@ -426,8 +438,14 @@ int UploadRings(xf_scene* s) {
     rc = DevAlloc(ctx, &s->rmeta, s->R, &s->device_bytes);
     if (!rc) rc = DevAlloc(ctx, &s->rinfo, s->R, &s->device_bytes);
     if (!rc) rc = DevAlloc(ctx, &s->rnodes, nodes.size(), &s->device_bytes);
+    if (!rc) rc = DevAlloc(ctx, &s->rincb, incb.size(), &s->device_bytes);
+    if (!rc) rc = DevAlloc(ctx, &s->rinc, inc.size(), &s->device_bytes);
     if (!rc) rc = DevAlloc(ctx, &s->rtet, rtet.size(), &s->device_bytes);
     if (rc) return rc;
+    XF_HIP(ctx, hipMemcpyAsync(s->rincb, incb.data(), sizeof(uint16_t) * incb.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    XF_HIP(ctx, hipMemcpyAsync(s->rinc, inc.data(), sizeof(uint16_t) * inc.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
     XF_HIP(ctx, hipMemcpyAsync(s->rmeta, meta.data(), sizeof(uint4) * s->R, hipMemcpyHostToDevice, ctx->stream));
     XF_HIP(ctx, hipMemcpyAsync(s->rinfo, info.data(), sizeof(uint2) * s->R, hipMemcpyHostToDevice, ctx->stream));
     XF_HIP(ctx, hipMemcpyAsync(s->rnodes, nodes.data(), sizeof(uint32_t) * nodes.size(), hipMemcpyHostToDevice,
@@ -471,7 +489,10 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
         // Geo3d::Substep with a vertex-ring energy: ring colours (the first
         // advances the clock), velocity, then GeoLinear3d::Damp's ring pass
         // and PBD tet pass (Geo.cpp:790-811), then the next prediction.
-        uint32_t rb = s->ring_block;
+        // rings per workgroup: the scene's workgroup size over the lanes per
+        // ring, halved until the rings' LDS fits the budget
+        uint32_t rb = std::max(1u, block / xf_ring_lanes());
+        while (rb > 1 && rb * xf_ring_team_bytes(s->rnmax, s->rtmax) > kRingLdsLimit) rb /= 2;
         int rc = node(0, false);
         if (rc) return rc;
         for (uint32_t k = 0; k < substeps; k++) {
@@ -483,9 +504,9 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                     XF_HIP(ctx, hipEventCreate(&e1));
                     XF_HIP(ctx, hipEventRecord(e0, st));
                 }
-                XF_HIP(ctx, xf_launch_rings(st, false, rb, slabs, begin, count, s->rnmax, s->rmeta, s->rinfo,
-                                            s->rnodes, s->rtet, s->X, s->O, s->V, s->bp, dt,
-                                            c == 0 ? s->clock : nullptr));
+                XF_HIP(ctx, xf_launch_rings(st, false, rb, slabs, begin, count, s->rnmax, s->rtmax, s->rmeta,
+                                            s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet, s->X, s->O, s->V,
+                                            s->bp, dt, c == 0 ? s->clock : nullptr));
                 if (timer) {
                     XF_HIP(ctx, hipEventRecord(e1, st));
                     timer->ev_proj.push_back(e0);
@@ -502,8 +523,9 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
             if (s->post & 1u)
                 for (uint32_t c = 0; c < s->rcol; c++) {
                     uint32_t begin = s->ring_colour_begin[c], count = s->ring_colour_begin[c + 1] - begin;
-                    XF_HIP(ctx, xf_launch_rings(st, true, rb, slabs, begin, count, s->rnmax, s->rmeta, s->rinfo,
-                                                s->rnodes, s->rtet, s->X, s->O, s->V, s->bp, dt, s->clock));
+                    XF_HIP(ctx, xf_launch_rings(st, true, rb, slabs, begin, count, s->rnmax, s->rtmax, s->rmeta,
+                                                s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet, s->X, s->O, s->V,
+                                                s->bp, dt, s->clock));
                 }
             if (s->post & 2u)
                 for (uint32_t c = 0; c < s->ncol; c++) {
@@ -713,7 +735,7 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     DropGraph(s);
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
-                    s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rtet};
+                    s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -1250,8 +1272,8 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
                                {XF_LAUNCH_PERSISTENT, 512}};
     xf_scene_opts keep = s->opts;
     Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size};
-    int maxb = 0;
-    XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
+    int maxb = 0;  // ring energies run per colour only
+    if (energy != kRingMode) XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
     float best = -1.0f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t he = copy_all(true);
