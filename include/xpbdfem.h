@@ -172,6 +172,8 @@ typedef struct xf_scene_opts {
                                    sweeps one spatial slab and node lines are fetched by one L2.
                                    Changes the sweep order inside colours (a valid serial order
                                    still; xf_scene_get_order reports it). */
+    uint32_t ring_lanes;        /* lanes per vertex ring in ring-energy launches: 16, 32, or 0 = 32
+                                   when some ring has more than 16 tets, else 16 */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {

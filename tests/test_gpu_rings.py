@@ -86,9 +86,11 @@ def test_ring_colouring_is_valid(ctx):
 
 @pytest.mark.parametrize("energy", [8, 10])
 @pytest.mark.parametrize("block,slabs", [(64, False), (256, False), (64, True), (128, True)])
-def test_ring_launch_shapes_bit_exact(ctx, energy, block, slabs):
+@pytest.mark.parametrize("lanes", [16, 32])
+def test_ring_launch_shapes_bit_exact(ctx, energy, block, slabs, lanes):
+    """Rings of 24 tets: with 16 lanes per ring a lane takes 1 or 2 tets."""
     wl = ring_beam(energy, "limit")
-    x, v, order, rorder, _, _, _ = run(ctx, wl, block_size=block, xcd_slabs=slabs)
+    x, v, order, rorder, _, _, _ = run(ctx, wl, block_size=block, xcd_slabs=slabs, ring_lanes=lanes)
     ob = oracle(wl, order, rorder, 2 * wl["substeps"])
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
 
@@ -124,9 +126,24 @@ def test_ring_batched_bodies_bit_exact(ctx):
     sc.close()
 
 
+@pytest.mark.parametrize("damping", ["paper", "post"])
+def test_ring_energies_sixteen_lanes(ctx, damping):
+    """Damped forms with 16 lanes per ring (the automatic choice here is 32)."""
+    for energy in (8, 9, 10):
+        wl = ring_beam(energy, damping)
+        x, v, order, rorder, _, _, _ = run(ctx, wl, ring_lanes=16)
+        ob = oracle(wl, order, rorder, 2 * wl["substeps"])
+        assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V), energy
+
+
 def test_ring_mode_rejections(ctx):
     wl = ring_beam(9)
     dt = float(wl["dt"])
+    sc = xf.Scene(ctx)
+    sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"])
+    with pytest.raises(xf.XfError):
+        sc.finalize(ring_lanes=8)
+    sc.close()
     sc = workloads.make_scene(ctx, wl, launch_mode=xf.LAUNCH_PERSISTENT)
     with pytest.raises(xf.XfError):
         sc.step(1, dt)

@@ -18,8 +18,11 @@
 //   vertex rings in sweep order (colour-major, body-major inside a colour),
 //   built when a body first uses a vertex-ring energy:
 //          rmeta[R] uint4 (node offset, node count, ring-tet offset, tet count)
-//          rinfo[R] uint2 (body id, position in the body's ring sweep order)
+//          rinfo[R] uint4 (body id, position in the body's ring sweep order,
+//            ring volume as float bits, 0)
 //          rnodes[]  uint32 global node ids, the ring's own vertex first
+//          rincb[] uint16 per ring node, rinc[4 * ring tets] uint16: each ring
+//            node's (ring tet << 2 | corner) slots in ring-tet order
 //          rtet[2 * ring tets] float4 pairs (QQ0 QQ1 QQ2 QR0 | QR1 QR2 V idx)
 //            idx = four ring-local node indices, one byte each, as float bits
 //   body table  XfBodyParams[B]

@@ -10,16 +10,18 @@
 // Rings of one colour share no node (the host colours the ring conflict
 // graph), so a colour is one launch with plain gathers and scatters, and
 // colours in order equal the reference's serial sweep over the colour-major
-// ring order. Inside a launch a team of kRingLanes lanes works on one ring,
-// in LDS, in phases separated by workgroup barriers:
-//   0  load the ring's FP64 node positions; one lane sums the ring volume;
+// ring order. Inside a launch a team of 16 or 32 lanes (a launch option)
+// works on one ring, in LDS, in phases separated by workgroup barriers:
+//   0  load the ring's FP64 node positions and its incidence lists (the ring
+//      volume, a sum of rest volumes, comes precomputed in the same order);
 //   1  per ring tet (lanes in parallel): I1 and J of the tet and their
 //      weighted gradient contributions at its 4 corners;
 //   2  per ring node (lanes in parallel): the contributions of the node's
-//      tets added in ring-tet order (host-built incidence lists); one lane
-//      adds the energy terms in ring-tet order;
-//   3  per ring node: SolveSimplex's scaling, then one lane forms the
-//      constraint sums in ring-node order and the Lagrange multipliers;
+//      tets added in ring-tet order (host-built incidence lists); two lanes
+//      add the two energy terms in ring-tet order;
+//   3  per ring node: SolveSimplex's scaling; then one lane per constraint
+//      sum forms that sum in ring-node order, and one lane the Lagrange
+//      multipliers;
 //   4  per ring node: the position (or velocity) update.
 // Every sum keeps the reference's order (ring tets in OneRing order, ring
 // nodes in OneRing order), compiled with -ffp-contract=off: a serial CPU
@@ -31,24 +33,26 @@
 
 namespace {
 
-constexpr uint32_t kRingLanes = 16;
 // Floats per ring tet in LDS: weighted I1, weighted J, then the I1 and J
 // gradient contributions at the 4 corners (4 x 3 each).
 constexpr uint32_t kTetFloats = 2 + 12 + 12;
-constexpr uint32_t kScalars = 8;
+constexpr uint32_t kScalars = 16;
 
 // LDS of one ring team: positions [3][nmax] double, inverse masses [nmax],
 // gradients [6][nmax] (g0 xyz, g1 xyz), per-node velocity-like vectors
 // [3][nmax] (X - O, or V), per-tet contributions [tmax][kTetFloats],
-// scalars [kScalars].
+// scalars [kScalars], incidence lists: per-node starts [nmax + 1] and
+// entries [4 tmax] (uint16).
 __host__ __device__ inline size_t ring_team_bytes(uint32_t nmax, uint32_t tmax) {
-    size_t b = 8 * 3 * (size_t)nmax + 4 * ((size_t)nmax * (1 + 6 + 3) + (size_t)tmax * kTetFloats + kScalars);
+    size_t b = 8 * 3 * (size_t)nmax + 4 * ((size_t)nmax * (1 + 6 + 3) + (size_t)tmax * kTetFloats + kScalars) +
+               2 * ((size_t)nmax + 1 + 4 * (size_t)tmax);
     return (b + 7) & ~(size_t)7;
 }
 
 struct Team {
     double* xs;
     float *ws, *g, *v, *c, *sc;
+    uint16_t *incb, *inc;
     uint32_t nmax;
     __device__ double& x(uint32_t r, uint32_t n) const { return xs[r * nmax + n]; }
     __device__ float& gr(uint32_t k, uint32_t r, uint32_t n) const { return g[(k * 3 + r) * nmax + n]; }
@@ -58,14 +62,6 @@ struct Team {
 
 __device__ __forceinline__ float dot3(const float* a, const float* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-
-__device__ __forceinline__ float gdot(const Team& T, uint32_t ka, uint32_t kb, uint32_t n) {
-    return T.gr(ka, 0, n) * T.gr(kb, 0, n) + T.gr(ka, 1, n) * T.gr(kb, 1, n) + T.gr(ka, 2, n) * T.gr(kb, 2, n);
-}
-
-__device__ __forceinline__ float vdot(const Team& T, uint32_t k, uint32_t n) {
-    return T.gr(k, 0, n) * T.vr(0, n) + T.gr(k, 1, n) * T.vr(1, n) + T.gr(k, 2, n) * T.vr(2, n);
 }
 
 // adjugate(mat3) and determinantFromAdjugate (Fem.cpp:284-306); F[c][r] is
@@ -135,10 +131,10 @@ __device__ void ring_tet(const Team& T, float4 a, float4 b, float vsum, float* o
 // (GeoLinear3d::Damp, Geo.cpp:802-804) of bodies with type Post or
 // PostAmortized and damping > 0, over the amortized eighth of the body's
 // ring order when PostAmortized (damping x 8).
-template <bool POST>
+template <bool POST, uint32_t LANES>
 __global__ __launch_bounds__(256) void xf_ring_colour(
     uint32_t begin, uint32_t count, uint32_t slabs, uint32_t nmax, uint32_t tmax, const uint4* __restrict__ rmeta,
-    const uint2* __restrict__ rinfo, const uint32_t* __restrict__ rnodes, const uint16_t* __restrict__ rincb,
+    const uint4* __restrict__ rinfo, const uint32_t* __restrict__ rnodes, const uint16_t* __restrict__ rincb,
     const uint16_t* __restrict__ rinc, const float4* __restrict__ rtet, double4* __restrict__ X,
     const double4* __restrict__ O, double4* __restrict__ V, const XfBodyParams* __restrict__ bodies, float dt,
     float* __restrict__ clock) {
@@ -148,7 +144,7 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
         uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
         tick[0] = tick[0] + 1u;
     }
-    const uint32_t lane = threadIdx.x % kRingLanes, team = threadIdx.x / kRingLanes;
+    const uint32_t lane = threadIdx.x % LANES, team = threadIdx.x / LANES;
     Team T;
     T.nmax = nmax;
     T.xs = reinterpret_cast<double*>(reinterpret_cast<char*>(xf_ring_lds) + team * ring_team_bytes(nmax, tmax));
@@ -157,16 +153,20 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
     T.v = T.g + 6 * nmax;
     T.c = T.v + 3 * nmax;
     T.sc = T.c + tmax * kTetFloats;
+    T.incb = reinterpret_cast<uint16_t*>(T.sc + kScalars);
+    T.inc = T.incb + nmax + 1;
 
     // Every lane reaches every barrier; a team without a ring (colour tail,
     // or a body the damping pass skips) has nn = nt = 0.
-    uint32_t i = xf::colour_slot(begin, count, slabs, kRingLanes);
+    uint32_t i = xf::colour_slot(begin, count, slabs, LANES);
     uint4 m = make_uint4(0, 0, 0, 0);
+    float volume = 0.0f;
     uint32_t energy = 0, rtype = 0;
     float damp = 0.0f, nu = 0.0f, compliance = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
     if (i != xf::kNoSlot) {
         m = rmeta[i];
-        uint2 info = rinfo[i];
+        uint4 info = rinfo[i];
+        volume = __uint_as_float(info.z);
         const XfBodyParams& bp = bodies[info.x];
         energy = bp.energy;
         rtype = bp.rtype;
@@ -190,35 +190,31 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
     const uint32_t nn = m.y, nt = m.w;
     const uint32_t* nodes = rnodes + m.x;
 
-    // Phase 0: positions and inverse masses; the ring volume.
-    for (uint32_t n = lane; n < nn; n += kRingLanes) {
+    // Phase 0: positions, inverse masses and incidence lists.
+    for (uint32_t n = lane; n < nn; n += LANES) {
         double4 x = X[nodes[n]];
         T.x(0, n) = x.x;
         T.x(1, n) = x.y;
         T.x(2, n) = x.z;
         T.ws[n] = (float)x.w;
+        T.incb[n] = rincb[m.x + n];
     }
-    if (lane == 0 && nt > 0) {
-        float vsum = 0.0f;
-        for (uint32_t k = 0; k < nt; k++) vsum = vsum + 0.25f * rtet[2 * (m.z + k) + 1].z;
-        T.sc[0] = vsum;
-    }
+    if (lane == 0) T.incb[nn] = (uint16_t)(4u * nt);
+    for (uint32_t e = lane; e < 4u * nt; e += LANES) T.inc[e] = rinc[4 * m.z + e];
     __syncthreads();
 
     // Phase 1: per ring tet.
-    if (nt > 0) {
-        float vsum = T.sc[0];
-        for (uint32_t k = lane; k < nt; k += kRingLanes)
-            ring_tet(T, rtet[2 * (m.z + k)], rtet[2 * (m.z + k) + 1], vsum, T.tet(k));
-    }
+    for (uint32_t k = lane; k < nt; k += LANES)
+        ring_tet(T, rtet[2 * (m.z + k)], rtet[2 * (m.z + k) + 1], volume, T.tet(k));
     __syncthreads();
 
     // Phase 2: ring-node gradients in ring-tet order; energy terms.
-    for (uint32_t n = lane; n < nn; n += kRingLanes) {
-        uint32_t e0 = rincb[m.x + n], e1 = n + 1 < nn ? rincb[m.x + n + 1] : 4u * nt;
+    for (uint32_t n = lane; n < nn; n += LANES) {
+        uint32_t e0 = T.incb[n], e1 = T.incb[n + 1];
         float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
         for (uint32_t e = e0; e < e1; e++) {
-            uint32_t code = rinc[4 * m.z + e];
+            uint32_t code = T.inc[e];
             const float* t = T.tet(code >> 2);
             uint32_t j = code & 3u;
             for (uint32_t r = 0; r < 3; r++) {
@@ -231,24 +227,21 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
             T.gr(1, r, n) = a[3 + r];
         }
     }
-    if (lane == 0 && nt > 0) {
-        float U0 = 0.0f, U1 = 0.0f;
-        for (uint32_t k = 0; k < nt; k++) {
-            U0 = U0 + T.tet(k)[0];
-            U1 = U1 + T.tet(k)[1];
-        }
-        T.sc[1] = U0;
-        T.sc[2] = U1;
+    if (lane >= LANES - 2 && nt > 0) {
+        uint32_t q = lane - (LANES - 2);  // 0: I1 term, 1: J term
+        float u = 0.0f;
+#pragma unroll 4
+        for (uint32_t k = 0; k < nt; k++) u = u + T.tet(k)[q];
+        T.sc[1 + q] = u;
     }
     __syncthreads();
 
     // Phase 3: SolveSimplex (Geo.cpp:13-64): constants, gradient scaling and
     // the per-node vectors of the damping terms.
-    float volume = 0.0f, I1 = 0.0f, J = 0.0f, U0 = 0.0f, U1 = 0.0f, comp0 = 0.0f, comp1 = 0.0f;
+    float I1 = 0.0f, J = 0.0f, U0 = 0.0f, U1 = 0.0f, comp0 = 0.0f, comp1 = 0.0f;
     bool two = energy != 8u;
     bool damped_proj = !POST && damp > 0.0f && rtype < 2u;
     if (nt > 0) {
-        volume = T.sc[0];
         I1 = T.sc[1];
         J = T.sc[2];
         if (!two) {
@@ -257,7 +250,7 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
             float a = 1.0f + mu / lambda;
             comp0 = compliance / volume;
             U0 = fmaxf(0.0000001f, (mu / 2.0f) * I1 + (lambda / 2.0f) * (J - a) * (J - a));
-            for (uint32_t n = lane; n < nn; n += kRingLanes)
+            for (uint32_t n = lane; n < nn; n += LANES)
                 for (uint32_t r = 0; r < 3; r++)
                     T.gr(0, r, n) = (mu / 2.0f) * T.gr(0, r, n) + (lambda / 2.0f) * 2.0f * (J - a) * T.gr(1, r, n);
         } else {
@@ -276,14 +269,14 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
             }
             U1 = (J - a) * (J - a);
             float s1 = 2.0f * (J - a);
-            for (uint32_t n = lane; n < nn; n += kRingLanes)
+            for (uint32_t n = lane; n < nn; n += LANES)
                 for (uint32_t r = 0; r < 3; r++) {
                     if (skin) T.gr(0, r, n) = T.gr(0, r, n) * sc;
                     T.gr(1, r, n) = T.gr(1, r, n) * s1;
                 }
         }
         if (POST || damped_proj)
-            for (uint32_t n = lane; n < nn; n += kRingLanes) {
+            for (uint32_t n = lane; n < nn; n += LANES) {
                 if (POST) {
                     double4 vd = V[nodes[n]];
                     T.vr(0, n) = (float)vd.x;
@@ -297,50 +290,59 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
     }
     __syncthreads();
 
-    // Phase 3b: one lane forms the sums in ring-node order and the multipliers.
+    // Phase 3b: one lane per constraint sum, each in ring-node order, all
+    // lanes running the same loop on their own operands (lane q: a = A_q,
+    // b = B_q, weighted by w or not, added or subtracted):
+    //   sc[5] = 1e-22 + sum w g0.g0     sc[6] = 1e-22 + sum w g1.g0
+    //   sc[7] = 1e-22 + sum w g1.g1     sc[8] = sum +-g0.v, sc[9] = sum +-g1.v
+    // with v = X - O (projection) or V (damping pass, where the two-constraint
+    // RayleighDamp sums -g.v), the sums of Xpbd.h:87-263. Multiplying by 1
+    // and negating are exact, so each sum rounds as the reference's does.
+    if (nt > 0 && lane < 5) {
+        const float* a = T.g + (lane == 1 || lane == 2 || lane == 4 ? 3 * nmax : 0);
+        const float* b = lane <= 1 ? T.g : lane == 2 ? T.g + 3 * nmax : T.v;
+        bool weighted = lane <= 2;
+        float sign = lane >= 3 && POST && two ? -1.0f : 1.0f;
+        float acc = weighted ? 1.0e-22f : 0.0f;
+#pragma unroll 4
+        for (uint32_t n = 0; n < nn; n++) {
+            float d = a[n] * b[n] + a[nmax + n] * b[nmax + n] + a[2 * nmax + n] * b[2 * nmax + n];
+            float term = weighted ? T.ws[n] * d : d;
+            acc = acc + sign * term;
+        }
+        T.sc[5 + lane] = acc;
+    }
+    __syncthreads();
+
+    // Phase 3c: one lane forms the Lagrange multipliers.
     if (lane == 0 && nt > 0) {
         float l0 = 0.0f, l1 = 0.0f;
+        float wgg[3] = {T.sc[5], T.sc[6], T.sc[7]};
+        float gV[2] = {T.sc[8], T.sc[9]};
         if (!two && !POST) {
             // EnergyXpbdConstrain (Xpbd.h:87-120)
             float alpha = comp0 / (dt * dt);
             if (damped_proj) {
                 float gamma = damp / dt;
-                float wgg = 1.0e-22f, gV = 0.0f;
-                for (uint32_t n = 0; n < nn; n++) {
-                    wgg = wgg + T.ws[n] * gdot(T, 0, 0, n);
-                    gV = gV + vdot(T, 0, n);
-                }
                 if (rtype == 0u) {
-                    float A = wgg * (1.0f + gamma) + (2.0f * U0 * alpha);
-                    l0 = (-2.0f * U0 - gamma * gV) / A;
+                    float A = wgg[0] * (1.0f + gamma) + (2.0f * U0 * alpha);
+                    l0 = (-2.0f * U0 - gamma * gV[0]) / A;
                 } else {
-                    l0 = (-2.0f * U0) / (wgg + (2.0f * U0 * alpha));
+                    l0 = (-2.0f * U0) / (wgg[0] + (2.0f * U0 * alpha));
                     float invBeta = comp0 / (dt * damp);
-                    float A = wgg + 2.0f * U0 * invBeta;
-                    float b = -gV - wgg * l0;
-                    b = b * (A / fmaxf(A, 4.0f * wgg));
+                    float A = wgg[0] + 2.0f * U0 * invBeta;
+                    float b = -gV[0] - wgg[0] * l0;
+                    b = b * (A / fmaxf(A, 4.0f * wgg[0]));
                     l0 = l0 + b / A;
                 }
             } else {
-                float wgg = 1.0e-22f;
-                for (uint32_t n = 0; n < nn; n++) wgg = wgg + T.ws[n] * gdot(T, 0, 0, n);
-                l0 = (-2.0f * U0) / (wgg + (2.0f * U0 * alpha));
+                l0 = (-2.0f * U0) / (wgg[0] + (2.0f * U0 * alpha));
             }
         } else if (!POST) {
             // EnergyXpbdConstrainSimultaneous<2> (Xpbd.h:123-214)
             float U[2] = {U0, U1}, comp[2] = {comp0, comp1};
             float alpha[2] = {comp0 / (dt * dt), comp1 / (dt * dt)};
             float gamma = damp / dt;
-            float gV[2] = {0.0f, 0.0f};
-            if (damped_proj)
-                for (uint32_t n = 0; n < nn; n++) {
-                    gV[0] = gV[0] + vdot(T, 0, n);
-                    gV[1] = gV[1] + vdot(T, 1, n);
-                }
-            float wgg[3] = {1.0e-22f, 1.0e-22f, 1.0e-22f};
-            for (uint32_t n = 0; n < nn; n++) wgg[0] = wgg[0] + T.ws[n] * gdot(T, 0, 0, n);
-            for (uint32_t n = 0; n < nn; n++) wgg[1] = wgg[1] + T.ws[n] * gdot(T, 1, 0, n);
-            for (uint32_t n = 0; n < nn; n++) wgg[2] = wgg[2] + T.ws[n] * gdot(T, 1, 1, n);
             float A[3] = {wgg[0], wgg[1], wgg[2]}, b[2];
             for (int k = 0; k < 2; k++) {
                 int d = 2 * k;
@@ -377,26 +379,14 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
         } else if (!two) {
             // RayleighDamp, one constraint (Xpbd.h:216-230)
             float invBeta = comp0 / (dt * damp);
-            float wgg = 1.0e-22f, gV = 0.0f;
-            for (uint32_t n = 0; n < nn; n++) {
-                wgg = wgg + T.ws[n] * gdot(T, 0, 0, n);
-                gV = gV + vdot(T, 0, n);
-            }
-            l0 = -gV / (2.0f * U0 * invBeta + wgg);
+            l0 = -gV[0] / (2.0f * U0 * invBeta + wgg[0]);
         } else {
-            // RayleighDamp, two coupled constraints (Xpbd.h:232-263)
+            // RayleighDamp, two coupled constraints (Xpbd.h:232-263); gV holds b
             float invBeta0 = comp0 / (dt * damp), invBeta1 = comp1 / (dt * damp);
-            float A0 = 1.0e-22f, A1 = 1.0e-22f, A2 = 1.0e-22f, b0 = 0.0f, b1 = 0.0f;
-            for (uint32_t n = 0; n < nn; n++) {
-                float wn = T.ws[n];
-                A0 = A0 + wn * gdot(T, 0, 0, n);
-                A1 = A1 + wn * gdot(T, 0, 1, n);
-                A2 = A2 + wn * gdot(T, 1, 1, n);
-                b0 = b0 - vdot(T, 0, n);
-                b1 = b1 - vdot(T, 1, n);
-            }
-            A0 = A0 + 2.0f * U0 * invBeta0;
-            A2 = A2 + 2.0f * U1 * invBeta1;
+            float A0 = wgg[0] + 2.0f * U0 * invBeta0;
+            float A1 = wgg[1];
+            float A2 = wgg[2] + 2.0f * U1 * invBeta1;
+            float b0 = gV[0], b1 = gV[1];
             float invA00 = 1.0f / A0;
             float invA11 = 1.0f / A2;
             float invDet = 1.0f / fmaxf(0.00000001f, 1.0f - (A1 * invA00) * (A1 * invA11));
@@ -411,22 +401,25 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
     // Phase 4: node updates.
     if (nt > 0) {
         float l0 = T.sc[3], l1 = T.sc[4];
-        for (uint32_t n = lane; n < nn; n += kRingLanes) {
+        for (uint32_t n = lane; n < nn; n += LANES) {
             float wn = T.ws[n];
             if (!POST) {
-                double4 x = X[nodes[n]];
-                double* xr[3] = {&x.x, &x.y, &x.z};
+                double xr[3];
                 for (uint32_t r = 0; r < 3; r++) {
                     if (!two) {
                         float s = wn * l0;
-                        *xr[r] = T.x(r, n) + (double)(s * T.gr(0, r, n));
+                        xr[r] = T.x(r, n) + (double)(s * T.gr(0, r, n));
                     } else {
                         float acc = 0.0f + l0 * T.gr(0, r, n);
                         acc = acc + l1 * T.gr(1, r, n);
-                        *xr[r] = T.x(r, n) + (double)(wn * acc);
+                        xr[r] = T.x(r, n) + (double)(wn * acc);
                     }
                 }
-                X[nodes[n]] = x;
+                // x, y, z only: w (the inverse mass) stays as stored
+                double* xn = reinterpret_cast<double*>(X + nodes[n]);
+                xn[0] = xr[0];
+                xn[1] = xr[1];
+                xn[2] = xr[2];
             } else {
                 double4 vd = V[nodes[n]];
                 double* vr[3] = {&vd.x, &vd.y, &vd.z};
@@ -442,28 +435,26 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
 
 }  // namespace
 
-// Lanes per ring and LDS bytes of one ring team (nmax ring nodes, tmax ring
-// tets at most).
-extern "C" uint32_t xf_ring_lanes(void) { return kRingLanes; }
+// LDS bytes of one ring team (nmax ring nodes, tmax ring tets at most).
 extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax) { return ring_team_bytes(nmax, tmax); }
 
 // One colour of the ring sweep (post = false) or of the ring damping pass
-// (post = true), `teams` rings per workgroup. clock: advanced by the first
-// colour of a substep (pass it for that launch only); read by the damping
-// pass for the amortized range.
-extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t teams, uint32_t slabs, uint32_t begin,
-                                      uint32_t count, uint32_t nmax, uint32_t tmax, const uint4* rmeta,
-                                      const uint2* rinfo, const uint32_t* rnodes, const uint16_t* rincb,
-                                      const uint16_t* rinc, const float4* rtet, double4* X, const double4* O,
-                                      double4* V, const XfBodyParams* bodies, float dt, float* clock) {
+// (post = true), `teams` rings per workgroup and `lanes` (16 or 32) lanes per
+// ring. clock: advanced by the first colour of a substep (pass it for that
+// launch only); read by the damping pass for the amortized range.
+extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t lanes, uint32_t teams, uint32_t slabs,
+                                      uint32_t begin, uint32_t count, uint32_t nmax, uint32_t tmax,
+                                      const uint4* rmeta, const uint4* rinfo, const uint32_t* rnodes,
+                                      const uint16_t* rincb, const uint16_t* rinc, const float4* rtet, double4* X,
+                                      const double4* O, double4* V, const XfBodyParams* bodies, float dt,
+                                      float* clock) {
     if (count == 0) return hipSuccess;
+    if ((lanes != 16 && lanes != 32) || teams == 0 || teams * lanes > 256) return hipErrorInvalidValue;
     size_t lds = (size_t)teams * ring_team_bytes(nmax, tmax);
-    dim3 grid(xf::colour_grid(count, teams, slabs)), block(teams * kRingLanes);
-    if (post)
-        hipLaunchKernelGGL(xf_ring_colour<true>, grid, block, lds, s, begin, count, slabs, nmax, tmax, rmeta, rinfo,
-                           rnodes, rincb, rinc, rtet, X, O, V, bodies, dt, clock);
-    else
-        hipLaunchKernelGGL(xf_ring_colour<false>, grid, block, lds, s, begin, count, slabs, nmax, tmax, rmeta, rinfo,
-                           rnodes, rincb, rinc, rtet, X, O, V, bodies, dt, clock);
+    dim3 grid(xf::colour_grid(count, teams, slabs)), block(teams * lanes);
+    auto k = post ? (lanes == 16 ? xf_ring_colour<true, 16> : xf_ring_colour<true, 32>)
+                  : (lanes == 16 ? xf_ring_colour<false, 16> : xf_ring_colour<false, 32>);
+    hipLaunchKernelGGL(k, grid, block, lds, s, begin, count, slabs, nmax, tmax, rmeta, rinfo, rnodes, rincb, rinc,
+                       rtet, X, O, V, bodies, dt, clock);
     return hipGetLastError();
 }

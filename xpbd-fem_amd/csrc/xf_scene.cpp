@@ -37,12 +37,12 @@ extern "C" hipError_t xf_launch_volume(hipStream_t s, uint32_t block, uint32_t s
                                        uint32_t pass,
                                        const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
                                        const float4* taux, double4* X, const XfBodyParams* bodies, float dt);
-extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t teams, uint32_t slabs, uint32_t begin,
-                                      uint32_t count, uint32_t nmax, uint32_t tmax, const uint4* rmeta,
-                                      const uint2* rinfo, const uint32_t* rnodes, const uint16_t* rincb,
-                                      const uint16_t* rinc, const float4* rtet, double4* X, const double4* O,
-                                      double4* V, const XfBodyParams* bodies, float dt, float* clock);
-extern "C" uint32_t xf_ring_lanes(void);
+extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t lanes, uint32_t teams, uint32_t slabs,
+                                      uint32_t begin, uint32_t count, uint32_t nmax, uint32_t tmax,
+                                      const uint4* rmeta, const uint4* rinfo, const uint32_t* rnodes,
+                                      const uint16_t* rincb, const uint16_t* rinc, const float4* rtet, double4* X,
+                                      const double4* O, double4* V, const XfBodyParams* bodies, float dt,
+                                      float* clock);
 extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
@@ -174,7 +174,7 @@ struct xf_scene {
     uint32_t R = 0, rcol = 0, rnmax = 0, rtmax = 0;
     std::vector<uint32_t> ring_colour_begin;
     uint4* rmeta = nullptr;
-    uint2* rinfo = nullptr;
+    uint4* rinfo = nullptr;  // body id, position in the body's ring order, ring volume bits, 0
     uint32_t* rnodes = nullptr;
     uint16_t* rincb = nullptr;  // per ring node: first entry of its incidence list (ring-local)
     uint16_t* rinc = nullptr;   // per ring: (ring tet << 2 | corner) of each node, node-major, tets ascending
@@ -390,7 +390,7 @@ int UploadRings(xf_scene* s) {
     if (rc) return rc;
     xf_ctx* ctx = s->ctx;
     std::vector<uint4> meta(s->R);
-    std::vector<uint2> info(s->R);
+    std::vector<uint4> info(s->R);
     std::vector<uint32_t> nodes;
     std::vector<uint16_t> incb, inc;
     std::vector<float4> rtet;
@@ -407,7 +407,12 @@ int UploadRings(xf_scene* s) {
                 uint32_t v = b.ring_order[k++];
                 uint32_t n0 = r.node_off[v], n1 = r.node_off[v + 1], t0 = r.tet_off[v], t1 = r.tet_off[v + 1];
                 meta[pos] = make_uint4((uint32_t)nodes.size(), n1 - n0, (uint32_t)(rtet.size() / 2), t1 - t0);
-                info[pos] = make_uint2((uint32_t)bi, k - 1);
+                // ring volume: sum of V_t / 4 in ring-tet order (SolveRing, Geo.cpp:160-163)
+                float vsum = 0.0f;
+                for (uint32_t q = t0; q < t1; q++) vsum = vsum + 0.25f * b.ec.vol[r.tets[q]];
+                uint32_t vbits;
+                memcpy(&vbits, &vsum, 4);
+                info[pos] = make_uint4((uint32_t)bi, k - 1, vbits, 0u);
                 for (uint32_t q = n0; q < n1; q++) nodes.push_back(b.node_base + r.nodes[q]);
                 // incidence lists: for each ring node, its (tet, corner) slots in ring-tet order
                 std::vector<uint16_t> cnt(n1 - n0 + 1, 0);
@@ -447,7 +452,7 @@ int UploadRings(xf_scene* s) {
     XF_HIP(ctx, hipMemcpyAsync(s->rinc, inc.data(), sizeof(uint16_t) * inc.size(), hipMemcpyHostToDevice,
                                ctx->stream));
     XF_HIP(ctx, hipMemcpyAsync(s->rmeta, meta.data(), sizeof(uint4) * s->R, hipMemcpyHostToDevice, ctx->stream));
-    XF_HIP(ctx, hipMemcpyAsync(s->rinfo, info.data(), sizeof(uint2) * s->R, hipMemcpyHostToDevice, ctx->stream));
+    XF_HIP(ctx, hipMemcpyAsync(s->rinfo, info.data(), sizeof(uint4) * s->R, hipMemcpyHostToDevice, ctx->stream));
     XF_HIP(ctx, hipMemcpyAsync(s->rnodes, nodes.data(), sizeof(uint32_t) * nodes.size(), hipMemcpyHostToDevice,
                                ctx->stream));
     XF_HIP(ctx, hipMemcpyAsync(s->rtet, rtet.data(), sizeof(float4) * rtet.size(), hipMemcpyHostToDevice,
@@ -489,9 +494,12 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
         // Geo3d::Substep with a vertex-ring energy: ring colours (the first
         // advances the clock), velocity, then GeoLinear3d::Damp's ring pass
         // and PBD tet pass (Geo.cpp:790-811), then the next prediction.
-        // rings per workgroup: the scene's workgroup size over the lanes per
-        // ring, halved until the rings' LDS fits the budget
-        uint32_t rb = std::max(1u, block / xf_ring_lanes());
+        // lanes per ring: the option, else 32 when rings have more than 16
+        // tets (a tet per lane in the gather); rings per workgroup: the
+        // scene's workgroup size over the lanes, halved until the rings' LDS
+        // fits the budget
+        uint32_t lanes = s->opts.ring_lanes ? s->opts.ring_lanes : (s->rtmax > 16 ? 32u : 16u);
+        uint32_t rb = std::max(1u, block / lanes);
         while (rb > 1 && rb * xf_ring_team_bytes(s->rnmax, s->rtmax) > kRingLdsLimit) rb /= 2;
         int rc = node(0, false);
         if (rc) return rc;
@@ -504,7 +512,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                     XF_HIP(ctx, hipEventCreate(&e1));
                     XF_HIP(ctx, hipEventRecord(e0, st));
                 }
-                XF_HIP(ctx, xf_launch_rings(st, false, rb, slabs, begin, count, s->rnmax, s->rtmax, s->rmeta,
+                XF_HIP(ctx, xf_launch_rings(st, false, lanes, rb, slabs, begin, count, s->rnmax, s->rtmax, s->rmeta,
                                             s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet, s->X, s->O, s->V,
                                             s->bp, dt, c == 0 ? s->clock : nullptr));
                 if (timer) {
@@ -523,7 +531,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
             if (s->post & 1u)
                 for (uint32_t c = 0; c < s->rcol; c++) {
                     uint32_t begin = s->ring_colour_begin[c], count = s->ring_colour_begin[c + 1] - begin;
-                    XF_HIP(ctx, xf_launch_rings(st, true, rb, slabs, begin, count, s->rnmax, s->rtmax, s->rmeta,
+                    XF_HIP(ctx, xf_launch_rings(st, true, lanes, rb, slabs, begin, count, s->rnmax, s->rtmax, s->rmeta,
                                                 s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet, s->X, s->O, s->V,
                                                 s->bp, dt, s->clock));
                 }
@@ -830,6 +838,8 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     if (s->opts.colour_order > XF_COLOUR_MINSTD) return Fail(ctx, XF_E_INVALID, "unknown colour order");
     if (s->opts.launch_mode > XF_LAUNCH_PERSISTENT) return Fail(ctx, XF_E_INVALID, "unknown launch mode");
     if (s->opts.xcd_slabs > 1) return Fail(ctx, XF_E_INVALID, "xcd_slabs must be 0 or 1");
+    if (s->opts.ring_lanes != 0 && s->opts.ring_lanes != 16 && s->opts.ring_lanes != 32)
+        return Fail(ctx, XF_E_INVALID, "ring_lanes must be 0, 16 or 32");
     XF_HIP(ctx, hipSetDevice(ctx->device));
 
     // Colour each body; the scene runs max(ncol) phases.

@@ -70,7 +70,8 @@ class BodyDesc(C.Structure):
 
 class SceneOpts(C.Structure):
     _fields_ = [("colour_order", C.c_uint32), ("block_size", C.c_uint32), ("use_graph", C.c_uint32),
-                ("launch_mode", C.c_uint32), ("persistent_blocks", C.c_uint32), ("xcd_slabs", C.c_uint32)]
+                ("launch_mode", C.c_uint32), ("persistent_blocks", C.c_uint32), ("xcd_slabs", C.c_uint32),
+                ("ring_lanes", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -386,9 +387,10 @@ class Scene:
         return bid.value
 
     def finalize(self, colour_order=COLOUR_NATURAL, block_size=64, use_graph=True, launch_mode=LAUNCH_PER_COLOUR,
-                 persistent_blocks=0, xcd_slabs=False):
+                 persistent_blocks=0, xcd_slabs=False, ring_lanes=0):
         o = SceneOpts()
         o.xcd_slabs = int(xcd_slabs)
+        o.ring_lanes = ring_lanes
         o.colour_order = colour_order
         o.block_size = block_size
         o.use_graph = 1 if use_graph else 0
