@@ -152,6 +152,11 @@ typedef struct xf_body_desc {
     const float* tet_mu;
     const uint32_t* tet_colour; /* nullable: caller's colouring (< 256 colours, node-disjoint
                                    classes, validated); else the scene's greedy colouring */
+    const double* xyz64;        /* nullable: FP64 positions (3 per node). When set they replace
+                                   xyz (which may then be NULL) for the initial node state and
+                                   the element constants -- the positions GeoLinear3d::Init
+                                   holds after autoResize (Geo.cpp:712-729), which float
+                                   cannot represent. */
 } xf_body_desc;
 
 /* Launch modes of xf_scene_step. */

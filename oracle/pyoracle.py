@@ -175,6 +175,8 @@ def ref():
         L = C.CDLL(REF_SO)
         L.xr_create.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float]
         L.xr_create.restype = C.c_void_p
+        L.xr_create_ex.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float, C.c_int]
+        L.xr_create_ex.restype = C.c_void_p
         L.xr_destroy.argtypes = [C.c_void_p]
         L.xr_vert_count.argtypes = [C.c_void_p]
         L.xr_vert_count.restype = C.c_uint32
@@ -384,12 +386,12 @@ def ref_armadillo_t4():
 class RefBody:
     """The compiled reference's GeoLinear3d driven through ref_harness."""
 
-    def __init__(self, xyz, tets, density, pinned=None, order=None):
+    def __init__(self, xyz, tets, density, pinned=None, order=None, autoresize=False):
         L = ref()
         self.n = xyz.size // 3
         self.t = tets.size // 4
-        self.h = L.xr_create(np.ascontiguousarray(xyz, np.float32), self.n, np.ascontiguousarray(tets, np.uint32),
-                             self.t, density)
+        self.h = L.xr_create_ex(np.ascontiguousarray(xyz, np.float32), self.n,
+                                np.ascontiguousarray(tets, np.uint32), self.t, density, int(autoresize))
         L.xr_set_pins(self.h, np.ascontiguousarray(pinned if pinned is not None else np.zeros(self.n), np.uint8))
         if order is not None:
             L.xr_set_order(self.h, np.ascontiguousarray(order, np.uint32))

@@ -66,8 +66,10 @@ Settings MakeSettings(const xr_settings* s) {
 extern "C" {
 
 // Builds a GeoLinear3d from xyz[n_nodes*3] and tets[n_tets*4] through the
-// reference's own ingestion path (index stream [4, i0, i1, i2, i3]*).
-void* xr_create(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density) {
+// reference's own ingestion path (index stream [4, i0, i1, i2, i3]*), with
+// Init's autoResize when auto_resize != 0.
+void* xr_create_ex(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                   int auto_resize) {
     RefBody* b = new RefBody();
     size_t bytes = (size_t)4096 * n_tets + (size_t)4096 * n_nodes + ((size_t)64 << 20);
     b->mem.resize(bytes);
@@ -78,8 +80,12 @@ void* xr_create(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32
         for (int c = 0; c < 4; c++) idx[5 * t + 1 + c] = tets[4 * t + c];
     }
     b->geo = b->alloc.New<GeoLinear3d>();
-    b->geo->Init(&b->alloc, density, xyz, 3 * n_nodes, idx.data(), (uint32_t)idx.size(), false);
+    b->geo->Init(&b->alloc, density, xyz, 3 * n_nodes, idx.data(), (uint32_t)idx.size(), auto_resize != 0);
     return b;
+}
+
+void* xr_create(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density) {
+    return xr_create_ex(xyz, n_nodes, tets, n_tets, density, 0);
 }
 
 void xr_destroy(void* h) { delete (RefBody*)h; }

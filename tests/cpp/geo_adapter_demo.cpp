@@ -35,7 +35,8 @@ struct DemoManipulator {
 
 int main(int argc, char** argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s OUT FRAMES SUBSTEPS ENERGY [RAYLEIGH_TYPE DAMPING PBD PBD_AMORTIZED]\n", argv[0]);
+        fprintf(stderr, "usage: %s OUT FRAMES SUBSTEPS ENERGY [RAYLEIGH_TYPE DAMPING PBD PBD_AMORTIZED [AUTORESIZE]]\n",
+                argv[0]);
         return 2;
     }
     const int frames = atoi(argv[2]), substeps = atoi(argv[3]);
@@ -51,7 +52,8 @@ int main(int argc, char** argv) {
         for (int c = 0; c < 4; c++) stream.push_back(tets[4 * t + c]);
     }
     xfgpu::GeoGpuLinear3d geo;
-    int rc = geo.Init(1000.0f, xyz.data(), (uint32_t)xyz.size(), stream.data(), (uint32_t)stream.size());
+    const bool auto_resize = argc >= 10 && atoi(argv[9]) != 0;  // GeoLinear3d::Init's autoResize
+    int rc = geo.Init(1000.0f, xyz.data(), (uint32_t)xyz.size(), stream.data(), (uint32_t)stream.size(), auto_resize);
     if (rc) {
         fprintf(stderr, "Init failed (%d): %s\n", rc, geo.LastError());
         return 1;

@@ -30,6 +30,12 @@ Fixtures (numpy .npz, no pickles):
                    substeps (tickId from 5) for the cases of its `cases` array
                    (energy, Poisson ratio, compliance, rayleigh type, damping,
                    pbd factor, amortized pbd factor).
+  ref_autoresize.npz
+                   the 12x4x4 beam through GeoLinear3d::Init with autoResize:
+                   the FP64 positions Init computes, and positions/velocities
+                   after 40 substeps (MixedSel, simultaneous, compliance 1e-5,
+                   nu 0.5, x = 0 face pinned, dt = 1/1200) over the greedy
+                   natural-order colour sweep order.
 """
 import os
 import sys
@@ -192,6 +198,19 @@ def rings():
     np.savez_compressed(os.path.join(HERE, "ref_rings.npz"), **out)
 
 
+def autoresize():
+    xyz, tets = po.tet_block(12, 4, 4, (0.1, 0.1, 0.1), (0.0, -0.2, -0.2))
+    pinned = (xyz[0::3] == 0.0).astype(np.uint8)
+    n = xyz.size // 3
+    order = natural_colour_order(n, tets)
+    rb = po.RefBody(xyz, tets, 1000.0, pinned, order, autoresize=True)
+    X0, _ = rb.state()
+    rb.substeps(rb.settings(1e-5, nu=0.5, energy=4, simultaneous=True), DT, 40)
+    X, V = rb.state()
+    np.savez_compressed(os.path.join(HERE, "ref_autoresize.npz"), xyz=xyz, tets=tets, pinned=pinned, order=order,
+                        X0=X0, X=X, V=V)
+
+
 if __name__ == "__main__":
     if not po.have_ref():
         sys.exit("oracle/_ref/libxfref.so is missing: build the oracle with the reference present")
@@ -199,5 +218,6 @@ if __name__ == "__main__":
     tet_kats()
     passes()
     rings()
-    for f in ("ref_beam12.npz", "ref_tet_kats.npz", "ref_passes.npz", "ref_rings.npz"):
+    autoresize()
+    for f in ("ref_beam12.npz", "ref_tet_kats.npz", "ref_passes.npz", "ref_rings.npz", "ref_autoresize.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

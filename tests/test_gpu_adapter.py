@@ -63,3 +63,37 @@ def test_adapter_ring_energies_bit_exact(ctx, tmp_path, energy):
     assert np.abs(X - wl["xyz"]).max() > 1e-4
     assert np.array_equal(X, ob.X)
     assert np.array_equal(V, ob.V)
+
+
+def _autoresize_golden():
+    return np.load(os.path.join(ROOT, "tests", "golden", "ref_autoresize.npz"))
+
+
+def test_scene_fp64_positions_autoresize_bit_exact(ctx):
+    """A body given FP64 (autoresized) positions: element constants and node
+    state from them reproduce the reference's Init(autoResize) trajectory."""
+    g = _autoresize_golden()
+    sc = xf.Scene(ctx)
+    sc.add_body(None, g["tets"], 1000.0, pinned=g["pinned"], xyz64=xf.autoresize(g["xyz"]))
+    sc.finalize()
+    sc.set_settings(0, energy=4, simultaneous=1, compliance=1e-5, poissons_ratio=0.5)
+    x0, _ = sc.get_state(0)
+    assert np.array_equal(x0, g["X0"])
+    assert np.array_equal(sc.order(0)[0], g["order"])
+    sc.step(40, float(np.float32(1.0) / np.float32(1200.0)))
+    x, v = sc.get_state(0)
+    sc.close()
+    assert np.array_equal(x, g["X"]) and np.array_equal(v, g["V"])
+
+
+def test_adapter_autoresize_bit_exact(tmp_path):
+    """GeoGpuLinear3d::Init(..., autoResize = true) driven like Sim::Update."""
+    g = _autoresize_golden()
+    out = tmp_path / "state.bin"
+    print(f"\nrunning {DEMO} with autoResize", flush=True)
+    r = subprocess.run([DEMO, str(out), "2", "20", "4", "0", "0", "0", "0", "1"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    n = g["xyz"].size // 3
+    data = np.fromfile(out, np.float64)
+    assert np.array_equal(data[:3 * n], g["X"]) and np.array_equal(data[3 * n:], g["V"])

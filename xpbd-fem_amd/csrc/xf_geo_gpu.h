@@ -46,8 +46,11 @@ class GeoGpuLinear3d {
     // [4, i0, i1, i2, i3]* (only tets). Nodes whose normalised x is below 2%
     // get the Left flag (Geo.cpp:724); they are pinned when the first
     // Substep sees Settings_LockLeft. Returns an xf_status.
+    // autoResize: positions become 0.085 (x - centre) / largest extent in FP64,
+    // as GeoLinear3d::Init computes them (Geo.cpp:712-729), after the 2 % pin
+    // rule has been applied to the original positions.
     int Init(float density, const float* nodeData, uint32_t nodeDataCount, const uint32_t* idxData,
-             uint32_t idxDataCount, int device = 0) {
+             uint32_t idxDataCount, bool autoResize = false, int device = 0) {
         if (nodeDataCount % 3) return XF_E_INVALID;
         nodes_.assign(nodeData, nodeData + nodeDataCount);
         uint32_t nt = 0;
@@ -65,6 +68,23 @@ class GeoGpuLinear3d {
         left_.assign(n_, 0);
         for (uint32_t i = 0; i < n_; i++)
             if ((nodes_[3 * i] - mn) / (mx - mn) < 0.02f) left_[i] = 1;
+        nodes64_.clear();
+        if (autoResize) {
+            float lo[3] = {1e24f, 1e24f, 1e24f}, hi[3] = {-1e24f, -1e24f, -1e24f};
+            for (uint32_t i = 0; i < n_; i++)
+                for (int r = 0; r < 3; r++) {
+                    lo[r] = std::fmin(lo[r], nodes_[3 * i + r]);
+                    hi[r] = std::fmax(hi[r], nodes_[3 * i + r]);
+                }
+            float dims[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+            double max_dim = std::fmax(dims[0], std::fmax(dims[1], dims[2]));
+            nodes64_.resize(3 * (size_t)n_);
+            for (uint32_t i = 0; i < n_; i++)
+                for (int r = 0; r < 3; r++) {
+                    double centre = 0.5 * (double)(lo[r] + hi[r]);
+                    nodes64_[3 * i + r] = 0.085 * (((double)nodes_[3 * i + r] - centre) / max_dim);
+                }
+        }
         return xf_ctx_create(device, &ctx_);
     }
 
@@ -137,7 +157,7 @@ class GeoGpuLinear3d {
     int GetState(double* X, double* V) {
         if (!scene_) {
             if (X)
-                for (uint32_t i = 0; i < 3 * n_; i++) X[i] = nodes_[i];
+                for (uint32_t i = 0; i < 3 * n_; i++) X[i] = nodes64_.empty() ? (double)nodes_[i] : nodes64_[i];
             if (V) memset(V, 0, sizeof(double) * 3 * n_);
             return XF_OK;
         }
@@ -155,7 +175,8 @@ class GeoGpuLinear3d {
         if (rc) return rc;
         xf_body_desc d;
         memset(&d, 0, sizeof(d));
-        d.xyz = nodes_.data();
+        if (nodes64_.empty()) d.xyz = nodes_.data();
+        else d.xyz64 = nodes64_.data();
         d.n_nodes = n_;
         d.tets = tets_.data();
         d.n_tets = (uint32_t)(tets_.size() / 4);
@@ -174,6 +195,7 @@ class GeoGpuLinear3d {
     xf_ctx* ctx_ = nullptr;
     xf_scene* scene_ = nullptr;
     std::vector<float> nodes_;
+    std::vector<double> nodes64_;  // autoresized FP64 positions (empty without autoResize)
     std::vector<uint32_t> tets_;
     std::vector<uint8_t> left_;
     float density_ = 0.0f;

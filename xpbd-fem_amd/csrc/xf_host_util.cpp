@@ -114,8 +114,10 @@ float Det3(const float m[3][3]) {
 float Dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 }  // namespace
 
-bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
-                  ElementConstants* out) {
+namespace {
+template <typename T>
+bool InitElementsT(const T* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                   ElementConstants* out) {
     out->qi.assign((size_t)9 * n_tets, 0.0f);
     out->qq.assign((size_t)3 * n_tets, 0.0f);
     out->qr.assign((size_t)3 * n_tets, 0.0f);
@@ -189,6 +191,17 @@ bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint
     }
     for (uint32_t i = 0; i < n_nodes; i++) out->w[i] = 1.0f / out->w[i];  // Geo.cpp:744
     return true;
+}
+}  // namespace
+
+bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                  ElementConstants* out) {
+    return InitElementsT(xyz, n_nodes, tets, n_tets, density, out);
+}
+
+bool InitElements(const double* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                  ElementConstants* out) {
+    return InitElementsT(xyz, n_nodes, tets, n_tets, density, out);
 }
 
 uint32_t GreedyColour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, const uint32_t* visit,

@@ -33,6 +33,10 @@ struct ElementConstants {
 // Geo.cpp:732-744). Returns false if a tet index is out of range.
 bool InitElements(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
                   ElementConstants* out);
+// The same from FP64 positions (GeoLinear3d::Init after autoResize holds
+// positions that float cannot represent; Geo.cpp:712-729).
+bool InitElements(const double* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
+                  ElementConstants* out);
 
 // Greedy node-disjoint colouring in visit order. Returns the colour count
 // (0 if more than kMaxColours would be needed).

@@ -106,6 +106,7 @@ struct Body {
     uint32_t n = 0, t = 0;
     float density = 0.0f;
     std::vector<float> xyz, vel, tet_mu;
+    std::vector<double> xyz64;  // FP64 positions when the caller gave them (xyz then holds their float values)
     std::vector<uint32_t> tets;
     std::vector<uint32_t> user_colour;  // caller-supplied colouring (empty: greedy)
     std::vector<uint8_t> pinned;
@@ -757,8 +758,8 @@ XF_API int xf_scene_add_body(xf_scene* s, const xf_body_desc* d, uint32_t* body_
     if (!s || !d) return XF_E_INVALID;
     xf_ctx* ctx = s->ctx;
     if (s->finalized) return Fail(ctx, XF_E_STATE, "scene already finalized");
-    if (!d->xyz || !d->tets || d->n_nodes < 4 || d->n_tets < 1)
-        return Fail(ctx, XF_E_INVALID, "body needs xyz, tets, >= 4 nodes and >= 1 tet");
+    if ((!d->xyz && !d->xyz64) || !d->tets || d->n_nodes < 4 || d->n_tets < 1)
+        return Fail(ctx, XF_E_INVALID, "body needs xyz (or xyz64), tets, >= 4 nodes and >= 1 tet");
     if (!(d->density > 0.0f)) return Fail(ctx, XF_E_INVALID, "density must be positive");
     if ((uint64_t)s->N + d->n_nodes >= (1ull << 31) || (uint64_t)s->T + d->n_tets >= (1ull << 31))
         return Fail(ctx, XF_E_INVALID, "scene too large");
@@ -767,7 +768,16 @@ XF_API int xf_scene_add_body(xf_scene* s, const xf_body_desc* d, uint32_t* body_
     b.n = d->n_nodes;
     b.t = d->n_tets;
     b.density = d->density;
-    b.xyz.assign(d->xyz, d->xyz + 3 * (size_t)b.n);
+    if (d->xyz64) {
+        b.xyz64.assign(d->xyz64, d->xyz64 + 3 * (size_t)b.n);
+        b.xyz.resize(3 * (size_t)b.n);
+        for (size_t i = 0; i < b.xyz.size(); i++) {
+            if (!std::isfinite(b.xyz64[i])) return Fail(ctx, XF_E_INVALID, "non-finite node position");
+            b.xyz[i] = (float)b.xyz64[i];  // ordering only (slab schedule)
+        }
+    } else {
+        b.xyz.assign(d->xyz, d->xyz + 3 * (size_t)b.n);
+    }
     b.tets.assign(d->tets, d->tets + 4 * (size_t)b.t);
     for (uint32_t v : b.tets)
         if (v >= b.n) return Fail(ctx, XF_E_INVALID, "tet index %u out of range (n_nodes %u)", v, b.n);
@@ -810,7 +820,8 @@ XF_API int xf_scene_add_body(xf_scene* s, const xf_body_desc* d, uint32_t* body_
             }
     }
     xf_settings_default(&b.settings);
-    if (!xf::InitElements(b.xyz.data(), b.n, b.tets.data(), b.t, b.density, &b.ec))
+    if (!(b.xyz64.empty() ? xf::InitElements(b.xyz.data(), b.n, b.tets.data(), b.t, b.density, &b.ec)
+                          : xf::InitElements(b.xyz64.data(), b.n, b.tets.data(), b.t, b.density, &b.ec)))
         return Fail(ctx, XF_E_INVALID, "bad tet indices");
     for (uint32_t t = 0; t < b.t; t++)
         if (!(b.ec.vol[t] > 0.0f)) return Fail(ctx, XF_E_INVALID, "tet %u has non-positive rest volume", t);
@@ -928,9 +939,20 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
             double flags = (double)(((uint32_t)bi << 1) | (b.pinned[i] ? 1u : 0u));
             s->h_w[g] = (double)b.ec.w[i];
             s->h_oflags[g] = flags;
-            // Node data enters as float and is held in double (Geo.cpp:713).
-            hX[g] = make_double4(b.xyz[3 * i], b.xyz[3 * i + 1], b.xyz[3 * i + 2], s->h_w[g]);
-            hO[g] = make_double4(b.xyz[3 * i], b.xyz[3 * i + 1], b.xyz[3 * i + 2], flags);
+            // Node data enters as float and is held in double (Geo.cpp:713),
+            // or as the caller's FP64 positions.
+            double px, py, pz;
+            if (b.xyz64.empty()) {
+                px = b.xyz[3 * i];
+                py = b.xyz[3 * i + 1];
+                pz = b.xyz[3 * i + 2];
+            } else {
+                px = b.xyz64[3 * i];
+                py = b.xyz64[3 * i + 1];
+                pz = b.xyz64[3 * i + 2];
+            }
+            hX[g] = make_double4(px, py, pz, s->h_w[g]);
+            hO[g] = make_double4(px, py, pz, flags);
             hV[g] = make_double4(b.vel[3 * i], b.vel[3 * i + 1], b.vel[3 * i + 2], 0.0);
         }
     }

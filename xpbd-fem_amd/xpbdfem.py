@@ -65,6 +65,7 @@ class BodyDesc(C.Structure):
         ("velocity", C.c_void_p),
         ("tet_mu", C.c_void_p),
         ("tet_colour", C.c_void_p),
+        ("xyz64", C.c_void_p),
     ]
 
 
@@ -218,6 +219,18 @@ def mesh_rings(tets):
     return out
 
 
+def autoresize(xyz):
+    """GeoLinear3d::Init's autoResize (Geo.cpp:712-729): FP64 positions
+    0.085 (x - centre) / largest bounding-box extent, with the box taken in
+    float and the arithmetic in double, as the reference computes them."""
+    p = np.asarray(xyz, np.float32).reshape(-1, 3)
+    mn, mx = p.min(axis=0), p.max(axis=0)
+    dims = mx - mn
+    max_dim = float(max(dims[0], max(dims[1], dims[2])))
+    centre = 0.5 * (mn + mx).astype(np.float64)
+    return (0.085 * ((p.astype(np.float64) - centre) / max_dim)).reshape(-1)
+
+
 def minstd_order(n):
     o = np.zeros(n, np.uint32)
     lib().xf_minstd_order(n, o)
@@ -355,11 +368,17 @@ class Scene:
         self.bodies = []  # (n_nodes, n_tets)
         self._keep = []
 
-    def add_body(self, xyz, tets, density, pinned=None, velocity=None, tet_mu=None, tet_colour=None):
-        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    def add_body(self, xyz, tets, density, pinned=None, velocity=None, tet_mu=None, tet_colour=None, xyz64=None):
+        """xyz: float32 positions; xyz64 (optional): FP64 positions that replace
+        them (e.g. autoresized ones, see autoresize())."""
         tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
         d = BodyDesc()
-        d.xyz = xyz.ctypes.data
+        if xyz64 is not None:
+            xyz = np.ascontiguousarray(xyz64, np.float64).reshape(-1)
+            d.xyz64 = xyz.ctypes.data
+        else:
+            xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+            d.xyz = xyz.ctypes.data
         d.n_nodes = xyz.size // 3
         d.tets = tets.ctypes.data
         d.n_tets = tets.size // 4
