@@ -40,6 +40,15 @@ typedef enum xf_status {
  * (Fem.cpp:437-564). Pixar (0-2), continuous (8-10) and cube (11-12)
  * energies are not implemented (XF_E_UNSUPPORTED). */
 enum {
+    /* Settings.h:41-53. Every 5-bit value is accepted and dispatched as
+     * TSolveElement does for linear tets (Fem.cpp:870-889): Pixar and
+     * PixarReduced coincide on a linear tet (one quadrature point), the Cube
+     * energies constrain quads and hexes only (tets stay free), values above
+     * 12 take the default, MixedSelective. Pixar and Cube run with per-colour
+     * launches; Pixar takes settings.compliance (no per-tet mu). */
+    XF_ENERGY_PIXAR = 0,
+    XF_ENERGY_PIXAR_REDUCED = 1,
+    XF_ENERGY_PIXAR_SEL = 2,
     XF_ENERGY_MIXED = 3,
     XF_ENERGY_MIXED_SEL = 4,
     XF_ENERGY_YEOH_SKIN = 5,
@@ -53,7 +62,9 @@ enum {
      * All bodies of a scene use ring energies or none does. */
     XF_ENERGY_CONTINUOUS_PIXAR = 8,
     XF_ENERGY_CONTINUOUS_MIXED = 9,
-    XF_ENERGY_CONTINUOUS_SKIN = 10
+    XF_ENERGY_CONTINUOUS_SKIN = 10,
+    XF_ENERGY_CUBE_NEO = 11,
+    XF_ENERGY_CUBE_SKIN = 12
 };
 
 /* Collision flags (no reference counterpart: the reference has no

@@ -327,7 +327,7 @@ class OracleBody:
         substep (as the device clock does) and the substep sees its end time.
         Vertex-ring energies (params.energy 8, 9, 10) sweep rings in
         ring_order; `order` is then the tet order of the PBD damping pass."""
-        rings = params.energy >= 8
+        rings = 8 <= params.energy <= 10
         if rings:
             if ring_order is None:
                 raise ValueError("vertex-ring energies need a ring order")

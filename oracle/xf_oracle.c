@@ -23,7 +23,10 @@
  *                        determinism; no reference counterpart
  *   xo_substep_ref64  == reference Geo3d::Substep on the same sweep order,
  *                        bit-exact, energies 3/4/5/6/7, serial and
- *                        simultaneous (committed golden fixtures + live)
+ *                        simultaneous (committed golden fixtures + live);
+ *                        Pixar 0/1/2 (every Rayleigh type, PBD), Cube 11/12
+ *                        and the default 13..31 (golden + live); Pixar Full
+ *                        == Reduced and Cube leaves tets in free fall
  *   xo_substep_f32    == GPU path bit-exact; ~= ref64 within tolerance
  *   xo_substep_f64    error-reporting oracle; ~= ref64 within tolerance
  *   volume passes     == reference SolveElementVolume sweeps inside

@@ -30,6 +30,12 @@ Fixtures (numpy .npz, no pickles):
                    substeps (tickId from 5) for the cases of its `cases` array
                    (energy, Poisson ratio, compliance, rayleigh type, damping,
                    pbd factor, amortized pbd factor).
+  ref_dispatch.npz the 6x3x3 block (x = 0 face pinned, MINSTD tOrder, tickId
+                   from 5): positions/velocities after 40 substeps for the rest
+                   of TSolveElement's switch (Pixar 0/1/2 with every Rayleigh
+                   type and PBD, Cube 11/12, defaults 13/31), per its `cases`
+                   array (energy, simultaneous, nu, compliance, rayleigh type,
+                   damping, pbd factor, amortized pbd factor).
   ref_autoresize.npz
                    the 12x4x4 beam through GeoLinear3d::Init with autoResize:
                    the FP64 positions Init computes, and positions/velocities
@@ -198,6 +204,36 @@ def rings():
     np.savez_compressed(os.path.join(HERE, "ref_rings.npz"), **out)
 
 
+DISPATCH_CASES = (
+    # energy, simultaneous, nu, compliance, rayleigh type, damping, pbd, pbd amortized
+    (0, 1, 0.45, 1e-6, 0, 0.0, 0.0, 0.0),
+    (1, 0, 0.3, 1e-6, 0, 0.0, 0.0, 0.0),
+    (2, 1, 0.45, 1e-6, 0, 0.0, 0.0, 0.0),
+    (0, 1, 0.45, 1e-6, 1, 0.02, 0.0, 0.0),
+    (2, 1, 0.45, 1e-6, 0, 0.02, 0.0, 0.0),
+    (0, 1, 0.45, 1e-6, 2, 0.02, 0.003, 0.0),
+    (2, 0, 0.45, 1e-6, 3, 0.02, 0.003, 0.012),
+    (11, 1, 0.45, 2e-5, 0, 0.0, 0.0, 0.0),
+    (12, 1, 0.45, 2e-5, 3, 0.02, 0.003, 0.012),
+    (13, 1, 0.45, 2e-5, 0, 0.0, 0.0, 0.0),
+    (31, 0, 0.45, 2e-5, 2, 0.02, 0.0, 0.0),
+)
+
+
+def dispatch():
+    xyz, tets = po.tet_block(6, 3, 3, (0.1, 0.1, 0.1), (0.0, -0.15, -0.15))
+    pinned = (xyz[0::3] == 0.0).astype(np.uint8)
+    order = po.minstd_order(tets.size // 4)
+    out = dict(xyz=xyz, tets=tets, pinned=pinned, order=order, cases=np.array(DISPATCH_CASES, np.float64))
+    for k, (energy, sim, nu, comp, rtype, damp, pbd, pbda) in enumerate(DISPATCH_CASES):
+        rb = po.RefBody(xyz, tets, 1000.0, pinned, order)
+        st = rb.settings(comp, nu=nu, energy=energy, simultaneous=bool(sim), damping=damp, rayleigh_type=rtype,
+                         pbd_damping=1.0 if pbd else 0.0, volume_pbd=pbd, amortized_volume_pbd=pbda, tick_id=5)
+        rb.substeps(st, DT, 40)
+        out[f"X{k}"], out[f"V{k}"] = rb.state()
+    np.savez_compressed(os.path.join(HERE, "ref_dispatch.npz"), **out)
+
+
 def autoresize():
     xyz, tets = po.tet_block(12, 4, 4, (0.1, 0.1, 0.1), (0.0, -0.2, -0.2))
     pinned = (xyz[0::3] == 0.0).astype(np.uint8)
@@ -219,5 +255,7 @@ if __name__ == "__main__":
     passes()
     rings()
     autoresize()
-    for f in ("ref_beam12.npz", "ref_tet_kats.npz", "ref_passes.npz", "ref_rings.npz", "ref_autoresize.npz"):
+    dispatch()
+    for f in ("ref_beam12.npz", "ref_tet_kats.npz", "ref_passes.npz", "ref_rings.npz", "ref_autoresize.npz",
+              "ref_dispatch.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

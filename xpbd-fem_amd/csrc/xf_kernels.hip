@@ -293,8 +293,52 @@ __device__ __forceinline__ void constrain2_damped(double x[4][3], const double o
 }
 
 // SolveElementMixed for a linear tet (Fem.cpp:437-564), damping off.
+// SolveElementPixar's energy and root-completed gradient for a linear tet
+// (Fem.cpp:362-432; one quadrature point of weight 1, so Full and Reduced
+// coincide): mu = 2, lambda = 2 / (1 - 2 nu) and a = 1 + mu / lambda come in
+// the per-tet material slots. SEL: PixarOptions::Selective (the J term at the
+// point plus the prefactored I1 term).
+template <bool SEL>
+__device__ __forceinline__ void pixar_ug(const float Qi[3][3], const float QQ[3], const float QR[3],
+                                         const float P[3][3], float lambda, float a, float& U, float g[4][3]) {
+    const float mu = 2.0f, weight = 1.0f;
+    float F[3][3], adj[3][3], G[3][3];
+    defgrad(Qi, P, F);
+    float u = 0.0f;
+    if constexpr (!SEL) {
+        float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+        u = u + weight * (mu / 2.0f) * I1;
+    }
+    float J = adjdet(F, adj);
+    u = u + weight * (lambda / 2.0f) * ((J - a) * (J - a));
+    float sJ = weight * (lambda / 2.0f) * 2.0f * (J - a);
+    float sI = weight * (mu / 2.0f) * 2.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) G[c][r] = SEL ? sJ * adj[c][r] : sI * F[r][c] + sJ * adj[c][r];
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) g[n][r] = 0.0f;
+    apply_grad(G, Qi, g);
+    root_grad(g);
+    if constexpr (SEL) {
+        float UI1, gI1[4][3];
+        neo_prefactored(QQ, QR, P, UI1, gI1);
+        u = u + (mu / 2.0f) * UI1;
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) g[n][r] = g[n][r] + (mu / 2.0f) * gI1[n][r];
+    }
+    U = u;
+}
+
 // E: 3 Mixed, 4 MixedSel, 5 YeohSkin, 6 YeohSkinSel, 7 YeohSkinFast,
-//    0 = MixedSel or YeohSkinFast chosen per tet by `yeohFast`.
+//    0 = MixedSel or YeohSkinFast chosen per tet by `yeohFast`,
+//    20 = Pixar / PixarReduced, 22 = PixarSel (comp0 = compliance / V,
+//    comp1 = lambda, a from the Pixar material).
 // DAMP: in-projection Rayleigh damping of the body (damp, rtype) along x - o.
 template <int E, bool SIM, bool DAMP = false>
 __device__ __forceinline__ void solve_tet(bool yeohFast, const float C[3], double x[4][3], const float w[4],
@@ -310,7 +354,11 @@ __device__ __forceinline__ void solve_tet(bool yeohFast, const float C[3], doubl
 #pragma unroll
         for (int r = 0; r < 3; r++) { g0[n][r] = 0.0f; g1[n][r] = 0.0f; }
 
-    if constexpr (E == 3 || E == 5 || E == 6) {
+    if constexpr (E == 20 || E == 22) {
+        pixar_ug<E == 22>(Qi, QQ, QR, P, comp1, a, U[0], g0);
+        if constexpr (DAMP) constrain1_damped(x, o, w, U[0], g0, dt, comp0, damp, rtype);
+        else constrain1(x, w, U[0], g0, dt, comp0);
+    } else if constexpr (E == 3 || E == 5 || E == 6) {
         i1_quad_term(E != 3, C, Qi, P, U[0], g0);
         if constexpr (SIM && E != 6) volume_term(Qi, P, a, U[1], g1);
         U[0] = fmaxf(0.0001f, U[0]);
@@ -397,8 +445,8 @@ __device__ __forceinline__ void volume_one(uint32_t i, uint32_t pass, const uint
 // (Fem.cpp:437-564 with the solve forced to Simultaneous), no projection.
 template <int E>
 __device__ __forceinline__ void element_ug(bool yeohFast, const float C[3], const double x[4][3], const float Qi[3][3],
-                                           const float QQ[3], const float QR[3], float a, float U[2], float g0[4][3],
-                                           float g1[4][3]) {
+                                           const float QQ[3], const float QR[3], float lambda, float a, float U[2],
+                                           float g0[4][3], float g1[4][3]) {
     float P[3][3];
     gather_p(x, P);
     U[0] = 0.0f;
@@ -407,7 +455,9 @@ __device__ __forceinline__ void element_ug(bool yeohFast, const float C[3], cons
     for (int n = 0; n < 4; n++)
 #pragma unroll
         for (int r = 0; r < 3; r++) { g0[n][r] = 0.0f; g1[n][r] = 0.0f; }
-    if constexpr (E == 3 || E == 5 || E == 6) {
+    if constexpr (E == 20 || E == 22) {
+        pixar_ug<E == 22>(Qi, QQ, QR, P, lambda, a, U[0], g0);
+    } else if constexpr (E == 3 || E == 5 || E == 6) {
         i1_quad_term(E != 3, C, Qi, P, U[0], g0);
         if constexpr (E != 6) volume_term(Qi, P, a, U[1], g1);
         U[0] = fmaxf(0.0001f, U[0]);
@@ -602,8 +652,10 @@ __device__ __forceinline__ void damp_one(uint32_t i, const uint4* __restrict__ t
     double v[4][3] = {{va.x, va.y, va.z}, {vb.x, vb.y, vb.z}, {vc.x, vc.y, vc.z}, {vd.x, vd.y, vd.z}};
     float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
     float U[2], g0[4][3], g1[4][3];
-    element_ug<E>(bp.energy == 7u, C, x, Qi, QQ, QR, a, U, g0, g1);
-    if (bp.simultaneous) {
+    element_ug<E>(bp.energy == 7u, C, x, Qi, QQ, QR, comp1, a, U, g0, g1);
+    if constexpr (E == 20 || E == 22) {
+        rayleigh1(v, w, U[0], g0, dt, q3.w, damp);  // SolveElementPixar in DampingMode::On
+    } else if (bp.simultaneous) {
         rayleigh2(v, w, U, g0, g1, dt, q3.w, comp1, damp);
     } else {
         rayleigh1(v, w, U[0], g0, dt, q3.w, damp);
@@ -673,6 +725,7 @@ __global__ __launch_bounds__(256) void xf_damp_colour(uint32_t begin, uint32_t c
                                                       const double4* __restrict__ X, double4* __restrict__ V,
                                                       const XfBodyParams* __restrict__ bodies, float comp1_u,
                                                       float a_u, float dt, const float* __restrict__ clock) {
+    if constexpr (E == 30) return;  // no constraint on tets, nothing to damp
     uint32_t i = colour_slot(begin, count, slabs);
     if (i == xf::kNoSlot) return;
     uint32_t tick = reinterpret_cast<const uint32_t*>(clock)[1] - 1u;
@@ -706,6 +759,7 @@ __global__ __launch_bounds__(256) void xf_project_colour(
         uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
         tick[0] = tick[0] + 1u;
     }
+    if constexpr (E == 30) return;  // CubeNeo / CubeSkin: no constraint on tets (Fem.cpp:567-760)
     uint32_t i = colour_slot(begin, count, slabs);
     if (i == xf::kNoSlot) return;
 
@@ -1166,6 +1220,9 @@ extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, boo
         case 5: fn = pick_aux<5>(sim, aux, damp); break;
         case 6: fn = pick_aux<6>(sim, aux, damp); break;
         case 7: fn = pick_aux<7>(sim, aux, damp); break;
+        case 20: fn = pick_aux<20>(sim, aux, damp); break;
+        case 22: fn = pick_aux<22>(sim, aux, damp); break;
+        case 30: fn = pick_aux<30>(sim, aux, damp); break;
         default: return hipErrorInvalidValue;
     }
     if (count == 0) return hipSuccess;
@@ -1199,6 +1256,9 @@ extern "C" hipError_t xf_launch_damp(hipStream_t s, int energy, bool aux, uint32
         case 5: fn = pick_damp<5>(aux); break;
         case 6: fn = pick_damp<6>(aux); break;
         case 7: fn = pick_damp<7>(aux); break;
+        case 20: fn = pick_damp<20>(aux); break;
+        case 22: fn = pick_damp<22>(aux); break;
+        case 30: fn = pick_damp<30>(aux); break;
         default: return hipErrorInvalidValue;
     }
     if (count == 0) return hipSuccess;

@@ -134,7 +134,23 @@ int DevAlloc(xf_ctx* ctx, T** p, size_t count, uint64_t* total) {
     return XF_OK;
 }
 
-bool EnergySupported(uint32_t e) { return e >= 3 && e <= 10; }
+// Settings energy field: 5 bits (Settings_EnergyMask), every value dispatched
+// as TSolveElement does for linear tets (Fem.cpp:870-889).
+bool EnergySupported(uint32_t e) { return e <= 31; }
+
+// Kernel variant of a tet energy: 0/1 Pixar / PixarReduced (one quadrature
+// point on a linear tet: the same code) -> 20, PixarSel -> 22, CubeNeo /
+// CubeSkin (quads and hexes only: no constraint on tets) -> 30, 3..7 as
+// they are, anything above 12 the switch's default MixedSelective -> 4.
+uint32_t KernelEnergy(uint32_t e) {
+    if (e <= 1) return 20;
+    if (e == 2) return 22;
+    if (e == 11 || e == 12) return 30;
+    if (e > 12) return 4;
+    return e;
+}
+
+bool PixarEnergy(uint32_t e) { return e <= 2; }
 
 // Energy_ContinuousPixar / ContinuousMixed / ContinuousSkin sweep vertex rings
 // instead of tets (GeoLinear3d::Constrain, Geo.cpp:774-788).
@@ -235,11 +251,11 @@ int ResolveMode(xf_scene* s, int* energy, bool* sim) {
     if (n_ring > 0)
         return Fail(s->ctx, XF_E_UNSUPPORTED,
                     "bodies of one scene must all use vertex-ring energies (8..10) or all tet energies (3..7)");
-    uint32_t e0 = s->bodies[0].settings.energy;
+    uint32_t e0 = KernelEnergy(s->bodies[0].settings.energy);
     uint32_t s0 = s->bodies[0].settings.simultaneous ? 1u : 0u;
     bool same = true, all47 = true;
     for (const Body& b : s->bodies) {
-        uint32_t e = b.settings.energy;
+        uint32_t e = KernelEnergy(b.settings.energy);
         uint32_t si = b.settings.simultaneous ? 1u : 0u;
         if (e != e0 || si != s0) same = false;
         if (!((e == 4 || e == 7) && si == 1)) all47 = false;
@@ -263,12 +279,22 @@ void ComputeMaterial(xf_scene* s) {
     bool compressible = false;
     for (Body& b : s->bodies) {
         float nu = b.settings.poissons_ratio;
+        bool pixar = PixarEnergy(b.settings.energy);
         for (uint32_t t = 0; t < b.t; t++) {
             float mu = b.tet_mu.empty() ? 1.0f / b.settings.compliance : b.tet_mu[t];
             float lambda = (2.0f * mu * nu) / (1.0f - 2.0f * nu);
             float a = 1.0f + mu / lambda;
             float comp0 = 1.0f / mu / b.ec.vol[t];
             float comp1 = 1.0f / lambda / b.ec.vol[t];
+            if (pixar) {
+                // SolveElementPixar (Fem.cpp:367-370): mu = 2; the slots hold
+                // compliance / V, lambda and a
+                float pmu = 2.0f;
+                float plambda = pmu / (1.0f - 2.0f * nu);
+                comp0 = b.settings.compliance / b.ec.vol[t];
+                comp1 = plambda;
+                a = 1.0f + pmu / plambda;
+            }
             uint32_t pos = b.sched_pos[t];
             s->h_tg3[pos].w = comp0;
             s->h_taux[pos].x = comp1;
@@ -635,6 +661,13 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) return Fail(s->ctx, XF_E_INVALID, "dt must be positive and finite");
     int rc = ResolveMode(s, energy, sim);
     if (rc) return rc;
+    if (*energy == 20 || *energy == 22 || *energy == 30) {
+        if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT)
+            return Fail(s->ctx, XF_E_UNSUPPORTED, "Pixar and Cube energies run with per-colour launches only");
+        for (const Body& b : s->bodies)
+            if (!b.tet_mu.empty() && PixarEnergy(b.settings.energy))
+                return Fail(s->ctx, XF_E_UNSUPPORTED, "Pixar energies take settings.compliance; a body has per-tet mu");
+    }
     if (*energy == kRingMode) {
         if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT)
             return Fail(s->ctx, XF_E_UNSUPPORTED, "vertex-ring energies run with per-colour launches only");
@@ -658,7 +691,7 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
 
 int CheckSettings(xf_ctx* ctx, const xf_settings* st) {
     if (!EnergySupported(st->energy))
-        return Fail(ctx, XF_E_UNSUPPORTED, "energy %u not implemented (3..10 are)", st->energy);
+        return Fail(ctx, XF_E_INVALID, "energy %u out of range (the Settings field has 5 bits)", st->energy);
     if (!(st->damping >= 0.0f) || !std::isfinite(st->damping) || !(st->pbd_damping >= 0.0f) ||
         !std::isfinite(st->pbd_damping) || !(st->pbd_damping_amortized >= 0.0f) ||
         !std::isfinite(st->pbd_damping_amortized))
@@ -1304,8 +1337,9 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
                                {XF_LAUNCH_PERSISTENT, 512}};
     xf_scene_opts keep = s->opts;
     Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size};
-    int maxb = 0;  // ring energies run per colour only
-    if (energy != kRingMode) XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
+    int maxb = 0;  // ring, Pixar and Cube energies run per colour only
+    if (energy != kRingMode && energy != 20 && energy != 22 && energy != 30)
+        XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
     float best = -1.0f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t he = copy_all(true);

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libxpbdfem.so")
 ENERGY_MIXED, ENERGY_MIXED_SEL, ENERGY_YEOH_SKIN, ENERGY_YEOH_SKIN_SEL, ENERGY_YEOH_SKIN_FAST = 3, 4, 5, 6, 7
 # vertex-ring ("continuous") energies: the sweep runs over vertex rings
 ENERGY_CONTINUOUS_PIXAR, ENERGY_CONTINUOUS_MIXED, ENERGY_CONTINUOUS_SKIN = 8, 9, 10
+ENERGY_PIXAR, ENERGY_PIXAR_REDUCED, ENERGY_PIXAR_SEL = 0, 1, 2
+ENERGY_CUBE_NEO, ENERGY_CUBE_SKIN = 11, 12  # no constraint on tets
 COLLIDE_GROUND, COLLIDE_SPHERE = 1, 2
 RAYLEIGH_PAPER, RAYLEIGH_LIMIT, RAYLEIGH_POST, RAYLEIGH_POST_AMORTIZED = 0, 1, 2, 3
 COLOUR_NATURAL, COLOUR_MINSTD = 0, 1
