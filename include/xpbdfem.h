@@ -328,7 +328,8 @@ XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
  * node_off[n_rings + 1], nodes[node_total] (each ring's own vertex first),
  * tet_off[n_rings + 1], ring_tets[4 n_tets] and idx[16 n_tets] (four
  * ring-local node indices per ring tet). XF_E_UNSUPPORTED if a ring would
- * exceed 256 nodes (kMaxRingNodes, Geo.cpp:10). */
+ * exceed 256 nodes (kMaxRingNodes, Geo.cpp:10) or 128 tets (the capacity of
+ * OneRingConstructor, Connectivity.h:151). */
 XF_API int xf_mesh_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_rings, uint32_t* node_total,
                          uint32_t* node_off, uint32_t* nodes, uint32_t* tet_off, uint32_t* ring_tets, uint8_t* idx,
                          uint32_t* largest);

@@ -336,7 +336,8 @@ uint32_t xo_colour_greedy(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
  * time they appear, starting from the ring's own vertex.
  * Call with node_off == NULL for the sizes: *n_rings and *node_total (ring
  * tets always total 4 * n_tets). Returns the largest ring's node count, or 0
- * when a ring would exceed 256 nodes (kMaxRingNodes, Geo.cpp:10). */
+ * when a ring would exceed 256 nodes (kMaxRingNodes, Geo.cpp:10) or 128 tets
+ * (OneRingConstructor's capacity, Connectivity.h:151). */
 uint32_t xo_build_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_rings, uint32_t* node_total,
                         uint32_t* node_off, uint32_t* nodes, uint32_t* tet_off, uint32_t* ring_tets, uint8_t* idx) {
     uint32_t nv = 0;
@@ -345,6 +346,8 @@ uint32_t xo_build_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_rings
     uint32_t* cnt = (uint32_t*)calloc((size_t)nv + 1, sizeof(uint32_t));
     for (uint32_t i = 0; i < 4 * n_tets; i++) cnt[tets[i] + 1]++;
     for (uint32_t v = 0; v < nv; v++) cnt[v + 1] += cnt[v];
+    for (uint32_t v = 0; v < nv; v++)
+        if (cnt[v + 1] - cnt[v] > 128) { free(cnt); return 0; } /* OneRingConstructor::polys[128] */
     uint32_t* list = (uint32_t*)malloc(sizeof(uint32_t) * (4 * (size_t)n_tets + 1));
     uint32_t* fill = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)nv + 1));
     memcpy(fill, cnt, sizeof(uint32_t) * ((size_t)nv + 1));

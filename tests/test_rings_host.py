@@ -34,3 +34,15 @@ def test_mesh_rings_match_oracle(mesh):
     assert r["largest"] == o.largest and r["node_off"].size == o.n_rings + 1
     for k in ("node_off", "nodes", "tet_off", "tets", "idx"):
         assert np.array_equal(r[k], getattr(o, k)), k
+
+
+def test_ring_limits_rejected():
+    """A vertex in 129 tets (a fan around node 0 with shared rim nodes: 133
+    nodes, under the 256-node limit) exceeds OneRingConstructor's 128 tets."""
+    fan = np.array([[0, 1 + i, 2 + i, 132] for i in range(129)], np.uint32).reshape(-1)
+    with pytest.raises(xf.XfError):
+        xf.mesh_rings(fan)
+    with pytest.raises(ValueError):
+        po.Rings(fan)
+    ok = np.array([[0, 1 + i, 2 + i, 132] for i in range(128)], np.uint32).reshape(-1)
+    assert xf.mesh_rings(ok)["tet_off"][1] == 128

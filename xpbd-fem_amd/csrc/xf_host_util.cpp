@@ -236,6 +236,8 @@ bool BuildRings(const uint32_t* tets, uint32_t n_tets, Rings* out) {
     std::vector<uint32_t> first(nv + 1, 0), list(4 * (size_t)n_tets);
     for (size_t i = 0; i < 4 * (size_t)n_tets; i++) first[tets[i] + 1]++;
     for (uint32_t v = 0; v < nv; v++) first[v + 1] += first[v];
+    for (uint32_t v = 0; v < nv; v++)
+        if (first[v + 1] - first[v] > kMaxRingTets) return false;
     std::vector<uint32_t> fill(first.begin(), first.end() - 1);
     for (uint32_t t = 0; t < n_tets; t++)
         for (int j = 0; j < 4; j++) list[fill[tets[4 * (size_t)t + j]]++] = t;

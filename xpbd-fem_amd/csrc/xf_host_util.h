@@ -55,12 +55,13 @@ uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
 // tets[tet_off[v] .. tet_off[v+1]); idx holds four ring-local node indices
 // per ring tet. Rings exist for vertices 0 .. max index in the tets.
 constexpr uint32_t kMaxRingNodes = 256;  // Geo.cpp:10
+constexpr uint32_t kMaxRingTets = 128;   // OneRingConstructor::polys (Connectivity.h:151)
 struct Rings {
     uint32_t n = 0, largest = 0;
     std::vector<uint32_t> node_off, nodes, tet_off, tets;
     std::vector<uint8_t> idx;
 };
-// Returns false if a ring would exceed kMaxRingNodes nodes.
+// Returns false if a ring would exceed kMaxRingNodes nodes or kMaxRingTets tets.
 bool BuildRings(const uint32_t* tets, uint32_t n_tets, Rings* out);
 
 // The reference's vOrder (Geo.cpp:759-771): the MINSTD state continues

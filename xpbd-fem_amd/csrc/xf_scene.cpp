@@ -389,7 +389,8 @@ int BuildRingsHost(xf_scene* s) {
     s->R = 0;
     for (Body& b : s->bodies) {
         if (!xf::BuildRings(b.tets.data(), b.t, &b.rings))
-            return Fail(ctx, XF_E_UNSUPPORTED, "a vertex ring has more than %u nodes", xf::kMaxRingNodes);
+            return Fail(ctx, XF_E_UNSUPPORTED, "a vertex ring has more than %u nodes or %u tets", xf::kMaxRingNodes,
+                        xf::kMaxRingTets);
         std::vector<uint32_t> visit(b.rings.n);
         if (s->opts.colour_order == XF_COLOUR_MINSTD) xf::MinstdRingOrder(b.t, b.rings.n, visit.data());
         else
