@@ -229,6 +229,38 @@ def rings_leg(ctx, frames=3, budget_s=8.0, xcd_slabs=True):
     return out
 
 
+def large_leg(ctx, frames=2, cells=(288, 72, 72), xcd_slabs=True):
+    """The same Yeoh beam at 288x72x72 cells (8,957,952 tets, 373,248 per
+    colour): the projection kernel with colours large enough that HBM
+    bandwidth, not the phase latency, bounds it. Reports tet-substeps/s and
+    the projection kernel's achieved algorithmic bandwidth (288 B per tet)."""
+    wl = workloads.beam_yeoh(*cells)
+    S, dt = wl["substeps"], float(wl["dt"])
+    sc = workloads.make_scene(ctx, wl, xcd_slabs=xcd_slabs)
+    info = sc.info()
+    T = info["n_tets"]
+    mode, size, _ = sc.autotune(S, dt, reps=1)
+    sc.step(S, dt)
+    sc.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sc.step(S, dt)
+    sc.sync()
+    el = time.perf_counter() - t0
+    finite = sc.count_nonfinite() == 0
+    ms_proj, _, n_launch = sc.profile(S, dt)
+    sc.close()
+    achieved = BYTES_PER_TET * T * S / (ms_proj * 1e-3) / 1e9 if not mode else None
+    return {"workload": f"{wl['name']}: {T} tets, {info['n_nodes']} nodes, {info['n_colours']} colours, "
+                        f"{T // info['n_colours']} tets per colour, {S} substeps/frame",
+            "tet_substeps_per_s": T * S * frames / el, "ms_per_frame": 1e3 * el / frames, "finite": finite,
+            "launch": {"mode": "persistent" if mode else "per-colour", "size": size, "xcd_slabs": xcd_slabs},
+            "roofline": {"bound": "hbm", "kernel": "xf_project_colour", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                         "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
+                         "bytes_per_launch": BYTES_PER_TET * T * S / max(n_launch, 1)}}
+
+
 def adjoint_leg(ctx, reps=3):
     """Adjoint of the 16x4x4 fitting beam (1,536 tets, per-tet mu, 30 frames of
     20 substeps, loss at frames 10, 20, 30, checkpoints every 20 substeps):
@@ -355,6 +387,21 @@ def rings_profile(frames=1, xcd_slabs=True):
                       "frames": frames, "xcd_slabs": xcd_slabs}))
 
 
+def large_profile(xcd_slabs=True, block=256):
+    """One frame of the 288x72x72 beam, per-colour launches of `block`
+    threads (short runs under rocprofv3 counter passes)."""
+    ctx = xf.Context(0)
+    wl = workloads.beam_yeoh(288, 72, 72)
+    sc = workloads.make_scene(ctx, wl, xcd_slabs=xcd_slabs, block_size=block)
+    sc.step(wl["substeps"], float(wl["dt"]))
+    sc.sync()
+    info = sc.info()
+    sc.close()
+    ctx.close()
+    print(json.dumps({"large_profile": wl["name"], "tets": info["n_tets"], "colours": info["n_colours"],
+                      "block": block, "xcd_slabs": xcd_slabs}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +411,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs (single beam and batched fit)")
     ap.add_argument("--no-rings", action="store_true", help="skip the vertex-ring energy leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 288x72x72 bandwidth leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="per-colour launches of 64 threads instead of the autotuned shape (short profiler passes)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -372,6 +420,8 @@ def main():
                     help="only run one frame of the 256-box batched scene (rocprofv3 counter passes)")
     ap.add_argument("--rings-profile", action="store_true",
                     help="only run one frame of the beam with the ContinuousSkin ring energy (rocprofv3 counter passes)")
+    ap.add_argument("--large-profile", action="store_true",
+                    help="only run one frame of the 288x72x72 beam (rocprofv3 counter passes)")
     ap.add_argument("--no-xcd-slabs", action="store_true",
                     help="plain blockIdx-linear schedule instead of the XCD-slab one (colour eighths per XCD)")
     args = ap.parse_args()
@@ -384,6 +434,9 @@ def main():
         return
     if args.rings_profile:
         rings_profile(xcd_slabs=args.xcd_slabs)
+        return
+    if args.large_profile:
+        large_profile(xcd_slabs=args.xcd_slabs)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
@@ -456,6 +509,8 @@ def main():
     legs = []
     if not args.no_batch:
         legs.append(("batched", lambda c: batched_leg(c, xcd_slabs=args.xcd_slabs)))
+    if not args.no_large:
+        legs.append(("large", lambda c: large_leg(c, xcd_slabs=args.xcd_slabs)))
     if not args.no_rings:
         legs.append(("rings", lambda c: rings_leg(c, xcd_slabs=args.xcd_slabs)))
     if not args.no_adjoint:

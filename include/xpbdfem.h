@@ -182,12 +182,13 @@ typedef struct xf_scene_opts {
     uint32_t use_graph;         /* capture each per-colour xf_scene_step call shape in a hipGraph */
     uint32_t launch_mode;       /* XF_LAUNCH_* */
     uint32_t persistent_blocks; /* grid size of the persistent launch, 0 = one per compute unit */
-    uint32_t xcd_slabs;         /* 1: inside each colour, order tets by (body, centroid x) and let
-                                   workgroup b work on eighth (b mod 8) of the colour, so each of
-                                   the 8 XCDs (workgroups are dispatched round-robin over them)
+    uint32_t xcd_slabs;         /* 1: inside each colour, cut each body's tets (rings) into eighths
+                                   by centroid (vertex) x, node-index ordered inside an eighth, and
+                                   let workgroup b work on eighth (b mod 8) of the colour, so each
+                                   of the 8 XCDs (workgroups are dispatched round-robin over them)
                                    sweeps one spatial slab and node lines are fetched by one L2.
                                    Changes the sweep order inside colours (a valid serial order
-                                   still; xf_scene_get_order reports it). */
+                                   still; xf_scene_get_order / xf_scene_get_ring_order report it). */
     uint32_t ring_lanes;        /* lanes per vertex ring in ring-energy launches: 16, 32, or 0 = 32
                                    when some ring has more than 16 tets, else 16 */
 } xf_scene_opts;

@@ -193,16 +193,25 @@ def test_user_colouring_validated_and_bit_exact(ctx):
 
 
 def check_slab_order(sc, wl):
-    """XCD-slab schedule: still colour-major, and inside each colour the tets
-    are ordered by centroid x."""
+    """XCD-slab schedule: still colour-major; each colour cut into eighths
+    [n r / 8, n (r + 1) / 8) that are slabs in centroid x (no tet of eighth r
+    lies right of one of eighth r + 1), node-index ordered inside."""
     order, colour = sc.order(0)
     t = wl["tets"].reshape(-1, 4).astype(np.int64)
     x = wl["xyz"].reshape(-1, 3)[:, 0].astype(np.float32)[t]
     cx = ((x[:, 0] + x[:, 1]) + x[:, 2]) + x[:, 3]
+    first = t.min(axis=1)
     co = colour[order]
     assert np.all(np.diff(co.astype(np.int64)) >= 0)
-    same = co[1:] == co[:-1]
-    assert np.all(np.diff(cx[order])[same] >= 0)
+    for c in np.unique(co):
+        seg = order[co == c]
+        n = seg.size
+        cuts = [(n * r) >> 3 for r in range(9)]
+        for r in range(8):
+            part = seg[cuts[r]:cuts[r + 1]]
+            assert np.all(np.diff(first[part]) >= 0)
+            if r < 7 and part.size and cuts[r + 2] > cuts[r + 1]:
+                assert cx[part].max() <= cx[seg[cuts[r + 1]:cuts[r + 2]]].min()
 
 
 @pytest.mark.parametrize("energy,sim", [(4, True), (7, True), (4, False), (3, True)])

@@ -162,6 +162,30 @@ constexpr int kRingMode = XF_ENERGY_CONTINUOUS_PIXAR;
 // LDS budget of one ring workgroup.
 constexpr size_t kRingLdsLimit = 65536;
 
+// XCD-slab order of a colour-major sweep order: inside each colour the items
+// are sorted by x (ties keep their order) and cut into the eighths the
+// per-colour launch maps to XCDs (positions [n r / 8, n (r + 1) / 8)); inside
+// an eighth they are then sorted by their smallest node index, so that
+// neighbouring lanes read neighbouring node records.
+void SlabOrder(std::vector<uint32_t>* order, const std::vector<uint32_t>& colour, const std::vector<float>& x,
+               const std::vector<uint32_t>& first_node) {
+    std::vector<uint32_t>& o = *order;
+    std::stable_sort(o.begin(), o.end(), [&](uint32_t a, uint32_t c) {
+        return colour[a] != colour[c] ? colour[a] < colour[c] : x[a] < x[c];
+    });
+    for (size_t b = 0; b < o.size();) {
+        size_t e = b;
+        while (e < o.size() && colour[o[e]] == colour[o[b]]) e++;
+        uint64_t n = e - b;
+        for (uint32_t r = 0; r < 8; r++) {
+            size_t lo = b + (size_t)((n * r) >> 3), hi = b + (size_t)((n * (r + 1)) >> 3);
+            std::stable_sort(o.begin() + lo, o.begin() + hi,
+                             [&](uint32_t a, uint32_t c) { return first_node[a] < first_node[c]; });
+        }
+        b = e;
+    }
+}
+
 // Volume passes per substep accepted by xf_scene_set_settings.
 constexpr uint32_t kMaxVolumePasses = 64;
 
@@ -397,11 +421,15 @@ int BuildRingsHost(xf_scene* s) {
             for (uint32_t v = 0; v < b.rings.n; v++) visit[v] = v;
         uint32_t nc = xf::RingColourSweepOrder(b.rings, b.n, visit.data(), &b.ring_colour, &b.ring_order);
         if (nc == 0) return Fail(ctx, XF_E_UNSUPPORTED, "mesh needs more than %u ring colours", xf::kMaxColours);
-        if (s->opts.xcd_slabs)
-            std::stable_sort(b.ring_order.begin(), b.ring_order.end(), [&](uint32_t a, uint32_t c) {
-                return b.ring_colour[a] != b.ring_colour[c] ? b.ring_colour[a] < b.ring_colour[c]
-                                                            : b.xyz[3 * (size_t)a] < b.xyz[3 * (size_t)c];
-            });
+        if (s->opts.xcd_slabs) {
+            std::vector<float> vx(b.rings.n);
+            std::vector<uint32_t> vid(b.rings.n);
+            for (uint32_t v = 0; v < b.rings.n; v++) {
+                vx[v] = b.xyz[3 * (size_t)v];
+                vid[v] = v;
+            }
+            SlabOrder(&b.ring_order, b.ring_colour, vx, vid);
+        }
         s->rcol = std::max(s->rcol, nc);
         s->rnmax = std::max(s->rnmax, b.rings.largest);
         for (uint32_t v = 0; v < b.rings.n; v++) s->rtmax = std::max(s->rtmax, b.rings.tet_off[v + 1] - b.rings.tet_off[v]);
@@ -905,18 +933,18 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         if (b.ncol == 0) return Fail(ctx, XF_E_UNSUPPORTED, "mesh needs more than %u colours", xf::kMaxColours);
         s->ncol = std::max(s->ncol, b.ncol);
     }
-    // XCD slabs: inside each colour, tets by centroid x (ties keep the order), so
-    // that eighths of a colour's schedule are spatial slabs.
+    // XCD slabs: eighths of each colour are spatial slabs (by centroid x),
+    // node-index ordered inside.
     if (s->opts.xcd_slabs)
         for (Body& b : s->bodies) {
             std::vector<float> cx(b.t);
+            std::vector<uint32_t> first(b.t);
             for (uint32_t t = 0; t < b.t; t++) {
                 const uint32_t* is = &b.tets[4 * (size_t)t];
                 cx[t] = b.xyz[3 * is[0]] + b.xyz[3 * is[1]] + b.xyz[3 * is[2]] + b.xyz[3 * is[3]];
+                first[t] = std::min(std::min(is[0], is[1]), std::min(is[2], is[3]));
             }
-            std::stable_sort(b.order.begin(), b.order.end(), [&](uint32_t a, uint32_t c) {
-                return b.colour[a] != b.colour[c] ? b.colour[a] < b.colour[c] : cx[a] < cx[c];
-            });
+            SlabOrder(&b.order, b.colour, cx, first);
         }
     // Schedule: colour-major, body-major within a colour, each body's tets
     // of that colour in its sweep order.

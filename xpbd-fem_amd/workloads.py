@@ -33,6 +33,10 @@ def beam_neo_12x4x4(energy=xf.ENERGY_MIXED_SEL, simultaneous=True, collide=True)
 
 
 def beam_yeoh_96x24x24(seed=1, h=0.1, structured=True):
+    return beam_yeoh(96, 24, 24, seed=seed, h=h, structured=structured)
+
+
+def beam_yeoh(W, H, D, seed=1, h=0.1, structured=True):
     """96x24x24 cells of h = 0.1 m (331,776 tets, 60,625 nodes; the beam spans
     x in [0, 9.6], y and z in [-1.2, 1.2]), Yeoh energy
     C10 (IM + 0.1 IM^2 + 0.01 IM^3) with C10 = 5e4, C20 = 5e3, C30 = 5e2 Pa,
@@ -48,8 +52,10 @@ def beam_yeoh_96x24x24(seed=1, h=0.1, structured=True):
     compliance = 1 / C10 describes the same polynomial, but the reference's
     simultaneous solve then turns chaotic within a few frames (node speeds
     above 100 m/s; also from rest, also without contacts), and it does so
-    for the skin-normalised form too at cell sizes of 0.02 m and below."""
-    W, H, D = 96, 24, 24
+    for the skin-normalised form too at cell sizes of 0.02 m and below.
+
+    Other W x H x D cell counts give the same material, pins and contacts
+    (the ground 1 m under the beam, the sphere at its underside)."""
     xyz, tets = xf.tet_block(W, H, D, (h, h, h), (0.0, -0.5 * H * h, -0.5 * D * h))
     n = xyz.size // 3
     vel = xf.normal_fill(seed, 0.05, 3 * n)
@@ -58,7 +64,7 @@ def beam_yeoh_96x24x24(seed=1, h=0.1, structured=True):
     k = xf.SKIN_YEOH[0]
     y_low = -0.5 * H * h
     return dict(
-        name="beam96x24x24_yeoh", xyz=xyz, tets=tets, density=1000.0, pinned=_pins_x0(xyz), velocity=vel,
+        name=f"beam{W}x{H}x{D}_yeoh", xyz=xyz, tets=tets, density=1000.0, pinned=_pins_x0(xyz), velocity=vel,
         tet_mu=None, tet_colour=xf.tet_block_colours(W, H, D) if structured else None,
         settings=dict(energy=xf.ENERGY_YEOH_SKIN_FAST, simultaneous=1, compliance=k / c10, poissons_ratio=0.5,
                       gravity=(0.0, -9.81), yeoh=(k, k * c20 / c10, k * c30 / c10),
