@@ -173,7 +173,10 @@ typedef struct xf_body_desc {
 /* Launch modes of xf_scene_step. */
 enum {
     XF_LAUNCH_PER_COLOUR = 0, /* one launch per colour phase and node phase (hipGraph-captured if use_graph) */
-    XF_LAUNCH_PERSISTENT = 1  /* one cooperative launch per step call, grid barriers between phases */
+    XF_LAUNCH_PERSISTENT = 1, /* one cooperative launch per step call, grid barriers between phases */
+    XF_LAUNCH_BLOCKS = 2      /* needs sweep_blocks: one projection launch per block launch colour
+                                 (at most 4 per substep), each workgroup sweeping one block through
+                                 its phases in LDS; other passes run per colour */
 };
 
 typedef struct xf_scene_opts {
@@ -191,6 +194,17 @@ typedef struct xf_scene_opts {
                                    still; xf_scene_get_order / xf_scene_get_ring_order report it). */
     uint32_t ring_lanes;        /* lanes per vertex ring in ring-energy launches: 16, 32, or 0 = 32
                                    when some ring has more than 16 tets, else 16 */
+    uint32_t sweep_blocks;      /* 1: colour the tets by sweep blocks. Node levels are breadth-first
+                                   distances over tet edges from the nodes of least x and of least
+                                   y; a tet's block is its pair of minimum levels and its launch
+                                   colour the pair's parities, so blocks of one launch colour share
+                                   no node. Inside a block, tets take greedy phases in visit order.
+                                   Colour (q, j) holds phase j of every block of launch colour q;
+                                   the order is colour-major, then body, block, visit order (a
+                                   valid serial order; xf_scene_get_order reports it). Needed by
+                                   XF_LAUNCH_BLOCKS; excludes xcd_slabs and tet_colour. A block
+                                   must fit the LDS of one workgroup (32 B per node + 88 B per tet
+                                   <= 64 KiB). */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {
@@ -201,6 +215,8 @@ typedef struct xf_scene_info {
     uint64_t device_bytes;   /* device memory held by the scene */
     uint32_t n_rings;        /* vertex rings (0 until a ring energy or xf_scene_get_ring_order builds them) */
     uint32_t n_ring_colours; /* ring colour phases per substep (max over bodies) */
+    uint32_t n_blocks;       /* sweep blocks over all bodies (0 without sweep_blocks) */
+    uint32_t block_launches; /* block launch colours that hold blocks (0 without sweep_blocks) */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */
@@ -271,8 +287,9 @@ XF_API int xf_scene_profile(xf_scene* scene, uint32_t substeps, float dt, float*
                             uint32_t* launches_project);
 /* Autotuner: times `reps` calls of xf_scene_step(substeps, dt) for each
  * launch shape -- per-colour launches with workgroup sizes 64, 128, 256 and
- * the persistent launch with 64, 128, 256 and 512 blocks (where co-resident)
- * -- and keeps the fastest in the scene options. The node state and clock are restored afterwards, so tuning
+ * the persistent launch with 64, 128, 256 and 512 blocks (where co-resident),
+ * and the sweep-block launches when the scene has sweep_blocks -- and keeps
+ * the fastest in the scene options. The node state and clock are restored afterwards, so tuning
  * does not advance the simulation. chosen / best_ms (ms per call) nullable.
  * No reference counterpart (the reference is single-threaded). */
 XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,
@@ -322,6 +339,14 @@ XF_API int xf_normal_fill(uint64_t seed, float sigma, uint32_t n, float* out);
  * the reference sweeps its MINSTD order serially (Geo.cpp:759-777). */
 XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, uint32_t colour_order,
                           uint32_t* order, uint32_t* colour, uint32_t* n_colours);
+/* The sweep blocks xf_scene_finalize builds with sweep_blocks, on the host
+ * (no reference counterpart): per tet its block (block_of_tet[n_tets]) and its
+ * phase inside the block (phase_of_tet[n_tets]); per block its launch colour
+ * (launch_of_block, capacity n_tets); *n_blocks; *max_phases (nullable) the
+ * most phases of any block. Visit order per colour_order (XF_COLOUR_*). */
+XF_API int xf_mesh_sweep_blocks(uint32_t n_nodes, const float* xyz, uint32_t n_tets, const uint32_t* tets,
+                                uint32_t colour_order, uint32_t* n_blocks, uint32_t* block_of_tet,
+                                uint32_t* phase_of_tet, uint32_t* launch_of_block, uint32_t* max_phases);
 /* The vertex rings the ring energies sweep, on the host: the reference's
  * OneRing of every vertex 0 .. largest index (Connectivity.h:106-155, built by
  * GeoLinear3d::Init, Geo.cpp:745-757) in CSR form. *n_rings, *node_total and

@@ -1,0 +1,192 @@
+"""Sweep-block launches (XF_LAUNCH_BLOCKS) on the GPU: each workgroup sweeps
+one block through its phases in LDS, blocks of one launch colour in
+parallel. Bar: bit-exact with the oracle's serial sweep in the order the
+scene reports (colour-major over the (launch colour, phase) colours), and
+with per-colour launches over the same colouring."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import workloads
+import xpbdfem as xf
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle(wl, order, substeps, tick=0):
+    ob = po.body_from_workload(wl, "ref64")
+    ob.tick = tick
+    ob.substep(po.params_from_settings(wl["dt"], wl["settings"]), order, count=substeps)
+    return ob
+
+
+def run(ctx, wl, frames=2, tick=0, **opts):
+    opts.setdefault("sweep_blocks", True)
+    opts.setdefault("launch_mode", xf.LAUNCH_BLOCKS)
+    sc = workloads.make_scene(ctx, wl, **opts)
+    sc.set_tick(tick)
+    S = wl["substeps"]
+    for _ in range(frames):
+        sc.step(S, float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, colour = sc.order(0)
+    info = sc.info()
+    t, k = sc.time(), sc.tick()
+    sc.close()
+    return x, v, order, colour, info, t, k
+
+
+def check_colouring(wl, order, colour):
+    t = wl["tets"].reshape(-1, 4)
+    co = colour[order]
+    assert np.all(np.diff(co.astype(np.int64)) >= 0)
+    pairs = np.stack([np.repeat(colour.astype(np.int64), 4), t.ravel().astype(np.int64)], 1)
+    assert len(np.unique(pairs, axis=0)) == len(pairs)
+
+
+@pytest.mark.parametrize("energy,sim", [(4, True), (7, True), (4, False), (3, True), (5, True), (6, False),
+                                        (7, False)])
+def test_blocks_bit_exact(ctx, energy, sim):
+    wl = workloads.beam_neo_12x4x4(energy=energy, simultaneous=sim)
+    x, v, order, colour, info, _, _ = run(ctx, wl)
+    assert info["n_blocks"] == 48 and info["block_launches"] == 4 and info["n_colours"] == 32
+    check_colouring(wl, order, colour)
+    ob = oracle(wl, order, 2 * wl["substeps"])
+    print(f"\nenergy {energy} sim {sim}: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_blocks_equal_per_colour_launches(ctx, use_graph):
+    """The same sweep-block colouring run with one launch per colour."""
+    wl = workloads.beam_neo_12x4x4(energy=7)
+    a = run(ctx, wl, use_graph=use_graph)
+    b = run(ctx, wl, launch_mode=xf.LAUNCH_PER_COLOUR, use_graph=use_graph)
+    assert np.array_equal(a[2], b[2])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert a[5] == b[5] and a[6] == b[6]  # clock advanced once per substep
+
+
+def test_blocks_big_beam_bit_exact(ctx):
+    """The 96x24x24 Yeoh beam (331,776 tets; 2,304 column blocks of 144 tets,
+    8 phases, 4 launches per substep) with ground and moving sphere."""
+    wl = workloads.beam_yeoh_96x24x24()
+    x, v, order, colour, info, _, _ = run(ctx, wl, frames=1)
+    assert info["n_blocks"] == 96 * 24 and info["n_colours"] == 32
+    ob = oracle(wl, order, wl["substeps"])
+    print(f"\nbig beam: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_blocks_minstd_visit_and_volume_passes(ctx):
+    wl = workloads.beam_neo_12x4x4(energy=4)
+    wl["settings"] = dict(wl["settings"], volume_passes=2)
+    x, v, order, colour, _, _, _ = run(ctx, wl, colour_order=xf.COLOUR_MINSTD)
+    check_colouring(wl, order, colour)
+    ob = oracle(wl, order, 2 * wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+@pytest.mark.parametrize("rtype", [0, 1, 2, 3])
+def test_blocks_damping_bit_exact(ctx, rtype):
+    """Rayleigh Paper / Limit damp inside the projection (O read from global
+    memory); Post / PostAmortized and PBD run per colour after the velocity
+    update."""
+    wl = workloads.beam_neo_12x4x4(energy=7)
+    wl["settings"] = dict(wl["settings"], damping=0.02, rayleigh_type=rtype, pbd_damping=0.003,
+                          pbd_damping_amortized=0.012)
+    x, v, order, _, _, _, k = run(ctx, wl, tick=5)
+    assert k == 5 + 2 * wl["substeps"]
+    ob = oracle(wl, order, 2 * wl["substeps"], tick=5)
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+@pytest.mark.parametrize("energy", [0, 2, 12, 31])
+def test_blocks_energy_dispatch_bit_exact(ctx, energy):
+    """Pixar (0), PixarSel (2), CubeSkin (12: no constraint on tets, the clock
+    still advances) and the default MixedSel (31)."""
+    wl = workloads.beam_neo_12x4x4(energy=energy)
+    wl["settings"] = dict(wl["settings"], compliance=1e-6 if energy <= 2 else 2e-5, poissons_ratio=0.45)
+    x, v, order, _, _, t, k = run(ctx, wl, tick=3)
+    assert k == 3 + 2 * wl["substeps"]
+    assert np.isfinite(t) and t > 0.0
+    ob = oracle(wl, order, 2 * wl["substeps"], tick=3)
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_blocks_per_tet_mu(ctx):
+    """Per-tet mu (element data with comp1 / a / body per tet)."""
+    wl = workloads.beam_fit_16x4x4()
+    x, v, order, _, _, _, _ = run(ctx, wl, frames=1)
+    ob = oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_blocks_batch_bit_exact(ctx):
+    """Many bodies, one launch per launch colour: each batched body equals the
+    same body alone and the oracle's sweep in its reported order."""
+    b = workloads.boxes_batch(n_bodies=6, seed=5, cells=(3, 6), frames=2)
+    dt, S = float(b["dt"]), b["substeps"]
+    sc = workloads.make_batch_scene(ctx, b, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    sc.step(S, dt)
+    sc.step(S, dt)
+    for i, wl in enumerate(b["bodies"]):
+        order, colour = sc.order(i)
+        check_colouring(wl, order, colour)
+        ob = po.body_from_workload(wl, "ref64")
+        ob.substep(po.params_from_settings(b["dt"], wl["settings"]), order, count=2 * S)
+        x, v = sc.get_state(i)
+        assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V), wl["name"]
+        single = workloads.make_batch_scene(ctx, b, bodies=[i], sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+        single.step(S, dt)
+        single.step(S, dt)
+        xs, vs = single.get_state(0)
+        single.close()
+        assert np.array_equal(xs, x) and np.array_equal(vs, v), wl["name"]
+    sc.close()
+
+
+def test_blocks_autotune_keeps_results(ctx):
+    wl = workloads.beam_neo_12x4x4(energy=7)
+    sc = workloads.make_scene(ctx, wl, sweep_blocks=True)
+    x0, v0 = sc.get_state(0)
+    mode, size, ms = sc.autotune(wl["substeps"], float(wl["dt"]), reps=2)
+    assert mode in (xf.LAUNCH_PER_COLOUR, xf.LAUNCH_PERSISTENT, xf.LAUNCH_BLOCKS) and ms > 0
+    x1, v1 = sc.get_state(0)
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1) and sc.tick() == 0
+    sc.step(wl["substeps"], float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, _ = sc.order(0)
+    sc.close()
+    ob = oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+def test_blocks_rejections(ctx):
+    wl = workloads.beam_neo_12x4x4()
+    cases = [dict(launch_mode=xf.LAUNCH_BLOCKS), dict(sweep_blocks=True, xcd_slabs=True),
+             dict(sweep_blocks=2), dict(launch_mode=3)]
+    for opts in cases:
+        sc = xf.Scene(ctx)
+        sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"])
+        with pytest.raises(xf.XfError):
+            sc.finalize(**opts)
+        sc.close()
+    sc = xf.Scene(ctx)
+    sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"], tet_colour=xf.tet_block_colours(12, 4, 4))
+    with pytest.raises(xf.XfError):
+        sc.finalize(sweep_blocks=True)
+    sc.close()
+    # a column of 120 cells: 720 tets and 484 nodes need 78,848 bytes of LDS
+    xyz, tets = xf.tet_block(1, 1, 120, (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))
+    sc = xf.Scene(ctx)
+    sc.add_body(xyz, tets, 1000.0)
+    with pytest.raises(xf.XfError, match="LDS"):
+        sc.finalize(sweep_blocks=True)
+    sc.close()
+    # vertex-ring energies run per colour only
+    sc = workloads.make_scene(ctx, wl, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    sc.set_settings(0, **dict(wl["settings"], energy=xf.ENERGY_CONTINUOUS_SKIN))
+    with pytest.raises(xf.XfError):
+        sc.step(1, float(wl["dt"]))
+    sc.close()

@@ -335,6 +335,136 @@ uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
     return ncol;
 }
 
+namespace {
+
+// Breadth-first levels over tet edges (nodes of one tet are adjacent) from
+// the used nodes of least coordinate `axis`; unreached components restart
+// at level 0 from their node of least coordinate (ties by index). Unused
+// nodes keep level -1.
+void AxisLevels(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets,
+                const std::vector<uint32_t>& first, const std::vector<uint32_t>& inc, int axis,
+                std::vector<int32_t>* level) {
+    std::vector<int32_t>& lev = *level;
+    lev.assign(n_nodes, -1);
+    std::vector<uint32_t> by_coord;
+    by_coord.reserve(n_nodes);
+    for (uint32_t v = 0; v < n_nodes; v++)
+        if (first[v + 1] > first[v]) by_coord.push_back(v);
+    std::stable_sort(by_coord.begin(), by_coord.end(),
+                     [&](uint32_t a, uint32_t b) { return xyz[3 * (size_t)a + axis] < xyz[3 * (size_t)b + axis]; });
+    std::vector<uint32_t> queue;
+    queue.reserve(n_nodes);
+    auto run = [&](size_t head) {
+        for (; head < queue.size(); head++) {
+            uint32_t u = queue[head];
+            for (uint32_t k = first[u]; k < first[u + 1]; k++) {
+                const uint32_t* is = tets + 4 * (size_t)inc[k];
+                for (int i = 0; i < 4; i++)
+                    if (lev[is[i]] < 0) {
+                        lev[is[i]] = lev[u] + 1;
+                        queue.push_back(is[i]);
+                    }
+            }
+        }
+    };
+    if (by_coord.empty()) return;
+    float lo = xyz[3 * (size_t)by_coord[0] + axis];
+    for (uint32_t v : by_coord) {
+        if (xyz[3 * (size_t)v + axis] != lo) break;
+        lev[v] = 0;
+        queue.push_back(v);
+    }
+    run(0);
+    for (uint32_t v : by_coord)
+        if (lev[v] < 0) {
+            size_t head = queue.size();
+            lev[v] = 0;
+            queue.push_back(v);
+            run(head);
+        }
+    (void)n_tets;
+}
+
+}  // namespace
+
+bool BuildSweepBlocks(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets,
+                      const uint32_t* visit, SweepBlocks* out, std::string* err) {
+    // node -> tets incidence
+    std::vector<uint32_t> first(n_nodes + 1, 0), inc(4 * (size_t)n_tets);
+    for (size_t i = 0; i < 4 * (size_t)n_tets; i++) {
+        if (tets[i] >= n_nodes) {
+            if (err) *err = "tet index out of range";
+            return false;
+        }
+        first[tets[i] + 1]++;
+    }
+    for (uint32_t v = 0; v < n_nodes; v++) first[v + 1] += first[v];
+    {
+        std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+        for (uint32_t t = 0; t < n_tets; t++)
+            for (int j = 0; j < 4; j++) inc[fill[tets[4 * (size_t)t + j]]++] = t;
+    }
+    std::vector<int32_t> l1, l2;
+    AxisLevels(xyz, n_nodes, tets, n_tets, first, inc, 0, &l1);
+    AxisLevels(xyz, n_nodes, tets, n_tets, first, inc, 1, &l2);
+    // block key per tet: minimum levels
+    std::vector<uint64_t> key(n_tets);
+    for (uint32_t t = 0; t < n_tets; t++) {
+        const uint32_t* is = tets + 4 * (size_t)t;
+        int32_t k1 = std::min(std::min(l1[is[0]], l1[is[1]]), std::min(l1[is[2]], l1[is[3]]));
+        int32_t k2 = std::min(std::min(l2[is[0]], l2[is[1]]), std::min(l2[is[2]], l2[is[3]]));
+        key[t] = ((uint64_t)(uint32_t)k1 << 32) | (uint32_t)k2;
+    }
+    auto launch_of = [](uint64_t k) -> uint32_t { return (uint32_t)((k >> 32) & 1u) | (uint32_t)((k & 1u) << 1); };
+    auto before = [&](uint64_t a, uint64_t b) {
+        uint32_t qa = launch_of(a), qb = launch_of(b);
+        return qa != qb ? qa < qb : a < b;
+    };
+    std::vector<uint64_t> keys(key);
+    std::sort(keys.begin(), keys.end(), before);
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    SweepBlocks& r = *out;
+    r.n_blocks = (uint32_t)keys.size();
+    r.launch.assign(r.n_blocks, 0);
+    for (uint32_t b = 0; b < r.n_blocks; b++) r.launch[b] = (uint8_t)launch_of(keys[b]);
+    r.block.assign(n_tets, 0);
+    for (uint32_t t = 0; t < n_tets; t++)
+        r.block[t] = (uint32_t)(std::lower_bound(keys.begin(), keys.end(), key[t], before) - keys.begin());
+    // tets of each block in visit order
+    std::vector<uint32_t> bstart(r.n_blocks + 1, 0), bl(n_tets);
+    for (uint32_t t = 0; t < n_tets; t++) bstart[r.block[t] + 1]++;
+    for (uint32_t b = 0; b < r.n_blocks; b++) bstart[b + 1] += bstart[b];
+    {
+        std::vector<uint32_t> fill(bstart.begin(), bstart.end() - 1);
+        for (uint32_t k = 0; k < n_tets; k++) bl[fill[r.block[visit[k]]]++] = visit[k];
+    }
+    // greedy phases inside each block (node masks are cleared after each block)
+    std::vector<uint64_t> mask(n_nodes, 0);
+    r.phase.assign(n_tets, 0);
+    r.phases.assign(r.n_blocks, 0);
+    for (uint32_t b = 0; b < r.n_blocks; b++) {
+        uint32_t np = 0;
+        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) {
+            const uint32_t* is = tets + 4 * (size_t)bl[k];
+            uint64_t used = mask[is[0]] | mask[is[1]] | mask[is[2]] | mask[is[3]];
+            if (used == ~0ull) {
+                if (err) *err = "a sweep block needs more than 64 phases";
+                return false;
+            }
+            uint32_t c = (uint32_t)__builtin_ctzll(~used);
+            r.phase[bl[k]] = c;
+            for (int i = 0; i < 4; i++) mask[is[i]] |= 1ull << c;
+            np = std::max(np, c + 1);
+        }
+        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) {
+            const uint32_t* is = tets + 4 * (size_t)bl[k];
+            for (int i = 0; i < 4; i++) mask[is[i]] = 0;
+        }
+        r.phases[b] = np;
+    }
+    return true;
+}
+
 int LdltSolve(uint32_t n, const float* A, const float* b, float* x) {
     // LDLTDecomposition / LDLTSolve (Ldlt.h:35-86) on packed lower-triangular A.
     if (n == 0 || n > 25) return -1;

@@ -48,6 +48,28 @@ uint32_t GreedyColour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, c
 uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, bool minstd_visit,
                           std::vector<uint32_t>* colour, std::vector<uint32_t>* order);
 
+// Sweep blocks: a partition of the tets into blocks whose launch colours
+// (0..3) make blocks of one launch colour node-disjoint, and a greedy
+// node-disjoint colouring (phases) inside each block. Node levels l1, l2 are
+// breadth-first distances over tet edges from the nodes of least x (l1) and
+// least y (l2); a component the search did not reach restarts from its node
+// of least coordinate. The nodes of one tet differ by at most one level, so a
+// tet with minimum levels (k1, k2) touches levels k1..k1+1 and k2..k2+1; its
+// block is (k1, k2) and its launch colour (k1 & 1) | (k2 & 1) << 1. Two blocks
+// of one launch colour differ by at least 2 in k1 or in k2 and share no node.
+// Blocks are numbered by (launch colour, k1, k2); phases take tets in visit
+// order. Returns false (with a message) if a block needs more than 64 phases.
+constexpr uint32_t kMaxBlockPhases = 64;
+struct SweepBlocks {
+    uint32_t n_blocks = 0;
+    std::vector<uint32_t> block;   // per tet
+    std::vector<uint32_t> phase;   // per tet: colour inside its block
+    std::vector<uint8_t> launch;   // per block: launch colour 0..3
+    std::vector<uint32_t> phases;  // per block: phase count
+};
+bool BuildSweepBlocks(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets,
+                      const uint32_t* visit, SweepBlocks* out, std::string* err);
+
 // Vertex rings of a linear-tet mesh as GeoLinear3d::Init builds them
 // (OneRing, Connectivity.h:106-155; Geo.cpp:745-757), in CSR form: ring v
 // has nodes nodes[node_off[v] .. node_off[v+1]) (v first, then every node of

@@ -44,6 +44,14 @@ extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t lanes, 
                                       const double4* O, double4* V, const XfBodyParams* bodies, float dt,
                                       float* clock);
 extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax);
+extern "C" size_t xf_block_lds_bytes(uint32_t nmax, uint32_t tmax, bool aux);
+extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
+                                       uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t tmax,
+                                       const uint32_t* tab, const uint32_t* noff, const uint32_t* bnode,
+                                       const uint2* tloc, const uint4* tidx, const float4* tg0, const float4* tg1,
+                                       const float4* tg2, const float4* tg3, const float4* taux, double4* X,
+                                       const double4* O, const XfBodyParams* bodies, float comp1_u, float a_u,
+                                       float dt, float* clock);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
@@ -220,6 +228,20 @@ struct xf_scene {
     uint16_t* rincb = nullptr;  // per ring node: first entry of its incidence list (ring-local)
     uint16_t* rinc = nullptr;   // per ring: (ring tet << 2 | corner) of each node, node-major, tets ascending
     float4* rtet = nullptr;
+    // sweep blocks (opts.sweep_blocks): per launch colour q, nslot blocks
+    // (body-major, then block number) and nph phases; device tables at
+    // blk_tab + tab_off (nph rows of nslot + 1 schedule positions) and
+    // blk_noff + noff_off (nslot + 1 offsets into blk_node, global node ids);
+    // tloc holds four block-local node indices per schedule position
+    struct BlockLaunch {
+        uint32_t nslot = 0, nph = 0, tab_off = 0, noff_off = 0;
+    };
+    BlockLaunch blk[4];
+    uint32_t n_blocks = 0, blk_nmax = 0, blk_tmax = 0, blk_threads = 64;
+    uint32_t* blk_tab = nullptr;
+    uint32_t* blk_noff = nullptr;
+    uint32_t* blk_node = nullptr;
+    uint2* tloc = nullptr;
     uint64_t device_bytes = 0;
     int n_cu = 256;
     // host mirrors
@@ -630,8 +652,33 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
     }
     int rc = node(0, false);
     if (rc) return rc;
+    const bool blocks = s->opts.launch_mode == XF_LAUNCH_BLOCKS;
     for (uint32_t k = 0; k < substeps; k++) {
-        for (uint32_t c = 0; c < s->ncol; c++) {
+        // sweep blocks: one launch per launch colour that holds blocks; the
+        // first advances the clock
+        bool first = true;
+        for (int q = 0; q < 4 && blocks; q++) {
+            const xf_scene::BlockLaunch& L = s->blk[q];
+            if (L.nslot == 0) continue;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timer) {
+                XF_HIP(ctx, hipEventCreate(&e0));
+                XF_HIP(ctx, hipEventCreate(&e1));
+                XF_HIP(ctx, hipEventRecord(e0, st));
+            }
+            XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, s->blk_threads, L.nslot, L.nph,
+                                         s->blk_nmax, s->blk_tmax, s->blk_tab + L.tab_off, s->blk_noff + L.noff_off,
+                                         s->blk_node, s->tloc, s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3],
+                                         s->taux, s->X, s->O, s->bp, s->comp1_u, s->a_u, dt,
+                                         first ? s->clock : nullptr));
+            first = false;
+            if (timer) {
+                XF_HIP(ctx, hipEventRecord(e1, st));
+                timer->ev_proj.push_back(e0);
+                timer->ev_proj.push_back(e1);
+            }
+        }
+        for (uint32_t c = 0; c < s->ncol && !blocks; c++) {
             uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (timer) {
@@ -698,7 +745,7 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
                 return Fail(s->ctx, XF_E_UNSUPPORTED, "Pixar energies take settings.compliance; a body has per-tet mu");
     }
     if (*energy == kRingMode) {
-        if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT)
+        if (s->opts.launch_mode != XF_LAUNCH_PER_COLOUR)
             return Fail(s->ctx, XF_E_UNSUPPORTED, "vertex-ring energies run with per-colour launches only");
         for (const Body& b : s->bodies)
             if (!b.tet_mu.empty())
@@ -806,7 +853,8 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     DropGraph(s);
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
-                    s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet};
+                    s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet,
+                    s->blk_tab, s->blk_noff, s->blk_node, s->tloc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -909,15 +957,74 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     if (s->opts.block_size % 64 || s->opts.block_size > 256)
         return Fail(ctx, XF_E_INVALID, "block_size must be 64, 128, 192 or 256");
     if (s->opts.colour_order > XF_COLOUR_MINSTD) return Fail(ctx, XF_E_INVALID, "unknown colour order");
-    if (s->opts.launch_mode > XF_LAUNCH_PERSISTENT) return Fail(ctx, XF_E_INVALID, "unknown launch mode");
+    if (s->opts.launch_mode > XF_LAUNCH_BLOCKS) return Fail(ctx, XF_E_INVALID, "unknown launch mode");
     if (s->opts.xcd_slabs > 1) return Fail(ctx, XF_E_INVALID, "xcd_slabs must be 0 or 1");
     if (s->opts.ring_lanes != 0 && s->opts.ring_lanes != 16 && s->opts.ring_lanes != 32)
         return Fail(ctx, XF_E_INVALID, "ring_lanes must be 0, 16 or 32");
+    if (s->opts.sweep_blocks > 1) return Fail(ctx, XF_E_INVALID, "sweep_blocks must be 0 or 1");
+    if (s->opts.sweep_blocks && s->opts.xcd_slabs)
+        return Fail(ctx, XF_E_INVALID, "sweep_blocks and xcd_slabs are exclusive");
+    if (s->opts.launch_mode == XF_LAUNCH_BLOCKS && !s->opts.sweep_blocks)
+        return Fail(ctx, XF_E_INVALID, "XF_LAUNCH_BLOCKS needs sweep_blocks");
+    if (s->opts.sweep_blocks)
+        for (const Body& b : s->bodies)
+            if (!b.user_colour.empty())
+                return Fail(ctx, XF_E_INVALID, "sweep_blocks colours the tets itself; a body has tet_colour");
     XF_HIP(ctx, hipSetDevice(ctx->device));
 
+    // Sweep blocks: colour (q, j) = phase j of the blocks of launch colour q,
+    // numbered over the launch colours that hold blocks.
+    std::vector<xf::SweepBlocks> sweep(s->opts.sweep_blocks ? s->bodies.size() : 0);
+    uint32_t blk_base[4] = {0, 0, 0, 0};
+    if (s->opts.sweep_blocks) {
+        uint32_t nph[4] = {0, 0, 0, 0};
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            Body& b = s->bodies[bi];
+            std::vector<uint32_t> visit(b.t);
+            if (s->opts.colour_order == XF_COLOUR_MINSTD) xf::MinstdOrder(b.t, visit.data());
+            else
+                for (uint32_t t = 0; t < b.t; t++) visit[t] = t;
+            std::string err;
+            if (!xf::BuildSweepBlocks(b.xyz.data(), b.n, b.tets.data(), b.t, visit.data(), &sweep[bi], &err))
+                return Fail(ctx, XF_E_UNSUPPORTED, "sweep blocks of body %zu: %s", bi, err.c_str());
+            const xf::SweepBlocks& sb = sweep[bi];
+            for (uint32_t k = 0; k < sb.n_blocks; k++) nph[sb.launch[k]] = std::max(nph[sb.launch[k]], sb.phases[k]);
+        }
+        s->ncol = 0;
+        for (int q = 0; q < 4; q++) {
+            blk_base[q] = s->ncol;
+            s->blk[q].nph = nph[q];
+            s->ncol += nph[q];
+        }
+        if (s->ncol > xf::kMaxColours)
+            return Fail(ctx, XF_E_UNSUPPORTED, "sweep blocks need %u colours (limit %u)", s->ncol, xf::kMaxColours);
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            Body& b = s->bodies[bi];
+            const xf::SweepBlocks& sb = sweep[bi];
+            std::vector<uint32_t> visit(b.t), rank(b.t);
+            if (s->opts.colour_order == XF_COLOUR_MINSTD) xf::MinstdOrder(b.t, visit.data());
+            else
+                for (uint32_t t = 0; t < b.t; t++) visit[t] = t;
+            for (uint32_t k = 0; k < b.t; k++) rank[visit[k]] = k;
+            b.colour.resize(b.t);
+            b.order.resize(b.t);
+            for (uint32_t t = 0; t < b.t; t++) {
+                b.colour[t] = blk_base[sb.launch[sb.block[t]]] + sb.phase[t];
+                b.order[t] = t;
+            }
+            b.ncol = 1 + *std::max_element(b.colour.begin(), b.colour.end());
+            std::sort(b.order.begin(), b.order.end(), [&](uint32_t a, uint32_t c) {
+                if (b.colour[a] != b.colour[c]) return b.colour[a] < b.colour[c];
+                if (sb.block[a] != sb.block[c]) return sb.block[a] < sb.block[c];
+                return rank[a] < rank[c];
+            });
+        }
+    }
+
     // Colour each body; the scene runs max(ncol) phases.
-    s->ncol = 0;
+    if (!s->opts.sweep_blocks) s->ncol = 0;
     for (Body& b : s->bodies) {
+        if (s->opts.sweep_blocks) break;
         if (!b.user_colour.empty()) {
             b.colour = b.user_colour;
             b.ncol = 1 + *std::max_element(b.colour.begin(), b.colour.end());
@@ -990,6 +1097,88 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     s->colour_begin[s->ncol] = pos;
     if (pos != s->T) return Fail(ctx, XF_E_INVALID, "internal: schedule covers %u of %u tets", pos, s->T);
 
+    // Sweep-block tables: slots of launch colour q are its blocks, body-major
+    // then by block number (the order of the blocks' tets inside a colour).
+    std::vector<uint2> h_tloc;
+    std::vector<uint32_t> h_btab, h_bnoff, h_bnode;
+    if (s->opts.sweep_blocks) {
+        uint32_t nslot[4] = {0, 0, 0, 0};
+        std::vector<std::vector<uint32_t>> slot(s->bodies.size());
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            const xf::SweepBlocks& sb = sweep[bi];
+            slot[bi].resize(sb.n_blocks);
+            for (uint32_t k = 0; k < sb.n_blocks; k++) slot[bi][k] = nslot[sb.launch[k]]++;
+        }
+        uint32_t slot_base[5] = {0, 0, 0, 0, 0};
+        for (int q = 0; q < 4; q++) slot_base[q + 1] = slot_base[q] + nslot[q];
+        s->n_blocks = slot_base[4];
+        std::vector<std::vector<uint32_t>> bnodes(s->n_blocks);
+        std::vector<uint32_t> btets(s->n_blocks, 0), sp(s->T, 0);
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            const Body& b = s->bodies[bi];
+            const xf::SweepBlocks& sb = sweep[bi];
+            for (uint32_t t = 0; t < b.t; t++) {
+                uint32_t k = sb.block[t], g = slot_base[sb.launch[k]] + slot[bi][k];
+                for (int i = 0; i < 4; i++) bnodes[g].push_back(b.node_base + b.tets[4 * (size_t)t + i]);
+                btets[g]++;
+                sp[b.sched_pos[t]] = slot[bi][k];
+            }
+        }
+        s->blk_nmax = s->blk_tmax = 0;
+        for (uint32_t g = 0; g < s->n_blocks; g++) {
+            std::vector<uint32_t>& v = bnodes[g];
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            s->blk_nmax = std::max(s->blk_nmax, (uint32_t)v.size());
+            s->blk_tmax = std::max(s->blk_tmax, btets[g]);
+        }
+        size_t lds = xf_block_lds_bytes(s->blk_nmax, s->blk_tmax, true);
+        if (lds > 65536)
+            return Fail(ctx, XF_E_UNSUPPORTED,
+                        "a sweep block needs %zu bytes of LDS (%u nodes, %u tets; limit 65536)", lds, s->blk_nmax,
+                        s->blk_tmax);
+        h_tloc.assign(s->T, make_uint2(0u, 0u));
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            const Body& b = s->bodies[bi];
+            const xf::SweepBlocks& sb = sweep[bi];
+            for (uint32_t t = 0; t < b.t; t++) {
+                uint32_t k = sb.block[t], g = slot_base[sb.launch[k]] + slot[bi][k];
+                const std::vector<uint32_t>& v = bnodes[g];
+                uint32_t l[4];
+                for (int i = 0; i < 4; i++)
+                    l[i] = (uint32_t)(std::lower_bound(v.begin(), v.end(), b.node_base + b.tets[4 * (size_t)t + i]) -
+                                      v.begin());
+                h_tloc[b.sched_pos[t]] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+            }
+        }
+        uint32_t pmax = 0;
+        for (int q = 0; q < 4; q++) {
+            xf_scene::BlockLaunch& L = s->blk[q];
+            L.nslot = nslot[q];
+            L.tab_off = (uint32_t)h_btab.size();
+            L.noff_off = (uint32_t)h_bnoff.size();
+            if (L.nslot == 0) L.nph = 0;
+            for (uint32_t j = 0; j < L.nph; j++) {
+                uint32_t c = blk_base[q] + j, p = s->colour_begin[c], end = s->colour_begin[c + 1];
+                for (uint32_t k = 0; k < L.nslot; k++) {
+                    h_btab.push_back(p);
+                    uint32_t p0 = p;
+                    while (p < end && sp[p] == k) p++;
+                    pmax = std::max(pmax, p - p0);
+                }
+                h_btab.push_back(end);
+                if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
+            }
+            for (uint32_t k = 0; k < L.nslot; k++) {
+                h_bnoff.push_back((uint32_t)h_bnode.size());
+                const std::vector<uint32_t>& v = bnodes[slot_base[q] + k];
+                h_bnode.insert(h_bnode.end(), v.begin(), v.end());
+            }
+            h_bnoff.push_back((uint32_t)h_bnode.size());
+        }
+        s->blk_threads = std::min(256u, 64u * std::max(1u, (pmax + 63) / 64));
+    }
+
     // Nodes.
     std::vector<double4> hX(s->N), hO(s->N), hV(s->N);
     s->h_w.assign(s->N, 0.0);
@@ -1032,6 +1221,12 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         (rc = DevAlloc(ctx, &s->arrive, 1, &s->device_bytes)) ||
         (rc = DevAlloc(ctx, &s->barrier_err, 1, &s->device_bytes)))
         return rc;
+    if (s->opts.sweep_blocks &&
+        ((rc = DevAlloc(ctx, &s->blk_tab, h_btab.size(), &s->device_bytes)) ||
+         (rc = DevAlloc(ctx, &s->blk_noff, h_bnoff.size(), &s->device_bytes)) ||
+         (rc = DevAlloc(ctx, &s->blk_node, std::max<size_t>(1, h_bnode.size()), &s->device_bytes)) ||
+         (rc = DevAlloc(ctx, &s->tloc, s->T, &s->device_bytes))))
+        return rc;
     XF_HIP(ctx, hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     hipStream_t st = ctx->stream;
     XF_HIP(ctx, hipMemcpyAsync(s->X, hX.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
@@ -1042,6 +1237,16 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     XF_HIP(ctx, hipMemcpyAsync(s->tg[1], h_tg1.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[2], h_tg2.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tdmp, h_tdmp.data(), sizeof(float2) * s->T, hipMemcpyHostToDevice, st));
+    if (s->opts.sweep_blocks) {
+        XF_HIP(ctx, hipMemcpyAsync(s->blk_tab, h_btab.data(), sizeof(uint32_t) * h_btab.size(), hipMemcpyHostToDevice,
+                                   st));
+        XF_HIP(ctx, hipMemcpyAsync(s->blk_noff, h_bnoff.data(), sizeof(uint32_t) * h_bnoff.size(),
+                                   hipMemcpyHostToDevice, st));
+        if (!h_bnode.empty())
+            XF_HIP(ctx, hipMemcpyAsync(s->blk_node, h_bnode.data(), sizeof(uint32_t) * h_bnode.size(),
+                                       hipMemcpyHostToDevice, st));
+        XF_HIP(ctx, hipMemcpyAsync(s->tloc, h_tloc.data(), sizeof(uint2) * s->T, hipMemcpyHostToDevice, st));
+    }
     uint32_t uzero = 0;
     static const uint32_t kClockZero[kClockWords] = {0u, 0u};  // time 0.0f, tick 0
     XF_HIP(ctx, hipMemcpyAsync(s->clock, kClockZero, sizeof(kClockZero), hipMemcpyHostToDevice, st));
@@ -1066,6 +1271,9 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     info->device_bytes = s->device_bytes;
     info->n_rings = s->rings_host ? s->R : 0;
     info->n_ring_colours = s->rings_host ? s->rcol : 0;
+    info->n_blocks = s->n_blocks;
+    info->block_launches = 0;
+    for (int q = 0; q < 4; q++) info->block_launches += s->blk[q].nslot ? 1u : 0u;
     return XF_OK;
 }
 
@@ -1363,9 +1571,11 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     };
     const Cand candidates[] = {{XF_LAUNCH_PER_COLOUR, 64},  {XF_LAUNCH_PER_COLOUR, 128}, {XF_LAUNCH_PER_COLOUR, 256},
                                {XF_LAUNCH_PERSISTENT, 64},  {XF_LAUNCH_PERSISTENT, 128}, {XF_LAUNCH_PERSISTENT, 256},
-                               {XF_LAUNCH_PERSISTENT, 512}};
+                               {XF_LAUNCH_PERSISTENT, 512}, {XF_LAUNCH_BLOCKS, 0}};
     xf_scene_opts keep = s->opts;
-    Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size};
+    Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks
+                                     : keep.launch_mode == XF_LAUNCH_BLOCKS   ? 0u
+                                                                              : keep.block_size};
     int maxb = 0;  // ring, Pixar and Cube energies run per colour only
     if (energy != kRingMode && energy != 20 && energy != 22 && energy != 30)
         XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
@@ -1379,9 +1589,10 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     for (const Cand& cand : candidates) {
         if (he != hipSuccess || rc) break;
         if (cand.mode == XF_LAUNCH_PERSISTENT && (int)cand.size > maxb) continue;
+        if (cand.mode == XF_LAUNCH_BLOCKS && (!s->opts.sweep_blocks || energy == kRingMode)) continue;
         s->opts.launch_mode = cand.mode;
         if (cand.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = cand.size;
-        else s->opts.block_size = cand.size;
+        else if (cand.mode == XF_LAUNCH_PER_COLOUR) s->opts.block_size = cand.size;
         DropGraph(s);
         if ((he = copy_all(false)) != hipSuccess) break;
         if ((rc = xf_scene_step(s, substeps, dt))) break;  // warm-up and graph capture
@@ -1407,7 +1618,7 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     s->opts = keep;
     s->opts.launch_mode = best_c.mode;
     if (best_c.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = best_c.size;
-    else s->opts.block_size = best_c.size;
+    else if (best_c.mode == XF_LAUNCH_PER_COLOUR) s->opts.block_size = best_c.size;
     DropGraph(s);
     if (rc) return rc;
     if (he != hipSuccess) return Fail(ctx, XF_E_HIP, "autotune: %s", hipGetErrorString(he));
@@ -1665,6 +1876,38 @@ XF_API int xf_mesh_colour(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
     return XF_OK;
 }
 
+namespace {
+thread_local std::string g_mesh_error;
+}
+
+XF_API int xf_mesh_sweep_blocks(uint32_t n_nodes, const float* xyz, uint32_t n_tets, const uint32_t* tets,
+                                uint32_t colour_order, uint32_t* n_blocks, uint32_t* block_of_tet,
+                                uint32_t* phase_of_tet, uint32_t* launch_of_block, uint32_t* max_phases) {
+    if (!xyz || !tets || !n_blocks || !block_of_tet || !phase_of_tet || !launch_of_block ||
+        colour_order > XF_COLOUR_MINSTD)
+        return XF_E_INVALID;
+    std::vector<uint32_t> visit(n_tets);
+    if (colour_order == XF_COLOUR_MINSTD) xf::MinstdOrder(n_tets, visit.data());
+    else
+        for (uint32_t t = 0; t < n_tets; t++) visit[t] = t;
+    xf::SweepBlocks sb;
+    std::string err;
+    if (!xf::BuildSweepBlocks(xyz, n_nodes, tets, n_tets, visit.data(), &sb, &err)) {
+        g_mesh_error = err;
+        return err == "tet index out of range" ? XF_E_INVALID : XF_E_UNSUPPORTED;
+    }
+    *n_blocks = sb.n_blocks;
+    memcpy(block_of_tet, sb.block.data(), sizeof(uint32_t) * n_tets);
+    memcpy(phase_of_tet, sb.phase.data(), sizeof(uint32_t) * n_tets);
+    uint32_t mp = 0;
+    for (uint32_t k = 0; k < sb.n_blocks; k++) {
+        launch_of_block[k] = sb.launch[k];
+        mp = std::max(mp, sb.phases[k]);
+    }
+    if (max_phases) *max_phases = mp;
+    return XF_OK;
+}
+
 XF_API int xf_mesh_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_rings, uint32_t* node_total,
                          uint32_t* node_off, uint32_t* nodes, uint32_t* tet_off, uint32_t* ring_tets, uint8_t* idx,
                          uint32_t* largest) {
@@ -1680,10 +1923,6 @@ XF_API int xf_mesh_rings(uint32_t n_tets, const uint32_t* tets, uint32_t* n_ring
     if (ring_tets) memcpy(ring_tets, r.tets.data(), sizeof(uint32_t) * r.tets.size());
     if (idx) memcpy(idx, r.idx.data(), r.idx.size());
     return XF_OK;
-}
-
-namespace {
-thread_local std::string g_mesh_error;
 }
 
 XF_API const char* xf_mesh_error(void) { return g_mesh_error.c_str(); }

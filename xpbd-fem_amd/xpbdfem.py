@@ -22,7 +22,7 @@ ENERGY_CUBE_NEO, ENERGY_CUBE_SKIN = 11, 12  # no constraint on tets
 COLLIDE_GROUND, COLLIDE_SPHERE = 1, 2
 RAYLEIGH_PAPER, RAYLEIGH_LIMIT, RAYLEIGH_POST, RAYLEIGH_POST_AMORTIZED = 0, 1, 2, 3
 COLOUR_NATURAL, COLOUR_MINSTD = 0, 1
-LAUNCH_PER_COLOUR, LAUNCH_PERSISTENT = 0, 1
+LAUNCH_PER_COLOUR, LAUNCH_PERSISTENT, LAUNCH_BLOCKS = 0, 1, 2
 SKIN_YEOH = (0.1095, 14.95, 4.595)  # Fem.cpp:526
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -74,13 +74,13 @@ class BodyDesc(C.Structure):
 class SceneOpts(C.Structure):
     _fields_ = [("colour_order", C.c_uint32), ("block_size", C.c_uint32), ("use_graph", C.c_uint32),
                 ("launch_mode", C.c_uint32), ("persistent_blocks", C.c_uint32), ("xcd_slabs", C.c_uint32),
-                ("ring_lanes", C.c_uint32)]
+                ("ring_lanes", C.c_uint32), ("sweep_blocks", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
     _fields_ = [("n_bodies", C.c_uint32), ("n_nodes", C.c_uint32), ("n_tets", C.c_uint32),
                 ("n_colours", C.c_uint32), ("device_bytes", C.c_uint64), ("n_rings", C.c_uint32),
-                ("n_ring_colours", C.c_uint32)]
+                ("n_ring_colours", C.c_uint32), ("n_blocks", C.c_uint32), ("block_launches", C.c_uint32)]
 
 
 _lib = None
@@ -141,6 +141,7 @@ def lib():
         "xf_scene_get_order": ([vp, C.c_uint32, vp, vp], C.c_int),
         "xf_scene_get_ring_order": ([vp, C.c_uint32, vp, vp, vp], C.c_int),
         "xf_mesh_rings": ([C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
+        "xf_mesh_sweep_blocks": ([C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, vp, vp, vp, vp], C.c_int),
         "xf_scene_get_elements": ([vp, C.c_uint32, vp, vp, vp, vp, vp], C.c_int),
         "xf_scene_profile": ([vp, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
                               C.POINTER(C.c_uint32)], C.c_int),
@@ -176,7 +177,7 @@ def exported_symbols():
         "xf_scene_get_elements", "xf_scene_profile", "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
         "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
         "xf_scene_get_tick", "xf_scene_set_tick", "xf_mesh_read_tetgen", "xf_mesh_tets_from_stream", "xf_mesh_error",
-        "xf_mesh_rings",
+        "xf_mesh_rings", "xf_mesh_sweep_blocks",
     ]
 
 
@@ -219,6 +220,22 @@ def mesh_rings(tets):
                         out["idx"].ctypes.data, C.byref(big))
     out["largest"] = big.value
     return out
+
+
+def mesh_sweep_blocks(xyz, tets, colour_order=COLOUR_NATURAL):
+    """Sweep blocks of a tet mesh as the scene builds them with sweep_blocks:
+    dict of block (per tet), phase (per tet, inside its block), launch (per
+    block, 0..3) and max_phases."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    tets = np.ascontiguousarray(tets, np.uint32)
+    n, t = xyz.size // 3, tets.size // 4
+    nb, mp = C.c_uint32(), C.c_uint32()
+    block, phase, launch = np.zeros(t, np.uint32), np.zeros(t, np.uint32), np.zeros(max(t, 1), np.uint32)
+    rc = lib().xf_mesh_sweep_blocks(n, xyz.ctypes.data, t, tets.ctypes.data, colour_order, C.byref(nb),
+                                    block.ctypes.data, phase.ctypes.data, launch.ctypes.data, C.byref(mp))
+    if rc:
+        raise XfError(f"xf_mesh_sweep_blocks failed ({rc}): {lib().xf_mesh_error().decode()}")
+    return dict(block=block, phase=phase, launch=launch[:nb.value].copy(), max_phases=mp.value)
 
 
 def autoresize(xyz):
@@ -408,8 +425,9 @@ class Scene:
         return bid.value
 
     def finalize(self, colour_order=COLOUR_NATURAL, block_size=64, use_graph=True, launch_mode=LAUNCH_PER_COLOUR,
-                 persistent_blocks=0, xcd_slabs=False, ring_lanes=0):
+                 persistent_blocks=0, xcd_slabs=False, ring_lanes=0, sweep_blocks=False):
         o = SceneOpts()
+        o.sweep_blocks = int(sweep_blocks)
         o.xcd_slabs = int(xcd_slabs)
         o.ring_lanes = ring_lanes
         o.colour_order = colour_order
@@ -423,7 +441,8 @@ class Scene:
         i = SceneInfo()
         self.ctx.check(lib().xf_scene_info_get(self.h, C.byref(i)), "xf_scene_info_get")
         return dict(n_bodies=i.n_bodies, n_nodes=i.n_nodes, n_tets=i.n_tets, n_colours=i.n_colours,
-                    device_bytes=i.device_bytes, n_rings=i.n_rings, n_ring_colours=i.n_ring_colours)
+                    device_bytes=i.device_bytes, n_rings=i.n_rings, n_ring_colours=i.n_ring_colours,
+                    n_blocks=i.n_blocks, block_launches=i.block_launches)
 
     def set_settings(self, body, settings=None, **kw):
         s = settings if settings is not None else default_settings(**kw)
@@ -502,12 +521,13 @@ class Scene:
 
     def autotune(self, substeps, dt, reps=3):
         """Picks the fastest launch shape; returns (launch_mode, size, ms per call),
-        size = workgroup size (per-colour) or grid blocks (persistent)."""
+        size = workgroup size (per-colour), grid blocks (persistent) or 0
+        (sweep blocks)."""
         o = SceneOpts()
         ms = C.c_float()
         self.ctx.check(lib().xf_scene_autotune(self.h, substeps, dt, reps, C.byref(o), C.byref(ms)),
                        "xf_scene_autotune")
-        size = o.persistent_blocks if o.launch_mode == LAUNCH_PERSISTENT else o.block_size
+        size = {LAUNCH_PERSISTENT: o.persistent_blocks, LAUNCH_BLOCKS: 0}.get(o.launch_mode, o.block_size)
         return o.launch_mode, size, ms.value
 
     def adjoint(self, substeps, dt, loss_at, targets, window=20, per_body=False):
