@@ -202,9 +202,8 @@ typedef struct xf_scene_opts {
                                    Colour (q, j) holds phase j of every block of launch colour q;
                                    the order is colour-major, then body, block, visit order (a
                                    valid serial order; xf_scene_get_order reports it). Needed by
-                                   XF_LAUNCH_BLOCKS; excludes xcd_slabs and tet_colour. A block
-                                   must fit the LDS of one workgroup (32 B per node + 88 B per tet
-                                   <= 64 KiB). */
+                                   XF_LAUNCH_BLOCKS; excludes xcd_slabs and tet_colour. The nodes
+                                   of a block must fit the LDS of one workgroup (at most 2048). */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {

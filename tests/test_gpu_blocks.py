@@ -177,8 +177,8 @@ def test_blocks_rejections(ctx):
     with pytest.raises(xf.XfError):
         sc.finalize(sweep_blocks=True)
     sc.close()
-    # a column of 120 cells: 720 tets and 484 nodes need 78,848 bytes of LDS
-    xyz, tets = xf.tet_block(1, 1, 120, (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))
+    # a column of 600 cells: 2,404 nodes in one block (LDS holds 2,048)
+    xyz, tets = xf.tet_block(1, 1, 600, (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))
     sc = xf.Scene(ctx)
     sc.add_body(xyz, tets, 1000.0)
     with pytest.raises(xf.XfError, match="LDS"):

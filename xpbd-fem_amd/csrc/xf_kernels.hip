@@ -810,82 +810,88 @@ __global__ __launch_bounds__(256) void xf_project_colour(
 }
 
 // Sweep-block projection: workgroup s of one launch colour sweeps block s
-// through all of its phases. The block's nodes and element data are staged
-// in LDS, phases are separated by workgroup barriers, and the nodes are
-// written back at the end. Blocks of one launch colour share no node, so
-// the launch equals the serial sweep of its blocks one after another, each
-// phase by phase.
-// tab: nph rows of nslot + 1 schedule positions (row j, entry s: the first
-// phase-j tet of block s; entry nslot: the end of that colour). noff: nslot
-// + 1 offsets into bnode (each block's nodes, global ids). tloc: per
+// through all of its phases. The block's nodes live in LDS for the whole
+// launch; phases are separated by workgroup barriers and the nodes are
+// written back at the end. Blocks of one launch colour share no node, so the
+// launch equals the serial sweep of its blocks one after another, each phase
+// by phase. Each lane holds one tet of the current phase and loads its
+// element data for the next phase before solving the current one.
+// rec: per block, nph (first schedule position, count) pairs. bnode: per
+// block, nmax global node ids (xf::kNoNode past the block's nodes). tloc: per
 // schedule position, the tet's four block-local node indices, 16 bits each
-// (x: corners 0 | 1 << 16, y: corners 2 | 3 << 16). nmax / tmax: the most
-// nodes / tets of any block (LDS layout). The first launch colour of a
-// substep advances the simulation clock (clock != nullptr).
+// (x: corners 0 | 1 << 16, y: corners 2 | 3 << 16). The first launch colour
+// of a substep advances the simulation clock (clock != nullptr).
+struct BlockTet {
+    float4 q0, q1, q2, q3, ax;
+    uint2 loc;
+};
+
+template <bool AUX>
+__device__ __forceinline__ BlockTet load_block_tet(uint32_t i, const uint2* __restrict__ tloc,
+                                                   const float4* __restrict__ tg0, const float4* __restrict__ tg1,
+                                                   const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+                                                   const float4* __restrict__ taux) {
+    BlockTet t;
+    t.q0 = tg0[i];
+    t.q1 = tg1[i];
+    t.q2 = tg2[i];
+    t.q3 = tg3[i];
+    t.ax = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (AUX) t.ax = taux[i];
+    t.loc = tloc[i];
+    return t;
+}
+
 template <int E, bool SIM, bool AUX, bool DAMP>
 __global__ __launch_bounds__(256) void xf_project_blocks(
-    uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t tmax, const uint32_t* __restrict__ tab,
-    const uint32_t* __restrict__ noff, const uint32_t* __restrict__ bnode, const uint2* __restrict__ tloc,
-    const uint4* __restrict__ tidx, const float4* __restrict__ tg0, const float4* __restrict__ tg1,
-    const float4* __restrict__ tg2, const float4* __restrict__ tg3, const float4* __restrict__ taux,
-    double4* __restrict__ X, const double4* __restrict__ O, const XfBodyParams* __restrict__ bodies, float comp1_u,
-    float a_u, float dt, float* __restrict__ clock) {
+    uint32_t nph, uint32_t nmax, const uint32_t* __restrict__ rec, const uint32_t* __restrict__ bnode,
+    const uint2* __restrict__ tloc, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+    const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+    const float4* __restrict__ taux, double4* __restrict__ X, const double4* __restrict__ O,
+    const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock) {
     if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
         clock[0] = clock[0] + dt;
         uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
         tick[0] = tick[0] + 1u;
     }
     if constexpr (E == 30) return;  // CubeNeo / CubeSkin: no constraint on tets (Fem.cpp:567-760)
-    extern __shared__ __align__(16) unsigned char xf_block_lds[];
-    double4* xs = reinterpret_cast<double4*>(xf_block_lds);
-    float4* l0 = reinterpret_cast<float4*>(xs + nmax);
-    float4* l1 = l0 + tmax;
-    float4* l2 = l1 + tmax;
-    float4* l3 = l2 + tmax;
-    float4* lx = l3 + tmax;  // AUX only
-    uint2* lc = reinterpret_cast<uint2*>(AUX ? lx + tmax : lx);
-    const uint32_t s = blockIdx.x, tid = threadIdx.x, bs = blockDim.x, row = nslot + 1;
-    const uint32_t n0 = noff[s], nn = noff[s + 1] - n0;
-    for (uint32_t k = tid; k < nn; k += bs) xs[k] = X[bnode[n0 + k]];
-    uint32_t off = 0;
-    for (uint32_t j = 0; j < nph; j++) {
-        uint32_t b = tab[j * row + s], cnt = tab[j * row + s + 1] - b;
-        for (uint32_t k = tid; k < cnt; k += bs) {
-            l0[off + k] = tg0[b + k];
-            l1[off + k] = tg1[b + k];
-            l2[off + k] = tg2[b + k];
-            l3[off + k] = tg3[b + k];
-            if constexpr (AUX) lx[off + k] = taux[b + k];
-            lc[off + k] = tloc[b + k];
-        }
-        off += cnt;
+    extern __shared__ double4 xf_block_nodes[];
+    double4* xs = xf_block_nodes;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t* r = rec + (size_t)blockIdx.x * 2 * nph;
+    const uint32_t* bn = bnode + (size_t)blockIdx.x * nmax;
+    BlockTet cur;
+    if (tid < r[1]) cur = load_block_tet<AUX>(r[0] + tid, tloc, tg0, tg1, tg2, tg3, taux);
+    for (uint32_t k = tid; k < nmax; k += bs) {
+        uint32_t g = bn[k];
+        if (g != xf::kNoNode) xs[k] = X[g];
     }
     __syncthreads();
-    off = 0;
     for (uint32_t j = 0; j < nph; j++) {
-        uint32_t b = tab[j * row + s], cnt = tab[j * row + s + 1] - b;
+        const uint32_t b = r[2 * j], cnt = r[2 * j + 1];
+        BlockTet nxt;
+        if (j + 1 < nph && tid < r[2 * j + 3]) nxt = load_block_tet<AUX>(r[2 * j + 2] + tid, tloc, tg0, tg1, tg2, tg3, taux);
         for (uint32_t k = tid; k < cnt; k += bs) {
-            uint32_t li = off + k;
-            uint2 c = lc[li];
-            uint32_t ia = c.x & 0xFFFFu, ib = c.x >> 16, ic = c.y & 0xFFFFu, id = c.y >> 16;
+            BlockTet t = k == tid ? cur : load_block_tet<AUX>(b + k, tloc, tg0, tg1, tg2, tg3, taux);
+            uint32_t ia = t.loc.x & 0xFFFFu, ib = t.loc.x >> 16, ic = t.loc.y & 0xFFFFu, id = t.loc.y >> 16;
             double4 na = xs[ia], nb = xs[ib], nc = xs[ic], nd = xs[id];
             double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
             float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
-            float4 ax = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if constexpr (AUX) ax = lx[li];
             uint4 gid = make_uint4(0u, 0u, 0u, 0u);
             if constexpr (DAMP) gid = tidx[b + k];
-            project_loaded<E, SIM, AUX, DAMP>(l0[li], l1[li], l2[li], l3[li], ax, gid, x, w, O, bodies, comp1_u,
-                                              a_u, dt);
+            project_loaded<E, SIM, AUX, DAMP>(t.q0, t.q1, t.q2, t.q3, t.ax, gid, x, w, O, bodies, comp1_u, a_u, dt);
             xs[ia] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
             xs[ib] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
             xs[ic] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
             xs[id] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
         }
-        off += cnt;
         __syncthreads();
+        cur = nxt;
     }
-    for (uint32_t k = tid; k < nn; k += bs) X[bnode[n0 + k]] = xs[k];
+    for (uint32_t k = tid; k < nmax; k += bs) {
+        uint32_t g = bn[k];
+        if (g != xf::kNoNode) X[g] = xs[k];
+    }
 }
 
 namespace {
@@ -1320,9 +1326,9 @@ extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, boo
     return hipGetLastError();
 }
 
-typedef void (*XfBlocksFn)(uint32_t, uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*, const uint32_t*,
-                           const uint2*, const uint4*, const float4*, const float4*, const float4*, const float4*,
-                           const float4*, double4*, const double4*, const XfBodyParams*, float, float, float, float*);
+typedef void (*XfBlocksFn)(uint32_t, uint32_t, const uint32_t*, const uint32_t*, const uint2*, const uint4*,
+                           const float4*, const float4*, const float4*, const float4*, const float4*, double4*,
+                           const double4*, const XfBodyParams*, float, float, float, float*);
 
 namespace {
 template <int E, bool SIM>
@@ -1336,20 +1342,18 @@ XfBlocksFn pick_blocks(bool sim, bool aux, bool damp) {
 }
 }  // namespace
 
-// LDS bytes of a sweep-block workgroup: nmax FP64 nodes (32 B), tmax tets of
-// element data (64 B, + 16 B with AUX) and local indices (8 B).
-extern "C" size_t xf_block_lds_bytes(uint32_t nmax, uint32_t tmax, bool aux) {
-    return 32 * (size_t)nmax + (size_t)tmax * (64 + (aux ? 16 : 0) + 8);
-}
+// LDS bytes of a sweep-block workgroup: its nodes in FP64 (32 B each).
+extern "C" size_t xf_block_lds_bytes(uint32_t nmax) { return 32 * (size_t)nmax; }
 
-// Host-callable: one sweep-block launch (nslot workgroups of `threads`).
+// Host-callable: one sweep-block launch (nslot workgroups of `threads`);
+// rec / bnode point at the launch colour's first block.
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
-                                       uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t tmax,
-                                       const uint32_t* tab, const uint32_t* noff, const uint32_t* bnode,
-                                       const uint2* tloc, const uint4* tidx, const float4* tg0, const float4* tg1,
-                                       const float4* tg2, const float4* tg3, const float4* taux, double4* X,
-                                       const double4* O, const XfBodyParams* bodies, float comp1_u, float a_u,
-                                       float dt, float* clock) {
+                                       uint32_t nslot, uint32_t nph, uint32_t nmax, const uint32_t* rec,
+                                       const uint32_t* bnode, const uint2* tloc, const uint4* tidx,
+                                       const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3,
+                                       const float4* taux, double4* X, const double4* O,
+                                       const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
+                                       float* clock) {
     XfBlocksFn fn = nullptr;
     switch (energy) {
         case 0: fn = pick_blocks<0>(sim, aux, damp); break;
@@ -1363,11 +1367,11 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
         case 30: fn = pick_blocks<30>(sim, aux, damp); break;
         default: return hipErrorInvalidValue;
     }
-    size_t lds = xf_block_lds_bytes(nmax, tmax, aux);
-    if (threads == 0 || threads > 256 || threads % 64 || lds > 65536) return hipErrorInvalidValue;
+    size_t lds = xf_block_lds_bytes(nmax);
+    if (threads == 0 || threads > 256 || threads % 64 || nph == 0 || lds > 65536) return hipErrorInvalidValue;
     if (nslot == 0) return hipSuccess;
-    hipLaunchKernelGGL(fn, dim3(nslot), dim3(threads), lds, s, nslot, nph, nmax, tmax, tab, noff, bnode, tloc, tidx,
-                       tg0, tg1, tg2, tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock);
+    hipLaunchKernelGGL(fn, dim3(nslot), dim3(threads), lds, s, nph, nmax, rec, bnode, tloc, tidx, tg0, tg1, tg2,
+                       tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock);
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@
 #include "../../include/xpbdfem.h"
 #include "xf_device.h"
 #include "xf_host_util.h"
+#include "xf_sched.h"
 
 extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t block,
                                         uint32_t slabs, uint32_t begin, uint32_t count, const uint4* tidx,
@@ -44,14 +45,14 @@ extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t lanes, 
                                       const double4* O, double4* V, const XfBodyParams* bodies, float dt,
                                       float* clock);
 extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax);
-extern "C" size_t xf_block_lds_bytes(uint32_t nmax, uint32_t tmax, bool aux);
+extern "C" size_t xf_block_lds_bytes(uint32_t nmax);
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
-                                       uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t tmax,
-                                       const uint32_t* tab, const uint32_t* noff, const uint32_t* bnode,
-                                       const uint2* tloc, const uint4* tidx, const float4* tg0, const float4* tg1,
-                                       const float4* tg2, const float4* tg3, const float4* taux, double4* X,
-                                       const double4* O, const XfBodyParams* bodies, float comp1_u, float a_u,
-                                       float dt, float* clock);
+                                       uint32_t nslot, uint32_t nph, uint32_t nmax, const uint32_t* rec,
+                                       const uint32_t* bnode, const uint2* tloc, const uint4* tidx,
+                                       const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3,
+                                       const float4* taux, double4* X, const double4* O,
+                                       const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
+                                       float* clock);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
@@ -229,17 +230,16 @@ struct xf_scene {
     uint16_t* rinc = nullptr;   // per ring: (ring tet << 2 | corner) of each node, node-major, tets ascending
     float4* rtet = nullptr;
     // sweep blocks (opts.sweep_blocks): per launch colour q, nslot blocks
-    // (body-major, then block number) and nph phases; device tables at
-    // blk_tab + tab_off (nph rows of nslot + 1 schedule positions) and
-    // blk_noff + noff_off (nslot + 1 offsets into blk_node, global node ids);
-    // tloc holds four block-local node indices per schedule position
+    // (body-major, then block number) and nph phases; blk_rec + rec_off holds
+    // per block nph (first schedule position, count) pairs, blk_node +
+    // node_off per block blk_nmax global node ids (xf::kNoNode padded); tloc
+    // holds four block-local node indices per schedule position
     struct BlockLaunch {
-        uint32_t nslot = 0, nph = 0, tab_off = 0, noff_off = 0;
+        uint32_t nslot = 0, nph = 0, rec_off = 0, node_off = 0;
     };
     BlockLaunch blk[4];
-    uint32_t n_blocks = 0, blk_nmax = 0, blk_tmax = 0, blk_threads = 64;
-    uint32_t* blk_tab = nullptr;
-    uint32_t* blk_noff = nullptr;
+    uint32_t n_blocks = 0, blk_nmax = 0, blk_threads = 64;
+    uint32_t* blk_rec = nullptr;
     uint32_t* blk_node = nullptr;
     uint2* tloc = nullptr;
     uint64_t device_bytes = 0;
@@ -667,10 +667,9 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                 XF_HIP(ctx, hipEventRecord(e0, st));
             }
             XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, s->blk_threads, L.nslot, L.nph,
-                                         s->blk_nmax, s->blk_tmax, s->blk_tab + L.tab_off, s->blk_noff + L.noff_off,
-                                         s->blk_node, s->tloc, s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3],
-                                         s->taux, s->X, s->O, s->bp, s->comp1_u, s->a_u, dt,
-                                         first ? s->clock : nullptr));
+                                         s->blk_nmax, s->blk_rec + L.rec_off, s->blk_node + L.node_off, s->tloc,
+                                         s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O,
+                                         s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr));
             first = false;
             if (timer) {
                 XF_HIP(ctx, hipEventRecord(e1, st));
@@ -854,7 +853,7 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
                     s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet,
-                    s->blk_tab, s->blk_noff, s->blk_node, s->tloc};
+                    s->blk_rec, s->blk_node, s->tloc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -1100,7 +1099,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     // Sweep-block tables: slots of launch colour q are its blocks, body-major
     // then by block number (the order of the blocks' tets inside a colour).
     std::vector<uint2> h_tloc;
-    std::vector<uint32_t> h_btab, h_bnoff, h_bnode;
+    std::vector<uint32_t> h_brec, h_bnode;
     if (s->opts.sweep_blocks) {
         uint32_t nslot[4] = {0, 0, 0, 0};
         std::vector<std::vector<uint32_t>> slot(s->bodies.size());
@@ -1113,30 +1112,25 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         for (int q = 0; q < 4; q++) slot_base[q + 1] = slot_base[q] + nslot[q];
         s->n_blocks = slot_base[4];
         std::vector<std::vector<uint32_t>> bnodes(s->n_blocks);
-        std::vector<uint32_t> btets(s->n_blocks, 0), sp(s->T, 0);
+        std::vector<uint32_t> sp(s->T, 0);
         for (size_t bi = 0; bi < s->bodies.size(); bi++) {
             const Body& b = s->bodies[bi];
             const xf::SweepBlocks& sb = sweep[bi];
             for (uint32_t t = 0; t < b.t; t++) {
                 uint32_t k = sb.block[t], g = slot_base[sb.launch[k]] + slot[bi][k];
                 for (int i = 0; i < 4; i++) bnodes[g].push_back(b.node_base + b.tets[4 * (size_t)t + i]);
-                btets[g]++;
                 sp[b.sched_pos[t]] = slot[bi][k];
             }
         }
-        s->blk_nmax = s->blk_tmax = 0;
+        s->blk_nmax = 0;
         for (uint32_t g = 0; g < s->n_blocks; g++) {
             std::vector<uint32_t>& v = bnodes[g];
             std::sort(v.begin(), v.end());
             v.erase(std::unique(v.begin(), v.end()), v.end());
             s->blk_nmax = std::max(s->blk_nmax, (uint32_t)v.size());
-            s->blk_tmax = std::max(s->blk_tmax, btets[g]);
         }
-        size_t lds = xf_block_lds_bytes(s->blk_nmax, s->blk_tmax, true);
-        if (lds > 65536)
-            return Fail(ctx, XF_E_UNSUPPORTED,
-                        "a sweep block needs %zu bytes of LDS (%u nodes, %u tets; limit 65536)", lds, s->blk_nmax,
-                        s->blk_tmax);
+        if (xf_block_lds_bytes(s->blk_nmax) > 65536)
+            return Fail(ctx, XF_E_UNSUPPORTED, "a sweep block has %u nodes (LDS holds 2048)", s->blk_nmax);
         h_tloc.assign(s->T, make_uint2(0u, 0u));
         for (size_t bi = 0; bi < s->bodies.size(); bi++) {
             const Body& b = s->bodies[bi];
@@ -1151,30 +1145,28 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
                 h_tloc[b.sched_pos[t]] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
             }
         }
+        h_bnode.assign((size_t)s->n_blocks * s->blk_nmax, xf::kNoNode);
+        for (uint32_t g = 0; g < s->n_blocks; g++)
+            std::copy(bnodes[g].begin(), bnodes[g].end(), h_bnode.begin() + (size_t)g * s->blk_nmax);
         uint32_t pmax = 0;
         for (int q = 0; q < 4; q++) {
             xf_scene::BlockLaunch& L = s->blk[q];
             L.nslot = nslot[q];
-            L.tab_off = (uint32_t)h_btab.size();
-            L.noff_off = (uint32_t)h_bnoff.size();
             if (L.nslot == 0) L.nph = 0;
+            L.rec_off = (uint32_t)h_brec.size();
+            L.node_off = slot_base[q] * s->blk_nmax;
+            h_brec.resize(h_brec.size() + 2 * (size_t)L.nslot * L.nph, 0u);
             for (uint32_t j = 0; j < L.nph; j++) {
                 uint32_t c = blk_base[q] + j, p = s->colour_begin[c], end = s->colour_begin[c + 1];
                 for (uint32_t k = 0; k < L.nslot; k++) {
-                    h_btab.push_back(p);
                     uint32_t p0 = p;
                     while (p < end && sp[p] == k) p++;
+                    h_brec[L.rec_off + 2 * ((size_t)k * L.nph + j)] = p0;
+                    h_brec[L.rec_off + 2 * ((size_t)k * L.nph + j) + 1] = p - p0;
                     pmax = std::max(pmax, p - p0);
                 }
-                h_btab.push_back(end);
                 if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
             }
-            for (uint32_t k = 0; k < L.nslot; k++) {
-                h_bnoff.push_back((uint32_t)h_bnode.size());
-                const std::vector<uint32_t>& v = bnodes[slot_base[q] + k];
-                h_bnode.insert(h_bnode.end(), v.begin(), v.end());
-            }
-            h_bnoff.push_back((uint32_t)h_bnode.size());
         }
         s->blk_threads = std::min(256u, 64u * std::max(1u, (pmax + 63) / 64));
     }
@@ -1221,11 +1213,9 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         (rc = DevAlloc(ctx, &s->arrive, 1, &s->device_bytes)) ||
         (rc = DevAlloc(ctx, &s->barrier_err, 1, &s->device_bytes)))
         return rc;
-    if (s->opts.sweep_blocks &&
-        ((rc = DevAlloc(ctx, &s->blk_tab, h_btab.size(), &s->device_bytes)) ||
-         (rc = DevAlloc(ctx, &s->blk_noff, h_bnoff.size(), &s->device_bytes)) ||
-         (rc = DevAlloc(ctx, &s->blk_node, std::max<size_t>(1, h_bnode.size()), &s->device_bytes)) ||
-         (rc = DevAlloc(ctx, &s->tloc, s->T, &s->device_bytes))))
+    if (s->opts.sweep_blocks && ((rc = DevAlloc(ctx, &s->blk_rec, h_brec.size(), &s->device_bytes)) ||
+                                 (rc = DevAlloc(ctx, &s->blk_node, h_bnode.size(), &s->device_bytes)) ||
+                                 (rc = DevAlloc(ctx, &s->tloc, s->T, &s->device_bytes))))
         return rc;
     XF_HIP(ctx, hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     hipStream_t st = ctx->stream;
@@ -1238,10 +1228,9 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     XF_HIP(ctx, hipMemcpyAsync(s->tg[2], h_tg2.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tdmp, h_tdmp.data(), sizeof(float2) * s->T, hipMemcpyHostToDevice, st));
     if (s->opts.sweep_blocks) {
-        XF_HIP(ctx, hipMemcpyAsync(s->blk_tab, h_btab.data(), sizeof(uint32_t) * h_btab.size(), hipMemcpyHostToDevice,
-                                   st));
-        XF_HIP(ctx, hipMemcpyAsync(s->blk_noff, h_bnoff.data(), sizeof(uint32_t) * h_bnoff.size(),
-                                   hipMemcpyHostToDevice, st));
+        if (!h_brec.empty())
+            XF_HIP(ctx, hipMemcpyAsync(s->blk_rec, h_brec.data(), sizeof(uint32_t) * h_brec.size(),
+                                       hipMemcpyHostToDevice, st));
         if (!h_bnode.empty())
             XF_HIP(ctx, hipMemcpyAsync(s->blk_node, h_bnode.data(), sizeof(uint32_t) * h_bnode.size(),
                                        hipMemcpyHostToDevice, st));

@@ -7,6 +7,7 @@
 namespace xf {
 
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;  // padding of a sweep block's node list
 
 // Schedule position of this lane's item in a colour [begin, begin + count),
 // or kNoSlot when the lane has none; `group` consecutive lanes share one item
