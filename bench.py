@@ -3,6 +3,10 @@
 Workload: the 96x24x24-cell Yeoh beam (331,776 linear tets, 60,625 nodes),
 1/60 s frames of 30 substeps, ground plane and kinematic sphere, inputs
 resident in HBM. One step = one frame (30 substeps, captured as one hipGraph).
+Schedule: sweep blocks by default (4 projection launches per substep, each
+workgroup sweeping one column block through its phases in LDS; the launch
+shape picked by the autotuner), or per-colour launches (--schedule slabs /
+plain); the per-colour XCD-slab frame time is reported beside the blocks.
 
 Prints one JSON line: value = whole-job tet-substeps/s, the roofline of the
 projection kernels (algorithmic bytes per launch / measured launch time,
@@ -33,6 +37,30 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_TET = 16 + 64 + 112 + 96
 # Per node and substep (predict + collisions + velocity): read X, V; write X, V (FP64).
 BYTES_PER_NODE = 96
+# Sweep-block launches: per tet its constants 64 + block-local node indices 8
+# (+ comp1 / a / body 16 with per-tet data); per block node its id 4 + FP64
+# position and inverse mass read 32 + written 32 (the block's nodes stay in
+# LDS through its phases).
+BYTES_PER_BLOCK_TET = 64 + 8
+BYTES_PER_BLOCK_NODE = 4 + 32 + 32
+# Scene options of the sweep schedules: sweep blocks (4 launches per substep,
+# each workgroup sweeping one block through its phases), or one launch per
+# colour phase with the XCD-slab order or the plain blockIdx-linear order.
+SCHEDULES = {"blocks": dict(sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS), "slabs": dict(xcd_slabs=True),
+             "plain": {}}
+MODE_NAMES = {xf.LAUNCH_PER_COLOUR: "per-colour", xf.LAUNCH_PERSISTENT: "persistent", xf.LAUNCH_BLOCKS: "blocks"}
+KERNELS = {xf.LAUNCH_PER_COLOUR: "xf_project_colour", xf.LAUNCH_PERSISTENT: "xf_frame_persistent",
+           xf.LAUNCH_BLOCKS: "xf_project_blocks"}
+
+
+def projection_bytes(info, mode, substeps, per_tet_data=False):
+    """Algorithmic bytes of the projection launches of `substeps` substeps
+    (the persistent launch also holds the node phases)."""
+    T, N = info["n_tets"], info["n_nodes"]
+    if mode == xf.LAUNCH_BLOCKS:
+        per_tet = BYTES_PER_BLOCK_TET + (16 if per_tet_data else 0)
+        return substeps * (per_tet * T + BYTES_PER_BLOCK_NODE * info["n_block_nodes"])
+    return substeps * (BYTES_PER_TET * T + (BYTES_PER_NODE * N if mode == xf.LAUNCH_PERSISTENT else 0))
 
 
 def rel_pos_err(x, x_ref, x0):
@@ -42,7 +70,7 @@ def rel_pos_err(x, x_ref, x0):
     return float(d / np.linalg.norm(p0.max(axis=0) - p0.min(axis=0)))
 
 
-def cpu_baseline(ctx, wl, budget_s=12.0):
+def cpu_baseline(ctx, wl, budget_s=12.0, opts=None):
     """The reference's own Geo3d::Substep (oracle/_ref, compiled from the
     reference sources) on the same mesh, pins and velocities, 1 thread,
     YeohSkinFast, its own MINSTD sweep order; falls back to the oracle's
@@ -51,9 +79,11 @@ def cpu_baseline(ctx, wl, budget_s=12.0):
     Also checks one frame of the GPU path: against the compiled reference
     (the GPU's colour order as tOrder, the reference's fixed Yeoh constants,
     no contacts: the reference has neither custom constants nor contacts)
-    and against the FP64 oracle with the workload's full settings."""
+    and against the FP64 oracle with the workload's full settings. opts:
+    the scene options of the GPU runs (the headline's schedule)."""
     import pyoracle as po
 
+    opts = opts or {}
     xyz, tets = wl["xyz"], wl["tets"]
     t = tets.size // 4
     dt = np.float32(wl["dt"])
@@ -62,7 +92,7 @@ def cpu_baseline(ctx, wl, budget_s=12.0):
     x0 = xyz.astype(np.float64)
     check = {"frames": 1}
     plain = dict(wl, settings=dict(s, collide=0, yeoh=xf.SKIN_YEOH))
-    g = workloads.make_scene(ctx, plain)
+    g = workloads.make_scene(ctx, plain, **opts)
     g.step(S, float(dt))
     xg, _ = g.get_state(0)
     order = g.order(0)[0]
@@ -73,7 +103,7 @@ def cpu_baseline(ctx, wl, budget_s=12.0):
         rb.substeps(rb.settings(s["compliance"], nu=s["poissons_ratio"], energy=7, simultaneous=True), dt, S)
         check["rel_pos_err_vs_reference"] = rel_pos_err(xg, rb.state()[0], xyz)
         del rb
-    g = workloads.make_scene(ctx, wl)
+    g = workloads.make_scene(ctx, wl, **opts)
     g.step(S, float(dt))
     xg, _ = g.get_state(0)
     order = g.order(0)[0]
@@ -108,14 +138,14 @@ def cpu_baseline(ctx, wl, budget_s=12.0):
                 sample=f"{what}: {n} substeps of the {t}-tet beam, single thread, {el:.1f} s", check=check)
 
 
-def batched_leg(ctx, frames=3, xcd_slabs=False):
-    """256 boxes: one batched scene (colour c of all bodies per launch,
-    autotuned workgroup size) against the plain per-body launch loop (one
-    scene per body, stepped one after another)."""
+def batched_leg(ctx, frames=3, schedule="slabs"):
+    """256 boxes: one batched scene (every body's blocks, or colour c of all
+    bodies, per launch; autotuned launch shape) against the plain per-body
+    launch loop (one per-colour scene per body, stepped one after another)."""
     b = workloads.boxes_batch()
     S, dt = b["substeps"], float(b["dt"])
     T = sum(x["tets"].size // 4 for x in b["bodies"])
-    sc = workloads.make_batch_scene(ctx, b, xcd_slabs=xcd_slabs)
+    sc = workloads.make_batch_scene(ctx, b, **SCHEDULES[schedule])
     mode, size, _ = sc.autotune(S, dt, reps=2)
     sc.step(S, dt)
     sc.sync()
@@ -146,7 +176,7 @@ def batched_leg(ctx, frames=3, xcd_slabs=False):
     return {"workload": f"{b['name']}: {len(b['bodies'])} bodies, {T} tets, {S} substeps/frame, {ncol} colours",
             "batched_tet_substeps_per_s": batched, "plain_loop_tet_substeps_per_s": plain,
             "batched_over_plain": batched / plain, "frames": frames,
-            "launch": {"mode": "persistent" if mode else "per-colour", "size": size, "xcd_slabs": xcd_slabs},
+            "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": schedule},
             "finite": finite}
 
 
@@ -229,36 +259,48 @@ def rings_leg(ctx, frames=3, budget_s=8.0, xcd_slabs=True):
     return out
 
 
-def large_leg(ctx, frames=2, cells=(288, 72, 72), xcd_slabs=True):
-    """The same Yeoh beam at 288x72x72 cells (8,957,952 tets, 373,248 per
-    colour): the projection kernel with colours large enough that HBM
-    bandwidth, not the phase latency, bounds it. Reports tet-substeps/s and
-    the projection kernel's achieved algorithmic bandwidth (288 B per tet)."""
-    wl = workloads.beam_yeoh(*cells)
-    S, dt = wl["substeps"], float(wl["dt"])
-    sc = workloads.make_scene(ctx, wl, xcd_slabs=xcd_slabs)
-    info = sc.info()
-    T = info["n_tets"]
-    mode, size, _ = sc.autotune(S, dt, reps=1)
+def timed_frames(sc, S, dt, frames):
+    """Milliseconds per frame of `frames` steps after one warm-up step."""
     sc.step(S, dt)
     sc.sync()
     t0 = time.perf_counter()
     for _ in range(frames):
         sc.step(S, dt)
     sc.sync()
-    el = time.perf_counter() - t0
+    return 1e3 * (time.perf_counter() - t0) / frames
+
+
+def large_leg(ctx, frames=2, cells=(288, 72, 72), schedule="blocks"):
+    """The same Yeoh beam at 288x72x72 cells (8,957,952 tets, 1,540,081
+    nodes): colours large enough that the projection is bound by throughput
+    (VALU and HBM), not by phase latency. Reports tet-substeps/s and the
+    projection kernel's achieved algorithmic bandwidth for the schedule, and
+    the per-colour XCD-slab schedule's frame time beside it."""
+    wl = workloads.beam_yeoh(*cells)
+    S, dt = wl["substeps"], float(wl["dt"])
+    sc = workloads.make_scene(ctx, wl, **SCHEDULES[schedule])
+    info = sc.info()
+    T = info["n_tets"]
+    mode, size, _ = sc.autotune(S, dt, reps=1)
+    ms = timed_frames(sc, S, dt, frames)
     finite = sc.count_nonfinite() == 0
     ms_proj, _, n_launch = sc.profile(S, dt)
     sc.close()
-    achieved = BYTES_PER_TET * T * S / (ms_proj * 1e-3) / 1e9 if not mode else None
-    return {"workload": f"{wl['name']}: {T} tets, {info['n_nodes']} nodes, {info['n_colours']} colours, "
-                        f"{T // info['n_colours']} tets per colour, {S} substeps/frame",
-            "tet_substeps_per_s": T * S * frames / el, "ms_per_frame": 1e3 * el / frames, "finite": finite,
-            "launch": {"mode": "persistent" if mode else "per-colour", "size": size, "xcd_slabs": xcd_slabs},
-            "roofline": {"bound": "hbm", "kernel": "xf_project_colour", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                         "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
-                         "bytes_per_launch": BYTES_PER_TET * T * S / max(n_launch, 1)}}
+    bytes_proj = projection_bytes(info, mode, S)
+    achieved = bytes_proj / (ms_proj * 1e-3) / 1e9 if mode != xf.LAUNCH_PERSISTENT else None
+    out = {"workload": f"{wl['name']}: {T} tets, {info['n_nodes']} nodes, {info['n_colours']} colours, "
+                       f"{info['n_blocks']} sweep blocks, {S} substeps/frame",
+           "tet_substeps_per_s": T * S / (ms * 1e-3), "ms_per_frame": ms, "finite": finite,
+           "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": schedule},
+           "roofline": {"bound": "hbm", "kernel": KERNELS[mode], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                        "launches": n_launch, "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
+                        "bytes_per_launch": bytes_proj / max(n_launch, 1)}}
+    if schedule != "slabs":
+        sc = workloads.make_scene(ctx, wl, **SCHEDULES["slabs"])
+        out["per_colour_slabs_ms_per_frame"] = timed_frames(sc, S, dt, frames)
+        sc.close()
+    return out
 
 
 def adjoint_leg(ctx, reps=3):
@@ -387,19 +429,36 @@ def rings_profile(frames=1, xcd_slabs=True):
                       "frames": frames, "xcd_slabs": xcd_slabs}))
 
 
-def large_profile(xcd_slabs=True, block=256):
-    """One frame of the 288x72x72 beam, per-colour launches of `block`
-    threads (short runs under rocprofv3 counter passes)."""
+def beam_profile(schedule="blocks"):
+    """One frame of the 96x24x24 beam in the schedule's fixed launch shape
+    (block launches, or per-colour launches of 64 threads; short runs under
+    rocprofv3 counter passes)."""
+    ctx = xf.Context(0)
+    wl = workloads.beam_yeoh_96x24x24()
+    sc = workloads.make_scene(ctx, wl, **SCHEDULES[schedule])
+    sc.step(wl["substeps"], float(wl["dt"]))
+    sc.sync()
+    info = sc.info()
+    sc.close()
+    ctx.close()
+    print(json.dumps({"beam_profile": wl["name"], "tets": info["n_tets"], "colours": info["n_colours"],
+                      "blocks": info["n_blocks"], "block_nodes": info["n_block_nodes"], "schedule": schedule}))
+
+
+def large_profile(schedule="blocks", block=256):
+    """One frame of the 288x72x72 beam in a fixed launch shape: block launches,
+    or per-colour launches of `block` threads (short runs under rocprofv3
+    counter passes)."""
     ctx = xf.Context(0)
     wl = workloads.beam_yeoh(288, 72, 72)
-    sc = workloads.make_scene(ctx, wl, xcd_slabs=xcd_slabs, block_size=block)
+    sc = workloads.make_scene(ctx, wl, block_size=block, **SCHEDULES[schedule])
     sc.step(wl["substeps"], float(wl["dt"]))
     sc.sync()
     info = sc.info()
     sc.close()
     ctx.close()
     print(json.dumps({"large_profile": wl["name"], "tets": info["n_tets"], "colours": info["n_colours"],
-                      "block": block, "xcd_slabs": xcd_slabs}))
+                      "blocks": info["n_blocks"], "block": block, "schedule": schedule}))
 
 
 def main():
@@ -423,7 +482,13 @@ def main():
     ap.add_argument("--large-profile", action="store_true",
                     help="only run one frame of the 288x72x72 beam (rocprofv3 counter passes)")
     ap.add_argument("--no-xcd-slabs", action="store_true",
-                    help="plain blockIdx-linear schedule instead of the XCD-slab one (colour eighths per XCD)")
+                    help="plain blockIdx-linear order in per-colour launches instead of the XCD-slab one")
+    ap.add_argument("--schedule", choices=sorted(SCHEDULES), default="blocks",
+                    help="sweep schedule of the beam, large and batched legs: sweep blocks (default) or "
+                         "per-colour launches in the XCD-slab or plain order")
+    ap.add_argument("--profile", action="store_true",
+                    help="only run one frame of the beam in the schedule's fixed launch shape "
+                         "(rocprofv3 counter passes)")
     args = ap.parse_args()
     args.xcd_slabs = not args.no_xcd_slabs
     if args.copy_probe:
@@ -436,7 +501,10 @@ def main():
         rings_profile(xcd_slabs=args.xcd_slabs)
         return
     if args.large_profile:
-        large_profile(xcd_slabs=args.xcd_slabs)
+        large_profile(schedule=args.schedule)
+        return
+    if args.profile:
+        beam_profile(schedule=args.schedule)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
@@ -445,11 +513,15 @@ def main():
 
     wl = workloads.beam_yeoh_96x24x24()
     ctx = xf.Context(0)
-    sc = workloads.make_scene(ctx, wl, use_graph=True, xcd_slabs=args.xcd_slabs)
+    opts = SCHEDULES[args.schedule]
+    sc = workloads.make_scene(ctx, wl, use_graph=True, **opts)
     info = sc.info()
     T, N, S = info["n_tets"], info["n_nodes"], wl["substeps"]
     dt = float(wl["dt"])
-    mode, size = (0, 64) if args.no_autotune else sc.autotune(S, dt, reps=2)[:2]
+    if args.no_autotune:
+        mode, size = opts.get("launch_mode", xf.LAUNCH_PER_COLOUR), 64
+    else:
+        mode, size = sc.autotune(S, dt, reps=2)[:2]
 
     for _ in range(args.warmup):
         sc.step(S, dt)
@@ -465,22 +537,26 @@ def main():
         sys.exit(1)
 
     # Device time of the dominant kernel (HIP events on the library stream
-    # around each launch of one frame): the per-colour projection launches,
-    # or the single persistent launch (which also holds the node phases).
+    # around each launch of one frame): the sweep-block or per-colour
+    # projection launches, or the single persistent launch (which also holds
+    # the node phases).
     ms_proj, ms_node, n_launch = sc.profile(S, dt)
-    bytes_proj = BYTES_PER_TET * T * S + (BYTES_PER_NODE * N * S if mode else 0)
+    bytes_proj = projection_bytes(info, mode, S)
     achieved = bytes_proj / (ms_proj * 1e-3) / 1e9
 
     # HBM traffic per projection launch: calibrated FETCH_SIZE + WRITE_SIZE of the
-    # rocprofv3 --pmc passes recorded for this workload (profiles/), when the
-    # launch shape matches.
+    # rocprofv3 --pmc passes recorded for this workload and schedule (profiles/),
+    # when the launch shape matches.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_beam96x24x24.json")
-    if os.path.exists(tpath) and not mode:
+    if os.path.exists(tpath):
         with open(tpath) as f:
             rec = json.load(f)
-        if rec.get("tets_per_dispatch") == T // max(info["n_colours"], 1):
-            traffic = float(rec["traffic_bytes_per_dispatch"] if args.xcd_slabs else
+        if mode == xf.LAUNCH_BLOCKS and "blocks" in rec and rec["blocks"].get("n_blocks") == info["n_blocks"]:
+            traffic = float(rec["blocks"]["traffic_bytes_per_dispatch"])
+        elif mode == xf.LAUNCH_PER_COLOUR and args.schedule != "blocks" and \
+                rec.get("tets_per_dispatch") == T // max(info["n_colours"], 1):
+            traffic = float(rec["traffic_bytes_per_dispatch"] if args.schedule == "slabs" else
                             rec["plain_schedule"]["traffic_bytes_per_dispatch"])
 
     out = {
@@ -497,26 +573,34 @@ def main():
         "dtype": "fp32 element math, fp64 node state",
         "data": "synthetic",
         "config": {"workload": f"{wl['name']}: {T} tets, {N} nodes, {S} substeps/frame, {info['n_colours']} "
-                               f"colours, ground + sphere", "step": "one frame",
-                   "launch": {"mode": "persistent" if mode else "per-colour", "size": size,
-                              "xcd_slabs": args.xcd_slabs}},
+                               f"colours, {info['n_blocks']} sweep blocks, ground + sphere", "step": "one frame",
+                   "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": args.schedule}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "xf_frame_persistent" if mode else "xf_project_colour", "launches": n_launch,
-                     "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": KERNELS[mode],
+                     "launches": n_launch, "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
                      "bytes_per_launch": bytes_proj / max(n_launch, 1)},
     }
+    if args.schedule == "blocks":
+        # the per-colour schedule on the same beam (structured 24 colours, XCD slabs)
+        pc = workloads.make_scene(ctx, wl, **SCHEDULES["slabs"])
+        pc.autotune(S, dt, reps=2)
+        ms_pc = timed_frames(pc, S, dt, 10)
+        pc.close()
+        out["per_colour_slabs"] = {"ms_per_frame": ms_pc, "tet_substeps_per_s": T * S / (ms_pc * 1e-3)}
     legs = []
     if not args.no_batch:
-        legs.append(("batched", lambda c: batched_leg(c, xcd_slabs=args.xcd_slabs)))
+        # many small rotated boxes: per-colour launches (158k tets per colour);
+        # their breadth-first blocks from single corner nodes are irregular
+        legs.append(("batched", lambda c: batched_leg(c, schedule="slabs" if args.schedule == "blocks"
+                                                      else args.schedule)))
     if not args.no_large:
-        legs.append(("large", lambda c: large_leg(c, xcd_slabs=args.xcd_slabs)))
+        legs.append(("large", lambda c: large_leg(c, schedule=args.schedule)))
     if not args.no_rings:
         legs.append(("rings", lambda c: rings_leg(c, xcd_slabs=args.xcd_slabs)))
     if not args.no_adjoint:
         legs += [("adjoint", adjoint_leg), ("fit", fit_leg)]
     if not args.no_cpu:
-        legs.append(("cpu_baseline", lambda c: cpu_baseline(c, wl)))
+        legs.append(("cpu_baseline", lambda c: cpu_baseline(c, wl, opts=opts)))
     for name, leg in legs:
         try:
             out[name] = leg(ctx)

@@ -175,13 +175,15 @@ enum {
     XF_LAUNCH_PER_COLOUR = 0, /* one launch per colour phase and node phase (hipGraph-captured if use_graph) */
     XF_LAUNCH_PERSISTENT = 1, /* one cooperative launch per step call, grid barriers between phases */
     XF_LAUNCH_BLOCKS = 2      /* needs sweep_blocks: one projection launch per block launch colour
-                                 (at most 4 per substep), each workgroup sweeping one block through
-                                 its phases in LDS; other passes run per colour */
+                                 (at most 4 per substep), each workgroup of block_size threads
+                                 sweeping one block through its phases with the block's nodes in
+                                 LDS; other passes run per colour */
 };
 
 typedef struct xf_scene_opts {
     uint32_t colour_order;      /* XF_COLOUR_* */
-    uint32_t block_size;        /* per-colour projection workgroup size, 0 = default (64) */
+    uint32_t block_size;        /* workgroup size of per-colour and sweep-block projection
+                                   launches (64, 128, 192, 256), 0 = default (64) */
     uint32_t use_graph;         /* capture each per-colour xf_scene_step call shape in a hipGraph */
     uint32_t launch_mode;       /* XF_LAUNCH_* */
     uint32_t persistent_blocks; /* grid size of the persistent launch, 0 = one per compute unit */
@@ -216,6 +218,8 @@ typedef struct xf_scene_info {
     uint32_t n_ring_colours; /* ring colour phases per substep (max over bodies) */
     uint32_t n_blocks;       /* sweep blocks over all bodies (0 without sweep_blocks) */
     uint32_t block_launches; /* block launch colours that hold blocks (0 without sweep_blocks) */
+    uint32_t n_block_nodes;  /* nodes of all sweep blocks, each block counting its own (a node shared
+                                by blocks of different launch colours counts once per block) */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */
@@ -287,8 +291,8 @@ XF_API int xf_scene_profile(xf_scene* scene, uint32_t substeps, float dt, float*
 /* Autotuner: times `reps` calls of xf_scene_step(substeps, dt) for each
  * launch shape -- per-colour launches with workgroup sizes 64, 128, 256 and
  * the persistent launch with 64, 128, 256 and 512 blocks (where co-resident),
- * and the sweep-block launches when the scene has sweep_blocks -- and keeps
- * the fastest in the scene options. The node state and clock are restored afterwards, so tuning
+ * and sweep-block launches with workgroup sizes 64, 128, 256 when the scene
+ * has sweep_blocks -- and keeps the fastest in the scene options. The node state and clock are restored afterwards, so tuning
  * does not advance the simulation. chosen / best_ms (ms per call) nullable.
  * No reference counterpart (the reference is single-threaded). */
 XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,

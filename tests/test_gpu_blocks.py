@@ -56,6 +56,36 @@ def test_blocks_bit_exact(ctx, energy, sim):
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
 
 
+def test_blocks_workgroup_sizes(ctx):
+    """64, 128 and 256 threads per block (lanes loop over a phase's tets when
+    it has more tets than lanes): the same results."""
+    wl = workloads.beam_neo_12x4x4(energy=7)
+    runs = [run(ctx, wl, block_size=b) for b in (64, 128, 256)]
+    ob = oracle(wl, runs[0][2], 2 * wl["substeps"])
+    for r in runs:
+        assert np.array_equal(r[2], runs[0][2])
+        assert np.array_equal(r[0], ob.X) and np.array_equal(r[1], ob.V)
+
+
+def test_blocks_more_tets_than_lanes(ctx):
+    """Columns of 180 tets (12x2x30 cells) and of 420 tets whose largest
+    phases hold 70 tets (4x4x70 cells), on 64-thread workgroups: lanes run
+    two tets in those phases."""
+    for cells in [(12, 2, 30), (4, 4, 70)]:
+        xyz, tets = xf.tet_block(*cells, (0.05, 0.05, 0.05), (0.0, 0.0, 0.0))
+        wl = workloads.beam_neo_12x4x4(energy=4)
+        wl = dict(wl, xyz=xyz, tets=tets, pinned=(xyz.reshape(-1, 3)[:, 0] == 0).astype(np.uint8),
+                  velocity=None, tet_colour=None)
+        sb = xf.mesh_sweep_blocks(xyz, tets)
+        big = np.bincount(sb["block"].astype(np.int64) * 64 + sb["phase"]).max()
+        x, v, order, _, _, _, _ = run(ctx, wl, frames=1, block_size=64)
+        ob = oracle(wl, order, wl["substeps"])
+        print(f"\ncells {cells}: largest phase {big} tets")
+        assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+        if cells[2] == 70:
+            assert big > 64
+
+
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_blocks_equal_per_colour_launches(ctx, use_graph):
     """The same sweep-block colouring run with one launch per colour."""

@@ -456,6 +456,27 @@ bool BuildSweepBlocks(const float* xyz, uint32_t n_nodes, const uint32_t* tets, 
             for (int i = 0; i < 4; i++) mask[is[i]] |= 1ull << c;
             np = std::max(np, c + 1);
         }
+        // Balance: tets of phases above ceil(n / np), in visit order, move to
+        // the first phase below that size that none of their nodes touches
+        // (a phase is node-disjoint, so the tet is the only one of its phase
+        // on each of its nodes).
+        uint32_t n = bstart[b + 1] - bstart[b], cap = (n + np - 1) / np;
+        uint32_t size[kMaxBlockPhases] = {};
+        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) size[r.phase[bl[k]]]++;
+        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) {
+            uint32_t t = bl[k], c = r.phase[t];
+            if (size[c] <= cap) continue;
+            const uint32_t* is = tets + 4 * (size_t)t;
+            uint64_t used = mask[is[0]] | mask[is[1]] | mask[is[2]] | mask[is[3]];
+            for (uint32_t d = 0; d < np; d++) {
+                if (size[d] >= cap || (used >> d & 1ull)) continue;
+                for (int i = 0; i < 4; i++) mask[is[i]] = (mask[is[i]] & ~(1ull << c)) | (1ull << d);
+                size[c]--;
+                size[d]++;
+                r.phase[t] = d;
+                break;
+            }
+        }
         for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) {
             const uint32_t* is = tets + 4 * (size_t)bl[k];
             for (int i = 0; i < 4; i++) mask[is[i]] = 0;

@@ -57,8 +57,10 @@ uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
 // tet with minimum levels (k1, k2) touches levels k1..k1+1 and k2..k2+1; its
 // block is (k1, k2) and its launch colour (k1 & 1) | (k2 & 1) << 1. Two blocks
 // of one launch colour differ by at least 2 in k1 or in k2 and share no node.
-// Blocks are numbered by (launch colour, k1, k2); phases take tets in visit
-// order. Returns false (with a message) if a block needs more than 64 phases.
+// Blocks are numbered by (launch colour, k1, k2); phases take tets greedily
+// in visit order, then tets of phases above ceil(tets / phases) move, in
+// visit order, to the first smaller phase none of their nodes touches.
+// Returns false (with a message) if a block needs more than 64 phases.
 constexpr uint32_t kMaxBlockPhases = 64;
 struct SweepBlocks {
     uint32_t n_blocks = 0;

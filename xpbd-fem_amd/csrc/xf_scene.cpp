@@ -238,7 +238,7 @@ struct xf_scene {
         uint32_t nslot = 0, nph = 0, rec_off = 0, node_off = 0;
     };
     BlockLaunch blk[4];
-    uint32_t n_blocks = 0, blk_nmax = 0, blk_threads = 64;
+    uint32_t n_blocks = 0, blk_nmax = 0, blk_nodes = 0;
     uint32_t* blk_rec = nullptr;
     uint32_t* blk_node = nullptr;
     uint2* tloc = nullptr;
@@ -666,7 +666,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                 XF_HIP(ctx, hipEventCreate(&e1));
                 XF_HIP(ctx, hipEventRecord(e0, st));
             }
-            XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, s->blk_threads, L.nslot, L.nph,
+            XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, block, L.nslot, L.nph,
                                          s->blk_nmax, s->blk_rec + L.rec_off, s->blk_node + L.node_off, s->tloc,
                                          s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O,
                                          s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr));
@@ -1101,12 +1101,31 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     std::vector<uint2> h_tloc;
     std::vector<uint32_t> h_brec, h_bnode;
     if (s->opts.sweep_blocks) {
+        // slot = workgroup of a launch. Blocks of a launch colour in (body,
+        // block number) order are cut into eighths and dealt round-robin, so
+        // that under round-robin dispatch over the 8 XCDs each XCD sweeps one
+        // contiguous run of blocks (neighbouring columns share node lines).
         uint32_t nslot[4] = {0, 0, 0, 0};
-        std::vector<std::vector<uint32_t>> slot(s->bodies.size());
+        std::vector<std::vector<uint32_t>> lay(s->bodies.size()), slot(s->bodies.size());
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            const xf::SweepBlocks& sb = sweep[bi];
+            lay[bi].resize(sb.n_blocks);
+            for (uint32_t k = 0; k < sb.n_blocks; k++) lay[bi][k] = nslot[sb.launch[k]]++;
+        }
+        std::vector<uint32_t> deal[4];  // per launch colour: position in (body, block) order -> slot
+        for (int q = 0; q < 4; q++) {
+            uint32_t n = nslot[q], next = 0;
+            deal[q].assign(n, 0);
+            for (uint32_t m = 0; next < n; m++)
+                for (uint32_t r = 0; r < 8; r++) {
+                    uint32_t lo = (uint32_t)(((uint64_t)n * r) / 8), hi = (uint32_t)(((uint64_t)n * (r + 1)) / 8);
+                    if (lo + m < hi) deal[q][lo + m] = next++;
+                }
+        }
         for (size_t bi = 0; bi < s->bodies.size(); bi++) {
             const xf::SweepBlocks& sb = sweep[bi];
             slot[bi].resize(sb.n_blocks);
-            for (uint32_t k = 0; k < sb.n_blocks; k++) slot[bi][k] = nslot[sb.launch[k]]++;
+            for (uint32_t k = 0; k < sb.n_blocks; k++) slot[bi][k] = deal[sb.launch[k]][lay[bi][k]];
         }
         uint32_t slot_base[5] = {0, 0, 0, 0, 0};
         for (int q = 0; q < 4; q++) slot_base[q + 1] = slot_base[q] + nslot[q];
@@ -1119,15 +1138,16 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
             for (uint32_t t = 0; t < b.t; t++) {
                 uint32_t k = sb.block[t], g = slot_base[sb.launch[k]] + slot[bi][k];
                 for (int i = 0; i < 4; i++) bnodes[g].push_back(b.node_base + b.tets[4 * (size_t)t + i]);
-                sp[b.sched_pos[t]] = slot[bi][k];
+                sp[b.sched_pos[t]] = lay[bi][k];
             }
         }
-        s->blk_nmax = 0;
+        s->blk_nmax = s->blk_nodes = 0;
         for (uint32_t g = 0; g < s->n_blocks; g++) {
             std::vector<uint32_t>& v = bnodes[g];
             std::sort(v.begin(), v.end());
             v.erase(std::unique(v.begin(), v.end()), v.end());
             s->blk_nmax = std::max(s->blk_nmax, (uint32_t)v.size());
+            s->blk_nodes += (uint32_t)v.size();
         }
         if (xf_block_lds_bytes(s->blk_nmax) > 65536)
             return Fail(ctx, XF_E_UNSUPPORTED, "a sweep block has %u nodes (LDS holds 2048)", s->blk_nmax);
@@ -1148,7 +1168,6 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         h_bnode.assign((size_t)s->n_blocks * s->blk_nmax, xf::kNoNode);
         for (uint32_t g = 0; g < s->n_blocks; g++)
             std::copy(bnodes[g].begin(), bnodes[g].end(), h_bnode.begin() + (size_t)g * s->blk_nmax);
-        uint32_t pmax = 0;
         for (int q = 0; q < 4; q++) {
             xf_scene::BlockLaunch& L = s->blk[q];
             L.nslot = nslot[q];
@@ -1157,18 +1176,17 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
             L.node_off = slot_base[q] * s->blk_nmax;
             h_brec.resize(h_brec.size() + 2 * (size_t)L.nslot * L.nph, 0u);
             for (uint32_t j = 0; j < L.nph; j++) {
+                // colour (q, j) holds the blocks' tets in (body, block) order
                 uint32_t c = blk_base[q] + j, p = s->colour_begin[c], end = s->colour_begin[c + 1];
-                for (uint32_t k = 0; k < L.nslot; k++) {
-                    uint32_t p0 = p;
-                    while (p < end && sp[p] == k) p++;
+                for (uint32_t m = 0; m < L.nslot; m++) {
+                    uint32_t p0 = p, k = deal[q][m];
+                    while (p < end && sp[p] == m) p++;
                     h_brec[L.rec_off + 2 * ((size_t)k * L.nph + j)] = p0;
                     h_brec[L.rec_off + 2 * ((size_t)k * L.nph + j) + 1] = p - p0;
-                    pmax = std::max(pmax, p - p0);
                 }
                 if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
             }
         }
-        s->blk_threads = std::min(256u, 64u * std::max(1u, (pmax + 63) / 64));
     }
 
     // Nodes.
@@ -1263,6 +1281,7 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     info->n_blocks = s->n_blocks;
     info->block_launches = 0;
     for (int q = 0; q < 4; q++) info->block_launches += s->blk[q].nslot ? 1u : 0u;
+    info->n_block_nodes = s->blk_nodes;
     return XF_OK;
 }
 
@@ -1560,11 +1579,10 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     };
     const Cand candidates[] = {{XF_LAUNCH_PER_COLOUR, 64},  {XF_LAUNCH_PER_COLOUR, 128}, {XF_LAUNCH_PER_COLOUR, 256},
                                {XF_LAUNCH_PERSISTENT, 64},  {XF_LAUNCH_PERSISTENT, 128}, {XF_LAUNCH_PERSISTENT, 256},
-                               {XF_LAUNCH_PERSISTENT, 512}, {XF_LAUNCH_BLOCKS, 0}};
+                               {XF_LAUNCH_PERSISTENT, 512}, {XF_LAUNCH_BLOCKS, 64},  {XF_LAUNCH_BLOCKS, 128},
+                               {XF_LAUNCH_BLOCKS, 256}};
     xf_scene_opts keep = s->opts;
-    Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks
-                                     : keep.launch_mode == XF_LAUNCH_BLOCKS   ? 0u
-                                                                              : keep.block_size};
+    Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size};
     int maxb = 0;  // ring, Pixar and Cube energies run per colour only
     if (energy != kRingMode && energy != 20 && energy != 22 && energy != 30)
         XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
@@ -1581,7 +1599,7 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
         if (cand.mode == XF_LAUNCH_BLOCKS && (!s->opts.sweep_blocks || energy == kRingMode)) continue;
         s->opts.launch_mode = cand.mode;
         if (cand.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = cand.size;
-        else if (cand.mode == XF_LAUNCH_PER_COLOUR) s->opts.block_size = cand.size;
+        else s->opts.block_size = cand.size;
         DropGraph(s);
         if ((he = copy_all(false)) != hipSuccess) break;
         if ((rc = xf_scene_step(s, substeps, dt))) break;  // warm-up and graph capture
@@ -1607,7 +1625,7 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     s->opts = keep;
     s->opts.launch_mode = best_c.mode;
     if (best_c.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = best_c.size;
-    else if (best_c.mode == XF_LAUNCH_PER_COLOUR) s->opts.block_size = best_c.size;
+    else s->opts.block_size = best_c.size;
     DropGraph(s);
     if (rc) return rc;
     if (he != hipSuccess) return Fail(ctx, XF_E_HIP, "autotune: %s", hipGetErrorString(he));

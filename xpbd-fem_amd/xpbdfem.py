@@ -80,7 +80,8 @@ class SceneOpts(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("n_bodies", C.c_uint32), ("n_nodes", C.c_uint32), ("n_tets", C.c_uint32),
                 ("n_colours", C.c_uint32), ("device_bytes", C.c_uint64), ("n_rings", C.c_uint32),
-                ("n_ring_colours", C.c_uint32), ("n_blocks", C.c_uint32), ("block_launches", C.c_uint32)]
+                ("n_ring_colours", C.c_uint32), ("n_blocks", C.c_uint32), ("block_launches", C.c_uint32),
+                ("n_block_nodes", C.c_uint32)]
 
 
 _lib = None
@@ -442,7 +443,7 @@ class Scene:
         self.ctx.check(lib().xf_scene_info_get(self.h, C.byref(i)), "xf_scene_info_get")
         return dict(n_bodies=i.n_bodies, n_nodes=i.n_nodes, n_tets=i.n_tets, n_colours=i.n_colours,
                     device_bytes=i.device_bytes, n_rings=i.n_rings, n_ring_colours=i.n_ring_colours,
-                    n_blocks=i.n_blocks, block_launches=i.block_launches)
+                    n_blocks=i.n_blocks, block_launches=i.block_launches, n_block_nodes=i.n_block_nodes)
 
     def set_settings(self, body, settings=None, **kw):
         s = settings if settings is not None else default_settings(**kw)
@@ -521,13 +522,13 @@ class Scene:
 
     def autotune(self, substeps, dt, reps=3):
         """Picks the fastest launch shape; returns (launch_mode, size, ms per call),
-        size = workgroup size (per-colour), grid blocks (persistent) or 0
-        (sweep blocks)."""
+        size = workgroup size (per-colour, sweep blocks) or grid blocks
+        (persistent)."""
         o = SceneOpts()
         ms = C.c_float()
         self.ctx.check(lib().xf_scene_autotune(self.h, substeps, dt, reps, C.byref(o), C.byref(ms)),
                        "xf_scene_autotune")
-        size = {LAUNCH_PERSISTENT: o.persistent_blocks, LAUNCH_BLOCKS: 0}.get(o.launch_mode, o.block_size)
+        size = o.persistent_blocks if o.launch_mode == LAUNCH_PERSISTENT else o.block_size
         return o.launch_mode, size, ms.value
 
     def adjoint(self, substeps, dt, loss_at, targets, window=20, per_body=False):
