@@ -197,10 +197,14 @@ typedef struct xf_scene_opts {
     uint32_t ring_lanes;        /* lanes per vertex ring in ring-energy launches: 16, 32, or 0 = 32
                                    when some ring has more than 16 tets, else 16 */
     uint32_t sweep_blocks;      /* 1: colour the tets by sweep blocks. Node levels are breadth-first
-                                   distances over tet edges from the nodes of least x and of least
-                                   y; a tet's block is its pair of minimum levels and its launch
-                                   colour the pair's parities, so blocks of one launch colour share
-                                   no node. Inside a block, tets take greedy phases in visit order.
+                                   distances over tet edges from the nodes of least d . p, for a
+                                   pair of directions d (x with y, y - z, y + z, z; y with z,
+                                   x -+ z; z with x -+ y: the pair whose blocks put the fewest tets
+                                   on one node); a tet's block is its pair of minimum levels and its
+                                   launch colour the pair's parities, so blocks of one launch
+                                   colour share no node. Inside a block, tets take DSatur phases
+                                   (ties in visit order). Uniform blocks of Freudenthal cells get
+                                   6 phases per block, 24 per substep.
                                    Colour (q, j) holds phase j of every block of launch colour q;
                                    the order is colour-major, then body, block, visit order (a
                                    valid serial order; xf_scene_get_order reports it). Needed by

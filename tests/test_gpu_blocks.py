@@ -49,7 +49,7 @@ def check_colouring(wl, order, colour):
 def test_blocks_bit_exact(ctx, energy, sim):
     wl = workloads.beam_neo_12x4x4(energy=energy, simultaneous=sim)
     x, v, order, colour, info, _, _ = run(ctx, wl)
-    assert info["n_blocks"] == 48 and info["block_launches"] == 4 and info["n_colours"] == 32
+    assert info["n_blocks"] == 96 and info["block_launches"] == 4 and info["n_colours"] == 24
     check_colouring(wl, order, colour)
     ob = oracle(wl, order, 2 * wl["substeps"])
     print(f"\nenergy {energy} sim {sim}: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
@@ -68,11 +68,11 @@ def test_blocks_workgroup_sizes(ctx):
 
 
 def test_blocks_more_tets_than_lanes(ctx):
-    """Columns of 180 tets (12x2x30 cells) and of 420 tets whose largest
-    phases hold 70 tets (4x4x70 cells), on 64-thread workgroups: lanes run
-    two tets in those phases."""
-    for cells in [(12, 2, 30), (4, 4, 70)]:
-        xyz, tets = xf.tet_block(*cells, (0.05, 0.05, 0.05), (0.0, 0.0, 0.0))
+    """A mirrored 4x4x150 block: no direction pair beats (x, y), whose
+    columns of 900 tets have 12+ phases of up to 75 tets; on 64-thread
+    workgroups lanes run two tets in those phases."""
+    for cells in [(4, 4, 150)]:
+        xyz, tets = xf.tet_block(*cells, (0.05, 0.05, 0.05), (0.0, 0.0, 0.0), mirrored=True)
         wl = workloads.beam_neo_12x4x4(energy=4)
         wl = dict(wl, xyz=xyz, tets=tets, pinned=(xyz.reshape(-1, 3)[:, 0] == 0).astype(np.uint8),
                   velocity=None, tet_colour=None)
@@ -82,8 +82,7 @@ def test_blocks_more_tets_than_lanes(ctx):
         ob = oracle(wl, order, wl["substeps"])
         print(f"\ncells {cells}: largest phase {big} tets")
         assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
-        if cells[2] == 70:
-            assert big > 64
+        assert big > 64
 
 
 @pytest.mark.parametrize("use_graph", [True, False])
@@ -98,11 +97,12 @@ def test_blocks_equal_per_colour_launches(ctx, use_graph):
 
 
 def test_blocks_big_beam_bit_exact(ctx):
-    """The 96x24x24 Yeoh beam (331,776 tets; 2,304 column blocks of 144 tets,
-    8 phases, 4 launches per substep) with ground and moving sphere."""
+    """The 96x24x24 Yeoh beam (331,776 tets; 4,608 blocks along x and y - z,
+    6 phases each, 4 launches per substep: 24 phases) with ground and moving
+    sphere."""
     wl = workloads.beam_yeoh_96x24x24()
     x, v, order, colour, info, _, _ = run(ctx, wl, frames=1)
-    assert info["n_blocks"] == 96 * 24 and info["n_colours"] == 32
+    assert info["n_blocks"] == 96 * 48 and info["n_colours"] == 24
     ob = oracle(wl, order, wl["substeps"])
     print(f"\nbig beam: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
@@ -207,8 +207,12 @@ def test_blocks_rejections(ctx):
     with pytest.raises(xf.XfError):
         sc.finalize(sweep_blocks=True)
     sc.close()
-    # a column of 600 cells: 2,404 nodes in one block (LDS holds 2,048)
-    xyz, tets = xf.tet_block(1, 1, 600, (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))
+    # 700 separate tets: every component's levels restart at 0, so all fall
+    # in one block of 2,800 nodes (LDS holds 2,048)
+    one, t4 = xf.tet_block(1, 1, 1, (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))
+    p = one.reshape(-1, 3)[t4[:4]]
+    xyz = np.concatenate([p + np.float32([0.3 * k, 0.0, 0.0]) for k in range(700)]).ravel().astype(np.float32)
+    tets = np.arange(4 * 700, dtype=np.uint32)
     sc = xf.Scene(ctx)
     sc.add_body(xyz, tets, 1000.0)
     with pytest.raises(xf.XfError, match="LDS"):

@@ -337,21 +337,28 @@ uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
 
 namespace {
 
-// Breadth-first levels over tet edges (nodes of one tet are adjacent) from
-// the used nodes of least coordinate `axis`; unreached components restart
-// at level 0 from their node of least coordinate (ties by index). Unused
-// nodes keep level -1.
-void AxisLevels(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets,
-                const std::vector<uint32_t>& first, const std::vector<uint32_t>& inc, int axis,
-                std::vector<int32_t>* level) {
+// Breadth-first levels over tet edges (the nodes of one tet are adjacent)
+// from the used nodes whose coordinate v = d . p lies within 1e-4 of its
+// range above the least; components the search did not reach restart at
+// level 0 from their node of least v (ties by index). Unused nodes keep
+// level -1.
+void DirectionLevels(const float* xyz, uint32_t n_nodes, const uint32_t* tets, const std::vector<uint32_t>& first,
+                     const std::vector<uint32_t>& inc, const int d[3], std::vector<int32_t>* level) {
     std::vector<int32_t>& lev = *level;
     lev.assign(n_nodes, -1);
-    std::vector<uint32_t> by_coord;
-    by_coord.reserve(n_nodes);
-    for (uint32_t v = 0; v < n_nodes; v++)
-        if (first[v + 1] > first[v]) by_coord.push_back(v);
-    std::stable_sort(by_coord.begin(), by_coord.end(),
-                     [&](uint32_t a, uint32_t b) { return xyz[3 * (size_t)a + axis] < xyz[3 * (size_t)b + axis]; });
+    std::vector<float> v(n_nodes, 0.0f);
+    float lo = 0.0f, hi = 0.0f;
+    bool any = false;
+    for (uint32_t n = 0; n < n_nodes; n++) {
+        if (first[n + 1] == first[n]) continue;
+        const float* p = xyz + 3 * (size_t)n;
+        v[n] = ((float)d[0] * p[0] + (float)d[1] * p[1]) + (float)d[2] * p[2];
+        if (!any || v[n] < lo) lo = v[n];
+        if (!any || v[n] > hi) hi = v[n];
+        any = true;
+    }
+    if (!any) return;
+    float tol = lo + 1e-4f * (hi - lo);
     std::vector<uint32_t> queue;
     queue.reserve(n_nodes);
     auto run = [&](size_t head) {
@@ -367,22 +374,35 @@ void AxisLevels(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32
             }
         }
     };
-    if (by_coord.empty()) return;
-    float lo = xyz[3 * (size_t)by_coord[0] + axis];
-    for (uint32_t v : by_coord) {
-        if (xyz[3 * (size_t)v + axis] != lo) break;
-        lev[v] = 0;
-        queue.push_back(v);
-    }
+    for (uint32_t n = 0; n < n_nodes; n++)
+        if (first[n + 1] > first[n] && v[n] <= tol) {
+            lev[n] = 0;
+            queue.push_back(n);
+        }
     run(0);
-    for (uint32_t v : by_coord)
-        if (lev[v] < 0) {
+    std::vector<uint32_t> rest;
+    for (uint32_t n = 0; n < n_nodes; n++)
+        if (first[n + 1] > first[n] && lev[n] < 0) rest.push_back(n);
+    std::stable_sort(rest.begin(), rest.end(), [&](uint32_t a, uint32_t b) { return v[a] < v[b]; });
+    for (uint32_t n : rest)
+        if (lev[n] < 0) {
             size_t head = queue.size();
-            lev[v] = 0;
-            queue.push_back(v);
+            lev[n] = 0;
+            queue.push_back(n);
             run(head);
         }
-    (void)n_tets;
+}
+
+// Level directions tried for the block partition: x with y, y - z, y + z,
+// z; y with z, x - z, x + z; z with x - y, x + y.
+constexpr int kDirs[9][3] = {{1, 0, 0}, {0, 1, 0},  {0, 0, 1},  {0, 1, -1}, {0, 1, 1},
+                             {1, 0, -1}, {1, 0, 1}, {1, -1, 0}, {1, 1, 0}};
+constexpr int kPairs[9][2] = {{0, 1}, {0, 3}, {0, 4}, {0, 2}, {1, 2}, {1, 5}, {1, 6}, {2, 7}, {2, 8}};
+
+uint64_t BlockKey(const std::vector<int32_t>& l1, const std::vector<int32_t>& l2, const uint32_t* is) {
+    int32_t k1 = std::min(std::min(l1[is[0]], l1[is[1]]), std::min(l1[is[2]], l1[is[3]]));
+    int32_t k2 = std::min(std::min(l2[is[0]], l2[is[1]]), std::min(l2[is[2]], l2[is[3]]));
+    return ((uint64_t)(uint32_t)k1 << 32) | (uint32_t)k2;
 }
 
 }  // namespace
@@ -404,32 +424,84 @@ bool BuildSweepBlocks(const float* xyz, uint32_t n_nodes, const uint32_t* tets, 
         for (uint32_t t = 0; t < n_tets; t++)
             for (int j = 0; j < 4; j++) inc[fill[tets[4 * (size_t)t + j]]++] = t;
     }
-    std::vector<int32_t> l1, l2;
-    AxisLevels(xyz, n_nodes, tets, n_tets, first, inc, 0, &l1);
-    AxisLevels(xyz, n_nodes, tets, n_tets, first, inc, 1, &l2);
-    // block key per tet: minimum levels
+    // Levels of each direction (computed when a pair needs them); the pair
+    // whose blocks put the fewest tets on one node (a lower bound on a
+    // block's phases) wins, the first on ties. A pair reaching a quarter of
+    // the largest node valence (the least any 4-colour block scheme can
+    // reach) ends the search.
+    std::vector<int32_t> lv[9];
+    bool have[9] = {};
+    uint32_t valence = 0;
+    for (uint32_t v = 0; v < n_nodes; v++) valence = std::max(valence, first[v + 1] - first[v]);
+    const uint32_t floor_bound = (valence + 3) / 4;
     std::vector<uint64_t> key(n_tets);
-    for (uint32_t t = 0; t < n_tets; t++) {
-        const uint32_t* is = tets + 4 * (size_t)t;
-        int32_t k1 = std::min(std::min(l1[is[0]], l1[is[1]]), std::min(l1[is[2]], l1[is[3]]));
-        int32_t k2 = std::min(std::min(l2[is[0]], l2[is[1]]), std::min(l2[is[2]], l2[is[3]]));
-        key[t] = ((uint64_t)(uint32_t)k1 << 32) | (uint32_t)k2;
+    int best = 0;
+    uint32_t best_bound = ~0u;
+    for (int c = 0; c < 9 && best_bound > floor_bound; c++) {
+        for (int a = 0; a < 2; a++)
+            if (!have[kPairs[c][a]]) {
+                DirectionLevels(xyz, n_nodes, tets, first, inc, kDirs[kPairs[c][a]], &lv[kPairs[c][a]]);
+                have[kPairs[c][a]] = true;
+            }
+        const std::vector<int32_t>&l1 = lv[kPairs[c][0]], &l2 = lv[kPairs[c][1]];
+        for (uint32_t t = 0; t < n_tets; t++) key[t] = BlockKey(l1, l2, tets + 4 * (size_t)t);
+        uint32_t bound = 0;
+        std::vector<uint64_t> ks;
+        for (uint32_t n = 0; n < n_nodes && bound < best_bound; n++) {
+            ks.clear();
+            for (uint32_t k = first[n]; k < first[n + 1]; k++) ks.push_back(key[inc[k]]);
+            std::sort(ks.begin(), ks.end());
+            for (size_t i = 0, j; i < ks.size(); i = j) {
+                for (j = i; j < ks.size() && ks[j] == ks[i]; j++) {
+                }
+                bound = std::max(bound, (uint32_t)(j - i));
+            }
+        }
+        if (bound < best_bound) {
+            best_bound = bound;
+            best = c;
+        }
     }
+    const std::vector<int32_t>&l1 = lv[kPairs[best][0]], &l2 = lv[kPairs[best][1]];
+    for (uint32_t t = 0; t < n_tets; t++) key[t] = BlockKey(l1, l2, tets + 4 * (size_t)t);
+
     auto launch_of = [](uint64_t k) -> uint32_t { return (uint32_t)((k >> 32) & 1u) | (uint32_t)((k & 1u) << 1); };
     auto before = [&](uint64_t a, uint64_t b) {
         uint32_t qa = launch_of(a), qb = launch_of(b);
         return qa != qb ? qa < qb : a < b;
     };
-    std::vector<uint64_t> keys(key);
-    std::sort(keys.begin(), keys.end(), before);
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     SweepBlocks& r = *out;
+    r.axes[0] = kPairs[best][0];
+    r.axes[1] = kPairs[best][1];
+    r.block.assign(n_tets, 0);
+    std::vector<uint64_t> keys;
+    uint32_t m1 = 0, m2 = 0;
+    for (uint32_t t = 0; t < n_tets; t++) {
+        m1 = std::max(m1, (uint32_t)(key[t] >> 32) + 1);
+        m2 = std::max(m2, (uint32_t)key[t] + 1);
+    }
+    if ((uint64_t)m1 * m2 <= 4 * (uint64_t)n_tets + 1024) {
+        // dense (k1, k2) grid: blocks numbered by launch colour, then k1, k2
+        std::vector<uint32_t> id((size_t)m1 * m2, ~0u);
+        for (uint32_t t = 0; t < n_tets; t++) id[(size_t)(key[t] >> 32) * m2 + (uint32_t)key[t]] = 0;
+        for (uint32_t q = 0; q < 4; q++)
+            for (uint32_t k1 = q & 1u; k1 < m1; k1 += 2)
+                for (uint32_t k2 = q >> 1; k2 < m2; k2 += 2)
+                    if (id[(size_t)k1 * m2 + k2] == 0) {
+                        id[(size_t)k1 * m2 + k2] = (uint32_t)keys.size() + 1;
+                        keys.push_back(((uint64_t)k1 << 32) | k2);
+                    }
+        for (uint32_t t = 0; t < n_tets; t++) r.block[t] = id[(size_t)(key[t] >> 32) * m2 + (uint32_t)key[t]] - 1;
+    } else {
+        keys = key;
+        std::sort(keys.begin(), keys.end(), before);
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        for (uint32_t t = 0; t < n_tets; t++)
+            r.block[t] = (uint32_t)(std::lower_bound(keys.begin(), keys.end(), key[t], before) - keys.begin());
+    }
     r.n_blocks = (uint32_t)keys.size();
     r.launch.assign(r.n_blocks, 0);
     for (uint32_t b = 0; b < r.n_blocks; b++) r.launch[b] = (uint8_t)launch_of(keys[b]);
-    r.block.assign(n_tets, 0);
-    for (uint32_t t = 0; t < n_tets; t++)
-        r.block[t] = (uint32_t)(std::lower_bound(keys.begin(), keys.end(), key[t], before) - keys.begin());
     // tets of each block in visit order
     std::vector<uint32_t> bstart(r.n_blocks + 1, 0), bl(n_tets);
     for (uint32_t t = 0; t < n_tets; t++) bstart[r.block[t] + 1]++;
@@ -438,48 +510,117 @@ bool BuildSweepBlocks(const float* xyz, uint32_t n_nodes, const uint32_t* tets, 
         std::vector<uint32_t> fill(bstart.begin(), bstart.end() - 1);
         for (uint32_t k = 0; k < n_tets; k++) bl[fill[r.block[visit[k]]]++] = visit[k];
     }
-    // greedy phases inside each block (node masks are cleared after each block)
+    // Phases inside each block: DSatur (the uncoloured tet with the most
+    // distinct phases among its neighbours -- tets of the block sharing a
+    // node --, then the most neighbours, then the earliest in visit order
+    // takes the lowest phase none of its neighbours has).
     std::vector<uint64_t> mask(n_nodes, 0);
     r.phase.assign(n_tets, 0);
     r.phases.assign(r.n_blocks, 0);
+    constexpr uint32_t kNoLocal = ~0u;
+    std::vector<uint32_t> slot_of(n_nodes, kNoLocal), nodes_seen, goff, gfill, gtet, noff, nbr, stamp, at;
+    std::vector<uint64_t> sat, keyv;
     for (uint32_t b = 0; b < r.n_blocks; b++) {
+        const uint32_t* L = bl.data() + bstart[b];
+        uint32_t n = bstart[b + 1] - bstart[b];
+        // neighbour lists (distinct): the block's tets grouped by node
+        // (counting sort over block-local node slots), deduplicated by stamp
+        nodes_seen.clear();
+        for (uint32_t i = 0; i < n; i++)
+            for (int c = 0; c < 4; c++) {
+                uint32_t g = tets[4 * (size_t)L[i] + c];
+                if (slot_of[g] == kNoLocal) {
+                    slot_of[g] = (uint32_t)nodes_seen.size();
+                    nodes_seen.push_back(g);
+                }
+            }
+        uint32_t nn = (uint32_t)nodes_seen.size();
+        goff.assign(nn + 1, 0);
+        for (uint32_t i = 0; i < n; i++)
+            for (int c = 0; c < 4; c++) goff[slot_of[tets[4 * (size_t)L[i] + c]] + 1]++;
+        for (uint32_t k = 0; k < nn; k++) goff[k + 1] += goff[k];
+        gtet.resize(4 * (size_t)n);
+        gfill.assign(goff.begin(), goff.end() - 1);
+        for (uint32_t i = 0; i < n; i++)
+            for (int c = 0; c < 4; c++) gtet[gfill[slot_of[tets[4 * (size_t)L[i] + c]]]++] = i;
+        noff.assign(n + 1, 0);
+        nbr.clear();
+        stamp.assign(n, ~0u);
+        for (uint32_t i = 0; i < n; i++) {
+            for (int c = 0; c < 4; c++) {
+                uint32_t sl = slot_of[tets[4 * (size_t)L[i] + c]];
+                for (uint32_t k = goff[sl]; k < goff[sl + 1]; k++) {
+                    uint32_t j = gtet[k];
+                    if (j != i && stamp[j] != i) {
+                        stamp[j] = i;
+                        nbr.push_back(j);
+                    }
+                }
+            }
+            noff[i + 1] = (uint32_t)nbr.size();
+        }
+        for (uint32_t g : nodes_seen) slot_of[g] = kNoLocal;
+        sat.assign(n, 0);
+        // selection key: saturation << 48 | degree << 32 | ~position, kept
+        // for the uncoloured tets only (at[] locates each tet's key)
+        keyv.resize(n);
+        at.resize(n);
+        for (uint32_t i = 0; i < n; i++) {
+            keyv[i] = ((uint64_t)(noff[i + 1] - noff[i]) << 32) | (uint64_t)(0xFFFFFFFFu - i);
+            at[i] = i;
+        }
         uint32_t np = 0;
-        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) {
-            const uint32_t* is = tets + 4 * (size_t)bl[k];
-            uint64_t used = mask[is[0]] | mask[is[1]] | mask[is[2]] | mask[is[3]];
-            if (used == ~0ull) {
+        for (uint32_t live = n; live > 0; live--) {
+            uint32_t top = 0;
+            for (uint32_t k = 1; k < live; k++)
+                if (keyv[k] > keyv[top]) top = k;
+            uint32_t i = 0xFFFFFFFFu - (uint32_t)keyv[top];
+            if (sat[i] == ~0ull) {
                 if (err) *err = "a sweep block needs more than 64 phases";
                 return false;
             }
-            uint32_t c = (uint32_t)__builtin_ctzll(~used);
-            r.phase[bl[k]] = c;
-            for (int i = 0; i < 4; i++) mask[is[i]] |= 1ull << c;
+            uint32_t c = (uint32_t)__builtin_ctzll(~sat[i]);
+            r.phase[L[i]] = c;
             np = std::max(np, c + 1);
+            // remove i's key (the last live key takes its place)
+            keyv[top] = keyv[live - 1];
+            at[0xFFFFFFFFu - (uint32_t)keyv[top]] = top;
+            at[i] = ~0u;
+            for (uint32_t k = noff[i]; k < noff[i + 1]; k++) {
+                uint32_t j = nbr[k];
+                if (at[j] == ~0u || (sat[j] >> c & 1ull)) continue;
+                sat[j] |= 1ull << c;
+                keyv[at[j]] += 1ull << 48;
+            }
         }
         // Balance: tets of phases above ceil(n / np), in visit order, move to
         // the first phase below that size that none of their nodes touches
         // (a phase is node-disjoint, so the tet is the only one of its phase
         // on each of its nodes).
-        uint32_t n = bstart[b + 1] - bstart[b], cap = (n + np - 1) / np;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t* is = tets + 4 * (size_t)L[i];
+            for (int c = 0; c < 4; c++) mask[is[c]] |= 1ull << r.phase[L[i]];
+        }
+        uint32_t cap = (n + np - 1) / np;
         uint32_t size[kMaxBlockPhases] = {};
-        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) size[r.phase[bl[k]]]++;
-        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) {
-            uint32_t t = bl[k], c = r.phase[t];
+        for (uint32_t i = 0; i < n; i++) size[r.phase[L[i]]]++;
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t t = L[i], c = r.phase[t];
             if (size[c] <= cap) continue;
             const uint32_t* is = tets + 4 * (size_t)t;
             uint64_t used = mask[is[0]] | mask[is[1]] | mask[is[2]] | mask[is[3]];
             for (uint32_t d = 0; d < np; d++) {
                 if (size[d] >= cap || (used >> d & 1ull)) continue;
-                for (int i = 0; i < 4; i++) mask[is[i]] = (mask[is[i]] & ~(1ull << c)) | (1ull << d);
+                for (int k = 0; k < 4; k++) mask[is[k]] = (mask[is[k]] & ~(1ull << c)) | (1ull << d);
                 size[c]--;
                 size[d]++;
                 r.phase[t] = d;
                 break;
             }
         }
-        for (uint32_t k = bstart[b]; k < bstart[b + 1]; k++) {
-            const uint32_t* is = tets + 4 * (size_t)bl[k];
-            for (int i = 0; i < 4; i++) mask[is[i]] = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t* is = tets + 4 * (size_t)L[i];
+            for (int c = 0; c < 4; c++) mask[is[c]] = 0;
         }
         r.phases[b] = np;
     }

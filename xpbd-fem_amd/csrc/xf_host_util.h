@@ -49,21 +49,27 @@ uint32_t ColourSweepOrder(uint32_t n_nodes, const uint32_t* tets, uint32_t n_tet
                           std::vector<uint32_t>* colour, std::vector<uint32_t>* order);
 
 // Sweep blocks: a partition of the tets into blocks whose launch colours
-// (0..3) make blocks of one launch colour node-disjoint, and a greedy
-// node-disjoint colouring (phases) inside each block. Node levels l1, l2 are
-// breadth-first distances over tet edges from the nodes of least x (l1) and
-// least y (l2); a component the search did not reach restarts from its node
-// of least coordinate. The nodes of one tet differ by at most one level, so a
-// tet with minimum levels (k1, k2) touches levels k1..k1+1 and k2..k2+1; its
-// block is (k1, k2) and its launch colour (k1 & 1) | (k2 & 1) << 1. Two blocks
-// of one launch colour differ by at least 2 in k1 or in k2 and share no node.
-// Blocks are numbered by (launch colour, k1, k2); phases take tets greedily
-// in visit order, then tets of phases above ceil(tets / phases) move, in
-// visit order, to the first smaller phase none of their nodes touches.
-// Returns false (with a message) if a block needs more than 64 phases.
+// (0..3) make blocks of one launch colour node-disjoint, and a node-disjoint
+// colouring (phases) inside each block. Node levels along a direction d are
+// breadth-first distances over tet edges from the nodes of least d . p
+// (within 1e-4 of its range); a component the search did not reach restarts
+// from its node of least d . p. The nodes of one tet differ by at most one
+// level, so a tet with minimum levels (k1, k2) in two directions touches
+// levels k1..k1+1 and k2..k2+1; its block is (k1, k2) and its launch colour
+// (k1 & 1) | (k2 & 1) << 1, and two blocks of one launch colour differ by at
+// least 2 in k1 or in k2 and share no node. The direction pair (x with y,
+// y - z, y + z, z; y with z, x - z, x + z; z with x - y, x + y) is the first
+// with the fewest tets of one block on one node (the least phases a block
+// can have); a pair reaching a quarter of the largest node valence ends the
+// search. Blocks are numbered by (launch colour, k1, k2). Phases: DSatur
+// inside each block (ties: more neighbours, then earlier in visit order),
+// then tets of phases above ceil(tets / phases) move, in visit order, to the
+// first smaller phase none of their nodes touches. Returns false (with a
+// message) if a block needs more than 64 phases.
 constexpr uint32_t kMaxBlockPhases = 64;
 struct SweepBlocks {
     uint32_t n_blocks = 0;
+    int axes[2] = {0, 1};          // direction pair (indices into x, y, z, y-z, y+z, x-z, x+z, x-y, x+y)
     std::vector<uint32_t> block;   // per tet
     std::vector<uint32_t> phase;   // per tet: colour inside its block
     std::vector<uint8_t> launch;   // per block: launch colour 0..3

@@ -842,9 +842,34 @@ __device__ __forceinline__ BlockTet load_block_tet(uint32_t i, const uint2* __re
     return t;
 }
 
+// One tet of a block phase: nodes from LDS, element data loaded, solved,
+// nodes written back to LDS.
 template <int E, bool SIM, bool AUX, bool DAMP>
+__device__ __forceinline__ void block_tet(const BlockTet& t, uint32_t i, double4* xs, const uint4* __restrict__ tidx,
+                                          const double4* __restrict__ O, const XfBodyParams* __restrict__ bodies,
+                                          float comp1_u, float a_u, float dt) {
+    uint32_t ia = t.loc.x & 0xFFFFu, ib = t.loc.x >> 16, ic = t.loc.y & 0xFFFFu, id = t.loc.y >> 16;
+    double4 na = xs[ia], nb = xs[ib], nc = xs[ic], nd = xs[id];
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    uint4 gid = make_uint4(0u, 0u, 0u, 0u);
+    if constexpr (DAMP) gid = tidx[i];
+    project_loaded<E, SIM, AUX, DAMP>(t.q0, t.q1, t.q2, t.q3, t.ax, gid, x, w, O, bodies, comp1_u, a_u, dt);
+    xs[ia] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
+    xs[ib] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
+    xs[ic] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
+    xs[id] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+}
+
+// ntet: tets in the schedule. Element loads are issued by every lane for
+// every phase pair, at clamped in-range positions (lanes without a tet, and
+// the phase after the last, load data they do not use), so that the loads in
+// flight are the same on every path and the wait before a phase covers
+// exactly that phase's loads. OVER: some block has more tets in a phase than
+// the workgroup has lanes (lanes then load and solve further tets in turn).
+template <int E, bool SIM, bool AUX, bool DAMP, bool OVER>
 __global__ __launch_bounds__(256) void xf_project_blocks(
-    uint32_t nph, uint32_t nmax, const uint32_t* __restrict__ rec, const uint32_t* __restrict__ bnode,
+    uint32_t nph, uint32_t nmax, uint32_t ntet, const uint32_t* __restrict__ rec, const uint32_t* __restrict__ bnode,
     const uint2* __restrict__ tloc, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
     const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
     const float4* __restrict__ taux, double4* __restrict__ X, const double4* __restrict__ O,
@@ -857,36 +882,37 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     if constexpr (E == 30) return;  // CubeNeo / CubeSkin: no constraint on tets (Fem.cpp:567-760)
     extern __shared__ double4 xf_block_nodes[];
     double4* xs = xf_block_nodes;
-    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x, last = ntet - 1u;
     const uint32_t* r = rec + (size_t)blockIdx.x * 2 * nph;
     const uint32_t* bn = bnode + (size_t)blockIdx.x * nmax;
-    BlockTet cur;
-    if (tid < r[1]) cur = load_block_tet<AUX>(r[0] + tid, tloc, tg0, tg1, tg2, tg3, taux);
+    const uint32_t jl = nph - 1u;
+    auto load = [&](uint32_t j) {
+        return load_block_tet<AUX>(min(r[2 * j] + tid, last), tloc, tg0, tg1, tg2, tg3, taux);
+    };
+    // each lane's first tet of the phase is t; with OVER, lanes go on to the
+    // tets beyond the workgroup's lanes
+    auto phase = [&](uint32_t j, const BlockTet& t) {
+        const uint32_t b = r[2 * j], cnt = r[2 * j + 1];
+        if (tid < cnt) block_tet<E, SIM, AUX, DAMP>(t, b + tid, xs, tidx, O, bodies, comp1_u, a_u, dt);
+        if constexpr (OVER)
+            for (uint32_t k = tid + bs; k < cnt; k += bs)
+                block_tet<E, SIM, AUX, DAMP>(load_block_tet<AUX>(b + k, tloc, tg0, tg1, tg2, tg3, taux), b + k, xs,
+                                             tidx, O, bodies, comp1_u, a_u, dt);
+        __syncthreads();
+    };
+    BlockTet ta = load(0), tb;
     for (uint32_t k = tid; k < nmax; k += bs) {
         uint32_t g = bn[k];
         if (g != xf::kNoNode) xs[k] = X[g];
     }
     __syncthreads();
-    for (uint32_t j = 0; j < nph; j++) {
-        const uint32_t b = r[2 * j], cnt = r[2 * j + 1];
-        BlockTet nxt;
-        if (j + 1 < nph && tid < r[2 * j + 3]) nxt = load_block_tet<AUX>(r[2 * j + 2] + tid, tloc, tg0, tg1, tg2, tg3, taux);
-        for (uint32_t k = tid; k < cnt; k += bs) {
-            BlockTet t = k == tid ? cur : load_block_tet<AUX>(b + k, tloc, tg0, tg1, tg2, tg3, taux);
-            uint32_t ia = t.loc.x & 0xFFFFu, ib = t.loc.x >> 16, ic = t.loc.y & 0xFFFFu, id = t.loc.y >> 16;
-            double4 na = xs[ia], nb = xs[ib], nc = xs[ic], nd = xs[id];
-            double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
-            float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
-            uint4 gid = make_uint4(0u, 0u, 0u, 0u);
-            if constexpr (DAMP) gid = tidx[b + k];
-            project_loaded<E, SIM, AUX, DAMP>(t.q0, t.q1, t.q2, t.q3, t.ax, gid, x, w, O, bodies, comp1_u, a_u, dt);
-            xs[ia] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
-            xs[ib] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
-            xs[ic] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
-            xs[id] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
-        }
-        __syncthreads();
-        cur = nxt;
+    // phases in pairs, element data of the next phase loaded before solving
+    // the current one (ta: even phases, tb: odd phases)
+    for (uint32_t j = 0; j < nph; j += 2) {
+        tb = load(min(j + 1u, jl));
+        phase(j, ta);
+        ta = load(min(j + 2u, jl));
+        if (j + 1u < nph) phase(j + 1u, tb);
     }
     for (uint32_t k = tid; k < nmax; k += bs) {
         uint32_t g = bn[k];
@@ -1326,19 +1352,20 @@ extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, boo
     return hipGetLastError();
 }
 
-typedef void (*XfBlocksFn)(uint32_t, uint32_t, const uint32_t*, const uint32_t*, const uint2*, const uint4*,
-                           const float4*, const float4*, const float4*, const float4*, const float4*, double4*,
-                           const double4*, const XfBodyParams*, float, float, float, float*);
+typedef void (*XfBlocksFn)(uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*, const uint2*,
+                           const uint4*, const float4*, const float4*, const float4*, const float4*, const float4*,
+                           double4*, const double4*, const XfBodyParams*, float, float, float, float*);
 
 namespace {
-template <int E, bool SIM>
+template <int E, bool SIM, bool OVER>
 XfBlocksFn pick_blocks_ad(bool aux, bool damp) {
-    if (aux) return damp ? xf_project_blocks<E, SIM, true, true> : xf_project_blocks<E, SIM, true, false>;
-    return damp ? xf_project_blocks<E, SIM, false, true> : xf_project_blocks<E, SIM, false, false>;
+    if (aux) return damp ? xf_project_blocks<E, SIM, true, true, OVER> : xf_project_blocks<E, SIM, true, false, OVER>;
+    return damp ? xf_project_blocks<E, SIM, false, true, OVER> : xf_project_blocks<E, SIM, false, false, OVER>;
 }
 template <int E>
-XfBlocksFn pick_blocks(bool sim, bool aux, bool damp) {
-    return sim ? pick_blocks_ad<E, true>(aux, damp) : pick_blocks_ad<E, false>(aux, damp);
+XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over) {
+    if (over) return sim ? pick_blocks_ad<E, true, true>(aux, damp) : pick_blocks_ad<E, false, true>(aux, damp);
+    return sim ? pick_blocks_ad<E, true, false>(aux, damp) : pick_blocks_ad<E, false, false>(aux, damp);
 }
 }  // namespace
 
@@ -1346,32 +1373,36 @@ XfBlocksFn pick_blocks(bool sim, bool aux, bool damp) {
 extern "C" size_t xf_block_lds_bytes(uint32_t nmax) { return 32 * (size_t)nmax; }
 
 // Host-callable: one sweep-block launch (nslot workgroups of `threads`);
-// rec / bnode point at the launch colour's first block.
+// rec / bnode point at the launch colour's first block; pmax: the most tets
+// of one block in one phase.
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
-                                       uint32_t nslot, uint32_t nph, uint32_t nmax, const uint32_t* rec,
+                                       uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
+                                       const uint32_t* rec,
                                        const uint32_t* bnode, const uint2* tloc, const uint4* tidx,
                                        const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3,
                                        const float4* taux, double4* X, const double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock) {
     XfBlocksFn fn = nullptr;
+    const bool over = pmax > threads;
     switch (energy) {
-        case 0: fn = pick_blocks<0>(sim, aux, damp); break;
-        case 3: fn = pick_blocks<3>(sim, aux, damp); break;
-        case 4: fn = pick_blocks<4>(sim, aux, damp); break;
-        case 5: fn = pick_blocks<5>(sim, aux, damp); break;
-        case 6: fn = pick_blocks<6>(sim, aux, damp); break;
-        case 7: fn = pick_blocks<7>(sim, aux, damp); break;
-        case 20: fn = pick_blocks<20>(sim, aux, damp); break;
-        case 22: fn = pick_blocks<22>(sim, aux, damp); break;
-        case 30: fn = pick_blocks<30>(sim, aux, damp); break;
+        case 0: fn = pick_blocks<0>(sim, aux, damp, over); break;
+        case 3: fn = pick_blocks<3>(sim, aux, damp, over); break;
+        case 4: fn = pick_blocks<4>(sim, aux, damp, over); break;
+        case 5: fn = pick_blocks<5>(sim, aux, damp, over); break;
+        case 6: fn = pick_blocks<6>(sim, aux, damp, over); break;
+        case 7: fn = pick_blocks<7>(sim, aux, damp, over); break;
+        case 20: fn = pick_blocks<20>(sim, aux, damp, over); break;
+        case 22: fn = pick_blocks<22>(sim, aux, damp, over); break;
+        case 30: fn = pick_blocks<30>(sim, aux, damp, over); break;
         default: return hipErrorInvalidValue;
     }
     size_t lds = xf_block_lds_bytes(nmax);
-    if (threads == 0 || threads > 256 || threads % 64 || nph == 0 || lds > 65536) return hipErrorInvalidValue;
+    if (threads == 0 || threads > 256 || threads % 64 || nph == 0 || ntet == 0 || lds > 65536)
+        return hipErrorInvalidValue;
     if (nslot == 0) return hipSuccess;
-    hipLaunchKernelGGL(fn, dim3(nslot), dim3(threads), lds, s, nph, nmax, rec, bnode, tloc, tidx, tg0, tg1, tg2,
-                       tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock);
+    hipLaunchKernelGGL(fn, dim3(nslot), dim3(threads), lds, s, nph, nmax, ntet, rec, bnode, tloc, tidx, tg0, tg1,
+                       tg2, tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock);
     return hipGetLastError();
 }
 

@@ -47,7 +47,8 @@ extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t lanes, 
 extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax);
 extern "C" size_t xf_block_lds_bytes(uint32_t nmax);
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
-                                       uint32_t nslot, uint32_t nph, uint32_t nmax, const uint32_t* rec,
+                                       uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
+                                       const uint32_t* rec,
                                        const uint32_t* bnode, const uint2* tloc, const uint4* tidx,
                                        const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3,
                                        const float4* taux, double4* X, const double4* O,
@@ -238,7 +239,7 @@ struct xf_scene {
         uint32_t nslot = 0, nph = 0, rec_off = 0, node_off = 0;
     };
     BlockLaunch blk[4];
-    uint32_t n_blocks = 0, blk_nmax = 0, blk_nodes = 0;
+    uint32_t n_blocks = 0, blk_nmax = 0, blk_nodes = 0, blk_pmax = 0;
     uint32_t* blk_rec = nullptr;
     uint32_t* blk_node = nullptr;
     uint2* tloc = nullptr;
@@ -666,8 +667,8 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                 XF_HIP(ctx, hipEventCreate(&e1));
                 XF_HIP(ctx, hipEventRecord(e0, st));
             }
-            XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, block, L.nslot, L.nph,
-                                         s->blk_nmax, s->blk_rec + L.rec_off, s->blk_node + L.node_off, s->tloc,
+            XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, block, L.nslot, L.nph, s->blk_nmax,
+                                         s->T, s->blk_pmax, s->blk_rec + L.rec_off, s->blk_node + L.node_off, s->tloc,
                                          s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O,
                                          s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr));
             first = false;
@@ -1096,25 +1097,88 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     s->colour_begin[s->ncol] = pos;
     if (pos != s->T) return Fail(ctx, XF_E_INVALID, "internal: schedule covers %u of %u tets", pos, s->T);
 
-    // Sweep-block tables: slots of launch colour q are its blocks, body-major
-    // then by block number (the order of the blocks' tets inside a colour).
+    // Sweep-block tables. Blocks of launch colour q in layout order (body,
+    // then block number: the order of their tets inside each colour q, j)
+    // are merged, consecutive ones, into workgroup slots while every phase of
+    // the slot holds at most kSlotLanes tets and its nodes fit LDS; a slot's
+    // tets of phase j are then one contiguous range. Slots are cut into
+    // eighths and dealt round-robin, so that under round-robin dispatch over
+    // the 8 XCDs each XCD sweeps one contiguous run of blocks (neighbouring
+    // blocks share node lines).
     std::vector<uint2> h_tloc;
     std::vector<uint32_t> h_brec, h_bnode;
     if (s->opts.sweep_blocks) {
-        // slot = workgroup of a launch. Blocks of a launch colour in (body,
-        // block number) order are cut into eighths and dealt round-robin, so
-        // that under round-robin dispatch over the 8 XCDs each XCD sweeps one
-        // contiguous run of blocks (neighbouring columns share node lines).
-        uint32_t nslot[4] = {0, 0, 0, 0};
-        std::vector<std::vector<uint32_t>> lay(s->bodies.size()), slot(s->bodies.size());
+        constexpr uint32_t kSlotLanes = 64, kSlotNodes = 2048;
+        uint32_t nblk[4] = {0, 0, 0, 0};
+        std::vector<std::vector<uint32_t>> lay(s->bodies.size());
         for (size_t bi = 0; bi < s->bodies.size(); bi++) {
             const xf::SweepBlocks& sb = sweep[bi];
             lay[bi].resize(sb.n_blocks);
-            for (uint32_t k = 0; k < sb.n_blocks; k++) lay[bi][k] = nslot[sb.launch[k]]++;
+            for (uint32_t k = 0; k < sb.n_blocks; k++) lay[bi][k] = nblk[sb.launch[k]]++;
         }
-        std::vector<uint32_t> deal[4];  // per launch colour: position in (body, block) order -> slot
+        s->n_blocks = nblk[0] + nblk[1] + nblk[2] + nblk[3];
+        // per block: sorted node list; per schedule position: the block's layout index
+        std::vector<std::vector<uint32_t>> bnodes[4];
+        for (int q = 0; q < 4; q++) bnodes[q].resize(nblk[q]);
+        std::vector<uint32_t> sp(s->T, 0);
+        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+            const Body& b = s->bodies[bi];
+            const xf::SweepBlocks& sb = sweep[bi];
+            for (uint32_t t = 0; t < b.t; t++) {
+                uint32_t k = sb.block[t], q = sb.launch[k], m = lay[bi][k];
+                for (int i = 0; i < 4; i++) bnodes[q][m].push_back(b.node_base + b.tets[4 * (size_t)t + i]);
+                sp[b.sched_pos[t]] = m;
+            }
+        }
+        for (int q = 0; q < 4; q++)
+            for (std::vector<uint32_t>& v : bnodes[q]) {
+                std::sort(v.begin(), v.end());
+                v.erase(std::unique(v.begin(), v.end()), v.end());
+                if (v.size() > kSlotNodes)
+                    return Fail(ctx, XF_E_UNSUPPORTED, "a sweep block has %zu nodes (LDS holds %u)", v.size(),
+                                kSlotNodes);
+            }
+        // per block and phase: first schedule position and count (walk of each colour)
+        std::vector<uint32_t> bst[4], bct[4];
         for (int q = 0; q < 4; q++) {
-            uint32_t n = nslot[q], next = 0;
+            uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
+            bst[q].assign((size_t)nblk[q] * nph, 0);
+            bct[q].assign((size_t)nblk[q] * nph, 0);
+            for (uint32_t j = 0; j < nph; j++) {
+                uint32_t c = blk_base[q] + j, p = s->colour_begin[c], end = s->colour_begin[c + 1];
+                for (uint32_t m = 0; m < nblk[q]; m++) {
+                    uint32_t p0 = p;
+                    while (p < end && sp[p] == m) p++;
+                    bst[q][(size_t)m * nph + j] = p0;
+                    bct[q][(size_t)m * nph + j] = p - p0;
+                }
+                if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
+            }
+        }
+        // slots: runs of consecutive blocks
+        std::vector<uint32_t> gfirst[4];  // per slot (in layout order): first block; sentinel at the end
+        for (int q = 0; q < 4; q++) {
+            uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
+            std::vector<uint32_t> lanes(nph, 0);
+            uint32_t nodes = 0;
+            for (uint32_t m = 0; m < nblk[q]; m++) {
+                bool fits = !gfirst[q].empty() && nodes + bnodes[q][m].size() <= kSlotNodes;
+                for (uint32_t j = 0; j < nph && fits; j++) fits = lanes[j] + bct[q][(size_t)m * nph + j] <= kSlotLanes;
+                if (!fits) {
+                    gfirst[q].push_back(m);
+                    std::fill(lanes.begin(), lanes.end(), 0u);
+                    nodes = 0;
+                }
+                for (uint32_t j = 0; j < nph; j++) lanes[j] += bct[q][(size_t)m * nph + j];
+                nodes += (uint32_t)bnodes[q][m].size();
+            }
+            gfirst[q].push_back(nblk[q]);
+        }
+        uint32_t nslot[4], slot_base[5] = {0, 0, 0, 0, 0};
+        std::vector<uint32_t> deal[4];  // per launch colour: slot in layout order -> dispatched slot
+        for (int q = 0; q < 4; q++) {
+            uint32_t n = nslot[q] = (uint32_t)gfirst[q].size() - 1, next = 0;
+            slot_base[q + 1] = slot_base[q] + n;
             deal[q].assign(n, 0);
             for (uint32_t m = 0; next < n; m++)
                 for (uint32_t r = 0; r < 8; r++) {
@@ -1122,42 +1186,30 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
                     if (lo + m < hi) deal[q][lo + m] = next++;
                 }
         }
-        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
-            const xf::SweepBlocks& sb = sweep[bi];
-            slot[bi].resize(sb.n_blocks);
-            for (uint32_t k = 0; k < sb.n_blocks; k++) slot[bi][k] = deal[sb.launch[k]][lay[bi][k]];
-        }
-        uint32_t slot_base[5] = {0, 0, 0, 0, 0};
-        for (int q = 0; q < 4; q++) slot_base[q + 1] = slot_base[q] + nslot[q];
-        s->n_blocks = slot_base[4];
-        std::vector<std::vector<uint32_t>> bnodes(s->n_blocks);
-        std::vector<uint32_t> sp(s->T, 0);
-        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
-            const Body& b = s->bodies[bi];
-            const xf::SweepBlocks& sb = sweep[bi];
-            for (uint32_t t = 0; t < b.t; t++) {
-                uint32_t k = sb.block[t], g = slot_base[sb.launch[k]] + slot[bi][k];
-                for (int i = 0; i < 4; i++) bnodes[g].push_back(b.node_base + b.tets[4 * (size_t)t + i]);
-                sp[b.sched_pos[t]] = lay[bi][k];
+        // slot node lists (member blocks' nodes are disjoint), records, local indices
+        std::vector<std::vector<uint32_t>> snodes(slot_base[4]);
+        std::vector<uint32_t> slot_of_block[4];
+        s->blk_nmax = s->blk_nodes = s->blk_pmax = 0;
+        for (int q = 0; q < 4; q++) {
+            slot_of_block[q].assign(nblk[q], 0);
+            for (uint32_t g = 0; g < nslot[q]; g++) {
+                std::vector<uint32_t>& v = snodes[slot_base[q] + deal[q][g]];
+                for (uint32_t m = gfirst[q][g]; m < gfirst[q][g + 1]; m++) {
+                    slot_of_block[q][m] = deal[q][g];
+                    v.insert(v.end(), bnodes[q][m].begin(), bnodes[q][m].end());
+                }
+                std::sort(v.begin(), v.end());
+                s->blk_nmax = std::max(s->blk_nmax, (uint32_t)v.size());
+                s->blk_nodes += (uint32_t)v.size();
             }
         }
-        s->blk_nmax = s->blk_nodes = 0;
-        for (uint32_t g = 0; g < s->n_blocks; g++) {
-            std::vector<uint32_t>& v = bnodes[g];
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-            s->blk_nmax = std::max(s->blk_nmax, (uint32_t)v.size());
-            s->blk_nodes += (uint32_t)v.size();
-        }
-        if (xf_block_lds_bytes(s->blk_nmax) > 65536)
-            return Fail(ctx, XF_E_UNSUPPORTED, "a sweep block has %u nodes (LDS holds 2048)", s->blk_nmax);
         h_tloc.assign(s->T, make_uint2(0u, 0u));
         for (size_t bi = 0; bi < s->bodies.size(); bi++) {
             const Body& b = s->bodies[bi];
             const xf::SweepBlocks& sb = sweep[bi];
             for (uint32_t t = 0; t < b.t; t++) {
-                uint32_t k = sb.block[t], g = slot_base[sb.launch[k]] + slot[bi][k];
-                const std::vector<uint32_t>& v = bnodes[g];
+                uint32_t k = sb.block[t], q = sb.launch[k];
+                const std::vector<uint32_t>& v = snodes[slot_base[q] + slot_of_block[q][lay[bi][k]]];
                 uint32_t l[4];
                 for (int i = 0; i < 4; i++)
                     l[i] = (uint32_t)(std::lower_bound(v.begin(), v.end(), b.node_base + b.tets[4 * (size_t)t + i]) -
@@ -1165,9 +1217,9 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
                 h_tloc[b.sched_pos[t]] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
             }
         }
-        h_bnode.assign((size_t)s->n_blocks * s->blk_nmax, xf::kNoNode);
-        for (uint32_t g = 0; g < s->n_blocks; g++)
-            std::copy(bnodes[g].begin(), bnodes[g].end(), h_bnode.begin() + (size_t)g * s->blk_nmax);
+        h_bnode.assign((size_t)slot_base[4] * s->blk_nmax, xf::kNoNode);
+        for (uint32_t g = 0; g < slot_base[4]; g++)
+            std::copy(snodes[g].begin(), snodes[g].end(), h_bnode.begin() + (size_t)g * s->blk_nmax);
         for (int q = 0; q < 4; q++) {
             xf_scene::BlockLaunch& L = s->blk[q];
             L.nslot = nslot[q];
@@ -1175,17 +1227,15 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
             L.rec_off = (uint32_t)h_brec.size();
             L.node_off = slot_base[q] * s->blk_nmax;
             h_brec.resize(h_brec.size() + 2 * (size_t)L.nslot * L.nph, 0u);
-            for (uint32_t j = 0; j < L.nph; j++) {
-                // colour (q, j) holds the blocks' tets in (body, block) order
-                uint32_t c = blk_base[q] + j, p = s->colour_begin[c], end = s->colour_begin[c + 1];
-                for (uint32_t m = 0; m < L.nslot; m++) {
-                    uint32_t p0 = p, k = deal[q][m];
-                    while (p < end && sp[p] == m) p++;
-                    h_brec[L.rec_off + 2 * ((size_t)k * L.nph + j)] = p0;
-                    h_brec[L.rec_off + 2 * ((size_t)k * L.nph + j) + 1] = p - p0;
+            for (uint32_t g = 0; g < L.nslot; g++)
+                for (uint32_t j = 0; j < L.nph; j++) {
+                    uint32_t first = gfirst[q][g], count = 0;
+                    for (uint32_t m = first; m < gfirst[q][g + 1]; m++) count += bct[q][(size_t)m * L.nph + j];
+                    size_t at = L.rec_off + 2 * ((size_t)deal[q][g] * L.nph + j);
+                    h_brec[at] = bst[q][(size_t)first * L.nph + j];
+                    h_brec[at + 1] = count;
+                    s->blk_pmax = std::max(s->blk_pmax, count);
                 }
-                if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
-            }
         }
     }
 
