@@ -2,7 +2,8 @@
 // objects (Demo.cpp:67-90): a Settings-shaped struct, the [4, i0..i3]* index
 // stream of GenerateTetBlock, and one Substep call per substep. Writes the
 // final FP64 positions and velocities to the file given as argv[1].
-// Usage: geo_adapter_demo OUT FRAMES SUBSTEPS ENERGY
+// Usage: geo_adapter_demo OUT FRAMES SUBSTEPS ENERGY [RAYLEIGH_TYPE DAMPING PBD
+// PBD_AMORTIZED [AUTORESIZE [SWEEP_BLOCKS]]]
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -35,7 +36,9 @@ struct DemoManipulator {
 
 int main(int argc, char** argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s OUT FRAMES SUBSTEPS ENERGY [RAYLEIGH_TYPE DAMPING PBD PBD_AMORTIZED [AUTORESIZE]]\n",
+        fprintf(stderr,
+                "usage: %s OUT FRAMES SUBSTEPS ENERGY [RAYLEIGH_TYPE DAMPING PBD PBD_AMORTIZED [AUTORESIZE "
+                "[SWEEP_BLOCKS]]]\n",
                 argv[0]);
         return 2;
     }
@@ -59,6 +62,7 @@ int main(int argc, char** argv) {
         return 1;
     }
     geo.SetGround(-0.5f, 0.5f);
+    if (argc >= 11 && atoi(argv[10]) != 0 && (rc = geo.SetSweepBlocks(true))) return 1;
     DemoSettings settings;
     settings.flags = (energy << xfgpu::kEnergyBit) | (1u << xfgpu::kXpbdSolveBit) | xfgpu::kLockLeft;
     if (argc >= 9) {  // damping: Settings flag bits 20-21 and the derived PBD factors

@@ -38,6 +38,33 @@ def test_adapter_substep_loop_bit_exact(tmp_path, energy):
     assert np.array_equal(V, ob.V)
 
 
+@pytest.mark.parametrize("energy,rtype,damp", [(7, 0, 0.0), (4, 0, 0.02), (7, 2, 0.02)])
+def test_adapter_sweep_blocks_bit_exact(ctx, tmp_path, energy, rtype, damp):
+    """GeoGpuLinear3d::SetSweepBlocks: the adapter's scene with sweep blocks
+    and block launches (with in-projection and post Rayleigh damping); a scene
+    of the same mesh and options gives the order the oracle replays."""
+    assert os.path.exists(DEMO), "build the adapter demo first (__graft_entry__.build())"
+    out = tmp_path / "state.bin"
+    frames, substeps = 2, 20
+    print(f"\nrunning {DEMO} energy {energy} rtype {rtype} with sweep blocks", flush=True)
+    r = subprocess.run([DEMO, str(out), str(frames), str(substeps), str(energy), str(rtype), str(damp), "0", "0", "0",
+                        "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    wl = workloads.beam_neo_12x4x4(energy=energy)
+    wl["settings"] = dict(wl["settings"], damping=damp, rayleigh_type=rtype)
+    n = wl["xyz"].size // 3
+    data = np.fromfile(out, np.float64)
+    X, V = data[:3 * n], data[3 * n:]
+    sc = workloads.make_scene(ctx, wl, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    order = sc.order(0)[0]
+    sc.close()
+    ob = po.body_from_workload(wl, "ref64")
+    ob.substep(po.params_from_settings(np.float32(1.0) / np.float32(1200.0), wl["settings"]), order,
+               count=frames * substeps)
+    assert np.array_equal(X, ob.X)
+    assert np.array_equal(V, ob.V)
+
+
 @pytest.mark.parametrize("energy", [9, 10])
 def test_adapter_ring_energies_bit_exact(ctx, tmp_path, energy):
     """Settings energy bits 9 / 10 through the adapter run the vertex-ring

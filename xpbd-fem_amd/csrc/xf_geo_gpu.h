@@ -167,6 +167,18 @@ class GeoGpuLinear3d {
 
     const char* LastError() const { return ctx_ ? xf_last_error(ctx_) : "no context"; }
 
+    // The sweep-block schedule (xf_scene_opts.sweep_blocks with
+    // XF_LAUNCH_BLOCKS: 4 projection launches per substep, each workgroup
+    // sweeping one block through its phases in LDS) for the scene created at
+    // the first Substep. Vertex-ring energies (Settings energy 8..10) then
+    // fail: they run per colour only. Returns XF_E_STATE after the first
+    // Substep.
+    int SetSweepBlocks(bool on) {
+        if (scene_) return XF_E_STATE;
+        sweep_blocks_ = on;
+        return XF_OK;
+    }
+
    private:
     int EnsureScene(uint32_t flags) {
         if (scene_) return XF_OK;
@@ -188,7 +200,8 @@ class GeoGpuLinear3d {
         o.colour_order = XF_COLOUR_NATURAL;
         o.block_size = 64;
         o.use_graph = 1;
-        o.launch_mode = XF_LAUNCH_PER_COLOUR;
+        o.launch_mode = sweep_blocks_ ? XF_LAUNCH_BLOCKS : XF_LAUNCH_PER_COLOUR;
+        o.sweep_blocks = sweep_blocks_ ? 1u : 0u;
         return xf_scene_finalize(scene_, &o);
     }
 
@@ -202,6 +215,7 @@ class GeoGpuLinear3d {
     uint32_t n_ = 0;
     uint32_t collide_ = 0;
     float ground_y_ = 0.0f, friction_ = 0.0f;
+    bool sweep_blocks_ = false;
 };
 
 }  // namespace xfgpu
