@@ -138,7 +138,7 @@ def cpu_baseline(ctx, wl, budget_s=12.0, opts=None):
                 sample=f"{what}: {n} substeps of the {t}-tet beam, single thread, {el:.1f} s", check=check)
 
 
-def batched_leg(ctx, frames=3, schedule="slabs"):
+def batched_leg(ctx, frames=3, schedule="blocks"):
     """256 boxes: one batched scene (every body's blocks, or colour c of all
     bodies, per launch; autotuned launch shape) against the plain per-body
     launch loop (one per-colour scene per body, stepped one after another)."""
@@ -589,10 +589,7 @@ def main():
         out["per_colour_slabs"] = {"ms_per_frame": ms_pc, "tet_substeps_per_s": T * S / (ms_pc * 1e-3)}
     legs = []
     if not args.no_batch:
-        # many small rotated boxes: per-colour launches (158k tets per colour);
-        # their breadth-first blocks from single corner nodes are irregular
-        legs.append(("batched", lambda c: batched_leg(c, schedule="slabs" if args.schedule == "blocks"
-                                                      else args.schedule)))
+        legs.append(("batched", lambda c: batched_leg(c, schedule=args.schedule)))
     if not args.no_large:
         legs.append(("large", lambda c: large_leg(c, schedule=args.schedule)))
     if not args.no_rings:
