@@ -96,6 +96,17 @@ def test_blocks_equal_per_colour_launches(ctx, use_graph):
     assert a[5] == b[5] and a[6] == b[6]  # clock advanced once per substep
 
 
+@pytest.mark.parametrize("energy,sim", [(7, True), (4, False)])
+def test_block_colouring_persistent_launch(ctx, energy, sim):
+    """The persistent launch over the sweep-block colouring (24 colours of
+    (launch colour, phase)) is a serial sweep in the reported order too."""
+    wl = workloads.beam_neo_12x4x4(energy=energy, simultaneous=sim)
+    x, v, order, colour, info, _, _ = run(ctx, wl, launch_mode=xf.LAUNCH_PERSISTENT)
+    assert info["n_colours"] == 24
+    ob = oracle(wl, order, 2 * wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
 def test_blocks_big_beam_bit_exact(ctx):
     """The 96x24x24 Yeoh beam (331,776 tets; 4,608 blocks along x and y - z,
     6 phases each, 4 launches per substep: 24 phases) with ground and moving
