@@ -445,7 +445,7 @@ def beam_profile(schedule="blocks"):
                       "blocks": info["n_blocks"], "block_nodes": info["n_block_nodes"], "schedule": schedule}))
 
 
-def large_profile(schedule="blocks", block=64):
+def large_profile(schedule="blocks", block=256):
     """One frame of the 288x72x72 beam in a fixed launch shape: block launches,
     or per-colour launches of `block` threads (short runs under rocprofv3
     counter passes)."""

@@ -210,6 +210,10 @@ typedef struct xf_scene_opts {
                                    valid serial order; xf_scene_get_order reports it). Needed by
                                    XF_LAUNCH_BLOCKS; excludes xcd_slabs and tet_colour. The nodes
                                    of a block must fit the LDS of one workgroup (at most 2048). */
+    uint32_t block_lanes;       /* sweep blocks: consecutive blocks of a launch colour share one
+                                   workgroup while each phase holds at most this many tets (64,
+                                   128, 256), or 0: the largest of these that leaves at least 4
+                                   workgroups per compute unit in every launch colour (else 64) */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {
@@ -222,8 +226,11 @@ typedef struct xf_scene_info {
     uint32_t n_ring_colours; /* ring colour phases per substep (max over bodies) */
     uint32_t n_blocks;       /* sweep blocks over all bodies (0 without sweep_blocks) */
     uint32_t block_launches; /* block launch colours that hold blocks (0 without sweep_blocks) */
-    uint32_t n_block_nodes;  /* nodes of all sweep blocks, each block counting its own (a node shared
-                                by blocks of different launch colours counts once per block) */
+    uint32_t n_block_nodes;  /* nodes of all sweep-block workgroups (slots), each counting its own (a
+                                node shared by slots of different launch colours counts once per slot) */
+    uint32_t block_lanes;    /* the slot lane cap in use (xf_scene_opts.block_lanes or its automatic
+                                choice; 0 without sweep_blocks) */
+    uint32_t n_block_slots;  /* workgroups of the block launches of one substep */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */

@@ -67,6 +67,22 @@ def test_blocks_workgroup_sizes(ctx):
         assert np.array_equal(r[0], ob.X) and np.array_equal(r[1], ob.V)
 
 
+@pytest.mark.parametrize("lanes,block", [(128, 64), (256, 256), (256, 128)])
+def test_blocks_slot_lanes(ctx, lanes, block):
+    """Blocks merged into slots of up to 128 / 256 tets per phase (the lanes
+    of smaller workgroups take several tets in turn) on a 24x8x8 Yeoh beam:
+    the same order, fewer workgroups, the oracle's result."""
+    wl = workloads.beam_yeoh(24, 8, 8, structured=False)
+    base = run(ctx, wl, frames=1)
+    r = run(ctx, wl, frames=1, block_lanes=lanes, block_size=block)
+    assert base[4]["block_lanes"] == 64
+    assert r[4]["block_lanes"] == lanes and r[4]["n_block_slots"] < base[4]["n_block_slots"]
+    assert np.array_equal(r[2], base[2])
+    ob = oracle(wl, r[2], wl["substeps"])
+    assert np.array_equal(r[0], ob.X) and np.array_equal(r[1], ob.V)
+    assert np.array_equal(base[0], ob.X) and np.array_equal(base[1], ob.V)
+
+
 def test_blocks_more_tets_than_lanes(ctx):
     """A mirrored 4x4x150 block: no direction pair beats (x, y), whose
     columns of 900 tets have 12+ phases of up to 75 tets; on 64-thread
@@ -114,6 +130,7 @@ def test_blocks_big_beam_bit_exact(ctx):
     wl = workloads.beam_yeoh_96x24x24()
     x, v, order, colour, info, _, _ = run(ctx, wl, frames=1)
     assert info["n_blocks"] == 96 * 48 and info["n_colours"] == 24
+    assert info["block_lanes"] == 64  # larger slots would leave fewer than 4 per compute unit
     ob = oracle(wl, order, wl["substeps"])
     print(f"\nbig beam: max|dx| vs oracle {np.abs(x - ob.X).max():.3e}")
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
@@ -206,7 +223,7 @@ def test_blocks_autotune_keeps_results(ctx):
 def test_blocks_rejections(ctx):
     wl = workloads.beam_neo_12x4x4()
     cases = [dict(launch_mode=xf.LAUNCH_BLOCKS), dict(sweep_blocks=True, xcd_slabs=True),
-             dict(sweep_blocks=2), dict(launch_mode=3)]
+             dict(sweep_blocks=2), dict(launch_mode=3), dict(sweep_blocks=True, block_lanes=100)]
     for opts in cases:
         sc = xf.Scene(ctx)
         sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"])

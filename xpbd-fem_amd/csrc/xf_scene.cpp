@@ -239,7 +239,7 @@ struct xf_scene {
         uint32_t nslot = 0, nph = 0, rec_off = 0, node_off = 0;
     };
     BlockLaunch blk[4];
-    uint32_t n_blocks = 0, blk_nmax = 0, blk_nodes = 0, blk_pmax = 0;
+    uint32_t n_blocks = 0, blk_nmax = 0, blk_nodes = 0, blk_pmax = 0, blk_lanes = 0;
     uint32_t* blk_rec = nullptr;
     uint32_t* blk_node = nullptr;
     uint2* tloc = nullptr;
@@ -962,6 +962,9 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     if (s->opts.ring_lanes != 0 && s->opts.ring_lanes != 16 && s->opts.ring_lanes != 32)
         return Fail(ctx, XF_E_INVALID, "ring_lanes must be 0, 16 or 32");
     if (s->opts.sweep_blocks > 1) return Fail(ctx, XF_E_INVALID, "sweep_blocks must be 0 or 1");
+    if (s->opts.block_lanes != 0 && s->opts.block_lanes != 64 && s->opts.block_lanes != 128 &&
+        s->opts.block_lanes != 256)
+        return Fail(ctx, XF_E_INVALID, "block_lanes must be 0, 64, 128 or 256");
     if (s->opts.sweep_blocks && s->opts.xcd_slabs)
         return Fail(ctx, XF_E_INVALID, "sweep_blocks and xcd_slabs are exclusive");
     if (s->opts.launch_mode == XF_LAUNCH_BLOCKS && !s->opts.sweep_blocks)
@@ -1100,7 +1103,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     // Sweep-block tables. Blocks of launch colour q in layout order (body,
     // then block number: the order of their tets inside each colour q, j)
     // are merged, consecutive ones, into workgroup slots while every phase of
-    // the slot holds at most kSlotLanes tets and its nodes fit LDS; a slot's
+    // the slot holds at most block_lanes tets and its nodes fit LDS; a slot's
     // tets of phase j are then one contiguous range. Slots are cut into
     // eighths and dealt round-robin, so that under round-robin dispatch over
     // the 8 XCDs each XCD sweeps one contiguous run of blocks (neighbouring
@@ -1108,7 +1111,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     std::vector<uint2> h_tloc;
     std::vector<uint32_t> h_brec, h_bnode;
     if (s->opts.sweep_blocks) {
-        constexpr uint32_t kSlotLanes = 64, kSlotNodes = 2048;
+        constexpr uint32_t kSlotNodes = 2048;
         uint32_t nblk[4] = {0, 0, 0, 0};
         std::vector<std::vector<uint32_t>> lay(s->bodies.size());
         for (size_t bi = 0; bi < s->bodies.size(); bi++) {
@@ -1155,25 +1158,46 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
                 if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
             }
         }
-        // slots: runs of consecutive blocks
+        // slots: runs of consecutive blocks, each phase at most `lanes_cap` tets
         std::vector<uint32_t> gfirst[4];  // per slot (in layout order): first block; sentinel at the end
-        for (int q = 0; q < 4; q++) {
-            uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
-            std::vector<uint32_t> lanes(nph, 0);
-            uint32_t nodes = 0;
-            for (uint32_t m = 0; m < nblk[q]; m++) {
-                bool fits = !gfirst[q].empty() && nodes + bnodes[q][m].size() <= kSlotNodes;
-                for (uint32_t j = 0; j < nph && fits; j++) fits = lanes[j] + bct[q][(size_t)m * nph + j] <= kSlotLanes;
-                if (!fits) {
-                    gfirst[q].push_back(m);
-                    std::fill(lanes.begin(), lanes.end(), 0u);
-                    nodes = 0;
+        auto group = [&](uint32_t lanes_cap) {
+            for (int q = 0; q < 4; q++) {
+                gfirst[q].clear();
+                uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
+                std::vector<uint32_t> lanes(nph, 0);
+                uint32_t nodes = 0;
+                for (uint32_t m = 0; m < nblk[q]; m++) {
+                    bool fits = !gfirst[q].empty() && nodes + bnodes[q][m].size() <= kSlotNodes;
+                    for (uint32_t j = 0; j < nph && fits; j++)
+                        fits = lanes[j] + bct[q][(size_t)m * nph + j] <= lanes_cap;
+                    if (!fits) {
+                        gfirst[q].push_back(m);
+                        std::fill(lanes.begin(), lanes.end(), 0u);
+                        nodes = 0;
+                    }
+                    for (uint32_t j = 0; j < nph; j++) lanes[j] += bct[q][(size_t)m * nph + j];
+                    nodes += (uint32_t)bnodes[q][m].size();
                 }
-                for (uint32_t j = 0; j < nph; j++) lanes[j] += bct[q][(size_t)m * nph + j];
-                nodes += (uint32_t)bnodes[q][m].size();
+                gfirst[q].push_back(nblk[q]);
             }
-            gfirst[q].push_back(nblk[q]);
+        };
+        int n_cu = 256;
+        XF_HIP(ctx, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        uint32_t lanes_cap = s->opts.block_lanes;
+        if (lanes_cap == 0) {
+            // the largest cap that leaves >= 4 slots per compute unit in every launch colour
+            for (uint32_t cap : {256u, 128u, 64u}) {
+                group(cap);
+                lanes_cap = cap;
+                bool enough = true;
+                for (int q = 0; q < 4; q++)
+                    if (nblk[q] && gfirst[q].size() - 1 < 4 * (size_t)n_cu) enough = false;
+                if (enough) break;
+            }
+        } else {
+            group(lanes_cap);
         }
+        s->blk_lanes = lanes_cap;
         uint32_t nslot[4], slot_base[5] = {0, 0, 0, 0, 0};
         std::vector<uint32_t> deal[4];  // per launch colour: slot in layout order -> dispatched slot
         for (int q = 0; q < 4; q++) {
@@ -1332,6 +1356,10 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     info->block_launches = 0;
     for (int q = 0; q < 4; q++) info->block_launches += s->blk[q].nslot ? 1u : 0u;
     info->n_block_nodes = s->blk_nodes;
+    info->block_lanes = s->blk_lanes;
+    uint32_t slots = 0;
+    for (int q = 0; q < 4; q++) slots += s->blk[q].nslot;
+    info->n_block_slots = slots;
     return XF_OK;
 }
 
