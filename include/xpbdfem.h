@@ -283,8 +283,11 @@ XF_API int xf_scene_get_order(const xf_scene* scene, uint32_t body, uint32_t* or
  * (nullable) gets them in sweep order, colour_of_ring (nullable) each
  * ring's colour. Sweeping this order serially reproduces the GPU result.
  * Replaces GeoLinear3d::vOrder (Geo.h:162, Geo.cpp:759-771), which the
- * reference shuffles with MINSTD; XF_COLOUR_MINSTD visits rings in that
- * order when colouring. Builds the rings on first use. */
+ * reference shuffles with MINSTD. Rings are coloured by DSatur (rings that
+ * share a node get different colours; ties: more conflicting rings, then
+ * earlier in the visit order: vertex order, or with XF_COLOUR_MINSTD the
+ * reference's vOrder); inside a colour they follow the visit order. Builds
+ * the rings on first use. */
 XF_API int xf_scene_get_ring_order(xf_scene* scene, uint32_t body, uint32_t* n_rings, uint32_t* order,
                                    uint32_t* colour_of_ring);
 /* Element constants of one body as computed by InitFiniteElement

@@ -284,25 +284,79 @@ void MinstdRingOrder(uint32_t n_tets, uint32_t n_rings, uint32_t* order) {
 
 uint32_t RingColourSweepOrder(const Rings& r, uint32_t n_nodes, const uint32_t* visit,
                               std::vector<uint32_t>* colour, std::vector<uint32_t>* order) {
-    // Each ring, in visit order, takes the lowest colour that no ring sharing
-    // one of its nodes has taken; node masks are kWords x 64 bits.
+    const uint32_t n = r.n;
+    std::vector<uint32_t> rank(n, 0);
+    for (uint32_t k = 0; k < n; k++) rank[visit[k]] = k;
+    // node -> rings holding it
+    std::vector<uint32_t> first(n_nodes + 1, 0);
+    for (uint32_t v = 0; v < n; v++)
+        for (uint32_t q = r.node_off[v]; q < r.node_off[v + 1]; q++) first[r.nodes[q] + 1]++;
+    for (uint32_t u = 0; u < n_nodes; u++) first[u + 1] += first[u];
+    std::vector<uint32_t> inc(first[n_nodes]);
+    {
+        std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+        for (uint32_t v = 0; v < n; v++)
+            for (uint32_t q = r.node_off[v]; q < r.node_off[v + 1]; q++) inc[fill[r.nodes[q]]++] = v;
+    }
+    // conflicting rings (sharing a node), distinct
+    std::vector<uint64_t> noff(n + 1, 0);
+    std::vector<uint32_t> nbr, stamp(n, ~0u);
+    for (uint32_t v = 0; v < n; v++) {
+        for (uint32_t q = r.node_off[v]; q < r.node_off[v + 1]; q++) {
+            uint32_t u = r.nodes[q];
+            for (uint32_t k = first[u]; k < first[u + 1]; k++) {
+                uint32_t w = inc[k];
+                if (w != v && stamp[w] != v) {
+                    stamp[w] = v;
+                    nbr.push_back(w);
+                }
+            }
+        }
+        noff[v + 1] = nbr.size();
+    }
+    // DSatur: the uncoloured ring with the most distinct colours among its
+    // conflicting rings, then the most conflicting rings, then the earliest
+    // in visit order, takes the lowest colour none of them has.
     const int kWords = 4;
-    std::vector<uint64_t> mask((size_t)n_nodes * kWords, 0);
-    colour->assign(r.n, 0);
+    std::vector<uint64_t> sat((size_t)n * kWords, 0);
+    std::vector<uint32_t> nsat(n, 0);
+    struct Item {
+        uint32_t sat, deg, rank, ring;
+        bool operator<(const Item& o) const {
+            if (sat != o.sat) return sat < o.sat;
+            if (deg != o.deg) return deg < o.deg;
+            return rank > o.rank;
+        }
+    };
+    std::vector<Item> heap;
+    heap.reserve(n);
+    for (uint32_t v = 0; v < n; v++) heap.push_back({0u, (uint32_t)(noff[v + 1] - noff[v]), rank[v], v});
+    std::make_heap(heap.begin(), heap.end());
+    colour->assign(n, kMaxColours);
     uint32_t ncol = 0;
-    for (uint32_t k = 0; k < r.n; k++) {
-        uint32_t v = visit[k];
+    while (!heap.empty()) {
+        std::pop_heap(heap.begin(), heap.end());
+        Item it = heap.back();
+        heap.pop_back();
+        uint32_t v = it.ring;
+        if ((*colour)[v] != kMaxColours || it.sat != nsat[v]) continue;
         uint32_t c = kMaxColours;
         for (int word = 0; word < kWords && c == kMaxColours; word++) {
-            uint64_t used = 0;
-            for (uint32_t q = r.node_off[v]; q < r.node_off[v + 1]; q++) used |= mask[(size_t)r.nodes[q] * kWords + word];
+            uint64_t used = sat[(size_t)v * kWords + word];
             if (used != ~0ull) c = 64 * word + (uint32_t)__builtin_ctzll(~used);
         }
         if (c == kMaxColours) return 0;
         (*colour)[v] = c;
-        for (uint32_t q = r.node_off[v]; q < r.node_off[v + 1]; q++)
-            mask[(size_t)r.nodes[q] * kWords + c / 64] |= 1ull << (c % 64);
         ncol = std::max(ncol, c + 1);
+        for (uint64_t k = noff[v]; k < noff[v + 1]; k++) {
+            uint32_t w = nbr[k];
+            uint64_t& m = sat[(size_t)w * kWords + c / 64];
+            if ((*colour)[w] != kMaxColours || (m >> (c % 64) & 1ull)) continue;
+            m |= 1ull << (c % 64);
+            nsat[w]++;
+            heap.push_back({nsat[w], (uint32_t)(noff[w + 1] - noff[w]), rank[w], w});
+            std::push_heap(heap.begin(), heap.end());
+        }
     }
     std::vector<uint32_t> start(ncol + 1, 0);
     for (uint32_t v = 0; v < r.n; v++) start[(*colour)[v] + 1]++;

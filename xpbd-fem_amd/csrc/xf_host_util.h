@@ -98,9 +98,10 @@ bool BuildRings(const uint32_t* tets, uint32_t n_tets, Rings* out);
 // from the n_tets draws of the tet shuffle, then shuffles n_rings vertices.
 void MinstdRingOrder(uint32_t n_tets, uint32_t n_rings, uint32_t* order);
 
-// Greedy colouring of rings in visit order (rings sharing a node get
-// different colours) and the colour-major ring order, stable in visit order.
-// Returns the colour count (0 if more than kMaxColours would be needed).
+// DSatur colouring of rings (rings sharing a node get different colours;
+// ties: more conflicting rings, then earlier in visit order) and the
+// colour-major ring order, stable in visit order. Returns the colour count
+// (0 if more than kMaxColours would be needed).
 uint32_t RingColourSweepOrder(const Rings& r, uint32_t n_nodes, const uint32_t* visit,
                               std::vector<uint32_t>* colour, std::vector<uint32_t>* order);
 
