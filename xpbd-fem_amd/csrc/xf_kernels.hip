@@ -1013,7 +1013,10 @@ __global__ __launch_bounds__(256) void xf_predict(uint32_t n, double4* __restric
 // End of a substep: pin (Geo.cpp:318-322), collisions, velocity
 // (Geo.cpp:342-344); then, if PREDICT, the next substep's prediction,
 // else the velocity is stored. clock holds the end time of the substep.
-template <bool PREDICT>
+// SINGLE: the scene has one body, whose parameters every node reads (loaded
+// once per wave, alongside the node loads, instead of through the node's
+// body id).
+template <bool PREDICT, bool SINGLE>
 __global__ __launch_bounds__(256) void xf_finish(uint32_t n, double4* __restrict__ X, double4* __restrict__ O,
                                                  double4* __restrict__ V, const XfBodyParams* __restrict__ bodies,
                                                  const float* __restrict__ clock) {
@@ -1022,7 +1025,7 @@ __global__ __launch_bounds__(256) void xf_finish(uint32_t n, double4* __restrict
     double4 xi = X[i];
     double4 oi = O[i];
     uint32_t flags = (uint32_t)oi.w;
-    const XfBodyParams& p = bodies[flags >> 1];
+    const XfBodyParams& p = bodies[SINGLE ? 0u : flags >> 1];
     double x[3] = {xi.x, xi.y, xi.z};
     double o[3] = {oi.x, oi.y, oi.z};
     if (flags & 1u) {
@@ -1475,14 +1478,12 @@ extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, d
     return hipGetLastError();
 }
 
-extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
-                                       double4* V,
-                                       const XfBodyParams* bodies, const float* clock) {
+extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
+                                       double4* O, double4* V, const XfBodyParams* bodies, const float* clock) {
     if (n == 0) return hipSuccess;
-    if (predict_next)
-        hipLaunchKernelGGL(xf_finish<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, clock);
-    else
-        hipLaunchKernelGGL(xf_finish<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, clock);
+    auto fn = predict_next ? (single_body ? xf_finish<true, true> : xf_finish<true, false>)
+                           : (single_body ? xf_finish<false, true> : xf_finish<false, false>);
+    hipLaunchKernelGGL(fn, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, clock);
     return hipGetLastError();
 }
 

@@ -56,8 +56,8 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
                                        float* clock);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
-extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, uint32_t n, double4* X, double4* O,
-                                       double4* V, const XfBodyParams* bodies, const float* clock);
+extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
+                                       double4* O, double4* V, const XfBodyParams* bodies, const float* clock);
 extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const double4* X, uint32_t* count);
 extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, bool damp, int* max_blocks);
 extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool aux, uint32_t begin, uint32_t count,
@@ -561,7 +561,9 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
             XF_HIP(ctx, hipEventRecord(e0, st));
         }
         if (kind == 0) XF_HIP(ctx, xf_launch_predict(st, s->N, s->X, s->O, s->V, s->bp));
-        else XF_HIP(ctx, xf_launch_finish(st, predict_next, s->N, s->X, s->O, s->V, s->bp, s->clock));
+        else
+            XF_HIP(ctx, xf_launch_finish(st, predict_next, s->bodies.size() == 1, s->N, s->X, s->O, s->V, s->bp,
+                                         s->clock));
         if (timer) {
             XF_HIP(ctx, hipEventRecord(e1, st));
             timer->ev_node.push_back(e0);
@@ -1863,7 +1865,8 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
             }
             XF_HIP(ctx, hipMemcpyAsync(xs + (size_t)N * (kk - k0), s->X, n4, hipMemcpyDeviceToDevice, st));
             XF_HIP(ctx, hipMemcpyAsync(oo + (size_t)N * (kk - k0), s->O, n4, hipMemcpyDeviceToDevice, st));
-            XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, N, s->X, s->O, s->V, s->bp, s->clock));
+            XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, s->bodies.size() == 1, N, s->X, s->O, s->V, s->bp,
+                                         s->clock));
         }
         for (uint32_t kk = k1; kk-- > k0;) {
             const double4* xl = nullptr;
