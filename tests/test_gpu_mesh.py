@@ -49,3 +49,25 @@ def test_irregular_mesh_bit_exact(ctx, tmp_path, seed, energy, mode):
     print(f"\nseed {seed}: {wl['tets'].size // 4} tets, {ncol} colours; max|dx| {np.abs(x - ob.X).max():.3e}")
     assert np.all(np.isfinite(x))
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
+@pytest.mark.parametrize("energy", [4, 7])
+def test_irregular_mesh_sweep_blocks_bit_exact(ctx, tmp_path, energy):
+    """Sweep blocks on the jittered Delaunay mesh: irregular blocks (coarse
+    breadth-first levels, up to ~34 phases per launch colour) still sweep in
+    the reported order."""
+    wl = delaunay_workload(tmp_path, 0, energy)
+    sc = workloads.make_scene(ctx, wl, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    S = wl["substeps"]
+    sc.step(S, float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, colour = sc.order(0)
+    info = sc.info()
+    sc.close()
+    t = wl["tets"].reshape(-1, 4)
+    pairs = np.stack([np.repeat(colour.astype(np.int64), 4), t.ravel().astype(np.int64)], 1)
+    assert len(np.unique(pairs, axis=0)) == len(pairs)  # colours node-disjoint
+    ob = po.body_from_workload(wl, "ref64")
+    ob.substep(po.params_from_settings(wl["dt"], wl["settings"]), order, count=S)
+    print(f"\n{t.shape[0]} tets: {info['n_blocks']} blocks, {info['n_colours']} phases per substep")
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
