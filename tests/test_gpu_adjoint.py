@@ -117,3 +117,28 @@ def test_batched_adjoint_equals_single(ctx, fit):
         one.close()
     assert np.isclose(L2, total, rtol=1e-14)
     both.close()
+
+
+def test_adjoint_over_sweep_block_colouring(ctx, fit):
+    """A scene coloured by sweep blocks: the adjoint runs per colour over the
+    (launch colour, phase) colours and equals the oracle's adjoint in the
+    reported order; the forward end state equals block-launch stepping."""
+    truth, guess, _ = fit
+    S = truth["substeps"]
+    sc = workloads.make_scene(ctx, guess, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    order = sc.order(0)[0]
+    loss_at = np.array([S, 2 * S, 3 * S], np.uint32)
+    targets = targets_for(truth, order, loss_at)
+    L, g = sc.adjoint(3 * S, float(truth["dt"]), loss_at, targets, window=20)
+    x_adj, v_adj = sc.get_state(0)
+    sc.close()
+    ob = po.body_from_workload(guess, "ref64")
+    Lo, glog = po.rollout_adjoint(ob, po.params_from_settings(truth["dt"], truth["settings"]), order, 3 * S,
+                                  loss_at, targets, window=20)
+    assert L == Lo
+    assert np.array_equal(g, -glog)
+    fwd = workloads.make_scene(ctx, guess, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    fwd.step(3 * S, float(truth["dt"]))
+    x, v = fwd.get_state(0)
+    fwd.close()
+    assert np.array_equal(x, x_adj) and np.array_equal(v, v_adj)
