@@ -214,6 +214,14 @@ typedef struct xf_scene_opts {
                                    workgroup while each phase holds at most this many tets (64,
                                    128, 256), or 0: the largest of these that leaves at least 4
                                    workgroups per compute unit in every launch colour (else 64) */
+    uint32_t separate_finish;   /* sweep blocks: 1 keeps the end of each substep (pinning, contact,
+                                   velocity, next prediction) as a node launch of its own; 0: the
+                                   last block launch of a substep does it for every node when at
+                                   least two launch colours hold blocks, the last has at most two
+                                   workgroups per compute unit (latency-bound) whose nodes fit LDS
+                                   with their O (68 B a node), no body has volume passes or
+                                   damping and the energy constrains tets (not in the last
+                                   substep of a step call, which stores the velocities) */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {
@@ -231,6 +239,9 @@ typedef struct xf_scene_info {
     uint32_t block_lanes;    /* the slot lane cap in use (xf_scene_opts.block_lanes or its automatic
                                 choice; 0 without sweep_blocks) */
     uint32_t n_block_slots;  /* workgroups of the block launches of one substep */
+    uint32_t fused_finish;   /* 1: under the current settings, step calls in XF_LAUNCH_BLOCKS end
+                                their substeps inside the last block launch (see separate_finish;
+                                the energy is not checked here) */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */

@@ -136,6 +136,49 @@ def test_blocks_big_beam_bit_exact(ctx):
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
 
 
+@pytest.mark.parametrize("energy", [7, 4])
+def test_blocks_fused_node_phase(ctx, energy):
+    """The end of each substep (pins, ground and moving-sphere contact with
+    friction, velocity, next prediction) done by the last block launch for
+    every node, its own block nodes from LDS and the rest from memory: the
+    same state as the separate node launch and as the oracle, over two step
+    calls (whose last substeps store the velocity instead)."""
+    wl = workloads.beam_neo_12x4x4(energy=energy)
+    wl["settings"] = dict(wl["settings"], ground_y=-0.2004, collide=xf.COLLIDE_GROUND | xf.COLLIDE_SPHERE,
+                          sphere_c0=(0.3, -0.45, 0.0), sphere_v=(0.0, 6.0, 0.0), sphere_r=0.3)
+    fused = run(ctx, wl, frames=3)
+    sep = run(ctx, wl, frames=3, separate_finish=True)
+    assert fused[4]["fused_finish"] == 1 and sep[4]["fused_finish"] == 0
+    assert np.array_equal(fused[2], sep[2])
+    ob = oracle(wl, fused[2], 3 * wl["substeps"])
+    assert np.array_equal(fused[0], ob.X) and np.array_equal(fused[1], ob.V)
+    assert np.array_equal(sep[0], ob.X) and np.array_equal(sep[1], ob.V)
+    assert fused[5] == sep[5] and fused[6] == sep[6] == 3 * wl["substeps"]
+    # contact and the sphere were active
+    assert np.any(np.abs(ob.X.reshape(-1, 3)[:, 1] - np.float32(-0.2004)) < 1e-9)
+
+
+def test_blocks_fused_node_phase_conditions(ctx):
+    """Volume passes or damping keep the node phase separate; so does a mesh
+    whose blocks fall in fewer than two launch colours (the clock would be
+    advanced and read in one launch)."""
+    wl = workloads.beam_neo_12x4x4(energy=7)
+    for extra in (dict(volume_passes=1), dict(damping=0.01, rayleigh_type=0), dict(pbd_damping=0.002)):
+        w = dict(wl, settings=dict(wl["settings"], **extra))
+        sc = workloads.make_scene(ctx, w, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+        assert sc.info()["fused_finish"] == 0, extra
+        sc.close()
+    xyz, tets = xf.tet_block(1, 1, 1, (0.1, 0.1, 0.1), (0.0, -0.05, -0.05))
+    corner = tets[:4].astype(np.int64)
+    one_tet = dict(xyz=xyz.reshape(-1, 3)[corner].ravel().copy(), tets=np.arange(4, dtype=np.uint32))
+    for name, mesh, launches, fused in (("cell", dict(xyz=xyz, tets=tets), 2, 1), ("tet", one_tet, 1, 0)):
+        w = dict(wl, name=name, pinned=np.zeros(mesh["xyz"].size // 3, np.uint8), **mesh)
+        x, v, order, _, info, _, _ = run(ctx, w, frames=2)
+        assert info["block_launches"] == launches and info["fused_finish"] == fused, name
+        ob = oracle(w, order, 2 * wl["substeps"])
+        assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V), name
+
+
 def test_blocks_minstd_visit_and_volume_passes(ctx):
     wl = workloads.beam_neo_12x4x4(energy=4)
     wl["settings"] = dict(wl["settings"], volume_passes=2)

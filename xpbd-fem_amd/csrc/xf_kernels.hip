@@ -826,6 +826,13 @@ struct BlockTet {
     uint2 loc;
 };
 
+namespace {
+// End of a substep and the next prediction for node i of body p, whose
+// position after the sweep is xi (defined with the node phase below).
+__device__ __forceinline__ void finish_predict(double4 xi, double4 oi, uint32_t i, double4* __restrict__ X,
+                                               double4* __restrict__ O, const XfBodyParams& p, float t_end);
+}  // namespace
+
 template <bool AUX>
 __device__ __forceinline__ BlockTet load_block_tet(uint32_t i, const uint2* __restrict__ tloc,
                                                    const float4* __restrict__ tg0, const float4* __restrict__ tg1,
@@ -867,13 +874,24 @@ __device__ __forceinline__ void block_tet(const BlockTet& t, uint32_t i, double4
 // flight are the same on every path and the wait before a phase covers
 // exactly that phase's loads. OVER: some block has more tets in a phase than
 // the workgroup has lanes (lanes then load and solve further tets in turn).
-template <int E, bool SIM, bool AUX, bool DAMP, bool OVER>
+// FIN: the launch is the substep's last; its workgroups end the substep and
+// predict the next one for every node of the scene (the node phase, fused).
+// fextra holds per workgroup nmax + fmax node ids (xf::kNoNode padded): its
+// block nodes as in bnode, then fmax nodes that no block of this launch
+// colour touches. All are staged in LDS with their O and finished from there
+// after the last phase. No other workgroup of the launch reads or writes
+// those nodes, and O is not read by the projection (FIN excludes DAMP).
+// tend: the clock, read for the end time of the substep (advanced by an
+// earlier launch). fsingle: the scene has one body.
+template <int E, bool SIM, bool AUX, bool DAMP, bool OVER, bool FIN>
 __global__ __launch_bounds__(256) void xf_project_blocks(
     uint32_t nph, uint32_t nmax, uint32_t ntet, const uint32_t* __restrict__ rec, const uint32_t* __restrict__ bnode,
     const uint2* __restrict__ tloc, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
     const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
-    const float4* __restrict__ taux, double4* __restrict__ X, const double4* __restrict__ O,
-    const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock) {
+    const float4* __restrict__ taux, double4* __restrict__ X, double4* __restrict__ O,
+    const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock,
+    const uint32_t* __restrict__ fextra, uint32_t fmax, const float* __restrict__ tend, uint32_t fsingle) {
+    static_assert(!(FIN && (DAMP || E == 30)), "the fused node phase needs a projection that leaves O unread");
     if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
         clock[0] = clock[0] + dt;
         uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
@@ -886,6 +904,12 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     const uint32_t* r = rec + (size_t)blockIdx.x * 2 * nph;
     const uint32_t* bn = bnode + (size_t)blockIdx.x * nmax;
     const uint32_t jl = nph - 1u;
+    // FIN: LDS holds ftot nodes (the block nodes, then the extra nodes), then
+    // their O, then their ids
+    const uint32_t ftot = nmax + fmax;
+    const uint32_t* fl = fextra + (size_t)blockIdx.x * ftot;
+    double4* os = xs + ftot;
+    uint32_t* fs = reinterpret_cast<uint32_t*>(os + ftot);
     auto load = [&](uint32_t j) {
         return load_block_tet<AUX>(min(r[2 * j] + tid, last), tloc, tg0, tg1, tg2, tg3, taux);
     };
@@ -901,9 +925,21 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
         __syncthreads();
     };
     BlockTet ta = load(0), tb;
-    for (uint32_t k = tid; k < nmax; k += bs) {
-        uint32_t g = bn[k];
-        if (g != xf::kNoNode) xs[k] = X[g];
+    if constexpr (FIN) {
+        // nodes, their O and their ids staged for the end of the substep
+        for (uint32_t k = tid; k < ftot; k += bs) {
+            uint32_t g = fl[k];
+            fs[k] = g;
+            if (g != xf::kNoNode) {
+                xs[k] = X[g];
+                os[k] = O[g];
+            }
+        }
+    } else {
+        for (uint32_t k = tid; k < nmax; k += bs) {
+            uint32_t g = bn[k];
+            if (g != xf::kNoNode) xs[k] = X[g];
+        }
     }
     __syncthreads();
     // phases in pairs, element data of the next phase loaded before solving
@@ -914,9 +950,27 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
         ta = load(min(j + 2u, jl));
         if (j + 1u < nph) phase(j + 1u, tb);
     }
-    for (uint32_t k = tid; k < nmax; k += bs) {
-        uint32_t g = bn[k];
-        if (g != xf::kNoNode) X[g] = xs[k];
+    if constexpr (FIN) {
+        // (single-body scenes read the body's parameters once)
+        const float t_end = tend[0];
+        auto finish_all = [&](auto body_of) {
+            for (uint32_t k = tid; k < ftot; k += bs) {
+                uint32_t g = fs[k];
+                if (g != xf::kNoNode) {
+                    double4 oi = os[k];
+                    finish_predict(xs[k], oi, g, X, O, body_of((uint32_t)oi.w), t_end);
+                }
+            }
+        };
+        if (fsingle)
+            finish_all([&](uint32_t) -> const XfBodyParams& { return bodies[0]; });
+        else
+            finish_all([&](uint32_t flags) -> const XfBodyParams& { return bodies[flags >> 1]; });
+    } else {
+        for (uint32_t k = tid; k < nmax; k += bs) {
+            uint32_t g = bn[k];
+            if (g != xf::kNoNode) X[g] = xs[k];
+        }
     }
 }
 
@@ -991,6 +1045,25 @@ __device__ __forceinline__ void predict(const XfBodyParams& p, double v[3], cons
     double dt = (double)p.dt;
     *Oo = make_double4(x[0], x[1], x[2], oflags);
     *Xo = make_double4(x[0] + v[0] * dt, x[1] + v[1] * dt, x[2] + v[2] * dt, w);
+}
+
+__device__ __forceinline__ void finish_predict(double4 xi, double4 oi, uint32_t i, double4* __restrict__ X,
+                                               double4* __restrict__ O, const XfBodyParams& p, float t_end) {
+    uint32_t flags = (uint32_t)oi.w;
+    double x[3] = {xi.x, xi.y, xi.z};
+    double o[3] = {oi.x, oi.y, oi.z};
+    if (flags & 1u) {
+        x[0] = o[0];
+        x[1] = o[1];
+        x[2] = o[2];
+    } else if (p.collide) {
+        collide_node(p, t_end, o, x);
+    }
+    double v[3];
+    double invdt = (double)p.invdt;
+#pragma unroll
+    for (int r = 0; r < 3; r++) v[r] = (x[r] - o[r]) * invdt;
+    predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
 }
 
 }  // namespace
@@ -1357,55 +1430,74 @@ extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, boo
 
 typedef void (*XfBlocksFn)(uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*, const uint2*,
                            const uint4*, const float4*, const float4*, const float4*, const float4*, const float4*,
-                           double4*, const double4*, const XfBodyParams*, float, float, float, float*);
+                           double4*, double4*, const XfBodyParams*, float, float, float, float*, const uint32_t*,
+                           uint32_t, const float*, uint32_t);
 
 namespace {
 template <int E, bool SIM, bool OVER>
-XfBlocksFn pick_blocks_ad(bool aux, bool damp) {
-    if (aux) return damp ? xf_project_blocks<E, SIM, true, true, OVER> : xf_project_blocks<E, SIM, true, false, OVER>;
-    return damp ? xf_project_blocks<E, SIM, false, true, OVER> : xf_project_blocks<E, SIM, false, false, OVER>;
+XfBlocksFn pick_blocks_ad(bool aux, bool damp, bool fin) {
+    if (fin) {
+        if constexpr (E == 30) return nullptr;
+        else if (damp) return nullptr;
+        else return aux ? xf_project_blocks<E, SIM, true, false, OVER, true>
+                        : xf_project_blocks<E, SIM, false, false, OVER, true>;
+    }
+    if (aux)
+        return damp ? xf_project_blocks<E, SIM, true, true, OVER, false> : xf_project_blocks<E, SIM, true, false, OVER, false>;
+    return damp ? xf_project_blocks<E, SIM, false, true, OVER, false> : xf_project_blocks<E, SIM, false, false, OVER, false>;
 }
 template <int E>
-XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over) {
-    if (over) return sim ? pick_blocks_ad<E, true, true>(aux, damp) : pick_blocks_ad<E, false, true>(aux, damp);
-    return sim ? pick_blocks_ad<E, true, false>(aux, damp) : pick_blocks_ad<E, false, false>(aux, damp);
+XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over, bool fin) {
+    if (over)
+        return sim ? pick_blocks_ad<E, true, true>(aux, damp, fin) : pick_blocks_ad<E, false, true>(aux, damp, fin);
+    return sim ? pick_blocks_ad<E, true, false>(aux, damp, fin) : pick_blocks_ad<E, false, false>(aux, damp, fin);
 }
 }  // namespace
 
 // LDS bytes of a sweep-block workgroup: its nodes in FP64 (32 B each).
 extern "C" size_t xf_block_lds_bytes(uint32_t nmax) { return 32 * (size_t)nmax; }
 
+// LDS bytes of a workgroup of the launch that ends the substep: its nodes and
+// extra nodes, each with O (32 B) and id (4 B).
+extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax) { return 68 * ((size_t)nmax + fmax); }
+
 // Host-callable: one sweep-block launch (nslot workgroups of `threads`);
 // rec / bnode point at the launch colour's first block; pmax: the most tets
-// of one block in one phase.
+// of one block in one phase. fin: the launch also ends the substep and
+// predicts the next for every node (fextra: per workgroup its nmax block
+// nodes and fmax extra nodes; tend: the clock, not advanced by this launch;
+// single_body: the scene has one body); not with damp or energy 30.
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
                                        uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
                                        const uint32_t* rec,
                                        const uint32_t* bnode, const uint2* tloc, const uint4* tidx,
                                        const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3,
-                                       const float4* taux, double4* X, const double4* O,
+                                       const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
-                                       float* clock) {
+                                       float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
+                                       const float* tend, bool single_body) {
     XfBlocksFn fn = nullptr;
     const bool over = pmax > threads;
     switch (energy) {
-        case 0: fn = pick_blocks<0>(sim, aux, damp, over); break;
-        case 3: fn = pick_blocks<3>(sim, aux, damp, over); break;
-        case 4: fn = pick_blocks<4>(sim, aux, damp, over); break;
-        case 5: fn = pick_blocks<5>(sim, aux, damp, over); break;
-        case 6: fn = pick_blocks<6>(sim, aux, damp, over); break;
-        case 7: fn = pick_blocks<7>(sim, aux, damp, over); break;
-        case 20: fn = pick_blocks<20>(sim, aux, damp, over); break;
-        case 22: fn = pick_blocks<22>(sim, aux, damp, over); break;
-        case 30: fn = pick_blocks<30>(sim, aux, damp, over); break;
+        case 0: fn = pick_blocks<0>(sim, aux, damp, over, fin); break;
+        case 3: fn = pick_blocks<3>(sim, aux, damp, over, fin); break;
+        case 4: fn = pick_blocks<4>(sim, aux, damp, over, fin); break;
+        case 5: fn = pick_blocks<5>(sim, aux, damp, over, fin); break;
+        case 6: fn = pick_blocks<6>(sim, aux, damp, over, fin); break;
+        case 7: fn = pick_blocks<7>(sim, aux, damp, over, fin); break;
+        case 20: fn = pick_blocks<20>(sim, aux, damp, over, fin); break;
+        case 22: fn = pick_blocks<22>(sim, aux, damp, over, fin); break;
+        case 30: fn = pick_blocks<30>(sim, aux, damp, over, fin); break;
         default: return hipErrorInvalidValue;
     }
-    size_t lds = xf_block_lds_bytes(nmax);
-    if (threads == 0 || threads > 256 || threads % 64 || nph == 0 || ntet == 0 || lds > 65536)
+    size_t lds = fin ? xf_block_fin_lds_bytes(nmax, fmax) : xf_block_lds_bytes(nmax);
+    if (fn == nullptr || threads == 0 || threads > 256 || threads % 64 || nph == 0 || ntet == 0 || lds > 65536)
         return hipErrorInvalidValue;
+    if (fin && (tend == nullptr || clock != nullptr || fextra == nullptr)) return hipErrorInvalidValue;
     if (nslot == 0) return hipSuccess;
     hipLaunchKernelGGL(fn, dim3(nslot), dim3(threads), lds, s, nph, nmax, ntet, rec, bnode, tloc, tidx, tg0, tg1,
-                       tg2, tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock);
+                       tg2, tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock, fextra, fmax, tend,
+                       single_body ? 1u : 0u);
     return hipGetLastError();
 }
 

@@ -46,14 +46,16 @@ extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t lanes, 
                                       float* clock);
 extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax);
 extern "C" size_t xf_block_lds_bytes(uint32_t nmax);
+extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax);
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
                                        uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
                                        const uint32_t* rec,
                                        const uint32_t* bnode, const uint2* tloc, const uint4* tidx,
                                        const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3,
-                                       const float4* taux, double4* X, const double4* O,
+                                       const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
-                                       float* clock);
+                                       float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
+                                       const float* tend, bool single_body);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
@@ -171,6 +173,8 @@ constexpr int kRingMode = XF_ENERGY_CONTINUOUS_PIXAR;
 
 // LDS budget of one ring workgroup.
 constexpr size_t kRingLdsLimit = 65536;
+// LDS budget of one sweep-block workgroup.
+constexpr size_t kBlockLdsLimit = 65536;
 
 // XCD-slab order of a colour-major sweep order: inside each colour the items
 // are sorted by x (ties keep their order) and cut into the eighths the
@@ -242,6 +246,15 @@ struct xf_scene {
     uint32_t n_blocks = 0, blk_nmax = 0, blk_nodes = 0, blk_pmax = 0, blk_lanes = 0;
     uint32_t* blk_rec = nullptr;
     uint32_t* blk_node = nullptr;
+    // fused node phase: the last launch colour with blocks (blk_last; -1 with
+    // fewer than two such colours, more than two of its slots per compute
+    // unit, or when its nodes with their O and ids do not fit LDS) ends the
+    // substep; blk_fin holds per slot of that launch its
+    // blk_nmax block nodes and blk_fmax nodes that none of the colour's
+    // blocks touches
+    int blk_last = -1;
+    uint32_t blk_fmax = 0;
+    uint32_t* blk_fin = nullptr;
     uint2* tloc = nullptr;
     uint64_t device_bytes = 0;
     int n_cu = 256;
@@ -656,7 +669,14 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
     int rc = node(0, false);
     if (rc) return rc;
     const bool blocks = s->opts.launch_mode == XF_LAUNCH_BLOCKS;
+    // the node phase rides on the substep's last block launch when nothing
+    // runs between the sweep and the velocity update (no volume passes), the
+    // velocities need not be stored (no damping passes, not the call's last
+    // substep) and the projection leaves O unread
+    const bool fuse = blocks && s->blk_last >= 0 && !s->opts.separate_finish && s->vpasses == 0 && s->post == 0 &&
+                      !s->dproj && energy != 30;
     for (uint32_t k = 0; k < substeps; k++) {
+        const bool fin = fuse && k + 1 < substeps;
         // sweep blocks: one launch per launch colour that holds blocks; the
         // first advances the clock
         bool first = true;
@@ -672,7 +692,9 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
             XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, block, L.nslot, L.nph, s->blk_nmax,
                                          s->T, s->blk_pmax, s->blk_rec + L.rec_off, s->blk_node + L.node_off, s->tloc,
                                          s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O,
-                                         s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr));
+                                         s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr,
+                                         fin && q == s->blk_last, s->blk_fin, s->blk_fmax, s->clock,
+                                         s->bodies.size() == 1));
             first = false;
             if (timer) {
                 XF_HIP(ctx, hipEventRecord(e1, st));
@@ -705,6 +727,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                                              s->taux, s->X, s->bp, dt));
             }
         if (s->post == 0) {
+            if (fin) continue;
             rc = node(1, k + 1 < substeps);
             if (rc) return rc;
             continue;
@@ -856,7 +879,7 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
                     s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet,
-                    s->blk_rec, s->blk_node, s->tloc};
+                    s->blk_rec, s->blk_node, s->tloc, s->blk_fin};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -964,6 +987,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     if (s->opts.ring_lanes != 0 && s->opts.ring_lanes != 16 && s->opts.ring_lanes != 32)
         return Fail(ctx, XF_E_INVALID, "ring_lanes must be 0, 16 or 32");
     if (s->opts.sweep_blocks > 1) return Fail(ctx, XF_E_INVALID, "sweep_blocks must be 0 or 1");
+    if (s->opts.separate_finish > 1) return Fail(ctx, XF_E_INVALID, "separate_finish must be 0 or 1");
     if (s->opts.block_lanes != 0 && s->opts.block_lanes != 64 && s->opts.block_lanes != 128 &&
         s->opts.block_lanes != 256)
         return Fail(ctx, XF_E_INVALID, "block_lanes must be 0, 64, 128 or 256");
@@ -1111,7 +1135,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     // the 8 XCDs each XCD sweeps one contiguous run of blocks (neighbouring
     // blocks share node lines).
     std::vector<uint2> h_tloc;
-    std::vector<uint32_t> h_brec, h_bnode;
+    std::vector<uint32_t> h_brec, h_bnode, h_bfin;
     if (s->opts.sweep_blocks) {
         constexpr uint32_t kSlotNodes = 2048;
         uint32_t nblk[4] = {0, 0, 0, 0};
@@ -1246,6 +1270,52 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         h_bnode.assign((size_t)slot_base[4] * s->blk_nmax, xf::kNoNode);
         for (uint32_t g = 0; g < slot_base[4]; g++)
             std::copy(snodes[g].begin(), snodes[g].end(), h_bnode.begin() + (size_t)g * s->blk_nmax);
+        // fused node phase: per slot of the last launch colour, its node row
+        // followed by its share of the nodes outside that colour's slots (in
+        // ascending order, dealt in equal runs)
+        int launches = 0;
+        s->blk_last = -1;
+        for (int q = 0; q < 4; q++)
+            if (nslot[q]) {
+                launches++;
+                s->blk_last = q;
+            }
+        s->blk_fmax = 0;
+        h_bfin.clear();
+        std::vector<uint32_t> extra;
+        int cus = 256;
+        XF_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        // fused only where the last launch is latency-bound (at most two
+        // workgroups per compute unit): with more, the longer write-back and
+        // the LDS of O cost more than the node launch they save
+        if (launches < 2 || nslot[s->blk_last] > 2 * (uint32_t)cus) {
+            s->blk_last = -1;
+        } else {
+            const int q = s->blk_last;
+            std::vector<uint8_t> covered(s->N, 0);
+            for (uint32_t g = slot_base[q]; g < slot_base[q + 1]; g++)
+                for (uint32_t v : snodes[g]) covered[v] = 1;
+            for (uint32_t v = 0; v < s->N; v++)
+                if (!covered[v]) extra.push_back(v);
+            s->blk_fmax = (uint32_t)((extra.size() + nslot[q] - 1) / nslot[q]);
+            // staged with O and ids in LDS; else the node phase stays separate
+            if (xf_block_fin_lds_bytes(s->blk_nmax, s->blk_fmax) > kBlockLdsLimit) {
+                s->blk_last = -1;
+                s->blk_fmax = 0;
+            }
+        }
+        if (s->blk_last >= 0) {
+            const int q = s->blk_last;
+            const uint32_t fmax = s->blk_fmax;
+            const size_t row = (size_t)s->blk_nmax + fmax;
+            h_bfin.assign((size_t)nslot[q] * row, xf::kNoNode);
+            for (uint32_t b = 0; b < nslot[q]; b++) {
+                const uint32_t* bn = h_bnode.data() + (size_t)(slot_base[q] + b) * s->blk_nmax;
+                std::copy(bn, bn + s->blk_nmax, h_bfin.begin() + b * row);
+                for (uint32_t j = 0; j < fmax && (size_t)b * fmax + j < extra.size(); j++)
+                    h_bfin[b * row + s->blk_nmax + j] = extra[(size_t)b * fmax + j];
+            }
+        }
         for (int q = 0; q < 4; q++) {
             xf_scene::BlockLaunch& L = s->blk[q];
             L.nslot = nslot[q];
@@ -1309,7 +1379,8 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         return rc;
     if (s->opts.sweep_blocks && ((rc = DevAlloc(ctx, &s->blk_rec, h_brec.size(), &s->device_bytes)) ||
                                  (rc = DevAlloc(ctx, &s->blk_node, h_bnode.size(), &s->device_bytes)) ||
-                                 (rc = DevAlloc(ctx, &s->tloc, s->T, &s->device_bytes))))
+                                 (rc = DevAlloc(ctx, &s->tloc, s->T, &s->device_bytes)) ||
+                                 (!h_bfin.empty() && (rc = DevAlloc(ctx, &s->blk_fin, h_bfin.size(), &s->device_bytes)))))
         return rc;
     XF_HIP(ctx, hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     hipStream_t st = ctx->stream;
@@ -1329,6 +1400,9 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
             XF_HIP(ctx, hipMemcpyAsync(s->blk_node, h_bnode.data(), sizeof(uint32_t) * h_bnode.size(),
                                        hipMemcpyHostToDevice, st));
         XF_HIP(ctx, hipMemcpyAsync(s->tloc, h_tloc.data(), sizeof(uint2) * s->T, hipMemcpyHostToDevice, st));
+        if (!h_bfin.empty())
+            XF_HIP(ctx, hipMemcpyAsync(s->blk_fin, h_bfin.data(), sizeof(uint32_t) * h_bfin.size(),
+                                       hipMemcpyHostToDevice, st));
     }
     uint32_t uzero = 0;
     static const uint32_t kClockZero[kClockWords] = {0u, 0u};  // time 0.0f, tick 0
@@ -1362,6 +1436,10 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     uint32_t slots = 0;
     for (int q = 0; q < 4; q++) slots += s->blk[q].nslot;
     info->n_block_slots = slots;
+    bool fused = s->blk_last >= 0 && !s->opts.separate_finish && s->opts.launch_mode == XF_LAUNCH_BLOCKS;
+    for (const Body& b : s->bodies)
+        if (b.settings.volume_passes != 0 || b.settings.damping > 0.0f || b.settings.pbd_damping > 0.0f) fused = false;
+    info->fused_finish = fused ? 1u : 0u;
     return XF_OK;
 }
 
