@@ -43,6 +43,8 @@ BYTES_PER_NODE = 96
 # LDS through its phases).
 BYTES_PER_BLOCK_TET = 64 + 8
 BYTES_PER_BLOCK_NODE = 4 + 32 + 32
+# fused node phase: O read and written (32 + 32)
+BYTES_FUSED_NODE = 32 + 32
 # Scene options of the sweep schedules: sweep blocks (4 launches per substep,
 # each workgroup sweeping one block through its phases), or one launch per
 # colour phase with the XCD-slab order or the plain blockIdx-linear order.
@@ -55,11 +57,15 @@ KERNELS = {xf.LAUNCH_PER_COLOUR: "xf_project_colour", xf.LAUNCH_PERSISTENT: "xf_
 
 def projection_bytes(info, mode, substeps, per_tet_data=False):
     """Algorithmic bytes of the projection launches of `substeps` substeps
-    (the persistent launch also holds the node phases)."""
+    (the persistent launch also holds the node phases; with the fused node
+    phase, the last block launch of every substep but the call's last reads
+    and writes each node's O, 64 B a node; the X of the few nodes outside its
+    blocks is not counted)."""
     T, N = info["n_tets"], info["n_nodes"]
     if mode == xf.LAUNCH_BLOCKS:
         per_tet = BYTES_PER_BLOCK_TET + (16 if per_tet_data else 0)
-        return substeps * (per_tet * T + BYTES_PER_BLOCK_NODE * info["n_block_nodes"])
+        fused = (substeps - 1) * BYTES_FUSED_NODE * N if info.get("fused_finish") else 0
+        return substeps * (per_tet * T + BYTES_PER_BLOCK_NODE * info["n_block_nodes"]) + fused
     return substeps * (BYTES_PER_TET * T + (BYTES_PER_NODE * N if mode == xf.LAUNCH_PERSISTENT else 0))
 
 
