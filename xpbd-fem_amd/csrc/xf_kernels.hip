@@ -906,10 +906,9 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     const uint32_t jl = nph - 1u;
     // FIN: LDS holds ftot nodes (the block nodes, then the extra nodes), then
     // their O, then their ids
-    const uint32_t ftot = nmax + fmax;
-    const uint32_t* fl = fextra + (size_t)blockIdx.x * ftot;
-    double4* os = xs + ftot;
-    uint32_t* fs = reinterpret_cast<uint32_t*>(os + ftot);
+    const uint32_t ftot = FIN ? nmax + fmax : 0u;
+    double4* os = FIN ? xs + ftot : nullptr;
+    uint32_t* fs = FIN ? reinterpret_cast<uint32_t*>(os + ftot) : nullptr;
     auto load = [&](uint32_t j) {
         return load_block_tet<AUX>(min(r[2 * j] + tid, last), tloc, tg0, tg1, tg2, tg3, taux);
     };
@@ -927,6 +926,7 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     BlockTet ta = load(0), tb;
     if constexpr (FIN) {
         // nodes, their O and their ids staged for the end of the substep
+        const uint32_t* fl = fextra + (size_t)blockIdx.x * ftot;
         for (uint32_t k = tid; k < ftot; k += bs) {
             uint32_t g = fl[k];
             fs[k] = g;
