@@ -547,6 +547,7 @@ def main():
     # projection launches, or the single persistent launch (which also holds
     # the node phases).
     ms_proj, ms_node, n_launch = sc.profile(S, dt)
+    fused_now = bool(sc.info()["fused_finish"])
     bytes_proj = projection_bytes(info, mode, S)
     achieved = bytes_proj / (ms_proj * 1e-3) / 1e9
 
@@ -580,7 +581,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{wl['name']}: {T} tets, {N} nodes, {S} substeps/frame, {info['n_colours']} "
                                f"colours, {info['n_blocks']} sweep blocks, ground + sphere", "step": "one frame",
-                   "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": args.schedule}},
+                   "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": args.schedule,
+                              "node_phase": "in the last projection launch" if fused_now else "own launch"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": KERNELS[mode],
                      "launches": n_launch, "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
