@@ -438,6 +438,18 @@ __global__ __launch_bounds__(256) void xf_zero_d4(uint32_t n, double4* __restric
     if (i < n) a[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
+// Copy of two double4 arrays of n entries (the replay's per-substep
+// snapshot of X and O) in one launch.
+__global__ __launch_bounds__(256) void xf_copy2_d4(uint32_t n, double4* __restrict__ d0,
+                                                   const double4* __restrict__ s0, double4* __restrict__ d1,
+                                                   const double4* __restrict__ s1) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        d0[i] = s0[i];
+        d1[i] = s1[i];
+    }
+}
+
 // ------------------------------------------------------ launch wrappers
 
 extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, uint32_t begin, uint32_t count,
@@ -476,6 +488,13 @@ extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const 
 extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog) {
     if (t == 0) return hipSuccess;
     hipLaunchKernelGGL(xf_adj_flush, dim3((t + 255) / 256), dim3(256), 0, s, t, gacc, glog);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_copy2_d4(hipStream_t s, uint32_t n, double4* d0, const double4* s0, double4* d1,
+                                         const double4* s1) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_copy2_d4, dim3((n + 255) / 256), dim3(256), 0, s, n, d0, s0, d1, s1);
     return hipGetLastError();
 }
 

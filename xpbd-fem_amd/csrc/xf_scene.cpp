@@ -80,6 +80,8 @@ extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const 
                                                const double4* Ob, const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog);
 extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a);
+extern "C" hipError_t xf_launch_copy2_d4(hipStream_t s, uint32_t n, double4* d0, const double4* s0, double4* d1,
+                                         const double4* s1);
 extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t blocks,
                                       uint32_t substeps, uint32_t ncol, uint32_t vpasses, uint32_t post,
                                       const float2* tdmp, const uint32_t* colour_begin,
@@ -1941,8 +1943,7 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
                                                      s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp, s->comp1_u, s->a_u,
                                                      dt, nullptr));
             }
-            XF_HIP(ctx, hipMemcpyAsync(xs + (size_t)N * (kk - k0), s->X, n4, hipMemcpyDeviceToDevice, st));
-            XF_HIP(ctx, hipMemcpyAsync(oo + (size_t)N * (kk - k0), s->O, n4, hipMemcpyDeviceToDevice, st));
+            XF_HIP(ctx, xf_launch_copy2_d4(st, N, xs + (size_t)N * (kk - k0), s->X, oo + (size_t)N * (kk - k0), s->O));
             XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, s->bodies.size() == 1, N, s->X, s->O, s->V, s->bp,
                                          s->clock));
         }
