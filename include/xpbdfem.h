@@ -36,11 +36,9 @@ typedef enum xf_status {
     XF_E_NONFINITE = -6    /* NaN/Inf detected in the node state */
 } xf_status;
 
-/* Energy ids (Settings.h:41-53); the linear-tet subset of SolveElementMixed
- * (Fem.cpp:437-564). Pixar (0-2), continuous (8-10) and cube (11-12)
- * energies are not implemented (XF_E_UNSUPPORTED). */
+/* Energy ids of the 5-bit energy field (Settings.h:41-53). */
 enum {
-    /* Settings.h:41-53. Every 5-bit value is accepted and dispatched as
+    /* Every 5-bit value is accepted and dispatched as
      * TSolveElement does for linear tets (Fem.cpp:870-889): Pixar and
      * PixarReduced coincide on a linear tet (one quadrature point), the Cube
      * energies constrain quads and hexes only (tets stay free), values above
@@ -313,6 +311,11 @@ XF_API int xf_scene_get_elements(const xf_scene* scene, uint32_t body, float* qi
  * projection launches. */
 XF_API int xf_scene_profile(xf_scene* scene, uint32_t substeps, float dt, float* ms_project, float* ms_node,
                             uint32_t* launches_project);
+/* Runs `calls` calls of xf_scene_step(substeps, dt) between two HIP events
+ * recorded on the context's stream and returns the device time between them
+ * in *ms (synchronous). The benchmark's timed region; no reference
+ * counterpart (the reference prints wall-clock sim time, Demo.cpp:92-117). */
+XF_API int xf_scene_time_steps(xf_scene* scene, uint32_t calls, uint32_t substeps, float dt, float* ms);
 /* Autotuner: times `reps` calls of xf_scene_step(substeps, dt) for each
  * launch shape -- per-colour launches with workgroup sizes 64, 128, 256 and
  * the persistent launch with 64, 128, 256 and 512 blocks (where co-resident),

@@ -229,8 +229,19 @@ def test_blocks_batch_bit_exact(ctx):
     b = workloads.boxes_batch(n_bodies=6, seed=5, cells=(3, 6), frames=2)
     dt, S = float(b["dt"]), b["substeps"]
     sc = workloads.make_batch_scene(ctx, b, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    # the multi-body form of the fused node phase (body parameters per node)
+    assert sc.info()["fused_finish"] == 1
     sc.step(S, dt)
     sc.step(S, dt)
+    sep = workloads.make_batch_scene(ctx, b, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS, separate_finish=True)
+    assert sep.info()["fused_finish"] == 0
+    sep.step(S, dt)
+    sep.step(S, dt)
+    for i in range(len(b["bodies"])):
+        xa, va = sc.get_state(i)
+        xb, vb = sep.get_state(i)
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    sep.close()
     for i, wl in enumerate(b["bodies"]):
         order, colour = sc.order(i)
         check_colouring(wl, order, colour)
@@ -295,3 +306,34 @@ def test_blocks_rejections(ctx):
     with pytest.raises(xf.XfError):
         sc.step(1, float(wl["dt"]))
     sc.close()
+
+
+def test_bench_launch_shape_bit_exact(ctx):
+    """The headline run's exact launch shape on the 96x24x24 Yeoh beam with
+    ground and moving sphere: sweep blocks, 256-thread workgroups (the shape
+    the autotuner picks there), the frame captured in a hipGraph and replayed,
+    the node phase fused into the last block launch of each substep. Two
+    frames (capture, then a replay of the same graph) against the oracle's
+    serial sweep, bit for bit; the sphere's kinetic friction is active."""
+    wl = workloads.beam_yeoh_96x24x24()
+    S = wl["substeps"]
+    sc = workloads.make_scene(ctx, wl, use_graph=True, block_size=256, sweep_blocks=True,
+                              launch_mode=xf.LAUNCH_BLOCKS)
+    info = sc.info()
+    assert info["fused_finish"] == 1 and info["n_colours"] == 24 and info["block_launches"] == 4
+    sc.step(S, float(wl["dt"]))
+    x1, v1 = sc.get_state(0)
+    sc.step(S, float(wl["dt"]))
+    x2, v2 = sc.get_state(0)
+    order = sc.order(0)[0]
+    sc.close()
+    ob = oracle(wl, order, S)
+    print(f"\nbench shape, frame 1: max|dx| vs oracle {np.abs(x1 - ob.X).max():.3e}")
+    assert np.array_equal(x1, ob.X) and np.array_equal(v1, ob.V)
+    ob.substep(po.params_from_settings(wl["dt"], wl["settings"]), order, count=S)
+    assert np.array_equal(x2, ob.X) and np.array_equal(v2, ob.V)
+    # nodes in contact with the moving sphere, slipping along it
+    s = wl["settings"]
+    c = np.array(s["sphere_c0"], np.float32) + np.array(s["sphere_v"], np.float32) * np.float32(2 * S * wl["dt"])
+    d = np.linalg.norm(x2.reshape(-1, 3) - c, axis=1)
+    assert np.sum(np.abs(d - s["sphere_r"]) < 1e-4) > 0

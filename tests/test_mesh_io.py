@@ -105,3 +105,22 @@ def test_armadillo_matches_reference(tmp_path):
     node, ele = xf.write_tetgen(tmp_path / "armadillo", xyz, tets, base=1)
     x2, t2 = xf.read_tetgen(node, ele, orient=False)
     assert np.array_equal(x2, xyz) and np.array_equal(t2, tets)
+
+
+@pytest.mark.parametrize("count", ["-1", "nan", "1e30", "2.5", "4294967297"])
+def test_tetgen_malformed_counts(tmp_path, count):
+    """Header counts that are negative, non-finite, fractional or beyond
+    2^31 are rejected with a message (no undefined integer cast)."""
+    xyz, tets = jittered_delaunay()
+    node, ele = xf.write_tetgen(tmp_path / "m", xyz, tets)
+    bad = tmp_path / "bad.node"
+    bad.write_text(f"{count} 3 0 0\n0 0 0 0\n")
+    with pytest.raises(xf.XfError, match="node count"):
+        xf.read_tetgen(bad, ele)
+    bad = tmp_path / "bad.ele"
+    bad.write_text(f"{count} 4 0\n0 0 1 2 3\n")
+    with pytest.raises(xf.XfError, match="tet count"):
+        xf.read_tetgen(node, bad)
+    bad.write_text("1 4 0\n0 0 1 2 nan\n")
+    with pytest.raises(xf.XfError, match="out of range"):
+        xf.read_tetgen(node, bad)

@@ -9,7 +9,9 @@ product), independent of the reference's own code.
   with numpy (catches a dropped term, a wrong sign or a transposed operand);
 * linear momentum and angular momentum of a projection are conserved;
 * energies 3 and 5 equal 4 and 7 for a linear tet (one quadrature point);
-* collision pass: non-penetration, static and kinetic friction;
+* collision pass: non-penetration, static and kinetic friction on the ground
+  and on the moving sphere (Coulomb geometry at many contact normals, each
+  check shown to reject planted mistakes);
 * colouring: node-disjoint colours, product == oracle;
 * LDLT usage example of Ldlt.h: x = (-3, 3, 1).
 """
@@ -186,6 +188,106 @@ def test_sphere_collision_cases():
     po.lib().xo_collide_f64(p, t, o, x)
     rel = (x - o) - np.array([1.0, 0.0, 0.0]) * DT
     assert np.abs(rel - n * rel.dot(n)).max() <= 1e-9
+
+
+def sphere_contact_error(c, r, mu, sv, dt, o, x_in, x_out):
+    """How far x_out is from the Coulomb contact of x_in with a moving sphere,
+    stated by its geometry (0 when right): the node ends on the tangent plane
+    at the closest surface point c + r n; the tangential slip of the substep
+    relative to the sphere surface, s = tangential part of (x - o) - sv dt,
+    is cancelled when |s| <= mu * depth (static) and otherwise shrinks by
+    exactly mu * depth along its own direction (kinetic)."""
+    d = x_in - c
+    n = d / np.linalg.norm(d)
+    depth = r - np.linalg.norm(d)
+    surf = c + r * n
+
+    def slip(x):
+        rel = (x - o) - sv * dt
+        return rel - n * rel.dot(n)
+
+    s_in, s_out = slip(surf), slip(x_out)
+    err = abs((x_out - c).dot(n) - r)  # normal coordinate: on the tangent plane
+    lim = mu * depth
+    if np.linalg.norm(s_in) <= lim:
+        err = max(err, np.linalg.norm(s_out))
+    else:
+        want = s_in * (1.0 - lim / np.linalg.norm(s_in))
+        err = max(err, np.linalg.norm(s_out - want))
+    return err
+
+
+def sphere_mutant(kind, c, r, mu, sv, dt, o, x):
+    """Plausible mistakes in a sphere contact pass (numpy, for the checks
+    above to reject): the sphere's own motion with the wrong sign, the normal
+    part left in the friction correction, the depth taken from the squared
+    distance, static friction that pins the node to o (right only for a
+    sphere at rest)."""
+    d = x - c
+    dist = np.linalg.norm(d)
+    n = d / dist
+    depth = (r * r - dist * dist) if kind == "depth_squared" else (r - dist)
+    xp = x + n * depth
+    sgn = -1.0 if kind == "sv_sign" else 1.0
+    rel = (xp - o) - sgn * sv * dt
+    rt = rel if kind == "normal_kept" else rel - n * rel.dot(n)
+    lim = mu * (r - dist)
+    t = np.linalg.norm(rt)
+    if t <= lim:
+        return o.copy() if kind == "static_to_o" else xp - rt
+    return xp - rt * (lim / t)
+
+
+def _sphere_case(rng, c, r, sv, dt, kinetic):
+    """A node inside the sphere near its surface with a previous position o
+    whose slip relative to the surface is well above (kinetic) or below
+    (static) mu * depth for mu = 0.5."""
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    depth = rng.uniform(2e-5, 2e-4)
+    x = c + n * (r - depth)
+    tang = rng.normal(size=3)
+    tang -= n * tang.dot(n)
+    tang /= np.linalg.norm(tang)
+    slip = 0.5 * depth * (rng.uniform(1.5, 10.0) if kinetic else rng.uniform(0.05, 0.8))
+    # o such that the slip of the pushed-out position relative to the sphere
+    # is `slip` along tang, with a normal part of 50-100 um either way
+    o = (c + n * r) - sv * dt - tang * slip + n * rng.choice([-1.0, 1.0]) * rng.uniform(5e-5, 1e-4)
+    return o, x
+
+
+@pytest.mark.parametrize("kinetic", [True, False])
+def test_sphere_friction_closed_form(kinetic):
+    """The sphere branch of the contact pass against the Coulomb geometry,
+    at the 96x24x24 beam's regime (0.3 m sphere at 1 m/s, dt = 1/1800, so
+    the sphere moves 0.56 mm per substep while contacts are a fraction of
+    that deep), over many contact normals; FP64 variant to 1e-12 and the
+    float element-math variant (the one the GPU reproduces) to float
+    rounding."""
+    dt = float(np.float32(1.0) / np.float32(1800.0))
+    # parameters the oracle holds as float, so both sides see the same values
+    r, mu, sv = float(np.float32(0.3)), 0.5, np.array([1.0, 0.0, 0.0])
+    t = 0.125
+    c0 = np.array([0.25, -0.125, 0.0625])
+    p = po.make_params(dt, collide=2, friction=mu, sphere_c0=tuple(c0), sphere_v=tuple(sv), sphere_r=r)
+    cf = c0 + sv * t  # exact in float and double
+    rng = np.random.default_rng(40 + kinetic)
+    for _ in range(50):
+        o, x_in = _sphere_case(rng, cf, r, sv, dt, kinetic)
+        x64 = x_in.copy()
+        po.lib().xo_collide_f64(p, t, o, x64)
+        assert sphere_contact_error(cf, r, mu, sv, dt, o, x_in, x64) <= 1e-12
+        x32 = x_in.copy()
+        po.lib().xo_collide_ref64(p, t, o, x32)
+        # float32 REAL: centre, normal and slip carry ~1e-7 relative rounding of 0.3 m
+        assert sphere_contact_error(cf, r, mu, sv, dt, o, x_in, x32) <= 2e-7
+        # the check is sharp enough: each planted mistake fails it by far more
+        for kind in ("sv_sign", "normal_kept", "depth_squared") + (() if kinetic else ("static_to_o",)):
+            bad = sphere_mutant(kind, cf, r, mu, sv, dt, o, x_in)
+            assert sphere_contact_error(cf, r, mu, sv, dt, o, x_in, bad) > 1e-6, kind
+        # and the numpy restatement without a mistake passes it
+        good = sphere_mutant("none", cf, r, mu, sv, dt, o, x_in)
+        assert sphere_contact_error(cf, r, mu, sv, dt, o, x_in, good) <= 1e-12
 
 
 @pytest.mark.parametrize("order", [xf.COLOUR_NATURAL, xf.COLOUR_MINSTD])

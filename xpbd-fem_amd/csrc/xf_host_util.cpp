@@ -754,6 +754,12 @@ bool ReadRows(const char* path, std::vector<std::vector<double>>* rows, std::str
     return ok;
 }
 
+// A header count: finite, integral and in [0, 2^31) (casts of anything else
+// to an integer are undefined, and counts near 2^64 would wrap in n + 1).
+bool CountOk(double v) {
+    return std::isfinite(v) && v >= 0.0 && v < 2147483648.0 && v == std::floor(v);
+}
+
 }  // namespace
 
 bool ReadTetgen(const char* node_path, const char* ele_path, bool orient, std::vector<float>* xyz,
@@ -762,6 +768,10 @@ bool ReadTetgen(const char* node_path, const char* ele_path, bool orient, std::v
     if (!ReadRows(node_path, &nr, err) || !ReadRows(ele_path, &er, err)) return false;
     if (nr.empty() || nr[0].size() < 2 || nr[0][1] != 3.0) {
         *err = "node file: header must read <count> 3 [attributes] [markers]";
+        return false;
+    }
+    if (!CountOk(nr[0][0])) {
+        *err = "node file: the node count must be a whole number in [0, 2^31)";
         return false;
     }
     size_t n = (size_t)nr[0][0];
@@ -791,6 +801,10 @@ bool ReadTetgen(const char* node_path, const char* ele_path, bool orient, std::v
         *err = "ele file: only linear tets (4 nodes per tet) are supported";
         return false;
     }
+    if (!CountOk(er[0][0])) {
+        *err = "ele file: the tet count must be a whole number in [0, 2^31)";
+        return false;
+    }
     size_t t = (size_t)er[0][0];
     if (er.size() < t + 1) {
         *err = "ele file: fewer tet lines than the header count";
@@ -805,7 +819,7 @@ bool ReadTetgen(const char* node_path, const char* ele_path, bool orient, std::v
         }
         for (int c = 0; c < 4; c++) {
             double v = r[1 + c] - base;
-            if (v < 0.0 || v >= (double)n || v != (double)(uint32_t)v) {
+            if (!CountOk(v) || v >= (double)n) {
                 *err = "ele file: node index out of range";
                 return false;
             }

@@ -292,8 +292,15 @@ struct xf_scene {
 
 namespace {
 
+// Called when the step call shape changes: a launch of the old executable
+// graph may still be running on the stream (xf_scene_step returns once the
+// work is enqueued), so the stream drains before the graph is destroyed.
 void DropGraph(xf_scene* s) {
-    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph_exec) {
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
+        (void)hipGraphExecDestroy(s->graph_exec);
+    }
     if (s->graph) (void)hipGraphDestroy(s->graph);
     s->graph_exec = nullptr;
     s->graph = nullptr;
@@ -1694,6 +1701,29 @@ XF_API int xf_scene_profile(xf_scene* s, uint32_t substeps, float dt, float* ms_
     if (ms_node) *ms_node = tn;
     if (launches_project) *launches_project = nproj;
     return XF_OK;
+}
+
+XF_API int xf_scene_time_steps(xf_scene* s, uint32_t calls, uint32_t substeps, float dt, float* ms) {
+    if (!s || !ms) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    xf_ctx* ctx = s->ctx;
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    XF_HIP(ctx, hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    int rc = XF_OK;
+    if (he == hipSuccess) he = hipEventRecord(e0, ctx->stream);
+    for (uint32_t k = 0; k < calls && he == hipSuccess && !rc; k++) rc = xf_scene_step(s, substeps, dt);
+    if (he == hipSuccess && !rc) he = hipEventRecord(e1, ctx->stream);
+    if (he == hipSuccess && !rc) he = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (he == hipSuccess && !rc) he = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (he != hipSuccess) return Fail(ctx, XF_E_HIP, "time_steps: %s", hipGetErrorString(he));
+    *ms = t;
+    return CheckBarrier(s);
 }
 
 XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,

@@ -149,6 +149,7 @@ def lib():
         "xf_scene_profile": ([vp, C.c_uint32, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
                               C.POINTER(C.c_uint32)], C.c_int),
         "xf_scene_count_nonfinite": ([vp, C.POINTER(C.c_uint32)], C.c_int),
+        "xf_scene_time_steps": ([vp, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float)], C.c_int),
         "xf_scene_adjoint": ([vp, C.c_uint32, C.c_float, C.c_uint32, _u32p, vp, C.c_uint32, C.POINTER(C.c_double),
                               vp, vp], C.c_int),
         "xf_scene_autotune": ([vp, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(SceneOpts), C.POINTER(C.c_float)],
@@ -180,7 +181,7 @@ def exported_symbols():
         "xf_scene_get_elements", "xf_scene_profile", "xf_scene_count_nonfinite", "xf_mesh_tet_block", "xf_minstd_order", "xf_normal_fill", "xf_ldlt_solve",
         "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
         "xf_scene_get_tick", "xf_scene_set_tick", "xf_mesh_read_tetgen", "xf_mesh_tets_from_stream", "xf_mesh_error",
-        "xf_mesh_rings", "xf_mesh_sweep_blocks",
+        "xf_mesh_rings", "xf_mesh_sweep_blocks", "xf_scene_time_steps",
     ]
 
 
@@ -526,6 +527,13 @@ class Scene:
         self.ctx.check(lib().xf_scene_profile(self.h, substeps, dt, C.byref(mp), C.byref(mn), C.byref(nl)),
                        "xf_scene_profile")
         return mp.value, mn.value, nl.value
+
+    def time_steps(self, calls, substeps, dt):
+        """Device ms of `calls` step calls between two HIP events on the
+        library stream (synchronous)."""
+        ms = C.c_float()
+        self.ctx.check(lib().xf_scene_time_steps(self.h, calls, substeps, dt, C.byref(ms)), "xf_scene_time_steps")
+        return ms.value
 
     def autotune(self, substeps, dt, reps=3):
         """Picks the fastest launch shape; returns (launch_mode, size, ms per call),
