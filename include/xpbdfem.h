@@ -343,6 +343,31 @@ XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint3
 XF_API int xf_scene_adjoint(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
                             const double* targets, uint32_t window, double* loss, double* grad_log_mu,
                             double* body_loss);
+/* xf_scene_adjoint without allocating: no hipMalloc, hipFree or other
+ * allocating call. No counterpart in the reference; for callers that cannot
+ * allocate (tight fitting loops, harnesses that forbid device allocation
+ * during a run). Same arguments, results and scope as xf_scene_adjoint
+ * (loss, gradients and body_loss equal it bit for bit), with two
+ * differences: the forward rollout runs as eager launches (no hipGraph is
+ * instantiated), and the device buffers of the call -- checkpoints, the
+ * window's sweep tape, adjoint state, loss staging -- come from a fixed
+ * 32 MiB scratch array that each device holds for the life of the process:
+ *   2 N 32 ceil(substeps / window) + 96 T min(window, substeps)
+ *   + 64 N min(window, substeps) + 160 N + 20 T + 88 N n_loss + 8 N
+ *   + 8 B (n_loss + 1) + 8 bytes (N nodes, T tets, B bodies; each of the 19
+ *   regions rounded up to 256 bytes)
+ * must fit, else XF_E_INVALID (the message gives the bytes needed); the
+ * 16x4x4 beam (425 nodes, 1,536 tets) over 600 substeps with window 20 needs
+ * about 4.3 MiB. XF_E_STATE when the scene's per-tet material storage would
+ * have to be allocated first (a multi-body or compressible scene whose
+ * settings changed since its last step): run xf_scene_step or
+ * xf_scene_adjoint once. Every call on a device shares that device's
+ * scratch and leaves its contents behind (what must start at zero is
+ * cleared by each call); calls that overlap on one device, from different
+ * contexts or threads, race. */
+XF_API int xf_scene_adjoint_noalloc(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss,
+                                    const uint32_t* loss_at, const double* targets, uint32_t window, double* loss,
+                                    double* grad_log_mu, double* body_loss);
 /* Counts non-finite node positions (0 = all finite). */
 XF_API int xf_scene_count_nonfinite(xf_scene* scene, uint32_t* count);
 

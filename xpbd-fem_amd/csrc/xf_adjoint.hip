@@ -438,6 +438,82 @@ __global__ __launch_bounds__(256) void xf_zero_d4(uint32_t n, double4* __restric
     if (i < n) a[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
+// Zero-fill of n 32-bit words (gradient accumulators; a kernel rather than
+// a memset so that the caller's stream may be under graph capture).
+__global__ __launch_bounds__(256) void xf_zero_u32(uint32_t n, uint32_t* __restrict__ a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+
+// Loss targets as the caller passes them (3 doubles a node, n entries over
+// all loss substeps) to the double4 layout of the node state.
+__global__ __launch_bounds__(256) void xf_expand_targets(uint32_t n, const double* __restrict__ raw,
+                                                         double4* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_double4(raw[3 * (size_t)i], raw[3 * (size_t)i + 1], raw[3 * (size_t)i + 2], 0.0);
+}
+
+// Per-node weight of the loss gradient, 2 / (nodes of the node's body); the
+// body id is O.w >> 1.
+__global__ __launch_bounds__(256) void xf_loss_weights(uint32_t n, const double4* __restrict__ O,
+                                                       const XfBodyParams* __restrict__ bodies,
+                                                       double* __restrict__ lw) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t body = (uint32_t)O[i].w >> 1;
+    lw[i] = 2.0 / (double)bodies[body].n_nodes;
+}
+
+// Squared-error sum of one body at one loss substep: thread (j, b) adds
+// |x - target|^2 over the body's nodes in node order, component by
+// component, into e[j * nb + b] (one thread per term keeps the summation
+// order of a serial loop).
+__global__ __launch_bounds__(64) void xf_body_sq_err(uint32_t n_loss, uint32_t nb, uint32_t n,
+                                                     const double4* __restrict__ xloss,
+                                                     const double4* __restrict__ tgt,
+                                                     const XfBodyParams* __restrict__ bodies,
+                                                     double* __restrict__ e) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_loss * nb) return;
+    uint32_t j = k / nb, b = k % nb;
+    const XfBodyParams& bp = bodies[b];
+    const double4* x = xloss + (size_t)n * j + bp.node_base;
+    const double4* t = tgt + (size_t)n * j + bp.node_base;
+    double acc = 0.0;
+    for (uint32_t i = 0; i < bp.n_nodes; i++) {
+        double d0 = x[i].x - t[i].x, d1 = x[i].y - t[i].y, d2 = x[i].z - t[i].z;
+        acc += d0 * d0;
+        acc += d1 * d1;
+        acc += d2 * d2;
+    }
+    e[k] = acc;
+}
+
+// L = sum over loss substeps, then bodies, of e / n_b, and each body's share,
+// in that order by one thread: out[0] = L, out[1 + b] = body b's terms.
+__global__ __launch_bounds__(64) void xf_loss_total(uint32_t n_loss, uint32_t nb, const double* __restrict__ e,
+                                                    const XfBodyParams* __restrict__ bodies,
+                                                    double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (uint32_t b = 0; b < nb; b++) out[1 + b] = 0.0;
+    double L = 0.0;
+    for (uint32_t j = 0; j < n_loss; j++)
+        for (uint32_t b = 0; b < nb; b++) {
+            double v = e[(size_t)j * nb + b] / (double)bodies[b].n_nodes;
+            L += v;
+            out[1 + b] += v;
+        }
+    out[0] = L;
+}
+
+// Gradient in the caller's order: schedule position p holds the tet whose
+// output slot is tout[p]; dL/dlog mu = -glog.
+__global__ __launch_bounds__(256) void xf_grad_out(uint32_t t, const double* __restrict__ glog,
+                                                   const uint32_t* __restrict__ tout, double* __restrict__ g) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < t) g[tout[p]] = -glog[p];
+}
+
 // Copy of two double4 arrays of n entries (the replay's per-substep
 // snapshot of X and O) in one launch.
 __global__ __launch_bounds__(256) void xf_copy2_d4(uint32_t n, double4* __restrict__ d0,
@@ -501,5 +577,42 @@ extern "C" hipError_t xf_launch_copy2_d4(hipStream_t s, uint32_t n, double4* d0,
 extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(xf_zero_d4, dim3((n + 255) / 256), dim3(256), 0, s, n, a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_zero_u32(hipStream_t s, uint32_t n, uint32_t* a) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, n, a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_expand_targets(hipStream_t s, uint32_t n, const double* raw, double4* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_expand_targets, dim3((n + 255) / 256), dim3(256), 0, s, n, raw, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_loss_weights(hipStream_t s, uint32_t n, const double4* O, const XfBodyParams* bodies,
+                                             double* lw) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_loss_weights, dim3((n + 255) / 256), dim3(256), 0, s, n, O, bodies, lw);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_loss(hipStream_t s, uint32_t n_loss, uint32_t nb, uint32_t n, const double4* xloss,
+                                     const double4* tgt, const XfBodyParams* bodies, double* e, double* out) {
+    uint32_t terms = n_loss * nb;
+    if (terms == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_body_sq_err, dim3((terms + 63) / 64), dim3(64), 0, s, n_loss, nb, n, xloss, tgt, bodies, e);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(xf_loss_total, dim3(1), dim3(64), 0, s, n_loss, nb, e, bodies, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_grad_out(hipStream_t s, uint32_t t, const double* glog, const uint32_t* tout,
+                                         double* g) {
+    if (t == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_grad_out, dim3((t + 255) / 256), dim3(256), 0, s, t, glog, tout, g);
     return hipGetLastError();
 }

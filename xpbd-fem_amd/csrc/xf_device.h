@@ -54,6 +54,9 @@ struct XfBodyParams {
     uint32_t n_rings;        // rings of the body (amortized range of the ring damping pass)
     float compliance;        // Settings::compliance
     float poisson;           // Settings::poissonsRatio
+    // Node range of the body in the scene (adjoint loss terms)
+    uint32_t node_base;
+    uint32_t n_nodes;
 };
 
 // The scene clock buffer: [0] simulation time (float), [1] tick (uint32,

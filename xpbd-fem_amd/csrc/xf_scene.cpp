@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -80,6 +81,14 @@ extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const 
                                                const double4* Ob, const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog);
 extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a);
+extern "C" hipError_t xf_launch_zero_u32(hipStream_t s, uint32_t n, uint32_t* a);
+extern "C" hipError_t xf_launch_expand_targets(hipStream_t s, uint32_t n, const double* raw, double4* out);
+extern "C" hipError_t xf_launch_loss_weights(hipStream_t s, uint32_t n, const double4* O, const XfBodyParams* bodies,
+                                             double* lw);
+extern "C" hipError_t xf_launch_loss(hipStream_t s, uint32_t n_loss, uint32_t nb, uint32_t n, const double4* xloss,
+                                     const double4* tgt, const XfBodyParams* bodies, double* e, double* out);
+extern "C" hipError_t xf_launch_grad_out(hipStream_t s, uint32_t t, const double* glog, const uint32_t* tout,
+                                         double* g);
 extern "C" hipError_t xf_launch_copy2_d4(hipStream_t s, uint32_t n, double4* d0, const double4* s0, double4* d1,
                                          const double4* s1);
 extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t blocks,
@@ -93,11 +102,39 @@ extern "C" hipError_t xf_launch_frame(hipStream_t s, int energy, bool sim, bool 
 struct xf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    unsigned char* scratch = nullptr;  // this device's copy of xf_adjoint_scratch (device address)
     std::string err;
     std::vector<xf_scene*> scenes;  // open scenes, destroyed with the context
 };
 
+// Device scratch of xf_scene_adjoint_noalloc: checkpoints, the window's
+// sweep tape, adjoint state, loss staging. Each device holds its own copy
+// for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress reports for the device (the name itself is a
+// host-side stand-in). Calls that overlap on one device share it and race.
+constexpr size_t kAdjointScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char xf_adjoint_scratch[kAdjointScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// Resolves this device's scratch address once per device (the device must
+// be current).
+hipError_t ResolveScratch(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(xf_adjoint_scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
 
 int Fail(xf_ctx* ctx, int code, const char* fmt, ...) {
     char buf[512];
@@ -258,12 +295,14 @@ struct xf_scene {
     uint32_t blk_fmax = 0;
     uint32_t* blk_fin = nullptr;
     uint2* tloc = nullptr;
+    uint32_t* tout = nullptr;  // per schedule position: the tet's slot in body-concatenated, body-local order
     uint64_t device_bytes = 0;
     int n_cu = 256;
     // host mirrors
     std::vector<float4> h_tg3, h_taux;
     std::vector<double> h_w;       // per node inverse mass
     std::vector<double> h_oflags;  // per node O.w: (body << 1) | pinned
+    std::vector<XfBodyParams> h_bp;  // body table staging (sized at finalize)
     bool material_dirty = true, params_dirty = true;
     bool aux = false;
     uint32_t vpasses = 0;  // max volume passes over bodies (taux is then always present)
@@ -406,7 +445,8 @@ int UploadMaterial(xf_scene* s) {
 
 int UploadParams(xf_scene* s, float dt) {
     xf_ctx* ctx = s->ctx;
-    std::vector<XfBodyParams> hp(s->bodies.size());
+    std::vector<XfBodyParams>& hp = s->h_bp;
+    hp.resize(s->bodies.size());
     for (size_t i = 0; i < s->bodies.size(); i++) {
         const xf_settings& st = s->bodies[i].settings;
         XfBodyParams& p = hp[i];
@@ -436,6 +476,8 @@ int UploadParams(xf_scene* s, float dt) {
         p.n_rings = s->bodies[i].rings.n;
         p.compliance = st.compliance;
         p.poisson = st.poissons_ratio;
+        p.node_base = s->bodies[i].node_base;
+        p.n_nodes = s->bodies[i].n;
     }
     XF_HIP(ctx, hipMemcpyAsync(s->bp, hp.data(), sizeof(XfBodyParams) * hp.size(), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -851,6 +893,11 @@ XF_API int xf_ctx_create(int device, xf_ctx** out) {
         delete ctx;
         return XF_E_HIP;
     }
+    if (ResolveScratch(device, &ctx->scratch) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return XF_E_HIP;
+    }
     *out = ctx;
     return XF_OK;
 }
@@ -888,7 +935,7 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
                     s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet,
-                    s->blk_rec, s->blk_node, s->tloc, s->blk_fin};
+                    s->blk_rec, s->blk_node, s->tloc, s->blk_fin, s->tout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -1384,7 +1431,8 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         if ((rc = DevAlloc(ctx, &s->tg[k], s->T, &s->device_bytes))) return rc;
     if ((rc = DevAlloc(ctx, &s->colour_begin_d, s->ncol + 1, &s->device_bytes)) ||
         (rc = DevAlloc(ctx, &s->arrive, 1, &s->device_bytes)) ||
-        (rc = DevAlloc(ctx, &s->barrier_err, 1, &s->device_bytes)))
+        (rc = DevAlloc(ctx, &s->barrier_err, 1, &s->device_bytes)) ||
+        (rc = DevAlloc(ctx, &s->tout, s->T, &s->device_bytes)))
         return rc;
     if (s->opts.sweep_blocks && ((rc = DevAlloc(ctx, &s->blk_rec, h_brec.size(), &s->device_bytes)) ||
                                  (rc = DevAlloc(ctx, &s->blk_node, h_bnode.size(), &s->device_bytes)) ||
@@ -1397,6 +1445,16 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     XF_HIP(ctx, hipMemcpyAsync(s->O, hO.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->V, hV.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tidx, h_tidx.data(), sizeof(uint4) * s->T, hipMemcpyHostToDevice, st));
+    s->h_bp.resize(s->bodies.size());
+    std::vector<uint32_t> h_tout(s->T);
+    {
+        uint32_t off = 0;
+        for (const Body& b : s->bodies) {
+            for (uint32_t t = 0; t < b.t; t++) h_tout[b.sched_pos[t]] = off + t;
+            off += b.t;
+        }
+    }
+    XF_HIP(ctx, hipMemcpyAsync(s->tout, h_tout.data(), sizeof(uint32_t) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[0], h_tg0.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[1], h_tg1.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[2], h_tg2.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
@@ -1840,10 +1898,23 @@ int Reserve(xf_scene* s, xf_scene::AdjBuf* b, size_t bytes) {
 }
 }  // namespace
 
-XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
-                            const double* targets, uint32_t window, double* loss, double* grad_log_mu,
-                            double* body_loss) {
-    if (!s) return XF_E_INVALID;
+namespace {
+
+// Device buffers of one adjoint call.
+struct AdjBufs {
+    double4 *ckx = nullptr, *ckv = nullptr;  // window-start checkpoints of X and V
+    double* tape = nullptr;                  // 12 doubles per tet and substep of a window
+    double4 *xs = nullptr, *oo = nullptr;    // X after the sweep and O, per substep of a window
+    double4 *Xb = nullptr, *Vb = nullptr, *Ob = nullptr;
+    float* gacc = nullptr;                   // per-tet gradient within a window
+    double* glog = nullptr;                  // per-tet gradient over windows
+    double4 *xloss = nullptr, *tgt = nullptr;
+    double* lw = nullptr;                    // per-node loss-gradient weight
+    double4 *save_x = nullptr, *save_v = nullptr;
+};
+
+int AdjointCheck(xf_scene* s, uint32_t substeps, uint32_t n_loss, const uint32_t* loss_at, const double* targets,
+                 uint32_t window) {
     xf_ctx* ctx = s->ctx;
     if (!s->finalized) return Fail(ctx, XF_E_STATE, "scene is not finalized");
     if (substeps == 0 || window == 0) return Fail(ctx, XF_E_INVALID, "substeps and window must be positive");
@@ -1859,10 +1930,124 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
                         "adjoint covers simultaneous MixedSel/YeohSkinFast with pins and ground contact "
                         "(no volume passes, no damping)");
     }
+    return XF_OK;
+}
+
+// Forward rollout with checkpoints at window starts and X at the loss
+// substeps; graph: through xf_scene_step (its hipGraph cache), else eager
+// launches (no graph instantiation). Leaves the end state saved in
+// save_x / save_v and the end clock in *clock_end.
+int AdjointForward(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, uint32_t n_loss,
+                   const uint32_t* loss_at, uint32_t window, const AdjBufs& b, bool graph, float* clock_end) {
+    xf_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const uint32_t N = s->N;
+    const size_t n4 = sizeof(double4) * N;
+    int rc;
+    uint32_t k = 0, j = 0;
+    while (k < substeps) {
+        if (k % window == 0) {
+            XF_HIP(ctx, hipMemcpyAsync(b.ckx + (size_t)N * (k / window), s->X, n4, hipMemcpyDeviceToDevice, st));
+            XF_HIP(ctx, hipMemcpyAsync(b.ckv + (size_t)N * (k / window), s->V, n4, hipMemcpyDeviceToDevice, st));
+        }
+        uint32_t next = std::min(substeps, (k / window + 1) * window);
+        if (j < n_loss) next = std::min(next, loss_at[j]);
+        if (graph) {
+            if ((rc = xf_scene_step(s, next - k, dt))) return rc;
+        } else {
+            if ((rc = Enqueue(s, next - k, dt, energy, sim, nullptr))) return rc;
+            for (uint32_t q = k; q < next; q++) s->time_host = s->time_host + dt;
+            s->tick_host += next - k;
+        }
+        k = next;
+        if (j < n_loss && loss_at[j] == k) {
+            XF_HIP(ctx, hipMemcpyAsync(b.xloss + (size_t)N * j, s->X, n4, hipMemcpyDeviceToDevice, st));
+            j++;
+        }
+    }
+    XF_HIP(ctx, hipMemcpyAsync(b.save_x, s->X, n4, hipMemcpyDeviceToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(b.save_v, s->V, n4, hipMemcpyDeviceToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(clock_end, s->clock, sizeof(float), hipMemcpyDeviceToHost, st));
+    XF_HIP(ctx, hipStreamSynchronize(st));
+    return CheckBarrier(s);
+}
+
+// Reverse pass, window by window from the end: each window is replayed from
+// its checkpoint (recording the tape and node states), then swept back.
+// Leaves the scene at the end of the forward rollout and glog holding
+// -dL/dlog mu per schedule position.
+int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_t n_loss, const uint32_t* loss_at,
+                   uint32_t window, const AdjBufs& b, const float* clock_end) {
+    xf_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const uint32_t N = s->N, T = s->T;
+    const size_t n4 = sizeof(double4) * N;
+    const uint32_t nwin = (substeps + window - 1) / window;
+    XF_HIP(ctx, xf_launch_zero_d4(st, N, b.Xb));
+    XF_HIP(ctx, xf_launch_zero_d4(st, N, b.Vb));
+    XF_HIP(ctx, xf_launch_zero_u32(st, T, reinterpret_cast<uint32_t*>(b.gacc)));
+    XF_HIP(ctx, xf_launch_zero_u32(st, 2 * T, reinterpret_cast<uint32_t*>(b.glog)));
+    uint32_t jl = n_loss;
+    for (uint32_t w = nwin; w-- > 0;) {
+        const uint32_t k0 = w * window, k1 = std::min(substeps, k0 + window);
+        XF_HIP(ctx, hipMemcpyAsync(s->X, b.ckx + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
+        XF_HIP(ctx, hipMemcpyAsync(s->V, b.ckv + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
+        // replay of the window, recording the sweep tape and the node states
+        XF_HIP(ctx, xf_launch_predict(st, N, s->X, s->O, s->V, s->bp));
+        for (uint32_t kk = k0; kk < k1; kk++) {
+            double* tp = b.tape + 12 * (size_t)T * (kk - k0);
+            for (uint32_t c = 0; c < s->ncol; c++) {
+                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                XF_HIP(ctx, xf_launch_project_record(st, energy, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1],
+                                                     s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp, s->comp1_u, s->a_u,
+                                                     dt, nullptr));
+            }
+            XF_HIP(ctx, xf_launch_copy2_d4(st, N, b.xs + (size_t)N * (kk - k0), s->X, b.oo + (size_t)N * (kk - k0),
+                                           s->O));
+            XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, s->bodies.size() == 1, N, s->X, s->O, s->V, s->bp,
+                                         s->clock));
+        }
+        for (uint32_t kk = k1; kk-- > k0;) {
+            const double4* xl = nullptr;
+            const double4* tg = nullptr;
+            if (jl > 0 && loss_at[jl - 1] == kk + 1) {
+                jl--;
+                xl = b.xloss + (size_t)N * jl;
+                tg = b.tgt + (size_t)N * jl;
+            }
+            XF_HIP(ctx, xf_launch_adj_node_end(st, N, s->O, b.xs + (size_t)N * (kk - k0), b.oo + (size_t)N * (kk - k0),
+                                               b.Xb, b.Vb, b.Ob, s->bp, xl, tg, b.lw));
+            const double* tp = b.tape + 12 * (size_t)T * (kk - k0);
+            for (uint32_t c = s->ncol; c-- > 0;) {
+                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                XF_HIP(ctx, xf_launch_project_reverse(st, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                                                      s->tg[3], s->taux, tp, s->X, b.Xb, b.gacc, s->bp, s->comp1_u,
+                                                      s->a_u, dt));
+            }
+            XF_HIP(ctx, xf_launch_adj_node_start(st, N, s->O, b.Xb, b.Vb, b.Ob, s->bp));
+        }
+        XF_HIP(ctx, xf_launch_adj_flush(st, T, b.gacc, b.glog));
+    }
+    // Leave the scene at the end of the forward rollout.
+    XF_HIP(ctx, hipMemcpyAsync(s->X, b.save_x, n4, hipMemcpyDeviceToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->V, b.save_v, n4, hipMemcpyDeviceToDevice, st));
+    XF_HIP(ctx, hipMemcpyAsync(s->clock, clock_end, sizeof(float), hipMemcpyHostToDevice, st));
+    return XF_OK;
+}
+
+}  // namespace
+
+XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
+                            const double* targets, uint32_t window, double* loss, double* grad_log_mu,
+                            double* body_loss) {
+    if (!s) return XF_E_INVALID;
+    xf_ctx* ctx = s->ctx;
+    int rc = AdjointCheck(s, substeps, n_loss, loss_at, targets, window);
+    if (rc) return rc;
     XF_HIP(ctx, hipSetDevice(ctx->device));
     int energy;
     bool sim;
-    int rc = Prepare(s, dt, &energy, &sim);
+    rc = Prepare(s, dt, &energy, &sim);
     if (rc) return rc;
     const uint32_t N = s->N, T = s->T;
     const uint32_t nwin = (substeps + window - 1) / window;
@@ -1877,19 +2062,22 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
         (rc = Reserve(s, &s->adj_lw, sizeof(double) * N)) || (rc = Reserve(s, &s->adj_save_x, n4)) ||
         (rc = Reserve(s, &s->adj_save_v, n4)))
         return rc;
-    double4* ckx = (double4*)s->adj_ck_x.p;
-    double4* ckv = (double4*)s->adj_ck_v.p;
-    double* tape = (double*)s->adj_tape.p;
-    double4* xs = (double4*)s->adj_xs.p;
-    double4* oo = (double4*)s->adj_o.p;
-    double4* Xb = (double4*)s->adj_xb.p;
-    double4* Vb = (double4*)s->adj_vb.p;
-    double4* Ob = (double4*)s->adj_ob.p;
-    float* gacc = (float*)s->adj_gacc.p;
-    double* glog = (double*)s->adj_glog.p;
-    double4* xloss = (double4*)s->adj_xloss.p;
-    double4* tgt = (double4*)s->adj_tgt.p;
-    double* lw = (double*)s->adj_lw.p;
+    AdjBufs b;
+    b.ckx = (double4*)s->adj_ck_x.p;
+    b.ckv = (double4*)s->adj_ck_v.p;
+    b.tape = (double*)s->adj_tape.p;
+    b.xs = (double4*)s->adj_xs.p;
+    b.oo = (double4*)s->adj_o.p;
+    b.Xb = (double4*)s->adj_xb.p;
+    b.Vb = (double4*)s->adj_vb.p;
+    b.Ob = (double4*)s->adj_ob.p;
+    b.gacc = (float*)s->adj_gacc.p;
+    b.glog = (double*)s->adj_glog.p;
+    b.xloss = (double4*)s->adj_xloss.p;
+    b.tgt = (double4*)s->adj_tgt.p;
+    b.lw = (double*)s->adj_lw.p;
+    b.save_x = (double4*)s->adj_save_x.p;
+    b.save_v = (double4*)s->adj_save_v.p;
     hipStream_t st = ctx->stream;
 
     // Targets (body-concatenated xyz) and per-node loss-gradient weights 2/n_b.
@@ -1898,120 +2086,147 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
         for (size_t k = 0; k < ht.size(); k++)
             ht[k] = make_double4(targets[3 * k], targets[3 * k + 1], targets[3 * k + 2], 0.0);
         std::vector<double> hw(N);
-        for (const Body& b : s->bodies)
-            for (uint32_t i = 0; i < b.n; i++) hw[b.node_base + i] = 2.0 / (double)b.n;
-        XF_HIP(ctx, hipMemcpyAsync(tgt, ht.data(), ht.size() * sizeof(double4), hipMemcpyHostToDevice, st));
-        XF_HIP(ctx, hipMemcpyAsync(lw, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        for (const Body& bd : s->bodies)
+            for (uint32_t i = 0; i < bd.n; i++) hw[bd.node_base + i] = 2.0 / (double)bd.n;
+        XF_HIP(ctx, hipMemcpyAsync(b.tgt, ht.data(), ht.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+        XF_HIP(ctx, hipMemcpyAsync(b.lw, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice, st));
         XF_HIP(ctx, hipStreamSynchronize(st));
     }
 
-    // Forward pass with checkpoints at window starts and states at loss substeps.
-    uint32_t k = 0, j = 0;
-    while (k < substeps) {
-        if (k % window == 0) {
-            XF_HIP(ctx, hipMemcpyAsync(ckx + (size_t)N * (k / window), s->X, n4, hipMemcpyDeviceToDevice, st));
-            XF_HIP(ctx, hipMemcpyAsync(ckv + (size_t)N * (k / window), s->V, n4, hipMemcpyDeviceToDevice, st));
-        }
-        uint32_t next = std::min(substeps, (k / window + 1) * window);
-        if (j < n_loss) next = std::min(next, loss_at[j]);
-        if ((rc = xf_scene_step(s, next - k, dt))) return rc;
-        k = next;
-        if (j < n_loss && loss_at[j] == k) {
-            XF_HIP(ctx, hipMemcpyAsync(xloss + (size_t)N * j, s->X, n4, hipMemcpyDeviceToDevice, st));
-            j++;
-        }
-    }
-    XF_HIP(ctx, hipMemcpyAsync(s->adj_save_x.p, s->X, n4, hipMemcpyDeviceToDevice, st));
-    XF_HIP(ctx, hipMemcpyAsync(s->adj_save_v.p, s->V, n4, hipMemcpyDeviceToDevice, st));
     float clock_end = 0.0f;
-    XF_HIP(ctx, hipMemcpyAsync(&clock_end, s->clock, sizeof(float), hipMemcpyDeviceToHost, st));
-    XF_HIP(ctx, hipStreamSynchronize(st));
-    if ((rc = CheckBarrier(s))) return rc;
+    if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, true, &clock_end))) return rc;
 
     // Loss: sum over loss substeps and bodies of (1/n_b) sum |x - target|^2.
     {
         std::vector<double4> hx((size_t)N * n_loss);
-        XF_HIP(ctx, hipMemcpy(hx.data(), xloss, hx.size() * sizeof(double4), hipMemcpyDeviceToHost));
+        XF_HIP(ctx, hipMemcpy(hx.data(), b.xloss, hx.size() * sizeof(double4), hipMemcpyDeviceToHost));
         double L = 0.0;
         if (body_loss)
             for (size_t bi = 0; bi < s->bodies.size(); bi++) body_loss[bi] = 0.0;
         for (uint32_t jj = 0; jj < n_loss; jj++)
             for (size_t bi = 0; bi < s->bodies.size(); bi++) {
-                const Body& b = s->bodies[bi];
+                const Body& bd = s->bodies[bi];
                 double e = 0.0;
-                for (uint32_t i = 0; i < b.n; i++) {
-                    size_t g = (size_t)N * jj + b.node_base + i;
+                for (uint32_t i = 0; i < bd.n; i++) {
+                    size_t g = (size_t)N * jj + bd.node_base + i;
                     const double* tg = targets + 3 * g;
                     double d0 = hx[g].x - tg[0], d1 = hx[g].y - tg[1], d2 = hx[g].z - tg[2];
                     e += d0 * d0;
                     e += d1 * d1;
                     e += d2 * d2;
                 }
-                L += e / (double)b.n;
-                if (body_loss) body_loss[bi] += e / (double)b.n;
+                L += e / (double)bd.n;
+                if (body_loss) body_loss[bi] += e / (double)bd.n;
             }
         if (loss) *loss = L;
     }
 
-    // Reverse pass, window by window from the end.
-    XF_HIP(ctx, xf_launch_zero_d4(st, N, Xb));
-    XF_HIP(ctx, xf_launch_zero_d4(st, N, Vb));
-    XF_HIP(ctx, hipMemsetAsync(gacc, 0, sizeof(float) * T, st));
-    XF_HIP(ctx, hipMemsetAsync(glog, 0, sizeof(double) * T, st));
-    uint32_t jl = n_loss;
-    for (uint32_t w = nwin; w-- > 0;) {
-        const uint32_t k0 = w * window, k1 = std::min(substeps, k0 + window);
-        XF_HIP(ctx, hipMemcpyAsync(s->X, ckx + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
-        XF_HIP(ctx, hipMemcpyAsync(s->V, ckv + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
-        // replay of the window, recording the sweep tape and the node states
-        XF_HIP(ctx, xf_launch_predict(st, N, s->X, s->O, s->V, s->bp));
-        for (uint32_t kk = k0; kk < k1; kk++) {
-            double* tp = tape + 12 * (size_t)T * (kk - k0);
-            for (uint32_t c = 0; c < s->ncol; c++) {
-                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_project_record(st, energy, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1],
-                                                     s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp, s->comp1_u, s->a_u,
-                                                     dt, nullptr));
-            }
-            XF_HIP(ctx, xf_launch_copy2_d4(st, N, xs + (size_t)N * (kk - k0), s->X, oo + (size_t)N * (kk - k0), s->O));
-            XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, s->bodies.size() == 1, N, s->X, s->O, s->V, s->bp,
-                                         s->clock));
-        }
-        for (uint32_t kk = k1; kk-- > k0;) {
-            const double4* xl = nullptr;
-            const double4* tg = nullptr;
-            if (jl > 0 && loss_at[jl - 1] == kk + 1) {
-                jl--;
-                xl = xloss + (size_t)N * jl;
-                tg = tgt + (size_t)N * jl;
-            }
-            XF_HIP(ctx, xf_launch_adj_node_end(st, N, s->O, xs + (size_t)N * (kk - k0), oo + (size_t)N * (kk - k0), Xb,
-                                               Vb, Ob, s->bp, xl, tg, lw));
-            const double* tp = tape + 12 * (size_t)T * (kk - k0);
-            for (uint32_t c = s->ncol; c-- > 0;) {
-                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_project_reverse(st, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1], s->tg[2],
-                                                      s->tg[3], s->taux, tp, s->X, Xb, gacc, s->bp, s->comp1_u,
-                                                      s->a_u, dt));
-            }
-            XF_HIP(ctx, xf_launch_adj_node_start(st, N, s->O, Xb, Vb, Ob, s->bp));
-        }
-        XF_HIP(ctx, xf_launch_adj_flush(st, T, gacc, glog));
-    }
-    // Leave the scene at the end of the forward rollout.
-    XF_HIP(ctx, hipMemcpyAsync(s->X, s->adj_save_x.p, n4, hipMemcpyDeviceToDevice, st));
-    XF_HIP(ctx, hipMemcpyAsync(s->V, s->adj_save_v.p, n4, hipMemcpyDeviceToDevice, st));
-    XF_HIP(ctx, hipMemcpyAsync(s->clock, &clock_end, sizeof(float), hipMemcpyHostToDevice, st));
+    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, &clock_end))) return rc;
     std::vector<double> hg(T);
-    XF_HIP(ctx, hipMemcpyAsync(hg.data(), glog, sizeof(double) * T, hipMemcpyDeviceToHost, st));
+    XF_HIP(ctx, hipMemcpyAsync(hg.data(), b.glog, sizeof(double) * T, hipMemcpyDeviceToHost, st));
     XF_HIP(ctx, hipStreamSynchronize(st));
     if (grad_log_mu) {
         size_t off = 0;
-        for (const Body& b : s->bodies) {
-            for (uint32_t t = 0; t < b.t; t++) grad_log_mu[off + t] = -hg[b.sched_pos[t]];
-            off += b.t;
+        for (const Body& bd : s->bodies) {
+            for (uint32_t t = 0; t < bd.t; t++) grad_log_mu[off + t] = -hg[bd.sched_pos[t]];
+            off += bd.t;
         }
     }
+    return XF_OK;
+}
+
+namespace {
+// Scratch bytes xf_scene_adjoint_noalloc carves for one call (each region
+// rounded up to 256 bytes).
+size_t AdjointScratchBytes(const xf_scene* s, uint32_t substeps, uint32_t n_loss, uint32_t window) {
+    const size_t N = s->N, T = s->T, B = s->bodies.size();
+    const size_t nwin = (substeps + window - 1) / window, wlen = std::min(window, substeps);
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t n4 = al(sizeof(double4) * N);
+    return 2 * al(n4 * nwin) + al(sizeof(double) * 12 * T * wlen) + 2 * al(n4 * wlen) + 5 * n4 +
+           al(sizeof(float) * T) + 2 * al(sizeof(double) * T) + 2 * al(n4 * n_loss) +
+           al(sizeof(double) * 3 * N * n_loss) + al(sizeof(double) * N) + al(sizeof(double) * B * n_loss) +
+           al(sizeof(double) * (1 + B));
+}
+}  // namespace
+
+XF_API int xf_scene_adjoint_noalloc(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss,
+                                    const uint32_t* loss_at, const double* targets, uint32_t window, double* loss,
+                                    double* grad_log_mu, double* body_loss) {
+    if (!s) return XF_E_INVALID;
+    xf_ctx* ctx = s->ctx;
+    int rc = AdjointCheck(s, substeps, n_loss, loss_at, targets, window);
+    if (rc) return rc;
+    const size_t need = AdjointScratchBytes(s, substeps, n_loss, window);
+    if (need > kAdjointScratchBytes)
+        return Fail(ctx, XF_E_INVALID,
+                    "adjoint needs %zu bytes of scratch, more than the %zu of the allocation-free entry point "
+                    "(use xf_scene_adjoint)",
+                    need, kAdjointScratchBytes);
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    // Per-tet material storage is allocated on first use; this entry point
+    // does not allocate it.
+    if (s->material_dirty && !s->taux) {
+        bool had = s->aux;
+        ComputeMaterial(s);
+        bool need_taux = s->aux || s->vpasses > 0;
+        s->aux = had;
+        if (need_taux)
+            return Fail(ctx, XF_E_STATE,
+                        "the scene's per-tet material storage is not allocated yet: run xf_scene_step or "
+                        "xf_scene_adjoint once after changing settings");
+    }
+    int energy;
+    bool sim;
+    rc = Prepare(s, dt, &energy, &sim);
+    if (rc) return rc;
+    const uint32_t N = s->N, T = s->T, B = (uint32_t)s->bodies.size();
+    const uint32_t nwin = (substeps + window - 1) / window;
+    const uint32_t wlen = std::min(window, substeps);
+    const size_t n4 = sizeof(double4) * N;
+    // carve the scratch (each region 256-byte aligned)
+    unsigned char* p = ctx->scratch;
+    auto take = [&p](size_t bytes) {
+        unsigned char* r = p;
+        p += (bytes + 255) & ~size_t(255);
+        return r;
+    };
+    AdjBufs b;
+    b.ckx = (double4*)take(n4 * nwin);
+    b.ckv = (double4*)take(n4 * nwin);
+    b.tape = (double*)take(sizeof(double) * 12 * (size_t)T * wlen);
+    b.xs = (double4*)take(n4 * wlen);
+    b.oo = (double4*)take(n4 * wlen);
+    b.Xb = (double4*)take(n4);
+    b.Vb = (double4*)take(n4);
+    b.Ob = (double4*)take(n4);
+    b.gacc = (float*)take(sizeof(float) * T);
+    b.glog = (double*)take(sizeof(double) * T);
+    b.xloss = (double4*)take(n4 * n_loss);
+    b.tgt = (double4*)take(n4 * n_loss);
+    b.lw = (double*)take(sizeof(double) * N);
+    b.save_x = (double4*)take(n4);
+    b.save_v = (double4*)take(n4);
+    double* traw = (double*)take(sizeof(double) * 3 * (size_t)N * n_loss);
+    double* eterm = (double*)take(sizeof(double) * (size_t)B * n_loss);
+    double* lout = (double*)take(sizeof(double) * (1 + (size_t)B));
+    double* gout = (double*)take(sizeof(double) * T);
+    hipStream_t st = ctx->stream;
+
+    // Targets and per-node loss-gradient weights 2/n_b, built on the device.
+    XF_HIP(ctx, hipMemcpyAsync(traw, targets, sizeof(double) * 3 * (size_t)N * n_loss, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, xf_launch_expand_targets(st, N * n_loss, traw, b.tgt));
+    XF_HIP(ctx, xf_launch_loss_weights(st, N, s->O, s->bp, b.lw));
+
+    float clock_end = 0.0f;
+    if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, false, &clock_end))) return rc;
+    XF_HIP(ctx, xf_launch_loss(st, n_loss, B, N, b.xloss, b.tgt, s->bp, eterm, lout));
+    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, &clock_end))) return rc;
+    XF_HIP(ctx, xf_launch_grad_out(st, T, b.glog, s->tout, gout));
+    if (loss) XF_HIP(ctx, hipMemcpyAsync(loss, lout, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (body_loss) XF_HIP(ctx, hipMemcpyAsync(body_loss, lout + 1, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    if (grad_log_mu) XF_HIP(ctx, hipMemcpyAsync(grad_log_mu, gout, sizeof(double) * T, hipMemcpyDeviceToHost, st));
+    XF_HIP(ctx, hipStreamSynchronize(st));
     return XF_OK;
 }
 
