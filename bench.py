@@ -435,20 +435,23 @@ def rings_profile(frames=1, xcd_slabs=True):
                       "frames": frames, "xcd_slabs": xcd_slabs}))
 
 
-def beam_profile(schedule="blocks"):
-    """One frame of the 96x24x24 beam in the schedule's fixed launch shape
-    (block launches, or per-colour launches of 64 threads; short runs under
-    rocprofv3 counter passes)."""
+def beam_profile(schedule="blocks", block=64, tet_lanes=1, frames=1, graph=False):
+    """Frames of the 96x24x24 beam in a fixed launch shape: block launches or
+    per-colour launches of `block` threads, `tet_lanes` lanes per tet in
+    block launches; eager launches unless graph (short runs under rocprofv3
+    counter and trace passes)."""
     ctx = xf.Context(0)
     wl = workloads.beam_yeoh_96x24x24()
-    sc = workloads.make_scene(ctx, wl, **SCHEDULES[schedule])
-    sc.step(wl["substeps"], float(wl["dt"]))
+    sc = workloads.make_scene(ctx, wl, block_size=block, tet_lanes=tet_lanes, use_graph=graph, **SCHEDULES[schedule])
+    for _ in range(frames):
+        sc.step(wl["substeps"], float(wl["dt"]))
     sc.sync()
     info = sc.info()
     sc.close()
     ctx.close()
     print(json.dumps({"beam_profile": wl["name"], "tets": info["n_tets"], "colours": info["n_colours"],
-                      "blocks": info["n_blocks"], "block_nodes": info["n_block_nodes"], "schedule": schedule}))
+                      "blocks": info["n_blocks"], "block_nodes": info["n_block_nodes"], "schedule": schedule,
+                      "block": block, "tet_lanes": tet_lanes, "frames": frames, "graph": graph}))
 
 
 def large_profile(schedule="blocks", block=256):
@@ -493,8 +496,12 @@ def main():
                     help="sweep schedule of the beam, large and batched legs: sweep blocks (default) or "
                          "per-colour launches in the XCD-slab or plain order")
     ap.add_argument("--profile", action="store_true",
-                    help="only run one frame of the beam in the schedule's fixed launch shape "
-                         "(rocprofv3 counter passes)")
+                    help="only run frames of the beam in a fixed launch shape (rocprofv3 counter and trace "
+                         "passes; see --block, --tet-lanes, --frames, --graph)")
+    ap.add_argument("--block", type=int, default=64, help="--profile: workgroup size")
+    ap.add_argument("--tet-lanes", type=int, default=1, help="--profile: lanes per tet of block launches")
+    ap.add_argument("--frames", type=int, default=1, help="--profile: frames")
+    ap.add_argument("--graph", action="store_true", help="--profile: replay each frame from a hipGraph")
     args = ap.parse_args()
     args.xcd_slabs = not args.no_xcd_slabs
     if args.copy_probe:
@@ -510,7 +517,8 @@ def main():
         large_profile(schedule=args.schedule)
         return
     if args.profile:
-        beam_profile(schedule=args.schedule)
+        beam_profile(schedule=args.schedule, block=args.block, tet_lanes=args.tet_lanes, frames=args.frames,
+                     graph=args.graph)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
@@ -593,6 +601,8 @@ def main():
         "config": {"workload": f"{wl['name']}: {T} tets, {N} nodes, {S} substeps/frame, {info['n_colours']} "
                                f"colours, {info['n_blocks']} sweep blocks, ground + sphere", "step": "one frame",
                    "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": args.schedule,
+                              "tet_lanes": sc.info()["tet_lanes"] if mode == xf.LAUNCH_BLOCKS else 1,
+                              "block_lanes": info["block_lanes"],
                               "node_phase": "in the last projection launch" if fused_now else "own launch"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": KERNELS[mode],
@@ -620,7 +630,7 @@ def main():
         legs += [("adjoint", adjoint_leg), ("fit", fit_leg)]
     if not args.no_cpu:
         # the one-frame checks run at the timed launch shape
-        shape = dict(opts, launch_mode=mode, use_graph=True,
+        shape = dict(opts, launch_mode=mode, use_graph=True, tet_lanes=sc.info()["tet_lanes"],
                      **({"persistent_blocks": size} if mode == xf.LAUNCH_PERSISTENT else {"block_size": size}))
         legs.append(("cpu_baseline", lambda c: cpu_baseline(c, wl, opts=shape)))
     for name, leg in legs:

@@ -209,9 +209,10 @@ typedef struct xf_scene_opts {
                                    XF_LAUNCH_BLOCKS; excludes xcd_slabs and tet_colour. The nodes
                                    of a block must fit the LDS of one workgroup (at most 2048). */
     uint32_t block_lanes;       /* sweep blocks: consecutive blocks of a launch colour share one
-                                   workgroup while each phase holds at most this many tets (64,
-                                   128, 256), or 0: the largest of these that leaves at least 4
-                                   workgroups per compute unit in every launch colour (else 64) */
+                                   workgroup while each phase holds at most this many tets (1 to
+                                   256; a block above it gets a workgroup of its own), or 0: the
+                                   largest of 64, 128, 256 that leaves at least 4 workgroups per
+                                   compute unit in every launch colour (else 64) */
     uint32_t separate_finish;   /* sweep blocks: 1 keeps the end of each substep (pinning, contact,
                                    velocity, next prediction) as a node launch of its own; 0: the
                                    last block launch of a substep does it for every node when at
@@ -220,6 +221,11 @@ typedef struct xf_scene_opts {
                                    with their O (68 B a node), no body has volume passes or
                                    damping and the energy constrains tets (not in the last
                                    substep of a step call, which stores the velocities) */
+    uint32_t tet_lanes;         /* sweep-block launches: lanes per tet, 1 (0 = 1) or 4 (four lanes of a
+                                   quad share one tet, each owning one node: the per-node work is
+                                   split four ways, for launches whose phases are latency-bound; the
+                                   simultaneous MixedSel / YeohSkinFast projection without damping,
+                                   else step calls fail with XF_E_UNSUPPORTED). Same results. */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {
@@ -240,6 +246,8 @@ typedef struct xf_scene_info {
     uint32_t fused_finish;   /* 1: under the current settings, step calls in XF_LAUNCH_BLOCKS end
                                 their substeps inside the last block launch (see separate_finish;
                                 the energy is not checked here) */
+    uint32_t tet_lanes;      /* lanes per tet of sweep-block launches in use (xf_scene_opts.tet_lanes,
+                                or the autotuner's choice) */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */
@@ -319,8 +327,9 @@ XF_API int xf_scene_time_steps(xf_scene* scene, uint32_t calls, uint32_t substep
 /* Autotuner: times `reps` calls of xf_scene_step(substeps, dt) for each
  * launch shape -- per-colour launches with workgroup sizes 64, 128, 256 and
  * the persistent launch with 64, 128, 256 and 512 blocks (where co-resident),
- * and sweep-block launches with workgroup sizes 64, 128, 256 when the scene
- * has sweep_blocks -- and keeps the fastest in the scene options. The node state and clock are restored afterwards, so tuning
+ * and sweep-block launches with workgroup sizes 64, 128, 256 and 1 or 4
+ * lanes per tet (4 where the projection allows it) when the scene has
+ * sweep_blocks -- and keeps the fastest in the scene options. The node state and clock are restored afterwards, so tuning
  * does not advance the simulation. chosen / best_ms (ms per call) nullable.
  * No reference counterpart (the reference is single-threaded). */
 XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,

@@ -277,7 +277,7 @@ def test_blocks_autotune_keeps_results(ctx):
 def test_blocks_rejections(ctx):
     wl = workloads.beam_neo_12x4x4()
     cases = [dict(launch_mode=xf.LAUNCH_BLOCKS), dict(sweep_blocks=True, xcd_slabs=True),
-             dict(sweep_blocks=2), dict(launch_mode=3), dict(sweep_blocks=True, block_lanes=100)]
+             dict(sweep_blocks=2), dict(launch_mode=3), dict(sweep_blocks=True, block_lanes=300)]
     for opts in cases:
         sc = xf.Scene(ctx)
         sc.add_body(wl["xyz"], wl["tets"], wl["density"], pinned=wl["pinned"])

@@ -868,6 +868,157 @@ __device__ __forceinline__ void block_tet(const BlockTet& t, uint32_t i, double4
     xs[id] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
 }
 
+// Value of v in lane L of this lane's quad (lanes 4k .. 4k + 3), by DPP
+// quad_perm broadcast.
+template <int L>
+__device__ __forceinline__ float quad_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), L * 0x55, 0xF, 0xF, false));
+}
+
+// One tet of a block phase on the four lanes of a quad: lane q owns node q.
+// SolveElementMixed with the simultaneous solve (MixedSel / YeohSkinFast;
+// E = 0 picks per body), damping off, in exactly the operations and order of
+// solve_tet: the per-tet scalars (I1, F, adjugate, J, the Cramer solve) are
+// computed by all four lanes alike, each node's gradients, its terms of the
+// sums of the 2x2 system and its position update by its own lane; the root
+// node's gradient -(g0 + g1 + g2) and the sums over nodes are assembled in
+// node order from the other lanes by DPP. Results equal block_tet bit for
+// bit.
+template <int E, bool AUX>
+__device__ __forceinline__ void block_tet_quad(const BlockTet& t, uint32_t q, double4* xs,
+                                               const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u,
+                                               float dt) {
+    static_assert(E == 0 || E == 4 || E == 7, "quad lanes take MixedSel / YeohSkinFast");
+    const uint32_t ia = t.loc.x & 0xFFFFu, ib = t.loc.x >> 16, ic = t.loc.y & 0xFFFFu, id = t.loc.y >> 16;
+    const double4 na = xs[ia], nb = xs[ib], nc = xs[ic], nd = xs[id];
+    const double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    const bool q0 = q == 0u, q1 = q == 1u, q2 = q == 2u, q3 = q == 3u;
+    const uint32_t own = q0 ? ia : (q1 ? ib : (q2 ? ic : id));
+    const double4 xo = q0 ? na : (q1 ? nb : (q2 ? nc : nd));
+    const float w = (float)xo.w;
+
+    float comp1 = comp1_u, a = a_u;
+    uint32_t body = 0;
+    if constexpr (AUX) {
+        comp1 = t.ax.x;
+        a = t.ax.y;
+        body = __float_as_uint(t.ax.z);
+    }
+    const XfBodyParams& bp = bodies[body];
+    const float C[3] = {bp.yeoh[0], bp.yeoh[1], bp.yeoh[2]};
+    const bool yf = (E == 7) || (E == 0 && bp.energy == 7);
+    const float Qi[3][3] = {{t.q0.x, t.q0.y, t.q0.z}, {t.q0.w, t.q1.x, t.q1.y}, {t.q1.z, t.q1.w, t.q2.x}};
+    const float QQ[3] = {t.q2.y, t.q2.z, t.q2.w};
+    const float QR[3] = {t.q3.x, t.q3.y, t.q3.z};
+    const float comp0 = t.q3.w;
+
+    float P[3][3];
+    gather_p(x, P);
+
+    // Prefactored I1 (neo_prefactored): the energy on every lane, the
+    // gradient of node q on lane q, g[i] = (2 QQ_i) P_i + QR_a P_a + QR_b P_b
+    // in the pair order (0,1), (0,2), (1,2).
+    float u = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) u = u + QQ[i] * dot3(P[i], P[i]);
+    {
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = i + 1; j < 3; j++) {
+                u = u + QR[k] * dot3(P[i], P[j]);
+                ++k;
+            }
+    }
+    const float cq = 2.0f * (q0 ? QQ[0] : (q1 ? QQ[1] : QQ[2]));
+    const float qa = q2 ? QR[1] : QR[0];
+    const float qb = q0 ? QR[1] : QR[2];
+    float g0[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float pn = q0 ? P[0][r] : (q1 ? P[1][r] : P[2][r]);
+        const float pa = q0 ? P[1][r] : P[0][r];
+        const float pb = q2 ? P[1][r] : P[2][r];
+        float g = cq * pn;
+        g = g + qa * pa;
+        g = g + qb * pb;
+        g0[r] = g;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float root = ((0.0f - quad_lane<0>(g0[r])) - quad_lane<1>(g0[r])) - quad_lane<2>(g0[r]);
+        g0[r] = q3 ? root : g0[r];
+    }
+    float U0 = u;
+    if (yf) {
+        const float IM = U0 - 3.0f;
+        U0 = fmaxf(0.0001f, C[0] * IM + C[1] * IM * IM + C[2] * IM * IM * IM);
+        const float gScale = C[0] + 2.0f * C[1] * IM + 3.0f * C[2] * IM * IM;
+#pragma unroll
+        for (int r = 0; r < 3; r++) g0[r] = g0[r] * gScale;
+    }
+
+    // (J - a)^2 volume term (volume_term): F, adjugate and J on every lane,
+    // node q's row of G Qi^T on lane q.
+    float F[3][3], adj[3][3], GJ[3][3];
+    defgrad(Qi, P, F);
+    const float J = adjdet(F, adj);
+    const float U1 = 0.0f + 1.0f * ((J - a) * (J - a));
+    const float sJ = 2.0f * (J - a);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) GJ[c][r] = adj[c][r] * sJ;
+    float Qn[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) Qn[c] = q0 ? Qi[0][c] : (q1 ? Qi[1][c] : Qi[2][c]);
+    float g1[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) g1[r] = 0.0f + (GJ[0][r] * Qn[0] + GJ[1][r] * Qn[1] + GJ[2][r] * Qn[2]);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float root = ((0.0f - quad_lane<0>(g1[r])) - quad_lane<1>(g1[r])) - quad_lane<2>(g1[r]);
+        g1[r] = q3 ? root : g1[r];
+    }
+
+    // EnergyXpbdConstrainSimultaneous<2>, Cramer branch (constrain2): the
+    // node sums in node order, then the solve on every lane.
+    const float alpha0 = comp0 / (dt * dt);
+    const float alpha1 = comp1 / (dt * dt);
+    const float t00 = w * dot3(g0, g0), t10 = w * dot3(g1, g0), t11 = w * dot3(g1, g1);
+    float A0 = 1.0e-22f, A1 = 1.0e-22f, A2 = 1.0e-22f;
+    A0 = A0 + quad_lane<0>(t00);
+    A0 = A0 + quad_lane<1>(t00);
+    A0 = A0 + quad_lane<2>(t00);
+    A0 = A0 + quad_lane<3>(t00);
+    A0 = A0 + 2.0f * U0 * alpha0;
+    const float b0 = -2.0f * U0;
+    A1 = A1 + quad_lane<0>(t10);
+    A1 = A1 + quad_lane<1>(t10);
+    A1 = A1 + quad_lane<2>(t10);
+    A1 = A1 + quad_lane<3>(t10);
+    A2 = A2 + quad_lane<0>(t11);
+    A2 = A2 + quad_lane<1>(t11);
+    A2 = A2 + quad_lane<2>(t11);
+    A2 = A2 + quad_lane<3>(t11);
+    A2 = A2 + 2.0f * U1 * alpha1;
+    const float b1 = -2.0f * U1;
+    const float invA00 = 1.0f / A0;
+    const float invA11 = 1.0f / A2;
+    const float invDet = 1.0f / fmaxf(0.00000001f, 1.0f - (A1 * invA00) * (A1 * invA11));
+    const float l0 = invDet * ((b0 * invA00) - (A1 * invA00) * (b1 * invA11));
+    const float l1 = invDet * ((b1 * invA11) - (b0 * invA00) * (A1 * invA11));
+    double xn[3] = {xo.x, xo.y, xo.z};
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        float acc = 0.0f + l0 * g0[r];
+        acc = acc + l1 * g1[r];
+        xn[r] = xn[r] + (double)(w * acc);
+    }
+    xs[own] = make_double4(xn[0], xn[1], xn[2], xo.w);
+}
+
 // ntet: tets in the schedule. Element loads are issued by every lane for
 // every phase pair, at clamped in-range positions (lanes without a tet, and
 // the phase after the last, load data they do not use), so that the loads in
@@ -883,7 +1034,7 @@ __device__ __forceinline__ void block_tet(const BlockTet& t, uint32_t i, double4
 // those nodes, and O is not read by the projection (FIN excludes DAMP).
 // tend: the clock, read for the end time of the substep (advanced by an
 // earlier launch). fsingle: the scene has one body.
-template <int E, bool SIM, bool AUX, bool DAMP, bool OVER, bool FIN>
+template <int E, bool SIM, bool AUX, bool DAMP, bool OVER, bool FIN, bool QUAD = false>
 __global__ __launch_bounds__(256) void xf_project_blocks(
     uint32_t nph, uint32_t nmax, uint32_t ntet, const uint32_t* __restrict__ rec, const uint32_t* __restrict__ bnode,
     const uint2* __restrict__ tloc, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
@@ -892,6 +1043,8 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt, float* __restrict__ clock,
     const uint32_t* __restrict__ fextra, uint32_t fmax, const float* __restrict__ tend, uint32_t fsingle) {
     static_assert(!(FIN && (DAMP || E == 30)), "the fused node phase needs a projection that leaves O unread");
+    static_assert(!QUAD || (SIM && !DAMP && (E == 0 || E == 4 || E == 7)),
+                  "quad lanes take the simultaneous MixedSel / YeohSkinFast projection without damping");
     if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
         clock[0] = clock[0] + dt;
         uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
@@ -909,18 +1062,24 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     const uint32_t ftot = FIN ? nmax + fmax : 0u;
     double4* os = FIN ? xs + ftot : nullptr;
     uint32_t* fs = FIN ? reinterpret_cast<uint32_t*>(os + ftot) : nullptr;
+    // QUAD: four lanes per tet (lane tid & 3 owns node tid & 3), bs / 4 tets
+    // at a time; else one lane per tet
+    const uint32_t lt = QUAD ? tid >> 2 : tid, per = QUAD ? bs >> 2 : bs;
     auto load = [&](uint32_t j) {
-        return load_block_tet<AUX>(min(r[2 * j] + tid, last), tloc, tg0, tg1, tg2, tg3, taux);
+        return load_block_tet<AUX>(min(r[2 * j] + lt, last), tloc, tg0, tg1, tg2, tg3, taux);
+    };
+    auto solve = [&](const BlockTet& t, uint32_t i) {
+        if constexpr (QUAD) block_tet_quad<E, AUX>(t, tid & 3u, xs, bodies, comp1_u, a_u, dt);
+        else block_tet<E, SIM, AUX, DAMP>(t, i, xs, tidx, O, bodies, comp1_u, a_u, dt);
     };
     // each lane's first tet of the phase is t; with OVER, lanes go on to the
     // tets beyond the workgroup's lanes
     auto phase = [&](uint32_t j, const BlockTet& t) {
         const uint32_t b = r[2 * j], cnt = r[2 * j + 1];
-        if (tid < cnt) block_tet<E, SIM, AUX, DAMP>(t, b + tid, xs, tidx, O, bodies, comp1_u, a_u, dt);
+        if (lt < cnt) solve(t, b + lt);
         if constexpr (OVER)
-            for (uint32_t k = tid + bs; k < cnt; k += bs)
-                block_tet<E, SIM, AUX, DAMP>(load_block_tet<AUX>(b + k, tloc, tg0, tg1, tg2, tg3, taux), b + k, xs,
-                                             tidx, O, bodies, comp1_u, a_u, dt);
+            for (uint32_t k = lt + per; k < cnt; k += per)
+                solve(load_block_tet<AUX>(b + k, tloc, tg0, tg1, tg2, tg3, taux), b + k);
         __syncthreads();
     };
     BlockTet ta = load(0), tb;
@@ -1434,6 +1593,17 @@ typedef void (*XfBlocksFn)(uint32_t, uint32_t, uint32_t, const uint32_t*, const 
                            uint32_t, const float*, uint32_t);
 
 namespace {
+template <int E, bool OVER>
+XfBlocksFn pick_blocks_quad(bool aux, bool fin) {
+    if constexpr (E == 0 || E == 4 || E == 7) {
+        if (fin)
+            return aux ? xf_project_blocks<E, true, true, false, OVER, true, true>
+                       : xf_project_blocks<E, true, false, false, OVER, true, true>;
+        return aux ? xf_project_blocks<E, true, true, false, OVER, false, true>
+                   : xf_project_blocks<E, true, false, false, OVER, false, true>;
+    }
+    return nullptr;
+}
 template <int E, bool SIM, bool OVER>
 XfBlocksFn pick_blocks_ad(bool aux, bool damp, bool fin) {
     if (fin) {
@@ -1447,7 +1617,11 @@ XfBlocksFn pick_blocks_ad(bool aux, bool damp, bool fin) {
     return damp ? xf_project_blocks<E, SIM, false, true, OVER, false> : xf_project_blocks<E, SIM, false, false, OVER, false>;
 }
 template <int E>
-XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over, bool fin) {
+XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over, bool fin, bool quad) {
+    if (quad) {
+        if (!sim || damp) return nullptr;
+        return over ? pick_blocks_quad<E, true>(aux, fin) : pick_blocks_quad<E, false>(aux, fin);
+    }
     if (over)
         return sim ? pick_blocks_ad<E, true, true>(aux, damp, fin) : pick_blocks_ad<E, false, true>(aux, damp, fin);
     return sim ? pick_blocks_ad<E, true, false>(aux, damp, fin) : pick_blocks_ad<E, false, false>(aux, damp, fin);
@@ -1466,7 +1640,9 @@ extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax) { return 
 // of one block in one phase. fin: the launch also ends the substep and
 // predicts the next for every node (fextra: per workgroup its nmax block
 // nodes and fmax extra nodes; tend: the clock, not advanced by this launch;
-// single_body: the scene has one body); not with damp or energy 30.
+// single_body: the scene has one body); not with damp or energy 30. quad:
+// four lanes per tet (simultaneous MixedSel / YeohSkinFast without damping,
+// energies 0, 4 and 7; anything else is hipErrorInvalidValue).
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
                                        uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
                                        const uint32_t* rec,
@@ -1475,19 +1651,19 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
                                        const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
-                                       const float* tend, bool single_body) {
+                                       const float* tend, bool single_body, bool quad) {
     XfBlocksFn fn = nullptr;
-    const bool over = pmax > threads;
+    const bool over = pmax > (quad ? threads / 4 : threads);
     switch (energy) {
-        case 0: fn = pick_blocks<0>(sim, aux, damp, over, fin); break;
-        case 3: fn = pick_blocks<3>(sim, aux, damp, over, fin); break;
-        case 4: fn = pick_blocks<4>(sim, aux, damp, over, fin); break;
-        case 5: fn = pick_blocks<5>(sim, aux, damp, over, fin); break;
-        case 6: fn = pick_blocks<6>(sim, aux, damp, over, fin); break;
-        case 7: fn = pick_blocks<7>(sim, aux, damp, over, fin); break;
-        case 20: fn = pick_blocks<20>(sim, aux, damp, over, fin); break;
-        case 22: fn = pick_blocks<22>(sim, aux, damp, over, fin); break;
-        case 30: fn = pick_blocks<30>(sim, aux, damp, over, fin); break;
+        case 0: fn = pick_blocks<0>(sim, aux, damp, over, fin, quad); break;
+        case 3: fn = pick_blocks<3>(sim, aux, damp, over, fin, quad); break;
+        case 4: fn = pick_blocks<4>(sim, aux, damp, over, fin, quad); break;
+        case 5: fn = pick_blocks<5>(sim, aux, damp, over, fin, quad); break;
+        case 6: fn = pick_blocks<6>(sim, aux, damp, over, fin, quad); break;
+        case 7: fn = pick_blocks<7>(sim, aux, damp, over, fin, quad); break;
+        case 20: fn = pick_blocks<20>(sim, aux, damp, over, fin, quad); break;
+        case 22: fn = pick_blocks<22>(sim, aux, damp, over, fin, quad); break;
+        case 30: fn = pick_blocks<30>(sim, aux, damp, over, fin, quad); break;
         default: return hipErrorInvalidValue;
     }
     size_t lds = fin ? xf_block_fin_lds_bytes(nmax, fmax) : xf_block_lds_bytes(nmax);

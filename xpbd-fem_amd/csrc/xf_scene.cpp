@@ -56,7 +56,7 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
                                        const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
-                                       const float* tend, bool single_body);
+                                       const float* tend, bool single_body, bool quad);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
@@ -745,7 +745,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                                          s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O,
                                          s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr,
                                          fin && q == s->blk_last, s->blk_fin, s->blk_fmax, s->clock,
-                                         s->bodies.size() == 1));
+                                         s->bodies.size() == 1, s->opts.tet_lanes == 4));
             first = false;
             if (timer) {
                 XF_HIP(ctx, hipEventRecord(e1, st));
@@ -834,6 +834,10 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
         rc = UploadMaterial(s);
         if (rc) return rc;
     }
+    if (s->opts.launch_mode == XF_LAUNCH_BLOCKS && s->opts.tet_lanes == 4 &&
+        (!(*energy == 0 || *energy == 4 || *energy == 7) || !*sim || s->dproj))
+        return Fail(s->ctx, XF_E_UNSUPPORTED,
+                    "tet_lanes 4 takes the simultaneous MixedSel / YeohSkinFast projection without damping");
     if (s->params_dirty || s->params_dt != dt) {
         rc = UploadParams(s, dt);
         if (rc) return rc;
@@ -1044,9 +1048,9 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         return Fail(ctx, XF_E_INVALID, "ring_lanes must be 0, 16 or 32");
     if (s->opts.sweep_blocks > 1) return Fail(ctx, XF_E_INVALID, "sweep_blocks must be 0 or 1");
     if (s->opts.separate_finish > 1) return Fail(ctx, XF_E_INVALID, "separate_finish must be 0 or 1");
-    if (s->opts.block_lanes != 0 && s->opts.block_lanes != 64 && s->opts.block_lanes != 128 &&
-        s->opts.block_lanes != 256)
-        return Fail(ctx, XF_E_INVALID, "block_lanes must be 0, 64, 128 or 256");
+    if (s->opts.tet_lanes == 0) s->opts.tet_lanes = 1;
+    if (s->opts.tet_lanes != 1 && s->opts.tet_lanes != 4) return Fail(ctx, XF_E_INVALID, "tet_lanes must be 0, 1 or 4");
+    if (s->opts.block_lanes > 256) return Fail(ctx, XF_E_INVALID, "block_lanes must be at most 256");
     if (s->opts.sweep_blocks && s->opts.xcd_slabs)
         return Fail(ctx, XF_E_INVALID, "sweep_blocks and xcd_slabs are exclusive");
     if (s->opts.launch_mode == XF_LAUNCH_BLOCKS && !s->opts.sweep_blocks)
@@ -1507,6 +1511,7 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     for (const Body& b : s->bodies)
         if (b.settings.volume_passes != 0 || b.settings.damping > 0.0f || b.settings.pbd_damping > 0.0f) fused = false;
     info->fused_finish = fused ? 1u : 0u;
+    info->tet_lanes = s->opts.tet_lanes;
     return XF_OK;
 }
 
@@ -1823,14 +1828,19 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
         return hipStreamSynchronize(st);
     };
     struct Cand {
-        uint32_t mode, size;
+        uint32_t mode, size, lanes;
     };
-    const Cand candidates[] = {{XF_LAUNCH_PER_COLOUR, 64},  {XF_LAUNCH_PER_COLOUR, 128}, {XF_LAUNCH_PER_COLOUR, 256},
-                               {XF_LAUNCH_PERSISTENT, 64},  {XF_LAUNCH_PERSISTENT, 128}, {XF_LAUNCH_PERSISTENT, 256},
-                               {XF_LAUNCH_PERSISTENT, 512}, {XF_LAUNCH_BLOCKS, 64},  {XF_LAUNCH_BLOCKS, 128},
-                               {XF_LAUNCH_BLOCKS, 256}};
+    const Cand candidates[] = {{XF_LAUNCH_PER_COLOUR, 64, 1},  {XF_LAUNCH_PER_COLOUR, 128, 1},
+                               {XF_LAUNCH_PER_COLOUR, 256, 1}, {XF_LAUNCH_PERSISTENT, 64, 1},
+                               {XF_LAUNCH_PERSISTENT, 128, 1}, {XF_LAUNCH_PERSISTENT, 256, 1},
+                               {XF_LAUNCH_PERSISTENT, 512, 1}, {XF_LAUNCH_BLOCKS, 64, 1},
+                               {XF_LAUNCH_BLOCKS, 128, 1},     {XF_LAUNCH_BLOCKS, 256, 1},
+                               {XF_LAUNCH_BLOCKS, 64, 4},      {XF_LAUNCH_BLOCKS, 128, 4},
+                               {XF_LAUNCH_BLOCKS, 256, 4}};
     xf_scene_opts keep = s->opts;
-    Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size};
+    Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size,
+                   keep.tet_lanes};
+    const bool quad_ok = (energy == 0 || energy == 4 || energy == 7) && sim && !s->dproj;
     int maxb = 0;  // ring, Pixar and Cube energies run per colour only
     if (energy != kRingMode && energy != 20 && energy != 22 && energy != 30)
         XF_HIP(ctx, xf_frame_max_blocks(energy, sim, s->aux, s->dproj, &maxb));
@@ -1845,7 +1855,9 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
         if (he != hipSuccess || rc) break;
         if (cand.mode == XF_LAUNCH_PERSISTENT && (int)cand.size > maxb) continue;
         if (cand.mode == XF_LAUNCH_BLOCKS && (!s->opts.sweep_blocks || energy == kRingMode)) continue;
+        if (cand.lanes == 4 && !quad_ok) continue;
         s->opts.launch_mode = cand.mode;
+        s->opts.tet_lanes = cand.lanes;
         if (cand.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = cand.size;
         else s->opts.block_size = cand.size;
         DropGraph(s);
@@ -1872,6 +1884,7 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     s->tick_host = tick_keep;
     s->opts = keep;
     s->opts.launch_mode = best_c.mode;
+    s->opts.tet_lanes = best_c.lanes;
     if (best_c.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = best_c.size;
     else s->opts.block_size = best_c.size;
     DropGraph(s);
