@@ -114,7 +114,11 @@ typedef struct xf_scene xf_scene;
  *                            (Xpbd.h:306-348, min(1, pbd_damping / area));
  *                            the pass runs when it is > 0
  *   pbd_damping_amortized <- amortizedVolumeAndTimeCorrectedPbdDamping, used
- *                            instead under XF_RAYLEIGH_POST_AMORTIZED */
+ *                            instead under XF_RAYLEIGH_POST_AMORTIZED
+ *   unlock_pins           <- Settings_LockLeft clear (Geo.cpp:318): 1 leaves
+ *                            the body's pinned nodes free in these substeps
+ *                            (a node that was locked before keeps inverse
+ *                            mass 0, as in the reference) */
 typedef struct xf_settings {
     uint32_t energy;
     uint32_t simultaneous;
@@ -134,6 +138,7 @@ typedef struct xf_settings {
     float sphere_r;
     uint32_t rayleigh_type;       /* XF_RAYLEIGH_* (Settings flag bits 20-21) */
     float pbd_damping_amortized;  /* amortizedVolumeAndTimeCorrectedPbdDamping */
+    uint32_t unlock_pins;         /* 1: pins not locked (Settings_LockLeft clear) */
 } xf_settings;
 
 /* Fills defaults: MixedSel, simultaneous, compliance 0.01, nu 0.45,
@@ -166,6 +171,11 @@ typedef struct xf_body_desc {
                                    the element constants -- the positions GeoLinear3d::Init
                                    holds after autoResize (Geo.cpp:712-729), which float
                                    cannot represent. */
+    uint32_t pin_first_substep; /* 1: pinned nodes keep inverse mass 1/m until the end of their
+                                   first locked substep, as the reference's Left nodes do
+                                   (w = 1/m from GeoLinear3d::Init, Geo.cpp:744, set to 0 by the
+                                   lock, Geo.cpp:318-322), so the first sweep moves them before
+                                   they are reset to O; 0: inverse mass 0 from the start */
 } xf_body_desc;
 
 /* Launch modes of xf_scene_step. */
@@ -377,6 +387,23 @@ XF_API int xf_scene_adjoint(xf_scene* scene, uint32_t substeps, float dt, uint32
 XF_API int xf_scene_adjoint_noalloc(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss,
                                     const uint32_t* loss_at, const double* targets, uint32_t window, double* loss,
                                     double* grad_log_mu, double* body_loss);
+/* Nodes held at kinematic targets: the n body-local nodes get X = O =
+ * xyz[3 k ..] (FP64) and inverse mass 0 after the sweep of every substep of
+ * later step calls, before the velocity update -- what Settings_LockRight
+ * does to the reference's Right nodes with their animated target
+ * origin + lockedRightTransform3d X0 (Geo.cpp:324-331). n = 0 clears. A
+ * caller that animates the targets steps one substep at a time and sets them
+ * before each. These nodes skip contact; the adjoint and the persistent
+ * launch reject scenes with kinematic nodes. */
+XF_API int xf_scene_set_kinematic(xf_scene* scene, uint32_t body, uint32_t n, const uint32_t* nodes,
+                                  const double* xyz);
+/* The manipulation constraint of Geo3d::Substep (Geo.cpp:334-339) on one
+ * body-local node: after the sweep of every substep of later step calls,
+ * X += (target - X) w / (max(1e-6, w) + 1.8 / dt^2) in FP32 as there.
+ * node = 0xFFFFFFFF switches it off. The reference recomputes the target per
+ * substep from the manipulator (Demo.cpp:84-85); a caller following it steps
+ * one substep at a time. */
+XF_API int xf_scene_set_manip(xf_scene* scene, uint32_t body, uint32_t node, const float target[3]);
 /* Counts non-finite node positions (0 = all finite). */
 XF_API int xf_scene_count_nonfinite(xf_scene* scene, uint32_t* count);
 

@@ -57,6 +57,7 @@ struct XfBodyParams {
     // Node range of the body in the scene (adjoint loss terms)
     uint32_t node_base;
     uint32_t n_nodes;
+    uint32_t unlock;         // 1: pins are not locked this step (Settings_LockLeft clear, Geo.cpp:318)
 };
 
 // The scene clock buffer: [0] simulation time (float), [1] tick (uint32,

@@ -12,9 +12,12 @@
 // volumeAndTimeCorrectedPbdDamping,
 // amortizedVolumeAndTimeCorrectedPbdDamping, timeCorrectedDrag,
 // volumePasses, tickId; Settings.h:79-102) works, so the adapter needs none
-// of the reference's headers. The manipulator is accepted for signature
-// compatibility; interactive picking is not supported (pickedGeo must not
-// point at this body).
+// of the reference's headers. What needs the reference's own vector math
+// (Transform, the Settings_LockRight targets, the manipulator's target
+// point) is done by the Geo subclass in GeoGpu.h, which hands the results
+// over through SetPositions, SetKinematic and SetManip; a step whose
+// settings ask for Settings_LockRight, or whose manipulator picked this
+// body, without them fails with XF_E_UNSUPPORTED.
 #pragma once
 
 #include <stdint.h>
@@ -30,7 +33,8 @@ namespace xfgpu {
 
 // Flag layout of Settings::flags (Settings.h:9-29).
 constexpr uint32_t kEnergyBit = 6, kEnergyMask = 31, kXpbdSolveBit = 11, kRayleighBit = 20, kRayleighMask = 3,
-                   kLockLeft = 1u << 26;
+                   kLockLeft = 1u << 26, kLockRight = 1u << 27;
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
 class GeoGpuLinear3d {
    public:
@@ -66,8 +70,11 @@ class GeoGpuLinear3d {
             mx = std::fmax(mx, nodes_[3 * i]);
         }
         left_.assign(n_, 0);
-        for (uint32_t i = 0; i < n_; i++)
+        right_.assign(n_, 0);
+        for (uint32_t i = 0; i < n_; i++) {
             if ((nodes_[3 * i] - mn) / (mx - mn) < 0.02f) left_[i] = 1;
+            if ((nodes_[3 * i] - mn) / (mx - mn) > 0.98f) right_[i] = 1;  // Geo.cpp:725
+        }
         nodes64_.clear();
         if (autoResize) {
             float lo[3] = {1e24f, 1e24f, 1e24f}, hi[3] = {-1e24f, -1e24f, -1e24f};
@@ -88,15 +95,21 @@ class GeoGpuLinear3d {
         return xf_ctx_create(device, &ctx_);
     }
 
-    // Geo3d::Substep: one substep of length dt.
+    // Geo3d::Substep: one substep of length dt. owner: the object the
+    // caller's manipulator points at when it picked this body (the Geo that
+    // holds the adapter), or null.
     template <class SettingsT, class ManipT>
-    int Substep(const SettingsT& settings, const ManipT& /*manip*/, float dt) {
-        return Frame(settings, 1, dt);
+    int Substep(const SettingsT& settings, const ManipT& manip, float dt, const void* owner = nullptr) {
+        if (owner && static_cast<const void*>(manip.pickedGeo) == owner && !manip_set_) return XF_E_UNSUPPORTED;
+        int rc = Frame(settings, 1, dt);
+        manip_set_ = kinematic_set_ = false;
+        return rc;
     }
 
     // `substeps` consecutive Substep calls, enqueued as one hipGraph.
     template <class SettingsT>
     int Frame(const SettingsT& settings, uint32_t substeps, float dt) {
+        if ((settings.flags & kLockRight) && !kinematic_set_) return XF_E_UNSUPPORTED;
         int rc = EnsureScene(settings.flags);
         if (rc) return rc;
         xf_settings s;
@@ -118,6 +131,9 @@ class GeoGpuLinear3d {
         s.collide = collide_;
         s.ground_y = ground_y_;
         s.friction = friction_;
+        // with the reference's pins the Left nodes are pinned whenever the
+        // step's settings lock them (Geo.cpp:318)
+        s.unlock_pins = reference_pins_ && !(settings.flags & kLockLeft) ? 1u : 0u;
         if ((rc = xf_scene_set_settings(scene_, 0, &s))) return rc;
         uint32_t tick = 0;
         if ((rc = xf_scene_get_tick(scene_, &tick))) return rc;
@@ -167,6 +183,64 @@ class GeoGpuLinear3d {
 
     const char* LastError() const { return ctx_ ? xf_last_error(ctx_) : "no context"; }
 
+    // The reference's pin behaviour (before the scene exists): the Left nodes
+    // are pinned in the substeps whose settings have Settings_LockLeft, and
+    // keep inverse mass 1/m until the end of their first locked substep
+    // (GeoLinear3d::Init sets w = 1/m for every node, Geo.cpp:744; the lock
+    // sets it to 0, Geo.cpp:318-322). Off: the Left nodes are pinned from the
+    // first substep with inverse mass 0 when its settings lock them.
+    int SetReferencePins(bool on) {
+        if (scene_) return XF_E_STATE;
+        reference_pins_ = on;
+        return XF_OK;
+    }
+
+    // Creates the scene now (it is otherwise created by the first Substep,
+    // whose settings decide the pins unless SetReferencePins is on).
+    int Prepare(uint32_t settings_flags) { return EnsureScene(settings_flags); }
+
+    // FP64 positions of every node (Geo3d::Transform's result, Geo.cpp:358-364).
+    int SetPositions(const double* X) {
+        int rc = EnsureScene(kLockLeft);
+        return rc ? rc : xf_scene_set_state(scene_, 0, X, nullptr);
+    }
+
+    // Nodes held at FP64 targets after the sweep of the next Substep
+    // (Settings_LockRight, Geo.cpp:324-331); n = 0 clears.
+    int SetKinematic(uint32_t n, const uint32_t* nodes, const double* xyz) {
+        int rc = EnsureScene(kLockLeft);
+        if (rc) return rc;
+        if (n || kinematic_n_) rc = xf_scene_set_kinematic(scene_, 0, n, nodes, xyz);
+        kinematic_n_ = n;
+        kinematic_set_ = rc == XF_OK;
+        return rc;
+    }
+
+    // The manipulation constraint on `node` toward `target` in the next
+    // Substep (Geo.cpp:334-339); node = kNoNode: none.
+    int SetManip(uint32_t node, const float target[3]) {
+        int rc = EnsureScene(kLockLeft);
+        if (rc) return rc;
+        if (node != kNoNode || manip_on_) rc = xf_scene_set_manip(scene_, 0, node, target);
+        manip_on_ = node != kNoNode;
+        manip_set_ = rc == XF_OK;
+        return rc;
+    }
+
+    // The scene's sweep order (the reference's tOrder that reproduces it).
+    int Order(uint32_t* order) {
+        int rc = EnsureScene(kLockLeft);
+        return rc ? rc : xf_scene_get_order(scene_, 0, order, nullptr);
+    }
+
+    // Rest positions (X0 of GeoLinear3d::Init, after autoResize), node flags.
+    double RestPosition(uint32_t i, int r) const {
+        return nodes64_.empty() ? (double)nodes_[3 * i + r] : nodes64_[3 * i + r];
+    }
+    bool IsLeft(uint32_t i) const { return left_[i] != 0; }
+    bool IsRight(uint32_t i) const { return right_[i] != 0; }
+    const std::vector<uint32_t>& Tets() const { return tets_; }
+
     // The sweep-block schedule (xf_scene_opts.sweep_blocks with
     // XF_LAUNCH_BLOCKS: 4 projection launches per substep, each workgroup
     // sweeping one block through its phases in LDS) for the scene created at
@@ -193,7 +267,8 @@ class GeoGpuLinear3d {
         d.tets = tets_.data();
         d.n_tets = (uint32_t)(tets_.size() / 4);
         d.density = density_;
-        d.pinned = (flags & kLockLeft) ? left_.data() : nullptr;
+        d.pinned = (reference_pins_ || (flags & kLockLeft)) ? left_.data() : nullptr;
+        d.pin_first_substep = reference_pins_ ? 1u : 0u;
         if ((rc = xf_scene_add_body(scene_, &d, nullptr))) return rc;
         xf_scene_opts o;
         memset(&o, 0, sizeof(o));
@@ -210,12 +285,15 @@ class GeoGpuLinear3d {
     std::vector<float> nodes_;
     std::vector<double> nodes64_;  // autoresized FP64 positions (empty without autoResize)
     std::vector<uint32_t> tets_;
-    std::vector<uint8_t> left_;
+    std::vector<uint8_t> left_, right_;
     float density_ = 0.0f;
     uint32_t n_ = 0;
     uint32_t collide_ = 0;
     float ground_y_ = 0.0f, friction_ = 0.0f;
     bool sweep_blocks_ = false;
+    bool reference_pins_ = false;
+    uint32_t kinematic_n_ = 0;
+    bool kinematic_set_ = false, manip_on_ = false, manip_set_ = false;
 };
 
 }  // namespace xfgpu

@@ -1206,23 +1206,34 @@ __device__ __forceinline__ void predict(const XfBodyParams& p, double v[3], cons
     *Xo = make_double4(x[0] + v[0] * dt, x[1] + v[1] * dt, x[2] + v[2] * dt, w);
 }
 
+// The lock of Geo3d::Substep (Geo.cpp:318-322) and the contact pass for one
+// node: a pinned node of a body whose pins are locked this step returns to
+// O and its inverse mass becomes 0 (from then on; it may be finite until its
+// first locked substep); other nodes with mass take contact. Returns the
+// node's inverse mass after the lock.
+__device__ __forceinline__ double lock_contact(const XfBodyParams& p, uint32_t flags, float t_end, double w,
+                                               const double o[3], double x[3]) {
+    if ((flags & 1u) && !p.unlock) {
+        x[0] = o[0];
+        x[1] = o[1];
+        x[2] = o[2];
+        return 0.0;
+    }
+    if (p.collide && w != 0.0) collide_node(p, t_end, o, x);
+    return w;
+}
+
 __device__ __forceinline__ void finish_predict(double4 xi, double4 oi, uint32_t i, double4* __restrict__ X,
                                                double4* __restrict__ O, const XfBodyParams& p, float t_end) {
     uint32_t flags = (uint32_t)oi.w;
     double x[3] = {xi.x, xi.y, xi.z};
     double o[3] = {oi.x, oi.y, oi.z};
-    if (flags & 1u) {
-        x[0] = o[0];
-        x[1] = o[1];
-        x[2] = o[2];
-    } else if (p.collide) {
-        collide_node(p, t_end, o, x);
-    }
+    const double w = lock_contact(p, flags, t_end, xi.w, o, x);
     double v[3];
     double invdt = (double)p.invdt;
 #pragma unroll
     for (int r = 0; r < 3; r++) v[r] = (x[r] - o[r]) * invdt;
-    predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
+    predict(p, v, x, &X[i], &O[i], w, oi.w);
 }
 
 }  // namespace
@@ -1260,23 +1271,45 @@ __global__ __launch_bounds__(256) void xf_finish(uint32_t n, double4* __restrict
     const XfBodyParams& p = bodies[SINGLE ? 0u : flags >> 1];
     double x[3] = {xi.x, xi.y, xi.z};
     double o[3] = {oi.x, oi.y, oi.z};
-    if (flags & 1u) {
-        x[0] = o[0];
-        x[1] = o[1];
-        x[2] = o[2];
-    } else if (p.collide) {
-        collide_node(p, clock[0], o, x);
-    }
+    const double w = lock_contact(p, flags, clock[0], xi.w, o, x);
     double v[3];
     double invdt = (double)p.invdt;
 #pragma unroll
     for (int r = 0; r < 3; r++) v[r] = (x[r] - o[r]) * invdt;
     if constexpr (PREDICT) {
-        predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
+        predict(p, v, x, &X[i], &O[i], w, oi.w);
     } else {
-        X[i] = make_double4(x[0], x[1], x[2], xi.w);
+        X[i] = make_double4(x[0], x[1], x[2], w);
         V[i] = make_double4(v[0], v[1], v[2], 0.0);
     }
+}
+
+// Nodes held at kinematic targets (Settings_LockRight, Geo.cpp:324-331):
+// after the sweep, X = O = target and the inverse mass becomes 0.
+__global__ __launch_bounds__(256) void xf_kinematic(uint32_t n, const uint32_t* __restrict__ idx,
+                                                    const double4* __restrict__ target, double4* __restrict__ X,
+                                                    double4* __restrict__ O) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t i = idx[k];
+    double4 t = target[k];
+    X[i] = make_double4(t.x, t.y, t.z, 0.0);
+    O[i] = make_double4(t.x, t.y, t.z, O[i].w);
+}
+
+// The manipulation constraint of Geo3d::Substep (Geo.cpp:334-339): m holds
+// the target point and, in w, the bits of the node id (xf::kNoNode: none).
+// X += (target - X) * w / (max(1e-6, w) + 1.8 / dt^2), in FP32 as there.
+__global__ __launch_bounds__(64) void xf_manip(const float4* __restrict__ m, float dt, double4* __restrict__ X) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const float4 t = m[0];
+    const uint32_t i = __float_as_uint(t.w);
+    if (i == xf::kNoNode) return;
+    double4 xi = X[i];
+    float w = (float)xi.w;
+    float s = w / (fmaxf(0.000001f, w) + 1.8f / (dt * dt));
+    float d0 = (t.x - (float)xi.x) * s, d1 = (t.y - (float)xi.y) * s, d2 = (t.z - (float)xi.z) * s;
+    X[i] = make_double4(xi.x + (double)d0, xi.y + (double)d1, xi.z + (double)d2, xi.w);
 }
 
 // Forward projection of one colour that also records each tet's 4 node
@@ -1412,21 +1445,15 @@ __device__ __forceinline__ void finish_one(uint32_t i, bool predict_next, double
     const XfBodyParams& p = bodies[flags >> 1];
     double x[3] = {xi.x, xi.y, xi.z};
     double o[3] = {oi.x, oi.y, oi.z};
-    if (flags & 1u) {
-        x[0] = o[0];
-        x[1] = o[1];
-        x[2] = o[2];
-    } else if (p.collide) {
-        collide_node(p, t_end, o, x);
-    }
+    const double w = lock_contact(p, flags, t_end, xi.w, o, x);
     double v[3];
     double invdt = (double)p.invdt;
 #pragma unroll
     for (int r = 0; r < 3; r++) v[r] = (x[r] - o[r]) * invdt;
     if (predict_next) {
-        predict(p, v, x, &X[i], &O[i], xi.w, oi.w);
+        predict(p, v, x, &X[i], &O[i], w, oi.w);
     } else {
-        X[i] = make_double4(x[0], x[1], x[2], xi.w);
+        X[i] = make_double4(x[0], x[1], x[2], w);
         V[i] = make_double4(v[0], v[1], v[2], 0.0);
     }
 }
@@ -1752,6 +1779,18 @@ extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool si
     auto fn = predict_next ? (single_body ? xf_finish<true, true> : xf_finish<true, false>)
                            : (single_body ? xf_finish<false, true> : xf_finish<false, false>);
     hipLaunchKernelGGL(fn, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, clock);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_kinematic(hipStream_t s, uint32_t n, const uint32_t* idx, const double4* target,
+                                          double4* X, double4* O) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_kinematic, dim3((n + 255) / 256), dim3(256), 0, s, n, idx, target, X, O);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_manip(hipStream_t s, const float4* target, float dt, double4* X) {
+    hipLaunchKernelGGL(xf_manip, dim3(1), dim3(64), 0, s, target, dt, X);
     return hipGetLastError();
 }
 

@@ -62,6 +62,9 @@ extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, d
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
                                        double4* O, double4* V, const XfBodyParams* bodies, const float* clock);
 extern "C" hipError_t xf_launch_count_nonfinite(hipStream_t s, uint32_t n, const double4* X, uint32_t* count);
+extern "C" hipError_t xf_launch_kinematic(hipStream_t s, uint32_t n, const uint32_t* idx, const double4* target,
+                                          double4* X, double4* O);
+extern "C" hipError_t xf_launch_manip(hipStream_t s, const float4* target, float dt, double4* X);
 extern "C" hipError_t xf_frame_max_blocks(int energy, bool sim, bool aux, bool damp, int* max_blocks);
 extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool aux, uint32_t begin, uint32_t count,
                                                const uint4* tidx, const float4* tg0, const float4* tg1,
@@ -161,6 +164,10 @@ struct Body {
     std::vector<uint32_t> tets;
     std::vector<uint32_t> user_colour;  // caller-supplied colouring (empty: greedy)
     std::vector<uint8_t> pinned;
+    bool pin_first_substep = false;  // pins keep 1/m until their first locked substep (Geo.cpp:318-322)
+    // kinematic targets (body-local node, FP64 position) and manipulator
+    std::vector<uint32_t> kin_nodes;
+    std::vector<double> kin_xyz;
     xf_settings settings{};
     // finalize products
     xf::ElementConstants ec;
@@ -296,6 +303,13 @@ struct xf_scene {
     uint32_t* blk_fin = nullptr;
     uint2* tloc = nullptr;
     uint32_t* tout = nullptr;  // per schedule position: the tet's slot in body-concatenated, body-local order
+    // kinematic targets of all bodies (global node ids, FP64 positions) and
+    // the manipulator (target xyz, node id bits in w; kNoNode: none)
+    uint32_t nkin = 0, kin_cap = 0;
+    uint32_t* kin_idx = nullptr;
+    double4* kin_pos = nullptr;
+    float4* manip = nullptr;
+    bool manip_on = false;
     uint64_t device_bytes = 0;
     int n_cu = 256;
     // host mirrors
@@ -326,7 +340,8 @@ struct xf_scene {
     float graph_dt = 0.0f;
     int graph_energy = -1;
     bool graph_sim = false, graph_aux = false, graph_dproj = false;
-    uint32_t graph_vpasses = 0, graph_post = 0;
+    uint32_t graph_vpasses = 0, graph_post = 0, graph_nkin = 0;
+    bool graph_manip = false;
 };
 
 namespace {
@@ -478,6 +493,7 @@ int UploadParams(xf_scene* s, float dt) {
         p.poisson = st.poissons_ratio;
         p.node_base = s->bodies[i].node_base;
         p.n_nodes = s->bodies[i].n;
+        p.unlock = st.unlock_pins;
     }
     XF_HIP(ctx, hipMemcpyAsync(s->bp, hp.data(), sizeof(XfBodyParams) * hp.size(), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -612,6 +628,17 @@ struct LaunchTimer {
     std::vector<hipEvent_t> ev_proj, ev_node;
 };
 
+// After the sweep of a substep: nodes held at kinematic targets (LockRight,
+// Geo.cpp:324-331), then the manipulation constraint (Geo.cpp:334-339). The
+// lock of pinned nodes (Geo.cpp:318-322) runs in the node phase after them;
+// it touches other nodes, or overrides them, as the reference's order does.
+int HeldNodes(xf_scene* s, float dt) {
+    xf_ctx* ctx = s->ctx;
+    if (s->nkin) XF_HIP(ctx, xf_launch_kinematic(ctx->stream, s->nkin, s->kin_idx, s->kin_pos, s->X, s->O));
+    if (s->manip_on) XF_HIP(ctx, xf_launch_manip(ctx->stream, s->manip, dt, s->X));
+    return XF_OK;
+}
+
 int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, LaunchTimer* timer) {
     xf_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
@@ -666,6 +693,8 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                     timer->ev_proj.push_back(e1);
                 }
             }
+            rc = HeldNodes(s, dt);
+            if (rc) return rc;
             if (s->post == 0) {
                 rc = node(1, k + 1 < substeps);
                 if (rc) return rc;
@@ -725,7 +754,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
     // velocities need not be stored (no damping passes, not the call's last
     // substep) and the projection leaves O unread
     const bool fuse = blocks && s->blk_last >= 0 && !s->opts.separate_finish && s->vpasses == 0 && s->post == 0 &&
-                      !s->dproj && energy != 30;
+                      !s->dproj && energy != 30 && s->nkin == 0 && !s->manip_on;
     for (uint32_t k = 0; k < substeps; k++) {
         const bool fin = fuse && k + 1 < substeps;
         // sweep blocks: one launch per launch colour that holds blocks; the
@@ -777,6 +806,8 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                 XF_HIP(ctx, xf_launch_volume(st, block, slabs, begin, count, pass, s->tidx, s->tg[0], s->tg[1], s->tg[2],
                                              s->taux, s->X, s->bp, dt));
             }
+        rc = HeldNodes(s, dt);
+        if (rc) return rc;
         if (s->post == 0) {
             if (fin) continue;
             rc = node(1, k + 1 < substeps);
@@ -834,6 +865,8 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
         rc = UploadMaterial(s);
         if (rc) return rc;
     }
+    if (s->opts.launch_mode == XF_LAUNCH_PERSISTENT && (s->nkin || s->manip_on))
+        return Fail(s->ctx, XF_E_UNSUPPORTED, "kinematic nodes and the manipulator need per-colour or block launches");
     if (s->opts.launch_mode == XF_LAUNCH_BLOCKS && s->opts.tet_lanes == 4 &&
         (!(*energy == 0 || *energy == 4 || *energy == 7) || !*sim || s->dproj))
         return Fail(s->ctx, XF_E_UNSUPPORTED,
@@ -862,6 +895,7 @@ int CheckSettings(xf_ctx* ctx, const xf_settings* st) {
     if (st->collide & ~3u) return Fail(ctx, XF_E_INVALID, "unknown collide bits");
     if ((st->collide & XF_COLLIDE_SPHERE) && !(st->sphere_r > 0.0f))
         return Fail(ctx, XF_E_INVALID, "sphere radius must be positive");
+    if (st->unlock_pins > 1) return Fail(ctx, XF_E_INVALID, "unlock_pins must be 0 or 1");
     return XF_OK;
 }
 
@@ -939,7 +973,7 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
                     s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet,
-                    s->blk_rec, s->blk_node, s->tloc, s->blk_fin, s->tout};
+                    s->blk_rec, s->blk_node, s->tloc, s->blk_fin, s->tout, s->kin_idx, s->kin_pos, s->manip};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
@@ -989,6 +1023,8 @@ XF_API int xf_scene_add_body(xf_scene* s, const xf_body_desc* d, uint32_t* body_
     b.pinned.assign(b.n, 0);
     if (d->pinned)
         for (uint32_t i = 0; i < b.n; i++) b.pinned[i] = d->pinned[i] ? 1 : 0;
+    if (d->pin_first_substep > 1) return Fail(ctx, XF_E_INVALID, "pin_first_substep must be 0 or 1");
+    b.pin_first_substep = d->pin_first_substep != 0;
     if (d->tet_mu) {
         b.tet_mu.assign(d->tet_mu, d->tet_mu + b.t);
         for (float m : b.tet_mu)
@@ -1022,7 +1058,7 @@ XF_API int xf_scene_add_body(xf_scene* s, const xf_body_desc* d, uint32_t* body_
         if (!(b.ec.vol[t] > 0.0f)) return Fail(ctx, XF_E_INVALID, "tet %u has non-positive rest volume", t);
     for (uint32_t i = 0; i < b.n; i++) {
         if (!std::isfinite(b.ec.w[i])) return Fail(ctx, XF_E_INVALID, "node %u belongs to no tet", i);
-        if (b.pinned[i]) b.ec.w[i] = 0.0f;
+        if (b.pinned[i] && !b.pin_first_substep) b.ec.w[i] = 0.0f;
     }
     b.node_base = s->N;
     s->N += b.n;
@@ -1510,6 +1546,7 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     bool fused = s->blk_last >= 0 && !s->opts.separate_finish && s->opts.launch_mode == XF_LAUNCH_BLOCKS;
     for (const Body& b : s->bodies)
         if (b.settings.volume_passes != 0 || b.settings.damping > 0.0f || b.settings.pbd_damping > 0.0f) fused = false;
+    if (s->nkin || s->manip_on) fused = false;
     info->fused_finish = fused ? 1u : 0u;
     info->tet_lanes = s->opts.tet_lanes;
     return XF_OK;
@@ -1552,7 +1589,8 @@ XF_API int xf_scene_step(xf_scene* s, uint32_t substeps, float dt) {
     } else {
         if (!s->graph_exec || s->graph_substeps != substeps || s->graph_dt != dt || s->graph_energy != energy ||
             s->graph_sim != sim || s->graph_aux != s->aux || s->graph_vpasses != s->vpasses ||
-            s->graph_dproj != s->dproj || s->graph_post != s->post) {
+            s->graph_dproj != s->dproj || s->graph_post != s->post || s->graph_nkin != s->nkin ||
+            s->graph_manip != s->manip_on) {
             DropGraph(s);
             XF_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             rc = Enqueue(s, substeps, dt, energy, sim, nullptr);
@@ -1573,6 +1611,8 @@ XF_API int xf_scene_step(xf_scene* s, uint32_t substeps, float dt) {
             s->graph_vpasses = s->vpasses;
             s->graph_dproj = s->dproj;
             s->graph_post = s->post;
+            s->graph_nkin = s->nkin;
+            s->graph_manip = s->manip_on;
         }
         XF_HIP(ctx, hipGraphLaunch(s->graph_exec, ctx->stream));
     }
@@ -1643,8 +1683,12 @@ XF_API int xf_scene_set_state(xf_scene* s, uint32_t body, const double* x, const
     XF_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<double4> tmp(b.n);
     if (x) {
-        for (uint32_t i = 0; i < b.n; i++)
-            tmp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], s->h_w[b.node_base + i]);
+        // positions change; inverse masses keep their current values (pins and
+        // kinematic nodes may have dropped theirs to 0)
+        XF_HIP(ctx, hipMemcpyAsync(tmp.data(), s->X + b.node_base, sizeof(double4) * b.n, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < b.n; i++) tmp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], tmp[i].w);
         XF_HIP(ctx, hipMemcpyAsync(s->X + b.node_base, tmp.data(), sizeof(double4) * b.n, hipMemcpyHostToDevice,
                                    ctx->stream));
         XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1942,7 +1986,11 @@ int AdjointCheck(xf_scene* s, uint32_t substeps, uint32_t n_loss, const uint32_t
             return Fail(ctx, XF_E_UNSUPPORTED,
                         "adjoint covers simultaneous MixedSel/YeohSkinFast with pins and ground contact "
                         "(no volume passes, no damping)");
+        if (st.unlock_pins || b.pin_first_substep)
+            return Fail(ctx, XF_E_UNSUPPORTED, "adjoint takes pins locked from the first substep");
     }
+    if (s->nkin || s->manip_on)
+        return Fail(ctx, XF_E_UNSUPPORTED, "adjoint takes no kinematic nodes or manipulator");
     return XF_OK;
 }
 
@@ -2240,6 +2288,87 @@ XF_API int xf_scene_adjoint_noalloc(xf_scene* s, uint32_t substeps, float dt, ui
     if (body_loss) XF_HIP(ctx, hipMemcpyAsync(body_loss, lout + 1, sizeof(double) * B, hipMemcpyDeviceToHost, st));
     if (grad_log_mu) XF_HIP(ctx, hipMemcpyAsync(grad_log_mu, gout, sizeof(double) * T, hipMemcpyDeviceToHost, st));
     XF_HIP(ctx, hipStreamSynchronize(st));
+    return XF_OK;
+}
+
+namespace {
+// Rebuilds the scene's kinematic list from the bodies' lists (global ids).
+int UploadKinematic(xf_scene* s) {
+    xf_ctx* ctx = s->ctx;
+    std::vector<uint32_t> idx;
+    std::vector<double4> pos;
+    for (const Body& b : s->bodies)
+        for (size_t k = 0; k < b.kin_nodes.size(); k++) {
+            idx.push_back(b.node_base + b.kin_nodes[k]);
+            pos.push_back(make_double4(b.kin_xyz[3 * k], b.kin_xyz[3 * k + 1], b.kin_xyz[3 * k + 2], 0.0));
+        }
+    const uint32_t n = (uint32_t)idx.size();
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    if (n > s->kin_cap) {
+        // the stream may still read the old lists
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (s->kin_idx) (void)hipFree(s->kin_idx);
+        if (s->kin_pos) (void)hipFree(s->kin_pos);
+        s->kin_idx = nullptr;
+        s->kin_pos = nullptr;
+        s->kin_cap = 0;
+        int rc;
+        if ((rc = DevAlloc(ctx, &s->kin_idx, n, &s->device_bytes)) || (rc = DevAlloc(ctx, &s->kin_pos, n, &s->device_bytes)))
+            return rc;
+        s->kin_cap = n;
+    }
+    if (n) {
+        XF_HIP(ctx, hipMemcpyAsync(s->kin_idx, idx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+        XF_HIP(ctx, hipMemcpyAsync(s->kin_pos, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice, ctx->stream));
+        XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    s->nkin = n;
+    return XF_OK;
+}
+}  // namespace
+
+XF_API int xf_scene_set_kinematic(xf_scene* s, uint32_t body, uint32_t n, const uint32_t* nodes, const double* xyz) {
+    if (!s) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    if (body >= s->bodies.size()) return Fail(s->ctx, XF_E_INVALID, "body %u out of range", body);
+    if (n && (!nodes || !xyz)) return Fail(s->ctx, XF_E_INVALID, "nodes and xyz are needed");
+    Body& b = s->bodies[body];
+    std::vector<uint8_t> seen(b.n, 0);
+    for (uint32_t k = 0; k < n; k++) {
+        if (nodes[k] >= b.n) return Fail(s->ctx, XF_E_INVALID, "kinematic node %u out of range", nodes[k]);
+        if (seen[nodes[k]]++) return Fail(s->ctx, XF_E_INVALID, "kinematic node %u given twice", nodes[k]);
+        for (int r = 0; r < 3; r++)
+            if (!std::isfinite(xyz[3 * (size_t)k + r])) return Fail(s->ctx, XF_E_INVALID, "non-finite target");
+    }
+    b.kin_nodes.assign(nodes, nodes + n);
+    b.kin_xyz.assign(xyz, xyz + 3 * (size_t)n);
+    return UploadKinematic(s);
+}
+
+XF_API int xf_scene_set_manip(xf_scene* s, uint32_t body, uint32_t node, const float target[3]) {
+    if (!s) return XF_E_INVALID;
+    if (!s->finalized) return Fail(s->ctx, XF_E_STATE, "scene is not finalized");
+    xf_ctx* ctx = s->ctx;
+    uint32_t g = xf::kNoNode;
+    float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (node != xf::kNoNode) {
+        if (body >= s->bodies.size()) return Fail(ctx, XF_E_INVALID, "body %u out of range", body);
+        if (node >= s->bodies[body].n) return Fail(ctx, XF_E_INVALID, "node %u out of range", node);
+        if (!target || !std::isfinite(target[0]) || !std::isfinite(target[1]) || !std::isfinite(target[2]))
+            return Fail(ctx, XF_E_INVALID, "the manipulator needs a finite target");
+        g = s->bodies[body].node_base + node;
+        m = make_float4(target[0], target[1], target[2], 0.0f);
+    }
+    memcpy(&m.w, &g, sizeof(float));
+    XF_HIP(ctx, hipSetDevice(ctx->device));
+    if (!s->manip) {
+        if (g == xf::kNoNode) return XF_OK;  // never enabled
+        int rc = DevAlloc(ctx, &s->manip, 1, &s->device_bytes);
+        if (rc) return rc;
+    }
+    XF_HIP(ctx, hipMemcpyAsync(s->manip, &m, sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    XF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->manip_on = g != xf::kNoNode;
     return XF_OK;
 }
 

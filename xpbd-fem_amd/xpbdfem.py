@@ -53,6 +53,7 @@ class Settings(C.Structure):
         ("sphere_r", C.c_float),
         ("rayleigh_type", C.c_uint32),
         ("pbd_damping_amortized", C.c_float),
+        ("unlock_pins", C.c_uint32),
     ]
 
 
@@ -68,6 +69,7 @@ class BodyDesc(C.Structure):
         ("tet_mu", C.c_void_p),
         ("tet_colour", C.c_void_p),
         ("xyz64", C.c_void_p),
+        ("pin_first_substep", C.c_uint32),
     ]
 
 
@@ -150,6 +152,8 @@ def lib():
                               C.POINTER(C.c_uint32)], C.c_int),
         "xf_scene_count_nonfinite": ([vp, C.POINTER(C.c_uint32)], C.c_int),
         "xf_scene_time_steps": ([vp, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_float)], C.c_int),
+        "xf_scene_set_kinematic": ([vp, C.c_uint32, C.c_uint32, vp, vp], C.c_int),
+        "xf_scene_set_manip": ([vp, C.c_uint32, C.c_uint32, _f32p], C.c_int),
         "xf_scene_adjoint": ([vp, C.c_uint32, C.c_float, C.c_uint32, _u32p, vp, C.c_uint32, C.POINTER(C.c_double),
                               vp, vp], C.c_int),
         "xf_scene_adjoint_noalloc": ([vp, C.c_uint32, C.c_float, C.c_uint32, _u32p, vp, C.c_uint32,
@@ -184,6 +188,7 @@ def exported_symbols():
         "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
         "xf_scene_get_tick", "xf_scene_set_tick", "xf_mesh_read_tetgen", "xf_mesh_tets_from_stream", "xf_mesh_error",
         "xf_mesh_rings", "xf_mesh_sweep_blocks", "xf_scene_time_steps", "xf_scene_adjoint_noalloc",
+        "xf_scene_set_kinematic", "xf_scene_set_manip",
     ]
 
 
@@ -393,11 +398,14 @@ class Scene:
         self.bodies = []  # (n_nodes, n_tets)
         self._keep = []
 
-    def add_body(self, xyz, tets, density, pinned=None, velocity=None, tet_mu=None, tet_colour=None, xyz64=None):
+    def add_body(self, xyz, tets, density, pinned=None, velocity=None, tet_mu=None, tet_colour=None, xyz64=None,
+                 pin_first_substep=False):
         """xyz: float32 positions; xyz64 (optional): FP64 positions that replace
-        them (e.g. autoresized ones, see autoresize())."""
+        them (e.g. autoresized ones, see autoresize()); pin_first_substep: pins
+        keep inverse mass 1/m until their first locked substep."""
         tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
         d = BodyDesc()
+        d.pin_first_substep = int(bool(pin_first_substep))
         if xyz64 is not None:
             xyz = np.ascontiguousarray(xyz64, np.float64).reshape(-1)
             d.xyz64 = xyz.ctypes.data
@@ -530,6 +538,19 @@ class Scene:
         self.ctx.check(lib().xf_scene_profile(self.h, substeps, dt, C.byref(mp), C.byref(mn), C.byref(nl)),
                        "xf_scene_profile")
         return mp.value, mn.value, nl.value
+
+    def set_kinematic(self, body, nodes, xyz):
+        """Nodes held at FP64 targets after every sweep (LockRight); empty clears."""
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        self.ctx.check(lib().xf_scene_set_kinematic(self.h, body, nodes.size, nodes.ctypes.data, xyz.ctypes.data),
+                       "xf_scene_set_kinematic")
+
+    def set_manip(self, body, node, target=(0.0, 0.0, 0.0)):
+        """The manipulation constraint on one node (node None: off)."""
+        t = np.ascontiguousarray(target, np.float32)
+        self.ctx.check(lib().xf_scene_set_manip(self.h, body, 0xFFFFFFFF if node is None else node, t),
+                       "xf_scene_set_manip")
 
     def time_steps(self, calls, substeps, dt):
         """Device ms of `calls` step calls between two HIP events on the
