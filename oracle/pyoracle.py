@@ -16,6 +16,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libxfref.so")
+# the same reference sources at -O3 -march=x86-64-v3 (timing only)
+REF_FAST_SO = os.path.join(HERE, "_ref", "libxfref_fast.so")
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -160,19 +162,21 @@ def lib():
     return _lib
 
 
-_ref = None
+_ref = {}
 
 
-def have_ref():
-    return os.path.exists(REF_SO)
+def have_ref(fast=False):
+    return os.path.exists(REF_FAST_SO if fast else REF_SO)
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        if not have_ref():
-            raise RuntimeError(f"reference harness missing: {REF_SO}")
-        L = C.CDLL(REF_SO)
+def ref(fast=False):
+    """The compiled reference with its harness: the parity build, or with
+    fast the -O3 -march=x86-64-v3 build (for timing)."""
+    if fast not in _ref:
+        path = REF_FAST_SO if fast else REF_SO
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference harness missing: {path}")
+        L = C.CDLL(path)
         L.xr_create.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float]
         L.xr_create.restype = C.c_void_p
         L.xr_create_ex.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float, C.c_int]
@@ -205,8 +209,9 @@ def ref():
         L.xr_get_vorder.argtypes = [C.c_void_p, _u32p]
         L.xr_set_vorder.argtypes = [C.c_void_p, _u32p]
         L.xr_get_ring.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        _ref = L
-    return _ref
+        L.xr_substeps_many.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32]
+        _ref[fast] = L
+    return _ref[fast]
 
 
 # ------------------------------------------------------------------ helpers
@@ -386,8 +391,9 @@ def ref_armadillo_t4():
 class RefBody:
     """The compiled reference's GeoLinear3d driven through ref_harness."""
 
-    def __init__(self, xyz, tets, density, pinned=None, order=None, autoresize=False):
-        L = ref()
+    def __init__(self, xyz, tets, density, pinned=None, order=None, autoresize=False, fast=False):
+        self.fast = fast
+        L = ref(fast)
         self.n = xyz.size // 3
         self.t = tets.size // 4
         self.h = L.xr_create_ex(np.ascontiguousarray(xyz, np.float32), self.n,
@@ -398,7 +404,7 @@ class RefBody:
 
     def __del__(self):
         if getattr(self, "h", None):
-            ref().xr_destroy(self.h)
+            ref(self.fast).xr_destroy(self.h)
             self.h = None
 
     def settings(self, compliance, nu=0.5, gravity=(0.0, -9.81), drag=0.0, energy=4, simultaneous=True,
@@ -421,18 +427,18 @@ class RefBody:
         return s
 
     def substeps(self, settings, dt, count):
-        ref().xr_substeps(self.h, C.byref(settings), dt, count)
+        ref(self.fast).xr_substeps(self.h, C.byref(settings), dt, count)
 
     def state(self):
         X = np.zeros(3 * self.n)
         V = np.zeros(3 * self.n)
-        ref().xr_get_state(self.h, X.ctypes.data, V.ctypes.data)
+        ref(self.fast).xr_get_state(self.h, X.ctypes.data, V.ctypes.data)
         return X, V
 
     def set_state(self, X=None, V=None):
         Xp = np.ascontiguousarray(X, np.float64) if X is not None else None
         Vp = np.ascontiguousarray(V, np.float64) if V is not None else None
-        ref().xr_set_state(self.h, Xp.ctypes.data if Xp is not None else None,
+        ref(self.fast).xr_set_state(self.h, Xp.ctypes.data if Xp is not None else None,
                            Vp.ctypes.data if Vp is not None else None)
 
     def elements(self):
@@ -441,37 +447,46 @@ class RefBody:
         qr = np.zeros(3 * self.t, np.float32)
         vol = np.zeros(self.t, np.float32)
         w = np.zeros(self.n, np.float32)
-        ref().xr_get_elements(self.h, qi, qq, qr, vol, w)
+        ref(self.fast).xr_get_elements(self.h, qi, qq, qr, vol, w)
         return dict(qi=qi, qq=qq, qr=qr, vol=vol, w=w)
 
     def area(self):
         a = np.zeros(self.t, np.float32)
-        ref().xr_get_area(self.h, a)
+        ref(self.fast).xr_get_area(self.h, a)
         return a
 
     def order(self):
         o = np.zeros(self.t, np.uint32)
-        ref().xr_get_order(self.h, o)
+        ref(self.fast).xr_get_order(self.h, o)
         return o
 
     def ring_order(self):
         """GeoLinear3d::vOrder (one entry per ring)."""
-        o = np.zeros(ref().xr_ring_count(self.h), np.uint32)
-        ref().xr_get_vorder(self.h, o)
+        o = np.zeros(ref(self.fast).xr_ring_count(self.h), np.uint32)
+        ref(self.fast).xr_get_vorder(self.h, o)
         return o
 
     def set_ring_order(self, order):
-        ref().xr_set_vorder(self.h, np.ascontiguousarray(order, np.uint32))
+        ref(self.fast).xr_set_vorder(self.h, np.ascontiguousarray(order, np.uint32))
 
     def ring(self, v):
         """(nodes, tets, idx[k, 4]) of the reference's OneRing of vertex v."""
         nn, nt = C.c_uint32(), C.c_uint32()
-        ref().xr_get_ring(self.h, v, C.byref(nn), C.byref(nt), None, None, None)
+        ref(self.fast).xr_get_ring(self.h, v, C.byref(nn), C.byref(nt), None, None, None)
         nodes = np.zeros(nn.value, np.uint32)
         tets = np.zeros(nt.value, np.uint32)
         idx = np.zeros(4 * nt.value, np.uint16)
-        ref().xr_get_ring(self.h, v, C.byref(nn), C.byref(nt), nodes.ctypes.data, tets.ctypes.data, idx.ctypes.data)
+        ref(self.fast).xr_get_ring(self.h, v, C.byref(nn), C.byref(nt), nodes.ctypes.data, tets.ctypes.data, idx.ctypes.data)
         return nodes, tets, idx.reshape(-1, 4)
+
+
+def substeps_many(bodies, settings, dt, count, threads):
+    """`count` reference substeps of every RefBody in `bodies` (one settings
+    object each), spread over `threads` host threads (all of one build)."""
+    fast = bodies[0].fast
+    hs = (C.c_void_p * len(bodies))(*[b.h for b in bodies])
+    ss = (XrSettings * len(bodies))(*settings)
+    ref(fast).xr_substeps_many(hs, ss, len(bodies), dt, count, threads)
 
 
 def rollout_adjoint(body, params, order, n_sub, loss_at, targets, window=20):

@@ -7,9 +7,12 @@
 // same sweep order as the GPU path. It calls only the reference's public
 // functions and writes only the public members the reference exposes
 // (tOrder, vOrder, flags, w, X, V; Geo.h:61-66, 161-162).
+#include <atomic>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "Geo.h"
@@ -23,7 +26,7 @@ void GenerateTetBlock(Allocator* alloc, uint32_t width, uint32_t height, uint32_
 
 namespace {
 struct RefBody {
-    std::vector<uint8_t> mem;
+    std::unique_ptr<uint8_t[]> mem;  // the bump allocator's buffer (pages touched only as used)
     Allocator alloc;
     GeoLinear3d* geo = nullptr;
 };
@@ -71,9 +74,9 @@ extern "C" {
 void* xr_create_ex(const float* xyz, uint32_t n_nodes, const uint32_t* tets, uint32_t n_tets, float density,
                    int auto_resize) {
     RefBody* b = new RefBody();
-    size_t bytes = (size_t)4096 * n_tets + (size_t)4096 * n_nodes + ((size_t)64 << 20);
-    b->mem.resize(bytes);
-    b->alloc.Initialize(b->mem.data(), bytes);
+    size_t bytes = (size_t)4096 * n_tets + (size_t)4096 * n_nodes + ((size_t)4 << 20);
+    b->mem.reset(new uint8_t[bytes]);
+    b->alloc.Initialize(b->mem.get(), bytes);
     std::vector<uint32_t> idx((size_t)5 * n_tets);
     for (uint32_t t = 0; t < n_tets; t++) {
         idx[5 * t] = 4;
@@ -173,6 +176,20 @@ void xr_substeps(void* h, const xr_settings* s, float dt, uint32_t count) {
         g->Substep(st, manip, dt);
         ++st.tickId;  // Sim::Update advances tickId once per substep (Demo.cpp:81-89)
     }
+}
+
+// `count` substeps of each of n bodies, bodies dealt to `threads` host
+// threads in turn (each body is the reference's single-threaded Substep;
+// bodies share nothing). settings: one per body.
+void xr_substeps_many(void** hs, const xr_settings* ss, uint32_t n, float dt, uint32_t count, uint32_t threads) {
+    std::atomic<uint32_t> next(0);
+    auto work = [&]() {
+        for (uint32_t i = next++; i < n; i = next++) xr_substeps(hs[i], ss + i, dt, count);
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t k = 1; k < threads; k++) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
 }
 
 // One SolveElement call (Fem.cpp:892) on a 4-node local state.
