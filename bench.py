@@ -701,7 +701,8 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             rec = json.load(f)
-        if mode == xf.LAUNCH_BLOCKS and "blocks" in rec and rec["blocks"].get("n_blocks") == info["n_blocks"]:
+        if mode == xf.LAUNCH_BLOCKS and "blocks" in rec and rec["blocks"].get("n_blocks") == info["n_blocks"] and \
+                rec["blocks"].get("block_size") == size and sc.info()["tet_lanes"] == 1:
             traffic = float(rec["blocks"]["traffic_bytes_per_dispatch"])
         elif mode == xf.LAUNCH_PER_COLOUR and args.schedule != "blocks" and \
                 rec.get("tets_per_dispatch") == T // max(info["n_colours"], 1):
