@@ -13,106 +13,54 @@
 #include <hip/hip_runtime.h>
 
 #include "xf_device.h"
+#include "xf_element.h"
 
 namespace {
 
-__device__ __forceinline__ float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+using xf::adjdet;
+using xf::apply_grad;
+using xf::defgrad;
+using xf::dot3;
+using xf::neo_prefactored;
+using xf::root_grad;
 
-__device__ __forceinline__ void defgrad(const float Qi[3][3], const float P[3][3], float F[3][3]) {
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) F[j][r] = Qi[0][r] * P[0][j] + Qi[1][r] * P[1][j] + Qi[2][r] * P[2][j];
-}
-
-__device__ __forceinline__ float adjdet(const float F[3][3], float adj[3][3]) {
-    float m00 = F[0][0], m01 = F[0][1], m02 = F[0][2];
-    float m10 = F[1][0], m11 = F[1][1], m12 = F[1][2];
-    float m20 = F[2][0], m21 = F[2][1], m22 = F[2][2];
-    adj[0][0] = +(m11 * m22 - m12 * m21);
-    adj[0][1] = -(m01 * m22 - m02 * m21);
-    adj[0][2] = +(m01 * m12 - m02 * m11);
-    adj[1][0] = -(m10 * m22 - m12 * m20);
-    adj[1][1] = +(m00 * m22 - m02 * m20);
-    adj[1][2] = -(m00 * m12 - m02 * m10);
-    adj[2][0] = +(m10 * m21 - m11 * m20);
-    adj[2][1] = -(m00 * m21 - m01 * m20);
-    adj[2][2] = +(m00 * m11 - m01 * m10);
-    return F[0][0] * adj[0][0] + F[0][1] * adj[1][0] + F[0][2] * adj[2][0];
-}
-
-__device__ __forceinline__ void apply_grad(const float G[3][3], const float Qi[3][3], float g[4][3]) {
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) g[j][r] = g[j][r] + (G[0][r] * Qi[j][0] + G[1][r] * Qi[j][1] + G[2][r] * Qi[j][2]);
-}
-
-__device__ __forceinline__ void root_grad(float g[4][3]) {
-#pragma unroll
-    for (int r = 0; r < 3; r++) g[3][r] = ((0.0f - g[0][r]) - g[1][r]) - g[2][r];
-}
-
-__device__ __forceinline__ void neo_prefactored(const float QQ[3], const float QR[3], const float P[3][3], float& U,
-                                                float g[4][3]) {
-    float u = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        u = u + QQ[i] * dot3(P[i], P[i]);
-#pragma unroll
-        for (int r = 0; r < 3; r++) g[i][r] = (2.0f * QQ[i]) * P[i][r];
-    }
-    int k = 0;
+// Adjoint of one simultaneous projection (forward: xf::solve_mixed_sim, as
+// xf_kernels.hip solve_tet for E = 4 / 7) in element precision R. x:
+// positions before it; xbar: in, adjoint of the positions after it; out,
+// adjoint of the positions before it. Returns d(loss)/d(log comp0) of this
+// projection.
+template <class R>
+__device__ __forceinline__ R adjoint_tet(bool yeoh, const R C[3], const double x[4][3], const float w[4],
+                                         const R Qi[3][3], const R QQ[3], const R QR[3], R comp0, R comp1, R a, R dt,
+                                         double xbar[4][3]) {
+    R P[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = i + 1; j < 3; j++) {
-            u = u + QR[k] * dot3(P[i], P[j]);
-#pragma unroll
-            for (int r = 0; r < 3; r++) g[i][r] = g[i][r] + QR[k] * P[j][r];
-#pragma unroll
-            for (int r = 0; r < 3; r++) g[j][r] = g[j][r] + QR[k] * P[i][r];
-            ++k;
-        }
-    root_grad(g);
-    U = u;
-}
-
-// Adjoint of one simultaneous projection (forward: xf_kernels.hip solve_tet
-// for E = 4 / 7). x: positions before it; xbar: in, adjoint of the positions
-// after it; out, adjoint of the positions before it. Returns
-// d(loss)/d(log comp0) of this projection.
-__device__ __forceinline__ float adjoint_tet(bool yeoh, const float C[3], const double x[4][3], const float w[4],
-                                             const float Qi[3][3], const float QQ[3], const float QR[3], float comp0,
-                                             float comp1, float a, float dt, double xbar[4][3]) {
-    float P[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) P[i][r] = (float)(x[i][r] - x[3][r]);
-    float g0n[4][3], U0n;
+        for (int r = 0; r < 3; r++) P[i][r] = (R)(x[i][r] - x[3][r]);
+    R g0n[4][3], U0n;
     neo_prefactored(QQ, QR, P, U0n, g0n);
-    float U0 = U0n, IM = 0.0f, cpoly = 0.0f, s = 1.0f, g0[4][3];
+    R U0 = U0n, IM = R(0.0), cpoly = R(0.0), s = R(1.0), g0[4][3];
 #pragma unroll
     for (int n = 0; n < 4; n++)
 #pragma unroll
         for (int r = 0; r < 3; r++) g0[n][r] = g0n[n][r];
     if (yeoh) {
-        IM = U0n - 3.0f;
+        IM = U0n - R(3.0);
         cpoly = C[0] * IM + C[1] * IM * IM + C[2] * IM * IM * IM;
-        U0 = fmaxf(0.0001f, cpoly);
-        s = C[0] + 2.0f * C[1] * IM + 3.0f * C[2] * IM * IM;
+        U0 = fmax(R(0.0001), cpoly);
+        s = C[0] + R(2.0) * C[1] * IM + R(3.0) * C[2] * IM * IM;
 #pragma unroll
         for (int n = 0; n < 4; n++)
 #pragma unroll
             for (int r = 0; r < 3; r++) g0[n][r] = g0n[n][r] * s;
     }
-    float F[3][3], adj[3][3], G[3][3], g1[4][3];
+    R F[3][3], adj[3][3], G[3][3], g1[4][3];
     defgrad(Qi, P, F);
-    float J = adjdet(F, adj);
-    float d = J - a;
-    float U1 = 0.0f + 1.0f * (d * d);
-    float sv = 2.0f * d;
+    R J = adjdet(F, adj);
+    R d = J - a;
+    R U1 = R(0.0) + R(1.0) * (d * d);
+    R sv = R(2.0) * d;
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
@@ -120,85 +68,85 @@ __device__ __forceinline__ float adjoint_tet(bool yeoh, const float C[3], const 
 #pragma unroll
     for (int n = 0; n < 4; n++)
 #pragma unroll
-        for (int r = 0; r < 3; r++) g1[n][r] = 0.0f;
+        for (int r = 0; r < 3; r++) g1[n][r] = R(0.0);
     apply_grad(G, Qi, g1);
     root_grad(g1);
-    float alpha0 = comp0 / (dt * dt), alpha1 = comp1 / (dt * dt);
-    float A0 = 1.0e-22f, A1 = 1.0e-22f, A2 = 1.0e-22f;
+    R alpha0 = comp0 / (dt * dt), alpha1 = comp1 / (dt * dt);
+    R A0 = R(1.0e-22), A1 = R(1.0e-22), A2 = R(1.0e-22);
 #pragma unroll
     for (int n = 0; n < 4; n++) A0 = A0 + w[n] * dot3(g0[n], g0[n]);
-    A0 = A0 + 2.0f * U0 * alpha0;
-    float b0 = -2.0f * U0;
+    A0 = A0 + R(2.0) * U0 * alpha0;
+    R b0 = -R(2.0) * U0;
 #pragma unroll
     for (int n = 0; n < 4; n++) A1 = A1 + w[n] * dot3(g1[n], g0[n]);
 #pragma unroll
     for (int n = 0; n < 4; n++) A2 = A2 + w[n] * dot3(g1[n], g1[n]);
-    A2 = A2 + 2.0f * U1 * alpha1;
-    float b1 = -2.0f * U1;
-    float u = 1.0f / A0, v = 1.0f / A2;
-    float p = A1 * u, q = A1 * v, be0 = b0 * u, be1 = b1 * v;
-    float Dr = 1.0f - p * q;
-    float invDet = 1.0f / fmaxf(0.00000001f, Dr);
-    float r0 = be0 - p * be1, r1 = be1 - be0 * q;
-    float l0 = invDet * r0, l1 = invDet * r1;
+    A2 = A2 + R(2.0) * U1 * alpha1;
+    R b1 = -R(2.0) * U1;
+    R u = R(1.0) / A0, v = R(1.0) / A2;
+    R p = A1 * u, q = A1 * v, be0 = b0 * u, be1 = b1 * v;
+    R Dr = R(1.0) - p * q;
+    R invDet = R(1.0) / fmax(R(0.00000001), Dr);
+    R r0 = be0 - p * be1, r1 = be1 - be0 * q;
+    R l0 = invDet * r0, l1 = invDet * r1;
 
-    float g0b[4][3], g1b[4][3];
-    float l0b = 0.0f, l1b = 0.0f;
+    R g0b[4][3], g1b[4][3];
+    R l0b = R(0.0), l1b = R(0.0);
 #pragma unroll
     for (int n = 0; n < 4; n++)
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            float accb = w[n] * (float)xbar[n][r];
+            R accb = w[n] * (R)xbar[n][r];
             l0b = l0b + g0[n][r] * accb;
             l1b = l1b + g1[n][r] * accb;
             g0b[n][r] = l0 * accb;
             g1b[n][r] = l1 * accb;
         }
-    float invDetb = l0b * r0 + l1b * r1;
-    float r0b = l0b * invDet, r1b = l1b * invDet;
-    float be0b = r0b - r1b * q, be1b = r1b - r0b * p;
-    float pb = 0.0f - r0b * be1, qb = 0.0f - r1b * be0;
-    float Drb = Dr > 0.00000001f ? (0.0f - invDetb * invDet * invDet) : 0.0f;
+    R invDetb = l0b * r0 + l1b * r1;
+    R r0b = l0b * invDet, r1b = l1b * invDet;
+    R be0b = r0b - r1b * q, be1b = r1b - r0b * p;
+    R pb = R(0.0) - r0b * be1, qb = R(0.0) - r1b * be0;
+    R Drb = Dr > R(0.00000001) ? (R(0.0) - invDetb * invDet * invDet) : R(0.0);
     pb = pb - Drb * q;
     qb = qb - Drb * p;
-    float A1b = pb * u + qb * v;
-    float ub = pb * A1 + be0b * b0, vb = qb * A1 + be1b * b1;
-    float b0b = be0b * u, b1b = be1b * v;
-    float A0b = 0.0f - ub * u * u, A2b = 0.0f - vb * v * v;
-    float U0b = -2.0f * b0b + 2.0f * alpha0 * A0b;
-    float U1b = -2.0f * b1b + 2.0f * alpha1 * A2b;
-    float alpha0b = 2.0f * U0 * A0b;
+    R A1b = pb * u + qb * v;
+    R ub = pb * A1 + be0b * b0, vb = qb * A1 + be1b * b1;
+    R b0b = be0b * u, b1b = be1b * v;
+    R A0b = R(0.0) - ub * u * u, A2b = R(0.0) - vb * v * v;
+    R U0b = -R(2.0) * b0b + R(2.0) * alpha0 * A0b;
+    R U1b = -R(2.0) * b1b + R(2.0) * alpha1 * A2b;
+    R alpha0b = R(2.0) * U0 * A0b;
 #pragma unroll
     for (int n = 0; n < 4; n++)
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            g0b[n][r] = g0b[n][r] + 2.0f * A0b * w[n] * g0[n][r] + A1b * w[n] * g1[n][r];
-            g1b[n][r] = g1b[n][r] + A1b * w[n] * g0[n][r] + 2.0f * A2b * w[n] * g1[n][r];
+            g0b[n][r] = g0b[n][r] + R(2.0) * A0b * w[n] * g0[n][r] + A1b * w[n] * g1[n][r];
+            g1b[n][r] = g1b[n][r] + A1b * w[n] * g0[n][r] + R(2.0) * A2b * w[n] * g1[n][r];
         }
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int r = 0; r < 3; r++) g1b[i][r] = g1b[i][r] - g1b[3][r];
-    float adjb[3][3], Fb[3][3], Pb[3][3];
-    float svb = 0.0f;
+    R adjb[3][3], Fb[3][3], Pb[3][3];
+    R svb = R(0.0);
 #pragma unroll
     for (int m = 0; m < 3; m++)
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            float Gb = g1b[0][r] * Qi[0][m] + g1b[1][r] * Qi[1][m] + g1b[2][r] * Qi[2][m];
+            R Gb = g1b[0][r] * Qi[0][m] + g1b[1][r] * Qi[1][m] + g1b[2][r] * Qi[2][m];
             adjb[m][r] = Gb * sv;
             svb = svb + Gb * adj[m][r];
         }
-    float db = 2.0f * svb + 2.0f * d * U1b;
+    R db = R(2.0) * svb + R(2.0) * d * U1b;
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int r = 0; r < 3; r++) Fb[c][r] = db * adj[r][c];
     {
-        float m00 = F[0][0], m01 = F[0][1], m02 = F[0][2];
-        float m10 = F[1][0], m11 = F[1][1], m12 = F[1][2];
-        float m20 = F[2][0], m21 = F[2][1], m22 = F[2][2];
-        float t;
+        R m00 = F[0][0], m01 = F[0][1], m02 = F[0][2];
+        R m10 = F[1][0], m11 = F[1][1], m12 = F[1][2];
+        R m20 = F[2][0], m21 = F[2][1], m22 = F[2][2];
+        R t;
         t = adjb[0][0]; Fb[1][1] += t * m22; Fb[2][2] += t * m11; Fb[1][2] -= t * m21; Fb[2][1] -= t * m12;
         t = adjb[0][1]; Fb[0][1] -= t * m22; Fb[2][2] -= t * m01; Fb[0][2] += t * m21; Fb[2][1] += t * m02;
         t = adjb[0][2]; Fb[0][1] += t * m12; Fb[1][2] += t * m01; Fb[0][2] -= t * m11; Fb[1][1] -= t * m02;
@@ -213,10 +161,10 @@ __device__ __forceinline__ float adjoint_tet(bool yeoh, const float C[3], const 
     for (int m = 0; m < 3; m++)
 #pragma unroll
         for (int j = 0; j < 3; j++) Pb[m][j] = Qi[m][0] * Fb[j][0] + Qi[m][1] * Fb[j][1] + Qi[m][2] * Fb[j][2];
-    float g0nb[4][3], U0nb;
+    R g0nb[4][3], U0nb;
     if (yeoh) {
-        float polyb = cpoly > 0.0001f ? U0b : 0.0f;
-        float sb = 0.0f;
+        R polyb = cpoly > R(0.0001) ? U0b : R(0.0);
+        R sb = R(0.0);
 #pragma unroll
         for (int n = 0; n < 4; n++)
 #pragma unroll
@@ -224,7 +172,7 @@ __device__ __forceinline__ float adjoint_tet(bool yeoh, const float C[3], const 
                 sb = sb + g0b[n][r] * g0n[n][r];
                 g0nb[n][r] = g0b[n][r] * s;
             }
-        U0nb = polyb * s + sb * (2.0f * C[1] + 6.0f * C[2] * IM);
+        U0nb = polyb * s + sb * (R(2.0) * C[1] + R(6.0) * C[2] * IM);
     } else {
 #pragma unroll
         for (int n = 0; n < 4; n++)
@@ -239,7 +187,7 @@ __device__ __forceinline__ float adjoint_tet(bool yeoh, const float C[3], const 
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int r = 0; r < 3; r++) Pb[i][r] = Pb[i][r] + U0nb * g0n[i][r] + 2.0f * QQ[i] * g0nb[i][r];
+        for (int r = 0; r < 3; r++) Pb[i][r] = Pb[i][r] + U0nb * g0n[i][r] + R(2.0) * QQ[i] * g0nb[i][r];
     {
         int k = 0;
 #pragma unroll

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "xf_device.h"
+#include "xf_element.h"
 #include "xf_sched.h"
 
 using xf::colour_grid;
@@ -22,62 +23,15 @@ using xf::colour_slot;
 
 namespace {
 
-__device__ __forceinline__ float dot3(const float* a, const float* b) {
-    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-
-// F = Qi * MatMul(dN, P), dN = e_x, e_y, e_z (Fem.cpp:312-323).
-__device__ __forceinline__ void defgrad(const float Qi[3][3], const float P[3][3], float F[3][3]) {
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) F[j][r] = Qi[0][r] * P[0][j] + Qi[1][r] * P[1][j] + Qi[2][r] * P[2][j];
-}
-
-// adjugate + determinantFromAdjugate (Fem.cpp:284-306).
-__device__ __forceinline__ float adjdet(const float F[3][3], float adj[3][3]) {
-    float m00 = F[0][0], m01 = F[0][1], m02 = F[0][2];
-    float m10 = F[1][0], m11 = F[1][1], m12 = F[1][2];
-    float m20 = F[2][0], m21 = F[2][1], m22 = F[2][2];
-    adj[0][0] = +(m11 * m22 - m12 * m21);
-    adj[0][1] = -(m01 * m22 - m02 * m21);
-    adj[0][2] = +(m01 * m12 - m02 * m11);
-    adj[1][0] = -(m10 * m22 - m12 * m20);
-    adj[1][1] = +(m00 * m22 - m02 * m20);
-    adj[1][2] = -(m00 * m12 - m02 * m10);
-    adj[2][0] = +(m10 * m21 - m11 * m20);
-    adj[2][1] = -(m00 * m21 - m01 * m20);
-    adj[2][2] = +(m00 * m11 - m01 * m10);
-    return F[0][0] * adj[0][0] + F[0][1] * adj[1][0] + F[0][2] * adj[2][0];
-}
-
-// ApplyEnergyGradient at the single point (Fem.cpp:339-354): g[n] += column n of G*Qi.
-__device__ __forceinline__ void apply_grad(const float G[3][3], const float Qi[3][3], float g[4][3]) {
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) g[j][r] = g[j][r] + (G[0][r] * Qi[j][0] + G[1][r] * Qi[j][1] + G[2][r] * Qi[j][2]);
-}
-
-__device__ __forceinline__ void root_grad(float g[4][3]) {
-#pragma unroll
-    for (int r = 0; r < 3; r++) g[3][r] = ((0.0f - g[0][r]) - g[1][r]) - g[2][r];
-}
-
-// (J - a)^2 volume term (Fem.cpp:537-547).
-__device__ __forceinline__ void volume_term(const float Qi[3][3], const float P[3][3], float a, float& U1,
-                                            float g1[4][3]) {
-    float F[3][3], adj[3][3], GJ[3][3];
-    defgrad(Qi, P, F);
-    float J = adjdet(F, adj);
-    U1 = U1 + 1.0f * ((J - a) * (J - a));
-    float s = 2.0f * (J - a);
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) GJ[c][r] = adj[c][r] * s;
-    apply_grad(GJ, Qi, g1);
-}
+using xf::adjdet;
+using xf::apply_grad;
+using xf::constrain2;
+using xf::defgrad;
+using xf::dot3;
+using xf::gather_p;
+using xf::neo_prefactored;
+using xf::root_grad;
+using xf::volume_term;
 
 // I1 term by one-point quadrature (energies 3/5/6, Fem.cpp:461-477).
 __device__ __forceinline__ void i1_quad_term(bool yeoh, const float C[3], const float Qi[3][3], const float P[3][3],
@@ -101,40 +55,6 @@ __device__ __forceinline__ void i1_quad_term(bool yeoh, const float C[3], const 
     apply_grad(G, Qi, g0);
 }
 
-// Prefactored I1 (Fem.cpp:164-192).
-__device__ __forceinline__ void neo_prefactored(const float QQ[3], const float QR[3], const float P[3][3], float& U,
-                                                float g[4][3]) {
-    float u = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        u = u + QQ[i] * dot3(P[i], P[i]);
-#pragma unroll
-        for (int r = 0; r < 3; r++) g[i][r] = (2.0f * QQ[i]) * P[i][r];
-    }
-    int k = 0;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-#pragma unroll
-        for (int j = i + 1; j < 3; j++) {
-            u = u + QR[k] * dot3(P[i], P[j]);
-#pragma unroll
-            for (int r = 0; r < 3; r++) g[i][r] = g[i][r] + QR[k] * P[j][r];
-#pragma unroll
-            for (int r = 0; r < 3; r++) g[j][r] = g[j][r] + QR[k] * P[i][r];
-            ++k;
-        }
-    }
-    root_grad(g);
-    U = u;
-}
-
-__device__ __forceinline__ void gather_p(const double x[4][3], float P[3][3]) {
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) P[i][r] = (float)(x[i][r] - x[3][r]);
-}
-
 // EnergyXpbdConstrain without damping (Xpbd.h:110-119).
 __device__ __forceinline__ void constrain1(double x[4][3], const float w[4], float U, const float g[4][3], float dt,
                                            float comp) {
@@ -148,38 +68,6 @@ __device__ __forceinline__ void constrain1(double x[4][3], const float w[4], flo
         float s = w[n] * lp;
 #pragma unroll
         for (int r = 0; r < 3; r++) x[n][r] = x[n][r] + (double)(s * g[n][r]);
-    }
-}
-
-// EnergyXpbdConstrainSimultaneous<2> without damping, Cramer branch (Xpbd.h:123-214).
-__device__ __forceinline__ void constrain2(double x[4][3], const float w[4], const float U[2], const float g0[4][3],
-                                           const float g1[4][3], float dt, float comp0, float comp1) {
-    float alpha0 = comp0 / (dt * dt);
-    float alpha1 = comp1 / (dt * dt);
-    float A0 = 1.0e-22f, A1 = 1.0e-22f, A2 = 1.0e-22f;
-#pragma unroll
-    for (int n = 0; n < 4; n++) A0 = A0 + w[n] * dot3(g0[n], g0[n]);
-    A0 = A0 + 2.0f * U[0] * alpha0;
-    float b0 = -2.0f * U[0];
-#pragma unroll
-    for (int n = 0; n < 4; n++) A1 = A1 + w[n] * dot3(g1[n], g0[n]);
-#pragma unroll
-    for (int n = 0; n < 4; n++) A2 = A2 + w[n] * dot3(g1[n], g1[n]);
-    A2 = A2 + 2.0f * U[1] * alpha1;
-    float b1 = -2.0f * U[1];
-    float invA00 = 1.0f / A0;
-    float invA11 = 1.0f / A2;
-    float invDet = 1.0f / fmaxf(0.00000001f, 1.0f - (A1 * invA00) * (A1 * invA11));
-    float l0 = invDet * ((b0 * invA00) - (A1 * invA00) * (b1 * invA11));
-    float l1 = invDet * ((b1 * invA11) - (b0 * invA00) * (A1 * invA11));
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            float acc = 0.0f + l0 * g0[n][r];
-            acc = acc + l1 * g1[n][r];
-            x[n][r] = x[n][r] + (double)(w[n] * acc);
-        }
     }
 }
 

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include "xf_device.h"
+#include "xf_element.h"
 #include "xf_sched.h"
 
 namespace {
@@ -60,27 +61,8 @@ struct Team {
     __device__ float* tet(uint32_t k) const { return c + k * kTetFloats; }
 };
 
-__device__ __forceinline__ float dot3(const float* a, const float* b) {
-    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-
-// adjugate(mat3) and determinantFromAdjugate (Fem.cpp:284-306); F[c][r] is
-// column c, row r.
-__device__ __forceinline__ float adjdet(const float F[3][3], float adj[3][3]) {
-    float m00 = F[0][0], m01 = F[0][1], m02 = F[0][2];
-    float m10 = F[1][0], m11 = F[1][1], m12 = F[1][2];
-    float m20 = F[2][0], m21 = F[2][1], m22 = F[2][2];
-    adj[0][0] = +(m11 * m22 - m12 * m21);
-    adj[0][1] = -(m01 * m22 - m02 * m21);
-    adj[0][2] = +(m01 * m12 - m02 * m11);
-    adj[1][0] = -(m10 * m22 - m12 * m20);
-    adj[1][1] = +(m00 * m22 - m02 * m20);
-    adj[1][2] = -(m00 * m12 - m02 * m10);
-    adj[2][0] = +(m10 * m21 - m11 * m20);
-    adj[2][1] = -(m00 * m21 - m01 * m20);
-    adj[2][2] = +(m00 * m11 - m01 * m10);
-    return F[0][0] * adj[0][0] + F[0][1] * adj[1][0] + F[0][2] * adj[2][0];
-}
+using xf::adjdet;
+using xf::dot3;
 
 // Phase 1 for one ring tet (CalculateElementEnergyAndGradients for a linear
 // tet, Fem.cpp:941-973): out[0] = w_t I1, out[1] = w_t J, out[2 + 3 j + r]
