@@ -659,6 +659,7 @@ def main():
         mode, size = opts.get("launch_mode", xf.LAUNCH_PER_COLOUR), 64
     else:
         mode, size = sc.autotune(S, dt, reps=2)[:2]
+    info = sc.info()  # the slot packing may have changed with the tuned cap
 
     for _ in range(args.warmup):
         sc.step(S, dt)
@@ -702,7 +703,8 @@ def main():
         with open(tpath) as f:
             rec = json.load(f)
         if mode == xf.LAUNCH_BLOCKS and "blocks" in rec and rec["blocks"].get("n_blocks") == info["n_blocks"] and \
-                rec["blocks"].get("block_size") == size and sc.info()["tet_lanes"] == 1:
+                rec["blocks"].get("block_size") == size and info["tet_lanes"] == 1 and \
+                rec["blocks"].get("block_lanes", 64) == info["block_lanes"]:
             traffic = float(rec["blocks"]["traffic_bytes_per_dispatch"])
         elif mode == xf.LAUNCH_PER_COLOUR and args.schedule != "blocks" and \
                 rec.get("tets_per_dispatch") == T // max(info["n_colours"], 1):
@@ -754,7 +756,8 @@ def main():
         legs += [("adjoint", adjoint_leg), ("fit", fit_leg)]
     if not args.no_cpu:
         # the one-frame checks run at the timed launch shape
-        shape = dict(opts, launch_mode=mode, use_graph=True, tet_lanes=sc.info()["tet_lanes"],
+        shape = dict(opts, launch_mode=mode, use_graph=True, tet_lanes=info["tet_lanes"],
+                     block_lanes=info["block_lanes"] if mode == xf.LAUNCH_BLOCKS else opts.get("block_lanes", 0),
                      **({"persistent_blocks": size} if mode == xf.LAUNCH_PERSISTENT else {"block_size": size}))
         legs.append(("cpu_baseline", lambda c: cpu_baseline(c, wl, opts=shape)))
     for name, leg in legs:

@@ -337,10 +337,13 @@ XF_API int xf_scene_time_steps(xf_scene* scene, uint32_t calls, uint32_t substep
 /* Autotuner: times `reps` calls of xf_scene_step(substeps, dt) for each
  * launch shape -- per-colour launches with workgroup sizes 64, 128, 256 and
  * the persistent launch with 64, 128, 256 and 512 blocks (where co-resident),
- * and sweep-block launches with workgroup sizes 64, 128, 256 and 1 or 4
- * lanes per tet (4 where the projection allows it) when the scene has
- * sweep_blocks -- and keeps the fastest in the scene options. The node state and clock are restored afterwards, so tuning
- * does not advance the simulation. chosen / best_ms (ms per call) nullable.
+ * and, when the scene has sweep_blocks, sweep-block launches with
+ * workgroup sizes 64, 128, 256, 1 or 4 lanes per tet (4 where the
+ * projection allows it) and slot caps (block_lanes) 32, 64, 128 and 256,
+ * the slots repacked for each cap -- and keeps the fastest in the scene
+ * options (block_lanes is set when the chosen cap differs from the one in
+ * use). The node state and clock are restored afterwards, so tuning does
+ * not advance the simulation. chosen / best_ms (ms per call) nullable.
  * No reference counterpart (the reference is single-threaded). */
 XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint32_t reps, xf_scene_opts* chosen,
                              float* best_ms);

@@ -274,6 +274,35 @@ def test_blocks_autotune_keeps_results(ctx):
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
 
 
+def test_blocks_autotune_slot_caps(ctx):
+    """The autotuner repacks the sweep-block slots for each slot cap it
+    tries: the beam steps bit-exactly at every cap set explicitly, and after
+    tuning at the cap it chose (reported by info and in the chosen options)."""
+    wl = workloads.beam_yeoh(24, 8, 8)
+    S, dt = wl["substeps"], float(wl["dt"])
+    ref = None
+    slots = {}
+    for cap in (32, 64, 128, 256):
+        sc = workloads.make_scene(ctx, wl, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS, block_lanes=cap)
+        sc.step(S, dt)
+        x, v = sc.get_state(0)
+        slots[cap] = sc.info()["n_block_slots"]
+        assert sc.info()["block_lanes"] == cap
+        sc.close()
+        if ref is None:
+            ref = (x, v)
+        assert np.array_equal(x, ref[0]) and np.array_equal(v, ref[1]), cap
+    assert slots[32] > slots[256], slots
+    sc = workloads.make_scene(ctx, wl, sweep_blocks=True)
+    mode, size, ms = sc.autotune(S, dt, reps=1)
+    info = sc.info()
+    assert info["block_lanes"] in slots and info["n_block_slots"] == slots[info["block_lanes"]]
+    sc.step(S, dt)
+    x, v = sc.get_state(0)
+    sc.close()
+    assert np.array_equal(x, ref[0]) and np.array_equal(v, ref[1])
+
+
 def test_blocks_rejections(ctx):
     wl = workloads.beam_neo_12x4x4()
     cases = [dict(launch_mode=xf.LAUNCH_BLOCKS), dict(sweep_blocks=True, xcd_slabs=True),

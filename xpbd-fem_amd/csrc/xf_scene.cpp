@@ -292,6 +292,11 @@ struct xf_scene {
     uint32_t n_blocks = 0, blk_nmax = 0, blk_nodes = 0, blk_pmax = 0, blk_lanes = 0;
     uint32_t* blk_rec = nullptr;
     uint32_t* blk_node = nullptr;
+    size_t blk_rec_n = 0, blk_node_n = 0, blk_fin_n = 0;  // their allocated lengths
+    // the blocks of every body and the first colour of each launch colour,
+    // kept for repacking the slots (xf_scene_autotune)
+    std::vector<xf::SweepBlocks> sweep;
+    uint32_t blk_base[4] = {0, 0, 0, 0};
     // fused node phase: the last launch colour with blocks (blk_last; -1 with
     // fewer than two such colours, more than two of its slots per compute
     // unit, or when its nodes with their O and ids do not fit LDS) ends the
@@ -878,6 +883,257 @@ int Prepare(xf_scene* s, float dt, int* energy, bool* sim) {
     return XF_OK;
 }
 
+// Host tables of the sweep-block launches (uploaded to blk_rec, blk_node,
+// tloc and blk_fin).
+struct BlockTables {
+    std::vector<uint2> tloc;
+    std::vector<uint32_t> rec, node, fin;
+};
+
+// Sweep-block tables for the slot cap lanes_opt (0: automatic). Blocks of
+// launch colour q in layout order (body, then block number: the order of
+// their tets inside each colour q, j) are merged, consecutive ones, into
+// workgroup slots while every phase of the slot holds at most lanes_opt
+// tets and its nodes fit LDS; a slot's
+// tets of phase j are then one contiguous range. Slots are cut into
+// eighths and dealt round-robin, so that under round-robin dispatch over
+// the 8 XCDs each XCD sweeps one contiguous run of blocks (neighbouring
+// blocks share node lines).
+int PackBlocks(xf_scene* s, uint32_t lanes_opt, BlockTables* bt) {
+    xf_ctx* ctx = s->ctx;
+    *bt = BlockTables();
+    constexpr uint32_t kSlotNodes = 2048;
+    uint32_t nblk[4] = {0, 0, 0, 0};
+    std::vector<std::vector<uint32_t>> lay(s->bodies.size());
+    for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+        const xf::SweepBlocks& sb = s->sweep[bi];
+        lay[bi].resize(sb.n_blocks);
+        for (uint32_t k = 0; k < sb.n_blocks; k++) lay[bi][k] = nblk[sb.launch[k]]++;
+    }
+    s->n_blocks = nblk[0] + nblk[1] + nblk[2] + nblk[3];
+    // per block: sorted node list; per schedule position: the block's layout index
+    std::vector<std::vector<uint32_t>> bnodes[4];
+    for (int q = 0; q < 4; q++) bnodes[q].resize(nblk[q]);
+    std::vector<uint32_t> sp(s->T, 0);
+    for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+        const Body& b = s->bodies[bi];
+        const xf::SweepBlocks& sb = s->sweep[bi];
+        for (uint32_t t = 0; t < b.t; t++) {
+            uint32_t k = sb.block[t], q = sb.launch[k], m = lay[bi][k];
+            for (int i = 0; i < 4; i++) bnodes[q][m].push_back(b.node_base + b.tets[4 * (size_t)t + i]);
+            sp[b.sched_pos[t]] = m;
+        }
+    }
+    for (int q = 0; q < 4; q++)
+        for (std::vector<uint32_t>& v : bnodes[q]) {
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            if (v.size() > kSlotNodes)
+                return Fail(ctx, XF_E_UNSUPPORTED, "a sweep block has %zu nodes (LDS holds %u)", v.size(),
+                            kSlotNodes);
+        }
+    // per block and phase: first schedule position and count (walk of each colour)
+    std::vector<uint32_t> bst[4], bct[4];
+    for (int q = 0; q < 4; q++) {
+        uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
+        bst[q].assign((size_t)nblk[q] * nph, 0);
+        bct[q].assign((size_t)nblk[q] * nph, 0);
+        for (uint32_t j = 0; j < nph; j++) {
+            uint32_t c = s->blk_base[q] + j, p = s->colour_begin[c], end = s->colour_begin[c + 1];
+            for (uint32_t m = 0; m < nblk[q]; m++) {
+                uint32_t p0 = p;
+                while (p < end && sp[p] == m) p++;
+                bst[q][(size_t)m * nph + j] = p0;
+                bct[q][(size_t)m * nph + j] = p - p0;
+            }
+            if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
+        }
+    }
+    // slots: runs of consecutive blocks, each phase at most `lanes_cap` tets
+    std::vector<uint32_t> gfirst[4];  // per slot (in layout order): first block; sentinel at the end
+    auto group = [&](uint32_t lanes_cap) {
+        for (int q = 0; q < 4; q++) {
+            gfirst[q].clear();
+            uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
+            std::vector<uint32_t> lanes(nph, 0);
+            uint32_t nodes = 0;
+            for (uint32_t m = 0; m < nblk[q]; m++) {
+                bool fits = !gfirst[q].empty() && nodes + bnodes[q][m].size() <= kSlotNodes;
+                for (uint32_t j = 0; j < nph && fits; j++)
+                    fits = lanes[j] + bct[q][(size_t)m * nph + j] <= lanes_cap;
+                if (!fits) {
+                    gfirst[q].push_back(m);
+                    std::fill(lanes.begin(), lanes.end(), 0u);
+                    nodes = 0;
+                }
+                for (uint32_t j = 0; j < nph; j++) lanes[j] += bct[q][(size_t)m * nph + j];
+                nodes += (uint32_t)bnodes[q][m].size();
+            }
+            gfirst[q].push_back(nblk[q]);
+        }
+    };
+    int n_cu = 256;
+    XF_HIP(ctx, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    uint32_t lanes_cap = lanes_opt;
+    if (lanes_cap == 0) {
+        // the largest cap that leaves >= 4 slots per compute unit in every launch colour
+        for (uint32_t cap : {256u, 128u, 64u}) {
+            group(cap);
+            lanes_cap = cap;
+            bool enough = true;
+            for (int q = 0; q < 4; q++)
+                if (nblk[q] && gfirst[q].size() - 1 < 4 * (size_t)n_cu) enough = false;
+            if (enough) break;
+        }
+    } else {
+        group(lanes_cap);
+    }
+    s->blk_lanes = lanes_cap;
+    uint32_t nslot[4], slot_base[5] = {0, 0, 0, 0, 0};
+    std::vector<uint32_t> deal[4];  // per launch colour: slot in layout order -> dispatched slot
+    for (int q = 0; q < 4; q++) {
+        uint32_t n = nslot[q] = (uint32_t)gfirst[q].size() - 1, next = 0;
+        slot_base[q + 1] = slot_base[q] + n;
+        deal[q].assign(n, 0);
+        for (uint32_t m = 0; next < n; m++)
+            for (uint32_t r = 0; r < 8; r++) {
+                uint32_t lo = (uint32_t)(((uint64_t)n * r) / 8), hi = (uint32_t)(((uint64_t)n * (r + 1)) / 8);
+                if (lo + m < hi) deal[q][lo + m] = next++;
+            }
+    }
+    // slot node lists (member blocks' nodes are disjoint), records, local indices
+    std::vector<std::vector<uint32_t>> snodes(slot_base[4]);
+    std::vector<uint32_t> slot_of_block[4];
+    s->blk_nmax = s->blk_nodes = s->blk_pmax = 0;
+    for (int q = 0; q < 4; q++) {
+        slot_of_block[q].assign(nblk[q], 0);
+        for (uint32_t g = 0; g < nslot[q]; g++) {
+            std::vector<uint32_t>& v = snodes[slot_base[q] + deal[q][g]];
+            for (uint32_t m = gfirst[q][g]; m < gfirst[q][g + 1]; m++) {
+                slot_of_block[q][m] = deal[q][g];
+                v.insert(v.end(), bnodes[q][m].begin(), bnodes[q][m].end());
+            }
+            std::sort(v.begin(), v.end());
+            s->blk_nmax = std::max(s->blk_nmax, (uint32_t)v.size());
+            s->blk_nodes += (uint32_t)v.size();
+        }
+    }
+    bt->tloc.assign(s->T, make_uint2(0u, 0u));
+    for (size_t bi = 0; bi < s->bodies.size(); bi++) {
+        const Body& b = s->bodies[bi];
+        const xf::SweepBlocks& sb = s->sweep[bi];
+        for (uint32_t t = 0; t < b.t; t++) {
+            uint32_t k = sb.block[t], q = sb.launch[k];
+            const std::vector<uint32_t>& v = snodes[slot_base[q] + slot_of_block[q][lay[bi][k]]];
+            uint32_t l[4];
+            for (int i = 0; i < 4; i++)
+                l[i] = (uint32_t)(std::lower_bound(v.begin(), v.end(), b.node_base + b.tets[4 * (size_t)t + i]) -
+                                  v.begin());
+            bt->tloc[b.sched_pos[t]] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+        }
+    }
+    bt->node.assign((size_t)slot_base[4] * s->blk_nmax, xf::kNoNode);
+    for (uint32_t g = 0; g < slot_base[4]; g++)
+        std::copy(snodes[g].begin(), snodes[g].end(), bt->node.begin() + (size_t)g * s->blk_nmax);
+    // fused node phase: per slot of the last launch colour, its node row
+    // followed by its share of the nodes outside that colour's slots (in
+    // ascending order, dealt in equal runs)
+    int launches = 0;
+    s->blk_last = -1;
+    for (int q = 0; q < 4; q++)
+        if (nslot[q]) {
+            launches++;
+            s->blk_last = q;
+        }
+    s->blk_fmax = 0;
+    bt->fin.clear();
+    std::vector<uint32_t> extra;
+    int cus = 256;
+    XF_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    // fused only where the last launch is latency-bound (at most two
+    // workgroups per compute unit): with more, the longer write-back and
+    // the LDS of O cost more than the node launch they save
+    if (launches < 2 || nslot[s->blk_last] > 2 * (uint32_t)cus) {
+        s->blk_last = -1;
+    } else {
+        const int q = s->blk_last;
+        std::vector<uint8_t> covered(s->N, 0);
+        for (uint32_t g = slot_base[q]; g < slot_base[q + 1]; g++)
+            for (uint32_t v : snodes[g]) covered[v] = 1;
+        for (uint32_t v = 0; v < s->N; v++)
+            if (!covered[v]) extra.push_back(v);
+        s->blk_fmax = (uint32_t)((extra.size() + nslot[q] - 1) / nslot[q]);
+        // staged with O and ids in LDS; else the node phase stays separate
+        if (xf_block_fin_lds_bytes(s->blk_nmax, s->blk_fmax) > kBlockLdsLimit) {
+            s->blk_last = -1;
+            s->blk_fmax = 0;
+        }
+    }
+    if (s->blk_last >= 0) {
+        const int q = s->blk_last;
+        const uint32_t fmax = s->blk_fmax;
+        const size_t row = (size_t)s->blk_nmax + fmax;
+        bt->fin.assign((size_t)nslot[q] * row, xf::kNoNode);
+        for (uint32_t b = 0; b < nslot[q]; b++) {
+            const uint32_t* bn = bt->node.data() + (size_t)(slot_base[q] + b) * s->blk_nmax;
+            std::copy(bn, bn + s->blk_nmax, bt->fin.begin() + b * row);
+            for (uint32_t j = 0; j < fmax && (size_t)b * fmax + j < extra.size(); j++)
+                bt->fin[b * row + s->blk_nmax + j] = extra[(size_t)b * fmax + j];
+        }
+    }
+    for (int q = 0; q < 4; q++) {
+        xf_scene::BlockLaunch& L = s->blk[q];
+        L.nslot = nslot[q];
+        if (L.nslot == 0) L.nph = 0;
+        L.rec_off = (uint32_t)bt->rec.size();
+        L.node_off = slot_base[q] * s->blk_nmax;
+        bt->rec.resize(bt->rec.size() + 2 * (size_t)L.nslot * L.nph, 0u);
+        for (uint32_t g = 0; g < L.nslot; g++)
+            for (uint32_t j = 0; j < L.nph; j++) {
+                uint32_t first = gfirst[q][g], count = 0;
+                for (uint32_t m = first; m < gfirst[q][g + 1]; m++) count += bct[q][(size_t)m * L.nph + j];
+                size_t at = L.rec_off + 2 * ((size_t)deal[q][g] * L.nph + j);
+                bt->rec[at] = bst[q][(size_t)first * L.nph + j];
+                bt->rec[at + 1] = count;
+                s->blk_pmax = std::max(s->blk_pmax, count);
+            }
+    }
+    return XF_OK;
+}
+
+// Uploads sweep-block tables, growing the device arrays when a packing needs
+// more room than they hold. The stream is idle on return.
+int UploadBlocks(xf_scene* s, const BlockTables& bt) {
+    xf_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    XF_HIP(ctx, hipStreamSynchronize(st));
+    auto fit = [&](uint32_t** p, size_t* have, size_t need) -> int {
+        if (*p && *have >= need) return XF_OK;
+        if (*p) {
+            (void)hipFree(*p);
+            s->device_bytes -= sizeof(uint32_t) * std::max<size_t>(*have, 1);
+            *p = nullptr;
+        }
+        *have = need;
+        return DevAlloc(ctx, p, need, &s->device_bytes);
+    };
+    int rc;
+    if ((rc = fit(&s->blk_rec, &s->blk_rec_n, bt.rec.size())) || (rc = fit(&s->blk_node, &s->blk_node_n, bt.node.size())) ||
+        (!bt.fin.empty() && (rc = fit(&s->blk_fin, &s->blk_fin_n, bt.fin.size()))))
+        return rc;
+    if (!s->tloc && (rc = DevAlloc(ctx, &s->tloc, s->T, &s->device_bytes))) return rc;
+    if (!bt.rec.empty())
+        XF_HIP(ctx, hipMemcpyAsync(s->blk_rec, bt.rec.data(), sizeof(uint32_t) * bt.rec.size(), hipMemcpyHostToDevice, st));
+    if (!bt.node.empty())
+        XF_HIP(ctx, hipMemcpyAsync(s->blk_node, bt.node.data(), sizeof(uint32_t) * bt.node.size(), hipMemcpyHostToDevice,
+                                   st));
+    XF_HIP(ctx, hipMemcpyAsync(s->tloc, bt.tloc.data(), sizeof(uint2) * s->T, hipMemcpyHostToDevice, st));
+    if (!bt.fin.empty())
+        XF_HIP(ctx, hipMemcpyAsync(s->blk_fin, bt.fin.data(), sizeof(uint32_t) * bt.fin.size(), hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, hipStreamSynchronize(st));
+    return XF_OK;
+}
+
 int CheckSettings(xf_ctx* ctx, const xf_settings* st) {
     if (!EnergySupported(st->energy))
         return Fail(ctx, XF_E_INVALID, "energy %u out of range (the Settings field has 5 bits)", st->energy);
@@ -1099,8 +1355,9 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
 
     // Sweep blocks: colour (q, j) = phase j of the blocks of launch colour q,
     // numbered over the launch colours that hold blocks.
-    std::vector<xf::SweepBlocks> sweep(s->opts.sweep_blocks ? s->bodies.size() : 0);
-    uint32_t blk_base[4] = {0, 0, 0, 0};
+    std::vector<xf::SweepBlocks>& sweep = s->sweep;
+    sweep.assign(s->opts.sweep_blocks ? s->bodies.size() : 0, xf::SweepBlocks());
+    uint32_t* blk_base = s->blk_base;
     if (s->opts.sweep_blocks) {
         uint32_t nph[4] = {0, 0, 0, 0};
         for (size_t bi = 0; bi < s->bodies.size(); bi++) {
@@ -1222,213 +1479,10 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     s->colour_begin[s->ncol] = pos;
     if (pos != s->T) return Fail(ctx, XF_E_INVALID, "internal: schedule covers %u of %u tets", pos, s->T);
 
-    // Sweep-block tables. Blocks of launch colour q in layout order (body,
-    // then block number: the order of their tets inside each colour q, j)
-    // are merged, consecutive ones, into workgroup slots while every phase of
-    // the slot holds at most block_lanes tets and its nodes fit LDS; a slot's
-    // tets of phase j are then one contiguous range. Slots are cut into
-    // eighths and dealt round-robin, so that under round-robin dispatch over
-    // the 8 XCDs each XCD sweeps one contiguous run of blocks (neighbouring
-    // blocks share node lines).
-    std::vector<uint2> h_tloc;
-    std::vector<uint32_t> h_brec, h_bnode, h_bfin;
+    BlockTables bt;
     if (s->opts.sweep_blocks) {
-        constexpr uint32_t kSlotNodes = 2048;
-        uint32_t nblk[4] = {0, 0, 0, 0};
-        std::vector<std::vector<uint32_t>> lay(s->bodies.size());
-        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
-            const xf::SweepBlocks& sb = sweep[bi];
-            lay[bi].resize(sb.n_blocks);
-            for (uint32_t k = 0; k < sb.n_blocks; k++) lay[bi][k] = nblk[sb.launch[k]]++;
-        }
-        s->n_blocks = nblk[0] + nblk[1] + nblk[2] + nblk[3];
-        // per block: sorted node list; per schedule position: the block's layout index
-        std::vector<std::vector<uint32_t>> bnodes[4];
-        for (int q = 0; q < 4; q++) bnodes[q].resize(nblk[q]);
-        std::vector<uint32_t> sp(s->T, 0);
-        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
-            const Body& b = s->bodies[bi];
-            const xf::SweepBlocks& sb = sweep[bi];
-            for (uint32_t t = 0; t < b.t; t++) {
-                uint32_t k = sb.block[t], q = sb.launch[k], m = lay[bi][k];
-                for (int i = 0; i < 4; i++) bnodes[q][m].push_back(b.node_base + b.tets[4 * (size_t)t + i]);
-                sp[b.sched_pos[t]] = m;
-            }
-        }
-        for (int q = 0; q < 4; q++)
-            for (std::vector<uint32_t>& v : bnodes[q]) {
-                std::sort(v.begin(), v.end());
-                v.erase(std::unique(v.begin(), v.end()), v.end());
-                if (v.size() > kSlotNodes)
-                    return Fail(ctx, XF_E_UNSUPPORTED, "a sweep block has %zu nodes (LDS holds %u)", v.size(),
-                                kSlotNodes);
-            }
-        // per block and phase: first schedule position and count (walk of each colour)
-        std::vector<uint32_t> bst[4], bct[4];
-        for (int q = 0; q < 4; q++) {
-            uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
-            bst[q].assign((size_t)nblk[q] * nph, 0);
-            bct[q].assign((size_t)nblk[q] * nph, 0);
-            for (uint32_t j = 0; j < nph; j++) {
-                uint32_t c = blk_base[q] + j, p = s->colour_begin[c], end = s->colour_begin[c + 1];
-                for (uint32_t m = 0; m < nblk[q]; m++) {
-                    uint32_t p0 = p;
-                    while (p < end && sp[p] == m) p++;
-                    bst[q][(size_t)m * nph + j] = p0;
-                    bct[q][(size_t)m * nph + j] = p - p0;
-                }
-                if (p != end) return Fail(ctx, XF_E_INVALID, "internal: sweep-block table of colour %u", c);
-            }
-        }
-        // slots: runs of consecutive blocks, each phase at most `lanes_cap` tets
-        std::vector<uint32_t> gfirst[4];  // per slot (in layout order): first block; sentinel at the end
-        auto group = [&](uint32_t lanes_cap) {
-            for (int q = 0; q < 4; q++) {
-                gfirst[q].clear();
-                uint32_t nph = nblk[q] ? s->blk[q].nph : 0;
-                std::vector<uint32_t> lanes(nph, 0);
-                uint32_t nodes = 0;
-                for (uint32_t m = 0; m < nblk[q]; m++) {
-                    bool fits = !gfirst[q].empty() && nodes + bnodes[q][m].size() <= kSlotNodes;
-                    for (uint32_t j = 0; j < nph && fits; j++)
-                        fits = lanes[j] + bct[q][(size_t)m * nph + j] <= lanes_cap;
-                    if (!fits) {
-                        gfirst[q].push_back(m);
-                        std::fill(lanes.begin(), lanes.end(), 0u);
-                        nodes = 0;
-                    }
-                    for (uint32_t j = 0; j < nph; j++) lanes[j] += bct[q][(size_t)m * nph + j];
-                    nodes += (uint32_t)bnodes[q][m].size();
-                }
-                gfirst[q].push_back(nblk[q]);
-            }
-        };
-        int n_cu = 256;
-        XF_HIP(ctx, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        uint32_t lanes_cap = s->opts.block_lanes;
-        if (lanes_cap == 0) {
-            // the largest cap that leaves >= 4 slots per compute unit in every launch colour
-            for (uint32_t cap : {256u, 128u, 64u}) {
-                group(cap);
-                lanes_cap = cap;
-                bool enough = true;
-                for (int q = 0; q < 4; q++)
-                    if (nblk[q] && gfirst[q].size() - 1 < 4 * (size_t)n_cu) enough = false;
-                if (enough) break;
-            }
-        } else {
-            group(lanes_cap);
-        }
-        s->blk_lanes = lanes_cap;
-        uint32_t nslot[4], slot_base[5] = {0, 0, 0, 0, 0};
-        std::vector<uint32_t> deal[4];  // per launch colour: slot in layout order -> dispatched slot
-        for (int q = 0; q < 4; q++) {
-            uint32_t n = nslot[q] = (uint32_t)gfirst[q].size() - 1, next = 0;
-            slot_base[q + 1] = slot_base[q] + n;
-            deal[q].assign(n, 0);
-            for (uint32_t m = 0; next < n; m++)
-                for (uint32_t r = 0; r < 8; r++) {
-                    uint32_t lo = (uint32_t)(((uint64_t)n * r) / 8), hi = (uint32_t)(((uint64_t)n * (r + 1)) / 8);
-                    if (lo + m < hi) deal[q][lo + m] = next++;
-                }
-        }
-        // slot node lists (member blocks' nodes are disjoint), records, local indices
-        std::vector<std::vector<uint32_t>> snodes(slot_base[4]);
-        std::vector<uint32_t> slot_of_block[4];
-        s->blk_nmax = s->blk_nodes = s->blk_pmax = 0;
-        for (int q = 0; q < 4; q++) {
-            slot_of_block[q].assign(nblk[q], 0);
-            for (uint32_t g = 0; g < nslot[q]; g++) {
-                std::vector<uint32_t>& v = snodes[slot_base[q] + deal[q][g]];
-                for (uint32_t m = gfirst[q][g]; m < gfirst[q][g + 1]; m++) {
-                    slot_of_block[q][m] = deal[q][g];
-                    v.insert(v.end(), bnodes[q][m].begin(), bnodes[q][m].end());
-                }
-                std::sort(v.begin(), v.end());
-                s->blk_nmax = std::max(s->blk_nmax, (uint32_t)v.size());
-                s->blk_nodes += (uint32_t)v.size();
-            }
-        }
-        h_tloc.assign(s->T, make_uint2(0u, 0u));
-        for (size_t bi = 0; bi < s->bodies.size(); bi++) {
-            const Body& b = s->bodies[bi];
-            const xf::SweepBlocks& sb = sweep[bi];
-            for (uint32_t t = 0; t < b.t; t++) {
-                uint32_t k = sb.block[t], q = sb.launch[k];
-                const std::vector<uint32_t>& v = snodes[slot_base[q] + slot_of_block[q][lay[bi][k]]];
-                uint32_t l[4];
-                for (int i = 0; i < 4; i++)
-                    l[i] = (uint32_t)(std::lower_bound(v.begin(), v.end(), b.node_base + b.tets[4 * (size_t)t + i]) -
-                                      v.begin());
-                h_tloc[b.sched_pos[t]] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
-            }
-        }
-        h_bnode.assign((size_t)slot_base[4] * s->blk_nmax, xf::kNoNode);
-        for (uint32_t g = 0; g < slot_base[4]; g++)
-            std::copy(snodes[g].begin(), snodes[g].end(), h_bnode.begin() + (size_t)g * s->blk_nmax);
-        // fused node phase: per slot of the last launch colour, its node row
-        // followed by its share of the nodes outside that colour's slots (in
-        // ascending order, dealt in equal runs)
-        int launches = 0;
-        s->blk_last = -1;
-        for (int q = 0; q < 4; q++)
-            if (nslot[q]) {
-                launches++;
-                s->blk_last = q;
-            }
-        s->blk_fmax = 0;
-        h_bfin.clear();
-        std::vector<uint32_t> extra;
-        int cus = 256;
-        XF_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        // fused only where the last launch is latency-bound (at most two
-        // workgroups per compute unit): with more, the longer write-back and
-        // the LDS of O cost more than the node launch they save
-        if (launches < 2 || nslot[s->blk_last] > 2 * (uint32_t)cus) {
-            s->blk_last = -1;
-        } else {
-            const int q = s->blk_last;
-            std::vector<uint8_t> covered(s->N, 0);
-            for (uint32_t g = slot_base[q]; g < slot_base[q + 1]; g++)
-                for (uint32_t v : snodes[g]) covered[v] = 1;
-            for (uint32_t v = 0; v < s->N; v++)
-                if (!covered[v]) extra.push_back(v);
-            s->blk_fmax = (uint32_t)((extra.size() + nslot[q] - 1) / nslot[q]);
-            // staged with O and ids in LDS; else the node phase stays separate
-            if (xf_block_fin_lds_bytes(s->blk_nmax, s->blk_fmax) > kBlockLdsLimit) {
-                s->blk_last = -1;
-                s->blk_fmax = 0;
-            }
-        }
-        if (s->blk_last >= 0) {
-            const int q = s->blk_last;
-            const uint32_t fmax = s->blk_fmax;
-            const size_t row = (size_t)s->blk_nmax + fmax;
-            h_bfin.assign((size_t)nslot[q] * row, xf::kNoNode);
-            for (uint32_t b = 0; b < nslot[q]; b++) {
-                const uint32_t* bn = h_bnode.data() + (size_t)(slot_base[q] + b) * s->blk_nmax;
-                std::copy(bn, bn + s->blk_nmax, h_bfin.begin() + b * row);
-                for (uint32_t j = 0; j < fmax && (size_t)b * fmax + j < extra.size(); j++)
-                    h_bfin[b * row + s->blk_nmax + j] = extra[(size_t)b * fmax + j];
-            }
-        }
-        for (int q = 0; q < 4; q++) {
-            xf_scene::BlockLaunch& L = s->blk[q];
-            L.nslot = nslot[q];
-            if (L.nslot == 0) L.nph = 0;
-            L.rec_off = (uint32_t)h_brec.size();
-            L.node_off = slot_base[q] * s->blk_nmax;
-            h_brec.resize(h_brec.size() + 2 * (size_t)L.nslot * L.nph, 0u);
-            for (uint32_t g = 0; g < L.nslot; g++)
-                for (uint32_t j = 0; j < L.nph; j++) {
-                    uint32_t first = gfirst[q][g], count = 0;
-                    for (uint32_t m = first; m < gfirst[q][g + 1]; m++) count += bct[q][(size_t)m * L.nph + j];
-                    size_t at = L.rec_off + 2 * ((size_t)deal[q][g] * L.nph + j);
-                    h_brec[at] = bst[q][(size_t)first * L.nph + j];
-                    h_brec[at + 1] = count;
-                    s->blk_pmax = std::max(s->blk_pmax, count);
-                }
-        }
+        int prc = PackBlocks(s, s->opts.block_lanes, &bt);
+        if (prc) return prc;
     }
 
     // Nodes.
@@ -1474,11 +1528,6 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
         (rc = DevAlloc(ctx, &s->barrier_err, 1, &s->device_bytes)) ||
         (rc = DevAlloc(ctx, &s->tout, s->T, &s->device_bytes)))
         return rc;
-    if (s->opts.sweep_blocks && ((rc = DevAlloc(ctx, &s->blk_rec, h_brec.size(), &s->device_bytes)) ||
-                                 (rc = DevAlloc(ctx, &s->blk_node, h_bnode.size(), &s->device_bytes)) ||
-                                 (rc = DevAlloc(ctx, &s->tloc, s->T, &s->device_bytes)) ||
-                                 (!h_bfin.empty() && (rc = DevAlloc(ctx, &s->blk_fin, h_bfin.size(), &s->device_bytes)))))
-        return rc;
     XF_HIP(ctx, hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     hipStream_t st = ctx->stream;
     XF_HIP(ctx, hipMemcpyAsync(s->X, hX.data(), sizeof(double4) * s->N, hipMemcpyHostToDevice, st));
@@ -1499,18 +1548,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     XF_HIP(ctx, hipMemcpyAsync(s->tg[1], h_tg1.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tg[2], h_tg2.data(), sizeof(float4) * s->T, hipMemcpyHostToDevice, st));
     XF_HIP(ctx, hipMemcpyAsync(s->tdmp, h_tdmp.data(), sizeof(float2) * s->T, hipMemcpyHostToDevice, st));
-    if (s->opts.sweep_blocks) {
-        if (!h_brec.empty())
-            XF_HIP(ctx, hipMemcpyAsync(s->blk_rec, h_brec.data(), sizeof(uint32_t) * h_brec.size(),
-                                       hipMemcpyHostToDevice, st));
-        if (!h_bnode.empty())
-            XF_HIP(ctx, hipMemcpyAsync(s->blk_node, h_bnode.data(), sizeof(uint32_t) * h_bnode.size(),
-                                       hipMemcpyHostToDevice, st));
-        XF_HIP(ctx, hipMemcpyAsync(s->tloc, h_tloc.data(), sizeof(uint2) * s->T, hipMemcpyHostToDevice, st));
-        if (!h_bfin.empty())
-            XF_HIP(ctx, hipMemcpyAsync(s->blk_fin, h_bfin.data(), sizeof(uint32_t) * h_bfin.size(),
-                                       hipMemcpyHostToDevice, st));
-    }
+    if (s->opts.sweep_blocks && (rc = UploadBlocks(s, bt))) return rc;
     uint32_t uzero = 0;
     static const uint32_t kClockZero[kClockWords] = {0u, 0u};  // time 0.0f, tick 0
     XF_HIP(ctx, hipMemcpyAsync(s->clock, kClockZero, sizeof(kClockZero), hipMemcpyHostToDevice, st));
@@ -1871,19 +1909,23 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
             return e;
         return hipStreamSynchronize(st);
     };
+    // Candidates: launch mode and workgroup size; for sweep blocks also
+    // lanes per tet and the slot cap (block_lanes: the slots are repacked
+    // for each cap, caps outermost).
     struct Cand {
-        uint32_t mode, size, lanes;
+        uint32_t mode, size, lanes, cap;
     };
-    const Cand candidates[] = {{XF_LAUNCH_PER_COLOUR, 64, 1},  {XF_LAUNCH_PER_COLOUR, 128, 1},
-                               {XF_LAUNCH_PER_COLOUR, 256, 1}, {XF_LAUNCH_PERSISTENT, 64, 1},
-                               {XF_LAUNCH_PERSISTENT, 128, 1}, {XF_LAUNCH_PERSISTENT, 256, 1},
-                               {XF_LAUNCH_PERSISTENT, 512, 1}, {XF_LAUNCH_BLOCKS, 64, 1},
-                               {XF_LAUNCH_BLOCKS, 128, 1},     {XF_LAUNCH_BLOCKS, 256, 1},
-                               {XF_LAUNCH_BLOCKS, 64, 4},      {XF_LAUNCH_BLOCKS, 128, 4},
-                               {XF_LAUNCH_BLOCKS, 256, 4}};
+    std::vector<Cand> candidates;
+    for (uint32_t size : {64u, 128u, 256u}) candidates.push_back({XF_LAUNCH_PER_COLOUR, size, 1, 0});
+    for (uint32_t size : {64u, 128u, 256u, 512u}) candidates.push_back({XF_LAUNCH_PERSISTENT, size, 1, 0});
+    if (s->opts.sweep_blocks && energy != kRingMode)
+        for (uint32_t cap : {32u, 64u, 128u, 256u})
+            for (uint32_t lanes : {1u, 4u})
+                for (uint32_t size : {64u, 128u, 256u}) candidates.push_back({XF_LAUNCH_BLOCKS, size, lanes, cap});
     xf_scene_opts keep = s->opts;
+    const uint32_t cap_keep = s->blk_lanes;
     Cand best_c = {keep.launch_mode, keep.launch_mode == XF_LAUNCH_PERSISTENT ? keep.persistent_blocks : keep.block_size,
-                   keep.tet_lanes};
+                   keep.tet_lanes, cap_keep};
     const bool quad_ok = (energy == 0 || energy == 4 || energy == 7) && sim && !s->dproj;
     int maxb = 0;  // ring, Pixar and Cube energies run per colour only
     if (energy != kRingMode && energy != 20 && energy != 22 && energy != 30)
@@ -1895,11 +1937,21 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     if (he == hipSuccess) he = hipEventCreate(&e1);
     float time_keep = s->time_host;
     uint32_t tick_keep = s->tick_host;
+    uint32_t packed = cap_keep;  // the slot cap the uploaded tables hold
+    auto repack = [&](uint32_t cap) -> int {
+        if (cap == packed) return XF_OK;
+        DropGraph(s);
+        BlockTables bt;
+        int r = PackBlocks(s, cap, &bt);
+        if (!r) r = UploadBlocks(s, bt);
+        if (!r) packed = cap;
+        return r;
+    };
     for (const Cand& cand : candidates) {
         if (he != hipSuccess || rc) break;
         if (cand.mode == XF_LAUNCH_PERSISTENT && (int)cand.size > maxb) continue;
-        if (cand.mode == XF_LAUNCH_BLOCKS && (!s->opts.sweep_blocks || energy == kRingMode)) continue;
         if (cand.lanes == 4 && !quad_ok) continue;
+        if (cand.mode == XF_LAUNCH_BLOCKS && (rc = repack(cand.cap))) break;
         s->opts.launch_mode = cand.mode;
         s->opts.tet_lanes = cand.lanes;
         if (cand.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = cand.size;
@@ -1917,6 +1969,7 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
         if (best < 0.0f || ms < best) {
             best = ms;
             best_c = cand;
+            if (cand.mode != XF_LAUNCH_BLOCKS) best_c.cap = cap_keep;
         }
         if ((rc = CheckBarrier(s))) break;
     }
@@ -1932,6 +1985,10 @@ XF_API int xf_scene_autotune(xf_scene* s, uint32_t substeps, float dt, uint32_t 
     if (best_c.mode == XF_LAUNCH_PERSISTENT) s->opts.persistent_blocks = best_c.size;
     else s->opts.block_size = best_c.size;
     DropGraph(s);
+    if (!rc && s->opts.sweep_blocks) {
+        rc = repack(best_c.cap);
+        if (!rc && best_c.cap != cap_keep) s->opts.block_lanes = best_c.cap;
+    }
     if (rc) return rc;
     if (he != hipSuccess) return Fail(ctx, XF_E_HIP, "autotune: %s", hipGetErrorString(he));
     if (he2 != hipSuccess) return Fail(ctx, XF_E_HIP, "autotune restore: %s", hipGetErrorString(he2));
