@@ -400,7 +400,9 @@ def adjoint_leg(ctx, reps=3):
     """Adjoint of the 16x4x4 fitting beam (1,536 tets, per-tet mu, 30 frames of
     20 substeps, loss at frames 10, 20, 30, checkpoints every 20 substeps):
     wall time of xf_scene_adjoint (forward + checkpointed reverse) against the
-    plain forward rollout of the same 600 substeps."""
+    plain forward rollout of the same 600 substeps; the same for the FP64
+    element model (xf_scene_adjoint_fp64) with the relative L2 distance
+    between the two gradients."""
     truth = workloads.beam_fit_16x4x4()
     guess = dict(truth)
     guess["tet_mu"] = np.full(truth["tets"].size // 4, 5e4, np.float32)
@@ -432,12 +434,22 @@ def adjoint_leg(ctx, reps=3):
         t0 = time.perf_counter()
         L, g = adj.adjoint(n_sub, dt, loss_at, targets)
         t_a.append(time.perf_counter() - t0)
+    adj.set_state(0, x0, v0)
+    adj.adjoint(n_sub, dt, loss_at, targets, fp64=True)  # captures its reverse-pass graph
+    t_64 = []
+    for _ in range(reps):
+        adj.set_state(0, x0, v0)
+        t0 = time.perf_counter()
+        L64, g64 = adj.adjoint(n_sub, dt, loss_at, targets, fp64=True)
+        t_64.append(time.perf_counter() - t0)
     fwd.close()
     adj.close()
-    tf, ta = min(t_f), min(t_a)
+    tf, ta, t64 = min(t_f), min(t_a), min(t_64)
     return {"workload": f"beam16x4x4_fit: {truth['tets'].size // 4} tets, {n_sub} substeps, loss at frames 10/20/30",
             "forward_ms": 1e3 * tf, "adjoint_ms": 1e3 * ta, "adjoint_over_forward": ta / tf,
-            "reverse_over_forward": ta / tf - 1.0, "loss": L, "grad_norm": float(np.linalg.norm(g))}
+            "reverse_over_forward": ta / tf - 1.0, "loss": L, "grad_norm": float(np.linalg.norm(g)),
+            "fp64_model": {"adjoint_ms": 1e3 * t64, "over_fp32_forward": t64 / tf, "loss": L64,
+                           "grad_rel_l2_fp32_vs_fp64": float(np.linalg.norm(g - g64) / np.linalg.norm(g64))}}
 
 
 def fit_leg(ctx, iters=5):

@@ -365,6 +365,19 @@ XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint3
 XF_API int xf_scene_adjoint(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
                             const double* targets, uint32_t window, double* loss, double* grad_log_mu,
                             double* body_loss);
+/* xf_scene_adjoint in the FP64 element model (no reference counterpart):
+ * the rollout and its adjoint with every element, contact and settings
+ * product in double -- element constants and settings widened from their
+ * FP32 values, gravity * dt and 1 - drag and 1 / dt formed in double,
+ * inverse masses in FP32 as stored -- and per-tet window sums in double.
+ * The loss and gradient are those of this FP64 model (the gradient of the
+ * FP32 element arithmetic carries its rounding noise; this one matches
+ * finite differences of the FP64 model). Same arguments, scope and end
+ * state (the FP64 rollout's) as xf_scene_adjoint; one colour launch per
+ * colour and substep (no sweep blocks, no hipGraph). */
+XF_API int xf_scene_adjoint_fp64(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss,
+                                 const uint32_t* loss_at, const double* targets, uint32_t window, double* loss,
+                                 double* grad_log_mu, double* body_loss);
 /* xf_scene_adjoint without allocating: no hipMalloc, hipFree or other
  * allocating call. No counterpart in the reference; for callers that cannot
  * allocate (tight fitting loops, harnesses that forbid device allocation

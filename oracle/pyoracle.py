@@ -489,6 +489,26 @@ def substeps_many(bodies, settings, dt, count, threads):
     ref(fast).xr_substeps_many(hs, ss, len(bodies), dt, count, threads)
 
 
+def rollout_loss(body, params, order, n_sub, loss_at, targets):
+    """Forward-only loss of xo_rollout_adjoint (oracle/xo_substep.inc):
+    L = sum_j (1/n) sum_i |X_i(after substep loss_at[j]) - target_j,i|^2,
+    each squared-error sum taken serially in node-component order; `body`
+    is advanced n_sub substeps. For finite differences of the adjoint."""
+    targets = np.asarray(targets).reshape(len(loss_at), -1)
+    L, done = 0.0, 0
+    for j, la in enumerate(loss_at):
+        body.substep(params, order, count=int(la) - done)
+        done = int(la)
+        d = (body.X.astype(np.float64) - targets[j]).tolist()
+        e = 0.0
+        for x in d:
+            e += x * x
+        L += e / float(body.n)
+    if n_sub > done:
+        body.substep(params, order, count=int(n_sub) - done)
+    return L
+
+
 def rollout_adjoint(body, params, order, n_sub, loss_at, targets, window=20):
     """Forward rollout of `body` (an OracleBody; its X, V are advanced) and the
     adjoint of the mean-squared position loss at substeps `loss_at`.

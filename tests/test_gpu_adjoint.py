@@ -7,7 +7,11 @@ Bars:
     stepping;
   * against the FP64 oracle adjoint (itself pinned by finite differences in
     test_oracle_adjoint.py): relative L2 distance of the gradients reported
-    and bounded by GRAD_REL_FP64.
+    and bounded by GRAD_REL_FP64;
+  * the FP64 element model (xf_scene_adjoint_fp64): loss, gradient and end
+    state bit-exact against the oracle's f64 adjoint, and its gradient
+    within 1e-3 (median over 32 seeded tets; 5e-3 at most) of central
+    finite differences (relative step 1e-4) of the FP64 forward.
 """
 import numpy as np
 import pytest
@@ -18,7 +22,7 @@ import xpbdfem as xf
 
 pytestmark = pytest.mark.gpu
 
-GRAD_REL_FP64 = 0.05
+GRAD_REL_FP64 = 0.01  # measured 6.3e-3 (cosine 0.99998)
 
 
 def targets_for(wl, order, loss_at, variant="ref64"):
@@ -142,3 +146,89 @@ def test_adjoint_over_sweep_block_colouring(ctx, fit):
     x, v = fwd.get_state(0)
     fwd.close()
     assert np.array_equal(x, x_adj) and np.array_equal(v, v_adj)
+
+
+def oracle_fp64(guess, order, n_sub, loss_at, targets, dt, settings, window=20):
+    ob = po.body_from_workload(guess, "f64")
+    L, glog = po.rollout_adjoint(ob, po.params_from_settings(dt, settings), order, n_sub, loss_at, targets,
+                                 window=window)
+    return L, glog, ob
+
+
+@pytest.mark.parametrize("frames,window", [(3, 20), (30, 20), (30, 7)])
+def test_adjoint_fp64_bit_exact_vs_oracle(ctx, fit, frames, window):
+    truth, guess, order = fit
+    S = truth["substeps"]
+    loss_at = np.array([f * S for f in (1, 2, 3)] if frames == 3 else [200, 400, 600], np.uint32)
+    targets = targets_for(truth, order, loss_at, "f64")
+    sc = workloads.make_scene(ctx, guess)
+    L, g = sc.adjoint(frames * S, float(truth["dt"]), loss_at, targets, window=window, fp64=True)
+    Lo, glog, ob = oracle_fp64(guess, order, frames * S, loss_at, targets, truth["dt"], truth["settings"], window)
+    print(f"\nframes {frames} window {window}: FP64 loss GPU {L:.15e} oracle {Lo:.15e}")
+    assert L == Lo
+    assert np.array_equal(g, -glog)
+    x, v = sc.get_state(0)
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+    assert sc.tick() == frames * S
+    sc.close()
+
+
+def test_adjoint_fp64_batched_equals_single(ctx, fit):
+    truth, guess, order = fit
+    S = truth["substeps"]
+    loss_at = np.array([S, 2 * S], np.uint32)
+    tg = targets_for(truth, order, loss_at, "f64")
+    other = dict(guess)
+    other["tet_mu"] = np.full(guess["tets"].size // 4, 8e4, np.float32)
+    both = workloads.make_batch_scene(ctx, dict(bodies=[guess, other], dt=truth["dt"], substeps=S))
+    L2, g2 = both.adjoint(2 * S, float(truth["dt"]), loss_at, np.concatenate([tg, tg], axis=1), window=20,
+                          fp64=True)
+    both.close()
+    t = guess["tets"].size // 4
+    total = 0.0
+    for i, wl in enumerate((guess, other)):
+        Lo, glog, _ = oracle_fp64(wl, order, 2 * S, loss_at, tg, truth["dt"], truth["settings"])
+        assert np.array_equal(g2[i * t:(i + 1) * t], -glog)
+        total += Lo
+    assert np.isclose(L2, total, rtol=1e-14)
+
+
+def test_adjoint_fp64_per_tet_fd(ctx, fit):
+    """The FP64-model gradient against central finite differences of the
+    FP64 forward (relative step 1e-4) on 32 seeded tets: 30 frames, loss at
+    frames 10, 20 and 30."""
+    truth, guess, order = fit
+    loss_at = np.array([200, 400, 600], np.uint32)
+    targets = targets_for(truth, order, loss_at, "f64")
+    sc = workloads.make_scene(ctx, guess)
+    _, g = sc.adjoint(600, float(truth["dt"]), loss_at, targets, window=20, fp64=True)
+    sc.close()
+    mu0 = np.asarray(guess["tet_mu"], np.float32)
+    t = mu0.size
+    p = po.params_from_settings(truth["dt"], truth["settings"])
+    ks = np.random.default_rng(32).choice(t, 32, replace=False)
+    rel = []
+    for k in ks:
+        fd = []
+        for sign in (1, -1):
+            w2 = dict(guess)
+            mu = mu0.copy()
+            mu[k] = np.float32(mu0[k] * (1 + sign * 1e-4))
+            w2["tet_mu"] = mu
+            fd.append((po.rollout_loss(po.body_from_workload(w2, "f64"), p, order, 600, loss_at, targets), mu[k]))
+        d = (fd[0][0] - fd[1][0]) / (float(fd[0][1]) - float(fd[1][1]))
+        an = g[k] / float(mu0[k])  # dL/dmu = dL/dlog mu / mu
+        rel.append(abs(an - d) / abs(d))
+    rel = np.array(rel)
+    print("\nFP64-model gradient vs FD, per-tet relative errors:", " ".join(f"{x:.1e}" for x in rel))
+    assert np.median(rel) <= 1e-3
+    assert rel.max() <= 5e-3
+
+
+def test_adjoint_fp64_rejects_noalloc(ctx, fit):
+    truth, guess, order = fit
+    sc = workloads.make_scene(ctx, guess)
+    with pytest.raises(ValueError):
+        sc.adjoint(20, float(truth["dt"]), np.array([20], np.uint32), np.zeros((1, 425 * 3)), noalloc=True,
+                   fp64=True)
+    sc.close()

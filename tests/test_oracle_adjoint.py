@@ -5,8 +5,8 @@ The adjoint has no reference counterpart; it is pinned by differentiation:
     against central FD of the FP64 forward;
   * whole rollouts of the 16x4x4 fitting beam (per-tet mu, ground contact,
     pinned face): a directional derivative along a random direction in
-    log mu, and per-tet derivatives on seeded tets with relative step 1e-4,
-    against central FD of the FP64 forward.
+    log mu, and per-tet derivatives on 32 seeded tets with relative step
+    1e-4, against central FD of the FP64 forward.
 """
 import numpy as np
 import pytest
@@ -99,8 +99,40 @@ def test_rollout_adjoint_directional_fd(fit):
     assert abs(an - fd) <= 1e-3 * abs(fd)
 
 
+def per_tet_fd(fit, mu0, loss_at, targets, tets, n_sub=600, rel_step=1e-4):
+    """Central finite differences dL/dmu_k of the FP64 oracle's forward loss
+    for each tet k (relative step rel_step, the perturbed mu rounded to FP32
+    as the material stores it)."""
+    wl, order, p, make = fit
+    out = []
+    for k in tets:
+        mp, mm = mu0.copy(), mu0.copy()
+        mp[k] = np.float32(mu0[k] * (1 + rel_step))
+        mm[k] = np.float32(mu0[k] * (1 - rel_step))
+        Lp = po.rollout_loss(make(mp), p, order, n_sub, loss_at, targets)
+        Lm = po.rollout_loss(make(mm), p, order, n_sub, loss_at, targets)
+        out.append((Lp - Lm) / (float(mp[k]) - float(mm[k])))
+    return np.array(out)
+
+
+def fd_tets(t):
+    """The 32 seeded random tets of the gradient check."""
+    return np.random.default_rng(32).choice(t, 32, replace=False)
+
+
+def test_rollout_loss_matches_adjoint_loss(fit):
+    wl, order, p, make = fit
+    t = wl["tets"].size // 4
+    loss_at = np.array([200, 400, 600], np.uint32)
+    targets = targets_for(fit, loss_at)
+    mu0 = np.full(t, 5e4, np.float32)
+    L, _ = po.rollout_adjoint(make(mu0), p, order, 600, loss_at, targets)
+    assert po.rollout_loss(make(mu0), p, order, 600, loss_at, targets) == L
+
+
 def test_rollout_adjoint_per_tet_fd(fit):
-    """30 frames, loss at frames 10, 20, 30; 8 seeded tets, relative step 1e-4."""
+    """30 frames, loss at frames 10, 20, 30; 32 seeded tets, relative step
+    1e-4, central differences of the FP64 forward."""
     wl, order, p, make = fit
     t = wl["tets"].size // 4
     loss_at = np.array([200, 400, 600], np.uint32)
@@ -108,15 +140,9 @@ def test_rollout_adjoint_per_tet_fd(fit):
     mu0 = np.full(t, 5e4, np.float32)
     _, glog = po.rollout_adjoint(make(mu0), p, order, 600, loss_at, targets)
     an = -glog / mu0
-    rel = []
-    for k in np.random.default_rng(32).choice(t, 8, replace=False):
-        mp, mm = mu0.copy(), mu0.copy()
-        mp[k] = np.float32(mu0[k] * (1 + 1e-4))
-        mm[k] = np.float32(mu0[k] * (1 - 1e-4))
-        Lp, _ = po.rollout_adjoint(make(mp), p, order, 600, loss_at, targets)
-        Lm, _ = po.rollout_adjoint(make(mm), p, order, 600, loss_at, targets)
-        fd = (Lp - Lm) / (float(mp[k]) - float(mm[k]))
-        rel.append(abs(an[k] - fd) / abs(fd))
+    ks = fd_tets(t)
+    fd = per_tet_fd(fit, mu0, loss_at, targets, ks)
+    rel = np.abs(an[ks] - fd) / np.abs(fd)
     print("\nper-tet relative errors:", " ".join(f"{x:.1e}" for x in rel))
     assert np.median(rel) <= 1e-3
-    assert max(rel) <= 1e-2
+    assert rel.max() <= 5e-3

@@ -8,6 +8,13 @@
 // forward expression is differentiated as written, clamps included. The
 // operation order matches the CPU oracle's adjoint of the same precision.
 //
+// FP64 element model (xf_scene_adjoint_fp64): the same rollout and adjoint
+// with every element, contact and settings product in double (element
+// constants and settings widened from their FP32 values, inverse masses kept
+// in FP32 as the reference stores them), per-tet window sums in double: the
+// trajectory and gradient of the FP64 model rather than of the reference's
+// FP32 element arithmetic.
+//
 // Scope: simultaneous MixedSel / YeohSkinFast (energy 4, 7 or per-body 0),
 // pins, ground contact. Sphere contact is not differentiated.
 #include <hip/hip_runtime.h>
@@ -245,13 +252,14 @@ __device__ __forceinline__ TetData load_tet(uint32_t i, const float4* tg0, const
 // Reverse sweep of one colour: tets [begin, begin+count) of the sweep order.
 // tape holds each tet's 4 node positions before its projection (12 doubles
 // per schedule position) for this substep. gacc accumulates d(loss)/d(log
-// comp0) per schedule position.
-template <bool AUX>
+// comp0) per schedule position. R: element precision (float, or double for
+// the FP64 element model: constants widened from their FP32 values).
+template <bool AUX, class R>
 __global__ __launch_bounds__(256) void xf_project_reverse(
     uint32_t begin, uint32_t count, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
     const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
     const float4* __restrict__ taux, const double* __restrict__ tape, const double4* __restrict__ X,
-    double4* __restrict__ Xb, float* __restrict__ gacc, const XfBodyParams* __restrict__ bodies, float comp1_u,
+    double4* __restrict__ Xb, R* __restrict__ gacc, const XfBodyParams* __restrict__ bodies, float comp1_u,
     float a_u, float dt) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
@@ -259,7 +267,15 @@ __global__ __launch_bounds__(256) void xf_project_reverse(
     uint4 id = tidx[i];
     TetData t = load_tet<AUX>(i, tg0, tg1, tg2, tg3, taux, comp1_u, a_u);
     const XfBodyParams& bp = bodies[t.body];
-    float C[3] = {bp.yeoh[0], bp.yeoh[1], bp.yeoh[2]};
+    R C[3], Qi[3][3], QQ[3], QR[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        C[c] = (R)bp.yeoh[c];
+        QQ[c] = (R)t.QQ[c];
+        QR[c] = (R)t.QR[c];
+#pragma unroll
+        for (int r = 0; r < 3; r++) Qi[c][r] = (R)t.Qi[c][r];
+    }
     const double* tp = tape + 12 * (size_t)i;
     double x[4][3];
 #pragma unroll
@@ -277,16 +293,140 @@ __global__ __launch_bounds__(256) void xf_project_reverse(
         xb[n][1] = b.y;
         xb[n][2] = b.z;
     }
-    float gl = adjoint_tet(bp.energy == 7, C, x, w, t.Qi, t.QQ, t.QR, t.comp0, t.comp1, t.a, dt, xb);
+    R gl = adjoint_tet<R>(bp.energy == 7, C, x, w, Qi, QQ, QR, (R)t.comp0, (R)t.comp1, (R)t.a, (R)dt, xb);
     gacc[i] = gacc[i] + gl;
 #pragma unroll
     for (int n = 0; n < 4; n++) Xb[nid[n]] = make_double4(xb[n][0], xb[n][1], xb[n][2], 0.0);
 }
 
+// ---------------------------------------------- FP64 element model forward
+//
+// Geo3d::Substep (Geo.cpp:305-356) with the element, contact and settings
+// arithmetic in double, as the oracle's f64 variant: predict, colour
+// launches of the simultaneous MixedSel / YeohSkinFast projection
+// (xf::solve_mixed_sim<double>), then pins, ground contact and velocity.
+
+// Prediction from V (Geo.cpp:307-312): v += g dt, v *= 1 - drag, O = X,
+// X += v dt, with g dt and 1 - drag formed in double. Thread 0 advances the
+// scene clock by dt (the substep's end time; not read by this model).
+__global__ __launch_bounds__(256) void xf_predict64(uint32_t n, double4* __restrict__ X, double4* __restrict__ O,
+                                                    const double4* __restrict__ V,
+                                                    const XfBodyParams* __restrict__ bodies, float dt,
+                                                    float* __restrict__ clock) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && clock != nullptr) {
+        clock[0] = clock[0] + dt;
+        uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
+        tick[0] = tick[0] + 1u;
+    }
+    if (i >= n) return;
+    double4 xi = X[i], oi = O[i], vi = V[i];
+    const XfBodyParams& p = bodies[(uint32_t)oi.w >> 1];
+    const double d = (double)p.dt;
+    const double gdx = (double)p.gravity[0] * d, gdy = (double)p.gravity[1] * d;
+    const double dragf = 1.0 - (double)p.drag;
+    double v[3] = {vi.x + gdx, vi.y + gdy, vi.z + 0.0};
+#pragma unroll
+    for (int r = 0; r < 3; r++) v[r] = v[r] * dragf;
+    O[i] = make_double4(xi.x, xi.y, xi.z, oi.w);
+    X[i] = make_double4(xi.x + v[0] * d, xi.y + v[1] * d, xi.z + v[2] * d, xi.w);
+}
+
+// One colour of the FP64 projection; REC: records each tet's four node
+// positions before its projection into tape (12 doubles per schedule
+// position), as xf_project_record does for the FP32 model.
+template <bool AUX, bool REC>
+__global__ __launch_bounds__(256) void xf_project64(uint32_t begin, uint32_t count, const uint4* __restrict__ tidx,
+                                                    const float4* __restrict__ tg0, const float4* __restrict__ tg1,
+                                                    const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+                                                    const float4* __restrict__ taux, double4* __restrict__ X,
+                                                    double* __restrict__ tape, const XfBodyParams* __restrict__ bodies,
+                                                    float comp1_u, float a_u, float dt) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    uint32_t i = begin + k;
+    uint4 id = tidx[i];
+    TetData t = load_tet<AUX>(i, tg0, tg1, tg2, tg3, taux, comp1_u, a_u);
+    const XfBodyParams& bp = bodies[t.body];
+    double C[3], Qi[3][3], QQ[3], QR[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        C[c] = (double)bp.yeoh[c];
+        QQ[c] = (double)t.QQ[c];
+        QR[c] = (double)t.QR[c];
+#pragma unroll
+        for (int r = 0; r < 3; r++) Qi[c][r] = (double)t.Qi[c][r];
+    }
+    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    if constexpr (REC) {
+        double* tp = tape + 12 * (size_t)i;
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) tp[3 * n + r] = x[n][r];
+    }
+    xf::solve_mixed_sim<double>(bp.energy == 7, C, x, w, Qi, QQ, QR, (double)t.comp0, (double)t.comp1, (double)t.a,
+                                (double)dt);
+    X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
+    X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
+    X[id.z] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
+    X[id.w] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+}
+
+// End of an FP64 substep for one node: a pinned node returns to O; other
+// nodes take the ground contact in double (static friction when the
+// tangential displacement is within mu * depth, else it shrinks by mu *
+// depth); then V = (X - O) / dt with 1 / dt in double.
+__global__ __launch_bounds__(256) void xf_finish64(uint32_t n, double4* __restrict__ X, const double4* __restrict__ O,
+                                                   double4* __restrict__ V, const XfBodyParams* __restrict__ bodies) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 xi = X[i], oi = O[i];
+    const uint32_t flags = (uint32_t)oi.w;
+    const XfBodyParams& p = bodies[flags >> 1];
+    double x[3] = {xi.x, xi.y, xi.z};
+    const double o[3] = {oi.x, oi.y, oi.z};
+    if (flags & 1u) {
+        x[0] = o[0];
+        x[1] = o[1];
+        x[2] = o[2];
+    } else if (p.collide & 1u) {
+        const double gy = (double)p.ground_y;
+        if (x[1] < gy) {
+            const double mu = (double)p.friction;
+            const double depth = gy - x[1];
+            x[1] = gy;
+            const double dx = x[0] - o[0], dz = x[2] - o[2];
+            const double t2 = dx * dx + dz * dz;
+            const double lim = mu * depth;
+            if (t2 <= lim * lim) {
+                x[0] = o[0];
+                x[2] = o[2];
+            } else {
+                const double sc = lim / sqrt(t2);
+                x[0] = x[0] - dx * sc;
+                x[2] = x[2] - dz * sc;
+            }
+        }
+    }
+    const double invdt = 1.0 / (double)p.dt;
+    X[i] = make_double4(x[0], x[1], x[2], xi.w);
+    V[i] = make_double4((x[0] - o[0]) * invdt, (x[1] - o[1]) * invdt, (x[2] - o[2]) * invdt, 0.0);
+}
+
+// Square root of the element precision.
+__device__ __forceinline__ float rsqrt_of(float x) { return sqrtf(x); }
+__device__ __forceinline__ double rsqrt_of(double x) { return sqrt(x); }
+
 // Reverse of the end of one substep (velocity, pin, ground contact) for one
 // node, with the loss injected first when this substep ends a loss frame:
 // Xb += lw_i * (x_loss - target). xs / o: the node's position before the
 // contact pass and at the substep start (recorded by the forward replay).
+// R: element precision of the forward (float: FP32 contact and 1.0f / dt;
+// double: the FP64 element model).
+template <class R>
 __global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4* __restrict__ O,
                                                        const double4* __restrict__ xs, const double4* __restrict__ o,
                                                        double4* __restrict__ Xb, double4* __restrict__ Vb,
@@ -308,7 +448,7 @@ __global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4
         xbv[2] = xbv[2] + wl * (xl.z - tg.z);
     }
     double vbv[3] = {vb4.x, vb4.y, vb4.z};
-    double invdt = (double)p.invdt;
+    const double invdt = sizeof(R) == 4 ? (double)p.invdt : 1.0 / (double)p.dt;
     double xcb[3], ob[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
@@ -324,14 +464,14 @@ __global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4
         }
     } else if (p.collide & 1u) {
         double4 s4 = xs[i], o4 = o[i];
-        float gy = p.ground_y;
-        float xy = (float)s4.y;
+        R gy = (R)p.ground_y;
+        R xy = (R)s4.y;
         if (xy < gy) {
-            float mu = p.friction;
-            float depth = gy - xy;
-            float dx = (float)(s4.x - o4.x), dz = (float)(s4.z - o4.z);
-            float t2 = dx * dx + dz * dz;
-            float lim = mu * depth;
+            R mu = (R)p.friction;
+            R depth = gy - xy;
+            R dx = (R)(s4.x - o4.x), dz = (R)(s4.z - o4.z);
+            R t2 = dx * dx + dz * dz;
+            R lim = mu * depth;
             out[1] = 0.0;
             if (t2 <= lim * lim) {
                 ob[0] = ob[0] + xcb[0];
@@ -339,12 +479,12 @@ __global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4
                 out[0] = 0.0;
                 out[2] = 0.0;
             } else {
-                float L = sqrtf(t2);
-                float ux = dx / L, uz = dz / L;
-                float bx = (float)xcb[0], bz = (float)xcb[2];
-                float ubv = ux * bx + uz * bz;
-                float kk = lim / L;
-                float px = kk * (bx - ux * ubv), pz = kk * (bz - uz * ubv);
+                R L = rsqrt_of(t2);
+                R ux = dx / L, uz = dz / L;
+                R bx = (R)xcb[0], bz = (R)xcb[2];
+                R ubv = ux * bx + uz * bz;
+                R kk = lim / L;
+                R px = kk * (bx - ux * ubv), pz = kk * (bz - uz * ubv);
                 out[0] = xcb[0] - (double)px;
                 out[2] = xcb[2] - (double)pz;
                 ob[0] = ob[0] + (double)px;
@@ -358,7 +498,9 @@ __global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4
 }
 
 // Reverse of the prediction (Geo.cpp:307-312) for one node: Xb <- Xb + Ob,
-// Vb <- Xb_pred * dt * (1 - drag).
+// Vb <- Xb_pred * dt * (1 - drag) (1 - drag: the FP32 value, or formed in
+// double for the FP64 element model).
+template <class R>
 __global__ __launch_bounds__(256) void xf_adj_node_start(uint32_t n, const double4* __restrict__ O,
                                                          double4* __restrict__ Xb, double4* __restrict__ Vb,
                                                          const double4* __restrict__ Ob,
@@ -367,17 +509,20 @@ __global__ __launch_bounds__(256) void xf_adj_node_start(uint32_t n, const doubl
     if (i >= n) return;
     const XfBodyParams& p = bodies[(uint32_t)O[i].w >> 1];
     double4 xb = Xb[i], obv = Ob[i];
-    double dt = (double)p.dt, dragf = (double)p.dragf;
+    const double dt = (double)p.dt;
+    const double dragf = sizeof(R) == 4 ? (double)p.dragf : 1.0 - (double)p.drag;
     Xb[i] = make_double4(xb.x + obv.x, xb.y + obv.y, xb.z + obv.z, 0.0);
     Vb[i] = make_double4((xb.x * dt) * dragf, (xb.y * dt) * dragf, (xb.z * dt) * dragf, 0.0);
 }
 
-// Window boundary: glog += gacc (FP32 window sum into the FP64 total); gacc = 0.
-__global__ __launch_bounds__(256) void xf_adj_flush(uint32_t t, float* __restrict__ gacc, double* __restrict__ glog) {
+// Window boundary: glog += gacc (the window sum in element precision into
+// the FP64 total); gacc = 0.
+template <class R>
+__global__ __launch_bounds__(256) void xf_adj_flush(uint32_t t, R* __restrict__ gacc, double* __restrict__ glog) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= t) return;
     glog[i] += (double)gacc[i];
-    gacc[i] = 0.0f;
+    gacc[i] = R(0.0);
 }
 
 // Zero-fill of a double4 array (adjoint state reset).
@@ -476,42 +621,99 @@ __global__ __launch_bounds__(256) void xf_copy2_d4(uint32_t n, double4* __restri
 
 // ------------------------------------------------------ launch wrappers
 
-extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, uint32_t begin, uint32_t count,
+// fp64: the FP64 element model (gacc then holds doubles).
+extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, bool fp64, uint32_t begin, uint32_t count,
                                                 const uint4* tidx, const float4* tg0, const float4* tg1,
                                                 const float4* tg2, const float4* tg3, const float4* taux,
-                                                const double* tape, const double4* X, double4* Xb, float* gacc,
+                                                const double* tape, const double4* X, double4* Xb, void* gacc,
                                                 const XfBodyParams* bodies, float comp1_u, float a_u, float dt) {
     if (count == 0) return hipSuccess;
     dim3 grid((count + 63) / 64), block(64);
-    if (aux)
-        hipLaunchKernelGGL(xf_project_reverse<true>, grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2, tg3, taux,
-                           tape, X, Xb, gacc, bodies, comp1_u, a_u, dt);
+    float* g32 = static_cast<float*>(gacc);
+    double* g64 = static_cast<double*>(gacc);
+    if (fp64 && aux)
+        hipLaunchKernelGGL((xf_project_reverse<true, double>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
+                           tg3, taux, tape, X, Xb, g64, bodies, comp1_u, a_u, dt);
+    else if (fp64)
+        hipLaunchKernelGGL((xf_project_reverse<false, double>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
+                           tg3, taux, tape, X, Xb, g64, bodies, comp1_u, a_u, dt);
+    else if (aux)
+        hipLaunchKernelGGL((xf_project_reverse<true, float>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
+                           tg3, taux, tape, X, Xb, g32, bodies, comp1_u, a_u, dt);
     else
-        hipLaunchKernelGGL(xf_project_reverse<false>, grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2, tg3,
-                           taux, tape, X, Xb, gacc, bodies, comp1_u, a_u, dt);
+        hipLaunchKernelGGL((xf_project_reverse<false, float>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
+                           tg3, taux, tape, X, Xb, g32, bodies, comp1_u, a_u, dt);
     return hipGetLastError();
 }
 
-extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, uint32_t n, const double4* O, const double4* xs,
-                                             const double4* o, double4* Xb, double4* Vb, double4* Ob,
-                                             const XfBodyParams* bodies, const double4* xloss, const double4* target,
-                                             const double* lw) {
+extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, bool fp64, uint32_t n, const double4* O,
+                                             const double4* xs, const double4* o, double4* Xb, double4* Vb,
+                                             double4* Ob, const XfBodyParams* bodies, const double4* xloss,
+                                             const double4* target, const double* lw) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(xf_adj_node_end, dim3((n + 255) / 256), dim3(256), 0, s, n, O, xs, o, Xb, Vb, Ob, bodies, xloss,
-                       target, lw);
+    if (fp64)
+        hipLaunchKernelGGL(xf_adj_node_end<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, xs, o, Xb, Vb, Ob,
+                           bodies, xloss, target, lw);
+    else
+        hipLaunchKernelGGL(xf_adj_node_end<float>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, xs, o, Xb, Vb, Ob,
+                           bodies, xloss, target, lw);
     return hipGetLastError();
 }
 
-extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const double4* O, double4* Xb, double4* Vb,
-                                               const double4* Ob, const XfBodyParams* bodies) {
+extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, bool fp64, uint32_t n, const double4* O, double4* Xb,
+                                               double4* Vb, const double4* Ob, const XfBodyParams* bodies) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(xf_adj_node_start, dim3((n + 255) / 256), dim3(256), 0, s, n, O, Xb, Vb, Ob, bodies);
+    if (fp64)
+        hipLaunchKernelGGL(xf_adj_node_start<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, Xb, Vb, Ob,
+                           bodies);
+    else
+        hipLaunchKernelGGL(xf_adj_node_start<float>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, Xb, Vb, Ob,
+                           bodies);
     return hipGetLastError();
 }
 
-extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog) {
+extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, bool fp64, uint32_t t, void* gacc, double* glog) {
     if (t == 0) return hipSuccess;
-    hipLaunchKernelGGL(xf_adj_flush, dim3((t + 255) / 256), dim3(256), 0, s, t, gacc, glog);
+    if (fp64)
+        hipLaunchKernelGGL(xf_adj_flush<double>, dim3((t + 255) / 256), dim3(256), 0, s, t, static_cast<double*>(gacc),
+                           glog);
+    else
+        hipLaunchKernelGGL(xf_adj_flush<float>, dim3((t + 255) / 256), dim3(256), 0, s, t, static_cast<float*>(gacc),
+                           glog);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_predict64(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
+                                          const XfBodyParams* bodies, float dt, float* clock) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_predict64, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, dt, clock);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_project64(hipStream_t s, bool aux, bool rec, uint32_t begin, uint32_t count,
+                                          const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
+                                          const float4* tg3, const float4* taux, double4* X, double* tape,
+                                          const XfBodyParams* bodies, float comp1_u, float a_u, float dt) {
+    if (count == 0) return hipSuccess;
+    dim3 grid((count + 63) / 64), block(64);
+#define XF_P64(A, RC)                                                                                              \
+    hipLaunchKernelGGL((xf_project64<A, RC>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2, tg3, taux, X, \
+                       tape, bodies, comp1_u, a_u, dt)
+    if (aux) {
+        if (rec) XF_P64(true, true);
+        else XF_P64(true, false);
+    } else {
+        if (rec) XF_P64(false, true);
+        else XF_P64(false, false);
+    }
+#undef XF_P64
+    return hipGetLastError();
+}
+
+extern "C" hipError_t xf_launch_finish64(hipStream_t s, uint32_t n, double4* X, const double4* O, double4* V,
+                                         const XfBodyParams* bodies) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_finish64, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies);
     return hipGetLastError();
 }
 

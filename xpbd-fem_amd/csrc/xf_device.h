@@ -58,6 +58,10 @@ struct XfBodyParams {
     uint32_t node_base;
     uint32_t n_nodes;
     uint32_t unlock;         // 1: pins are not locked this step (Settings_LockLeft clear, Geo.cpp:318)
+    // Settings as given, for the FP64 element model of the adjoint (which
+    // forms gravity * dt and 1 - drag in double)
+    float gravity[2];
+    float drag;
 };
 
 // The scene clock buffer: [0] simulation time (float), [1] tick (uint32,

@@ -71,18 +71,26 @@ extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool a
                                                const float4* tg2, const float4* tg3, const float4* taux, double4* X,
                                                double* tape, const XfBodyParams* bodies, float comp1_u, float a_u,
                                                float dt, float* clock);
-extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, uint32_t begin, uint32_t count,
+extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, bool fp64, uint32_t begin, uint32_t count,
                                                 const uint4* tidx, const float4* tg0, const float4* tg1,
                                                 const float4* tg2, const float4* tg3, const float4* taux,
-                                                const double* tape, const double4* X, double4* Xb, float* gacc,
+                                                const double* tape, const double4* X, double4* Xb, void* gacc,
                                                 const XfBodyParams* bodies, float comp1_u, float a_u, float dt);
-extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, uint32_t n, const double4* O, const double4* xs,
-                                             const double4* o, double4* Xb, double4* Vb, double4* Ob,
-                                             const XfBodyParams* bodies, const double4* xloss, const double4* target,
-                                             const double* lw);
-extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, uint32_t n, const double4* O, double4* Xb, double4* Vb,
-                                               const double4* Ob, const XfBodyParams* bodies);
-extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, uint32_t t, float* gacc, double* glog);
+extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, bool fp64, uint32_t n, const double4* O,
+                                             const double4* xs, const double4* o, double4* Xb, double4* Vb,
+                                             double4* Ob, const XfBodyParams* bodies, const double4* xloss,
+                                             const double4* target, const double* lw);
+extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, bool fp64, uint32_t n, const double4* O, double4* Xb,
+                                               double4* Vb, const double4* Ob, const XfBodyParams* bodies);
+extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, bool fp64, uint32_t t, void* gacc, double* glog);
+extern "C" hipError_t xf_launch_predict64(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
+                                          const XfBodyParams* bodies, float dt, float* clock);
+extern "C" hipError_t xf_launch_project64(hipStream_t s, bool aux, bool rec, uint32_t begin, uint32_t count,
+                                          const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
+                                          const float4* tg3, const float4* taux, double4* X, double* tape,
+                                          const XfBodyParams* bodies, float comp1_u, float a_u, float dt);
+extern "C" hipError_t xf_launch_finish64(hipStream_t s, uint32_t n, double4* X, const double4* O, double4* V,
+                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a);
 extern "C" hipError_t xf_launch_zero_u32(hipStream_t s, uint32_t n, uint32_t* a);
 extern "C" hipError_t xf_launch_expand_targets(hipStream_t s, uint32_t n, const double* raw, double4* out);
@@ -347,6 +355,11 @@ struct xf_scene {
     bool graph_sim = false, graph_aux = false, graph_dproj = false;
     uint32_t graph_vpasses = 0, graph_post = 0, graph_nkin = 0;
     bool graph_manip = false;
+    // graph of the adjoint's reverse pass (window replays and reverse
+    // sweeps), kept for the call shape in adj_key (sizes, loss substeps,
+    // precision, scene constants baked into the launches, buffer addresses)
+    hipGraphExec_t adj_exec = nullptr;
+    std::vector<uint64_t> adj_key;
 };
 
 namespace {
@@ -354,6 +367,16 @@ namespace {
 // Called when the step call shape changes: a launch of the old executable
 // graph may still be running on the stream (xf_scene_step returns once the
 // work is enqueued), so the stream drains before the graph is destroyed.
+void DropAdjGraph(xf_scene* s) {
+    if (s->adj_exec) {
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
+        (void)hipGraphExecDestroy(s->adj_exec);
+    }
+    s->adj_exec = nullptr;
+    s->adj_key.clear();
+}
+
 void DropGraph(xf_scene* s) {
     if (s->graph_exec) {
         (void)hipSetDevice(s->ctx->device);
@@ -499,6 +522,9 @@ int UploadParams(xf_scene* s, float dt) {
         p.node_base = s->bodies[i].node_base;
         p.n_nodes = s->bodies[i].n;
         p.unlock = st.unlock_pins;
+        p.gravity[0] = st.gravity[0];
+        p.gravity[1] = st.gravity[1];
+        p.drag = st.drag;
     }
     XF_HIP(ctx, hipMemcpyAsync(s->bp, hp.data(), sizeof(XfBodyParams) * hp.size(), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -1226,6 +1252,7 @@ XF_API void xf_scene_destroy(xf_scene* s) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     DropGraph(s);
+    DropAdjGraph(s);
     void* ptrs[] = {s->X,  s->O,     s->V,     s->tidx,  s->tg[0],         s->tg[1],  s->tg[2],
                     s->tg[3], s->taux, s->bp, s->clock, s->counter, s->colour_begin_d, s->arrive,
                     s->barrier_err, s->tdmp, s->rmeta, s->rinfo, s->rnodes, s->rincb, s->rinc, s->rtet,
@@ -2020,7 +2047,7 @@ struct AdjBufs {
     double* tape = nullptr;                  // 12 doubles per tet and substep of a window
     double4 *xs = nullptr, *oo = nullptr;    // X after the sweep and O, per substep of a window
     double4 *Xb = nullptr, *Vb = nullptr, *Ob = nullptr;
-    float* gacc = nullptr;                   // per-tet gradient within a window
+    void* gacc = nullptr;                    // per-tet gradient within a window (float; double when fp64)
     double* glog = nullptr;                  // per-tet gradient over windows
     double4 *xloss = nullptr, *tgt = nullptr;
     double* lw = nullptr;                    // per-node loss-gradient weight
@@ -2051,12 +2078,42 @@ int AdjointCheck(xf_scene* s, uint32_t substeps, uint32_t n_loss, const uint32_t
     return XF_OK;
 }
 
+// `substeps` substeps of the FP64 element model from (X, V): prediction,
+// the colours in sweep order, pins / ground contact / velocity. With tape,
+// each substep k records its sweep tape at tape + 12 T k and its node state
+// before the contact pass and O at xs / oo + N k (the replay of a window).
+// advance: the scene clock advances as under xf_scene_step (not for a
+// replay).
+int Forward64(xf_scene* s, uint32_t substeps, float dt, bool advance, double* tape, double4* xs, double4* oo) {
+    xf_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const uint32_t N = s->N, T = s->T;
+    for (uint32_t k = 0; k < substeps; k++) {
+        XF_HIP(ctx, xf_launch_predict64(st, N, s->X, s->O, s->V, s->bp, dt, advance ? s->clock : nullptr));
+        double* tp = tape ? tape + 12 * (size_t)T * k : nullptr;
+        for (uint32_t c = 0; c < s->ncol; c++) {
+            uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+            XF_HIP(ctx, xf_launch_project64(st, s->aux, tp != nullptr, begin, count, s->tidx, s->tg[0], s->tg[1],
+                                            s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp, s->comp1_u, s->a_u, dt));
+        }
+        if (tape) XF_HIP(ctx, xf_launch_copy2_d4(st, N, xs + (size_t)N * k, s->X, oo + (size_t)N * k, s->O));
+        XF_HIP(ctx, xf_launch_finish64(st, N, s->X, s->O, s->V, s->bp));
+        if (advance) {
+            s->time_host = s->time_host + dt;
+            s->tick_host += 1;
+        }
+    }
+    return XF_OK;
+}
+
 // Forward rollout with checkpoints at window starts and X at the loss
 // substeps; graph: through xf_scene_step (its hipGraph cache), else eager
-// launches (no graph instantiation). Leaves the end state saved in
-// save_x / save_v and the end clock in *clock_end.
+// launches (no graph instantiation); fp64: the FP64 element model
+// (Forward64). Leaves the end state saved in save_x / save_v and the end
+// clock in *clock_end.
 int AdjointForward(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, uint32_t n_loss,
-                   const uint32_t* loss_at, uint32_t window, const AdjBufs& b, bool graph, float* clock_end) {
+                   const uint32_t* loss_at, uint32_t window, const AdjBufs& b, bool graph, bool fp64,
+                   float* clock_end) {
     xf_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     const uint32_t N = s->N;
@@ -2070,7 +2127,9 @@ int AdjointForward(xf_scene* s, uint32_t substeps, float dt, int energy, bool si
         }
         uint32_t next = std::min(substeps, (k / window + 1) * window);
         if (j < n_loss) next = std::min(next, loss_at[j]);
-        if (graph) {
+        if (fp64) {
+            if ((rc = Forward64(s, next - k, dt, true, nullptr, nullptr, nullptr))) return rc;
+        } else if (graph) {
             if ((rc = xf_scene_step(s, next - k, dt))) return rc;
         } else {
             if ((rc = Enqueue(s, next - k, dt, energy, sim, nullptr))) return rc;
@@ -2090,40 +2149,43 @@ int AdjointForward(xf_scene* s, uint32_t substeps, float dt, int energy, bool si
     return CheckBarrier(s);
 }
 
-// Reverse pass, window by window from the end: each window is replayed from
+// Launches of the reverse pass, window by window from the end: each window is replayed from
 // its checkpoint (recording the tape and node states), then swept back.
 // Leaves the scene at the end of the forward rollout and glog holding
 // -dL/dlog mu per schedule position.
-int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_t n_loss, const uint32_t* loss_at,
-                   uint32_t window, const AdjBufs& b, const float* clock_end) {
+int ReverseLaunches(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_t n_loss, const uint32_t* loss_at,
+                    uint32_t window, const AdjBufs& b, bool fp64) {
     xf_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     const uint32_t N = s->N, T = s->T;
-    const size_t n4 = sizeof(double4) * N;
     const uint32_t nwin = (substeps + window - 1) / window;
     XF_HIP(ctx, xf_launch_zero_d4(st, N, b.Xb));
     XF_HIP(ctx, xf_launch_zero_d4(st, N, b.Vb));
-    XF_HIP(ctx, xf_launch_zero_u32(st, T, reinterpret_cast<uint32_t*>(b.gacc)));
+    XF_HIP(ctx, xf_launch_zero_u32(st, fp64 ? 2 * T : T, reinterpret_cast<uint32_t*>(b.gacc)));
     XF_HIP(ctx, xf_launch_zero_u32(st, 2 * T, reinterpret_cast<uint32_t*>(b.glog)));
     uint32_t jl = n_loss;
     for (uint32_t w = nwin; w-- > 0;) {
         const uint32_t k0 = w * window, k1 = std::min(substeps, k0 + window);
-        XF_HIP(ctx, hipMemcpyAsync(s->X, b.ckx + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
-        XF_HIP(ctx, hipMemcpyAsync(s->V, b.ckv + (size_t)N * w, n4, hipMemcpyDeviceToDevice, st));
+        XF_HIP(ctx, xf_launch_copy2_d4(st, N, s->X, b.ckx + (size_t)N * w, s->V, b.ckv + (size_t)N * w));
         // replay of the window, recording the sweep tape and the node states
-        XF_HIP(ctx, xf_launch_predict(st, N, s->X, s->O, s->V, s->bp));
-        for (uint32_t kk = k0; kk < k1; kk++) {
-            double* tp = b.tape + 12 * (size_t)T * (kk - k0);
-            for (uint32_t c = 0; c < s->ncol; c++) {
-                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_project_record(st, energy, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1],
-                                                     s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp, s->comp1_u, s->a_u,
-                                                     dt, nullptr));
+        if (fp64) {
+            int rc = Forward64(s, k1 - k0, dt, false, b.tape, b.xs, b.oo);
+            if (rc) return rc;
+        } else {
+            XF_HIP(ctx, xf_launch_predict(st, N, s->X, s->O, s->V, s->bp));
+            for (uint32_t kk = k0; kk < k1; kk++) {
+                double* tp = b.tape + 12 * (size_t)T * (kk - k0);
+                for (uint32_t c = 0; c < s->ncol; c++) {
+                    uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
+                    XF_HIP(ctx, xf_launch_project_record(st, energy, s->aux, begin, count, s->tidx, s->tg[0],
+                                                         s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp,
+                                                         s->comp1_u, s->a_u, dt, nullptr));
+                }
+                XF_HIP(ctx, xf_launch_copy2_d4(st, N, b.xs + (size_t)N * (kk - k0), s->X,
+                                               b.oo + (size_t)N * (kk - k0), s->O));
+                XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, s->bodies.size() == 1, N, s->X, s->O, s->V, s->bp,
+                                             s->clock));
             }
-            XF_HIP(ctx, xf_launch_copy2_d4(st, N, b.xs + (size_t)N * (kk - k0), s->X, b.oo + (size_t)N * (kk - k0),
-                                           s->O));
-            XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, s->bodies.size() == 1, N, s->X, s->O, s->V, s->bp,
-                                         s->clock));
         }
         for (uint32_t kk = k1; kk-- > k0;) {
             const double4* xl = nullptr;
@@ -2133,18 +2195,71 @@ int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_
                 xl = b.xloss + (size_t)N * jl;
                 tg = b.tgt + (size_t)N * jl;
             }
-            XF_HIP(ctx, xf_launch_adj_node_end(st, N, s->O, b.xs + (size_t)N * (kk - k0), b.oo + (size_t)N * (kk - k0),
+            XF_HIP(ctx, xf_launch_adj_node_end(st, fp64, N, s->O, b.xs + (size_t)N * (kk - k0),
+                                               b.oo + (size_t)N * (kk - k0),
                                                b.Xb, b.Vb, b.Ob, s->bp, xl, tg, b.lw));
             const double* tp = b.tape + 12 * (size_t)T * (kk - k0);
             for (uint32_t c = s->ncol; c-- > 0;) {
                 uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_project_reverse(st, s->aux, begin, count, s->tidx, s->tg[0], s->tg[1], s->tg[2],
-                                                      s->tg[3], s->taux, tp, s->X, b.Xb, b.gacc, s->bp, s->comp1_u,
-                                                      s->a_u, dt));
+                XF_HIP(ctx, xf_launch_project_reverse(st, s->aux, fp64, begin, count, s->tidx, s->tg[0], s->tg[1],
+                                                      s->tg[2], s->tg[3], s->taux, tp, s->X, b.Xb, b.gacc, s->bp,
+                                                      s->comp1_u, s->a_u, dt));
             }
-            XF_HIP(ctx, xf_launch_adj_node_start(st, N, s->O, b.Xb, b.Vb, b.Ob, s->bp));
+            XF_HIP(ctx, xf_launch_adj_node_start(st, fp64, N, s->O, b.Xb, b.Vb, b.Ob, s->bp));
         }
-        XF_HIP(ctx, xf_launch_adj_flush(st, T, b.gacc, b.glog));
+        XF_HIP(ctx, xf_launch_adj_flush(st, fp64, T, b.gacc, b.glog));
+    }
+    return XF_OK;
+}
+
+// Reverse pass: the launches of ReverseLaunches, eagerly or (graph) as one
+// hipGraph kept for the call shape and replayed by later calls of that
+// shape. Leaves the scene at the end of the forward rollout and glog
+// holding -dL/dlog mu per schedule position.
+int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_t n_loss, const uint32_t* loss_at,
+                   uint32_t window, const AdjBufs& b, bool fp64, bool graph, const float* clock_end) {
+    xf_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const size_t n4 = sizeof(double4) * s->N;
+    if (!graph) {
+        int rc = ReverseLaunches(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64);
+        if (rc) return rc;
+    } else {
+        auto bits = [](float f) {
+            uint32_t u;
+            memcpy(&u, &f, 4);
+            return (uint64_t)u;
+        };
+        std::vector<uint64_t> key = {substeps, window, n_loss, fp64, (uint64_t)energy, s->aux, bits(dt),
+                                     bits(s->comp1_u), bits(s->a_u), s->ncol, s->N, s->T};
+        key.insert(key.end(), loss_at, loss_at + n_loss);
+        for (const void* q : {(const void*)b.ckx, (const void*)b.ckv, (const void*)b.tape, (const void*)b.xs,
+                              (const void*)b.oo, (const void*)b.Xb, (const void*)b.Vb, (const void*)b.Ob,
+                              (const void*)b.gacc,
+                              (const void*)b.glog, (const void*)b.xloss, (const void*)b.tgt, (const void*)b.lw,
+                              (const void*)s->X, (const void*)s->O, (const void*)s->V, (const void*)s->bp,
+                              (const void*)s->clock})
+            key.push_back((uint64_t)(uintptr_t)q);
+        if (!s->adj_exec || s->adj_key != key) {
+            DropAdjGraph(s);
+            XF_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = ReverseLaunches(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return Fail(ctx, XF_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&s->adj_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                s->adj_exec = nullptr;
+                return Fail(ctx, XF_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            }
+            s->adj_key = key;
+        }
+        XF_HIP(ctx, hipGraphLaunch(s->adj_exec, st));
     }
     // Leave the scene at the end of the forward rollout.
     XF_HIP(ctx, hipMemcpyAsync(s->X, b.save_x, n4, hipMemcpyDeviceToDevice, st));
@@ -2155,9 +2270,11 @@ int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_
 
 }  // namespace
 
-XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
-                            const double* targets, uint32_t window, double* loss, double* grad_log_mu,
-                            double* body_loss) {
+namespace {
+// xf_scene_adjoint (fp64 false) and xf_scene_adjoint_fp64 (fp64 true).
+int AdjointRun(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
+               const double* targets, uint32_t window, double* loss, double* grad_log_mu, double* body_loss,
+               bool fp64) {
     if (!s) return XF_E_INVALID;
     xf_ctx* ctx = s->ctx;
     int rc = AdjointCheck(s, substeps, n_loss, loss_at, targets, window);
@@ -2175,7 +2292,8 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
         (rc = Reserve(s, &s->adj_tape, sizeof(double) * 12 * (size_t)T * wlen)) ||
         (rc = Reserve(s, &s->adj_xs, n4 * wlen)) || (rc = Reserve(s, &s->adj_o, n4 * wlen)) ||
         (rc = Reserve(s, &s->adj_xb, n4)) || (rc = Reserve(s, &s->adj_vb, n4)) || (rc = Reserve(s, &s->adj_ob, n4)) ||
-        (rc = Reserve(s, &s->adj_gacc, sizeof(float) * T)) || (rc = Reserve(s, &s->adj_glog, sizeof(double) * T)) ||
+        (rc = Reserve(s, &s->adj_gacc, (fp64 ? sizeof(double) : sizeof(float)) * T)) ||
+        (rc = Reserve(s, &s->adj_glog, sizeof(double) * T)) ||
         (rc = Reserve(s, &s->adj_xloss, n4 * n_loss)) || (rc = Reserve(s, &s->adj_tgt, n4 * n_loss)) ||
         (rc = Reserve(s, &s->adj_lw, sizeof(double) * N)) || (rc = Reserve(s, &s->adj_save_x, n4)) ||
         (rc = Reserve(s, &s->adj_save_v, n4)))
@@ -2189,7 +2307,7 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
     b.Xb = (double4*)s->adj_xb.p;
     b.Vb = (double4*)s->adj_vb.p;
     b.Ob = (double4*)s->adj_ob.p;
-    b.gacc = (float*)s->adj_gacc.p;
+    b.gacc = s->adj_gacc.p;
     b.glog = (double*)s->adj_glog.p;
     b.xloss = (double4*)s->adj_xloss.p;
     b.tgt = (double4*)s->adj_tgt.p;
@@ -2212,7 +2330,8 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
     }
 
     float clock_end = 0.0f;
-    if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, true, &clock_end))) return rc;
+    if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, true, fp64, &clock_end)))
+        return rc;
 
     // Loss: sum over loss substeps and bodies of (1/n_b) sum |x - target|^2.
     {
@@ -2239,7 +2358,8 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
         if (loss) *loss = L;
     }
 
-    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, &clock_end))) return rc;
+    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64, true, &clock_end)))
+        return rc;
     std::vector<double> hg(T);
     XF_HIP(ctx, hipMemcpyAsync(hg.data(), b.glog, sizeof(double) * T, hipMemcpyDeviceToHost, st));
     XF_HIP(ctx, hipStreamSynchronize(st));
@@ -2251,6 +2371,19 @@ XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n
         }
     }
     return XF_OK;
+}
+}  // namespace
+
+XF_API int xf_scene_adjoint(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
+                            const double* targets, uint32_t window, double* loss, double* grad_log_mu,
+                            double* body_loss) {
+    return AdjointRun(s, substeps, dt, n_loss, loss_at, targets, window, loss, grad_log_mu, body_loss, false);
+}
+
+XF_API int xf_scene_adjoint_fp64(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
+                                 const double* targets, uint32_t window, double* loss, double* grad_log_mu,
+                                 double* body_loss) {
+    return AdjointRun(s, substeps, dt, n_loss, loss_at, targets, window, loss, grad_log_mu, body_loss, true);
 }
 
 namespace {
@@ -2318,7 +2451,7 @@ XF_API int xf_scene_adjoint_noalloc(xf_scene* s, uint32_t substeps, float dt, ui
     b.Xb = (double4*)take(n4);
     b.Vb = (double4*)take(n4);
     b.Ob = (double4*)take(n4);
-    b.gacc = (float*)take(sizeof(float) * T);
+    b.gacc = take(sizeof(float) * T);
     b.glog = (double*)take(sizeof(double) * T);
     b.xloss = (double4*)take(n4 * n_loss);
     b.tgt = (double4*)take(n4 * n_loss);
@@ -2337,9 +2470,11 @@ XF_API int xf_scene_adjoint_noalloc(xf_scene* s, uint32_t substeps, float dt, ui
     XF_HIP(ctx, xf_launch_loss_weights(st, N, s->O, s->bp, b.lw));
 
     float clock_end = 0.0f;
-    if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, false, &clock_end))) return rc;
+    if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, false, false, &clock_end)))
+        return rc;
     XF_HIP(ctx, xf_launch_loss(st, n_loss, B, N, b.xloss, b.tgt, s->bp, eterm, lout));
-    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, &clock_end))) return rc;
+    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, false, false, &clock_end)))
+        return rc;
     XF_HIP(ctx, xf_launch_grad_out(st, T, b.glog, s->tout, gout));
     if (loss) XF_HIP(ctx, hipMemcpyAsync(loss, lout, sizeof(double), hipMemcpyDeviceToHost, st));
     if (body_loss) XF_HIP(ctx, hipMemcpyAsync(body_loss, lout + 1, sizeof(double) * B, hipMemcpyDeviceToHost, st));

@@ -158,6 +158,8 @@ def lib():
                               vp, vp], C.c_int),
         "xf_scene_adjoint_noalloc": ([vp, C.c_uint32, C.c_float, C.c_uint32, _u32p, vp, C.c_uint32,
                                       C.POINTER(C.c_double), vp, vp], C.c_int),
+        "xf_scene_adjoint_fp64": ([vp, C.c_uint32, C.c_float, C.c_uint32, _u32p, vp, C.c_uint32,
+                                   C.POINTER(C.c_double), vp, vp], C.c_int),
         "xf_scene_autotune": ([vp, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(SceneOpts), C.POINTER(C.c_float)],
                               C.c_int),
         "xf_mesh_tet_block": ([C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, _u32p],
@@ -188,7 +190,7 @@ def exported_symbols():
         "xf_mesh_colour", "xf_scene_autotune", "xf_scene_adjoint", "xf_mesh_tet_block_colours",
         "xf_scene_get_tick", "xf_scene_set_tick", "xf_mesh_read_tetgen", "xf_mesh_tets_from_stream", "xf_mesh_error",
         "xf_mesh_rings", "xf_mesh_sweep_blocks", "xf_scene_time_steps", "xf_scene_adjoint_noalloc",
-        "xf_scene_set_kinematic", "xf_scene_set_manip",
+        "xf_scene_set_kinematic", "xf_scene_set_manip", "xf_scene_adjoint_fp64",
     ]
 
 
@@ -570,19 +572,22 @@ class Scene:
         size = o.persistent_blocks if o.launch_mode == LAUNCH_PERSISTENT else o.block_size
         return o.launch_mode, size, ms.value
 
-    def adjoint(self, substeps, dt, loss_at, targets, window=20, per_body=False, noalloc=False):
+    def adjoint(self, substeps, dt, loss_at, targets, window=20, per_body=False, noalloc=False, fp64=False):
         """Forward rollout + adjoint: returns (loss, dL/dlog mu per tet, all
         bodies concatenated), plus each body's loss term if per_body.
         targets: [n_loss, total nodes * 3] float64. noalloc: through
         xf_scene_adjoint_noalloc (device scratch of fixed size, no
-        allocation)."""
+        allocation). fp64: through xf_scene_adjoint_fp64 (the FP64 element
+        model)."""
+        if noalloc and fp64:
+            raise ValueError("the allocation-free adjoint runs the FP32 element model only")
         loss_at = np.ascontiguousarray(loss_at, np.uint32)
         targets = np.ascontiguousarray(targets, np.float64)
         n_tets = sum(t for _, t in self.bodies)
         grad = np.zeros(n_tets, np.float64)
         body_loss = np.zeros(len(self.bodies), np.float64)
         L = C.c_double()
-        name = "xf_scene_adjoint_noalloc" if noalloc else "xf_scene_adjoint"
+        name = "xf_scene_adjoint_noalloc" if noalloc else ("xf_scene_adjoint_fp64" if fp64 else "xf_scene_adjoint")
         self.ctx.check(getattr(lib(), name)(self.h, substeps, dt, loss_at.size, loss_at, targets.ctypes.data,
                                             window, C.byref(L), grad.ctypes.data, body_loss.ctypes.data), name)
         return (L.value, grad, body_loss) if per_body else (L.value, grad)
