@@ -396,7 +396,7 @@ def large_leg(ctx, frames=2, cells=(288, 72, 72), schedule="blocks"):
     return out
 
 
-def adjoint_leg(ctx, reps=3):
+def adjoint_leg(ctx, reps=3, use_graph=True):
     """Adjoint of the 16x4x4 fitting beam (1,536 tets, per-tet mu, 30 frames of
     20 substeps, loss at frames 10, 20, 30, checkpoints every 20 substeps):
     wall time of xf_scene_adjoint (forward + checkpointed reverse) against the
@@ -409,7 +409,7 @@ def adjoint_leg(ctx, reps=3):
     dt, S = float(truth["dt"]), truth["substeps"]
     n_sub = S * truth["frames"]
     loss_at = np.array([10 * S, 20 * S, 30 * S], np.uint32)
-    ref = workloads.make_scene(ctx, truth)
+    ref = workloads.make_scene(ctx, truth, use_graph=use_graph)
     targets, done = [], 0
     for la in loss_at:
         ref.step(int(la) - done, dt)
@@ -417,8 +417,8 @@ def adjoint_leg(ctx, reps=3):
         targets.append(ref.get_state(0)[0])
     ref.close()
     targets = np.array(targets)
-    fwd = workloads.make_scene(ctx, guess)
-    adj = workloads.make_scene(ctx, guess)
+    fwd = workloads.make_scene(ctx, guess, use_graph=use_graph)
+    adj = workloads.make_scene(ctx, guess, use_graph=use_graph)
     x0, v0 = fwd.get_state(0)
     fwd.step(n_sub, dt)
     fwd.sync()
@@ -637,6 +637,9 @@ def main():
     ap.add_argument("--tet-lanes", type=int, default=1, help="--profile: lanes per tet of block launches")
     ap.add_argument("--frames", type=int, default=1, help="--profile: frames")
     ap.add_argument("--graph", action="store_true", help="--profile: replay each frame from a hipGraph")
+    ap.add_argument("--adjoint-profile", action="store_true",
+                    help="only run the adjoint leg's forward and adjoint with eager launches (rocprofv3 kernel "
+                         "traces)")
     args = ap.parse_args()
     args.xcd_slabs = not args.no_xcd_slabs
     if args.copy_probe:
@@ -650,6 +653,11 @@ def main():
         return
     if args.large_profile:
         large_profile(schedule=args.schedule)
+        return
+    if args.adjoint_profile:
+        ctx = xf.Context(0)
+        print(json.dumps({"adjoint_profile": adjoint_leg(ctx, reps=2, use_graph=False)}))
+        ctx.close()
         return
     if args.profile:
         beam_profile(schedule=args.schedule, block=args.block, tet_lanes=args.tet_lanes, frames=args.frames,

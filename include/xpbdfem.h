@@ -361,7 +361,11 @@ XF_API int xf_scene_autotune(xf_scene* scene, uint32_t substeps, float dt, uint3
  * contact and serial mode return XF_E_UNSUPPORTED). The scene is left at the
  * end of the forward rollout. body_loss (nullable) receives each body's
  * term of L (one double per body), so a caller can tell which bodies of a
- * batch went non-finite. */
+ * batch went non-finite. With the scene option use_graph, the forward runs
+ * through xf_scene_step's graphs and the reverse pass (window replays and
+ * reverse sweeps) is captured as one hipGraph, kept for the call shape
+ * (substeps, window, loss substeps, buffers) and replayed by later calls of
+ * that shape; without it every launch is eager. */
 XF_API int xf_scene_adjoint(xf_scene* scene, uint32_t substeps, float dt, uint32_t n_loss, const uint32_t* loss_at,
                             const double* targets, uint32_t window, double* loss, double* grad_log_mu,
                             double* body_loss);

@@ -249,21 +249,20 @@ __device__ __forceinline__ TetData load_tet(uint32_t i, const float4* tg0, const
 
 }  // namespace
 
-// Reverse sweep of one colour: tets [begin, begin+count) of the sweep order.
-// tape holds each tet's 4 node positions before its projection (12 doubles
-// per schedule position) for this substep. gacc accumulates d(loss)/d(log
-// comp0) per schedule position. R: element precision (float, or double for
-// the FP64 element model: constants widened from their FP32 values).
+// Reverse of the projection of the tet at schedule position i. tape holds
+// each tet's 4 node positions before its projection (12 doubles per
+// schedule position) for this substep; W holds the inverse masses (.w);
+// gacc accumulates d(loss)/d(log comp0) per schedule position. R: element
+// precision (float, or double for the FP64 element model: constants
+// widened from their FP32 values).
 template <bool AUX, class R>
-__global__ __launch_bounds__(256) void xf_project_reverse(
-    uint32_t begin, uint32_t count, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
-    const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
-    const float4* __restrict__ taux, const double* __restrict__ tape, const double4* __restrict__ X,
-    double4* __restrict__ Xb, R* __restrict__ gacc, const XfBodyParams* __restrict__ bodies, float comp1_u,
-    float a_u, float dt) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    uint32_t i = begin + k;
+__device__ __forceinline__ void reverse_one(uint32_t i, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+                                            const float4* __restrict__ tg1, const float4* __restrict__ tg2,
+                                            const float4* __restrict__ tg3, const float4* __restrict__ taux,
+                                            const double* __restrict__ tape, const double4* __restrict__ W,
+                                            double4* __restrict__ Xb, R* __restrict__ gacc,
+                                            const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u,
+                                            float dt) {
     uint4 id = tidx[i];
     TetData t = load_tet<AUX>(i, tg0, tg1, tg2, tg3, taux, comp1_u, a_u);
     const XfBodyParams& bp = bodies[t.body];
@@ -287,7 +286,7 @@ __global__ __launch_bounds__(256) void xf_project_reverse(
     double xb[4][3];
 #pragma unroll
     for (int n = 0; n < 4; n++) {
-        w[n] = (float)X[nid[n]].w;
+        w[n] = (float)W[nid[n]].w;
         double4 b = Xb[nid[n]];
         xb[n][0] = b.x;
         xb[n][1] = b.y;
@@ -297,6 +296,85 @@ __global__ __launch_bounds__(256) void xf_project_reverse(
     gacc[i] = gacc[i] + gl;
 #pragma unroll
     for (int n = 0; n < 4; n++) Xb[nid[n]] = make_double4(xb[n][0], xb[n][1], xb[n][2], 0.0);
+}
+
+// Reverse sweep of one colour: tets [begin, begin+count) of the sweep order.
+template <bool AUX, class R>
+__global__ __launch_bounds__(64) void xf_project_reverse(
+    uint32_t begin, uint32_t count, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+    const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+    const float4* __restrict__ taux, const double* __restrict__ tape, const double4* __restrict__ W,
+    double4* __restrict__ Xb, R* __restrict__ gacc, const XfBodyParams* __restrict__ bodies, float comp1_u,
+    float a_u, float dt) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count)
+        reverse_one<AUX, R>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, tape, W, Xb, gacc, bodies, comp1_u, a_u, dt);
+}
+
+// Projection of the tet at schedule position i in element precision R
+// (xf::solve_mixed_sim: the simultaneous MixedSel / YeohSkinFast solve of
+// solve_tet in xf_kernels.hip, constants widened for R = double); REC
+// records its four node positions before the projection into tape (12
+// doubles per schedule position).
+template <bool AUX, class R, bool REC>
+__device__ __forceinline__ void project_one(uint32_t i, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+                                            const float4* __restrict__ tg1, const float4* __restrict__ tg2,
+                                            const float4* __restrict__ tg3, const float4* __restrict__ taux,
+                                            double4* __restrict__ X, double* __restrict__ tape,
+                                            const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u,
+                                            float dt) {
+    uint4 id = tidx[i];
+    TetData t = load_tet<AUX>(i, tg0, tg1, tg2, tg3, taux, comp1_u, a_u);
+    const XfBodyParams& bp = bodies[t.body];
+    R C[3], Qi[3][3], QQ[3], QR[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        C[c] = (R)bp.yeoh[c];
+        QQ[c] = (R)t.QQ[c];
+        QR[c] = (R)t.QR[c];
+#pragma unroll
+        for (int r = 0; r < 3; r++) Qi[c][r] = (R)t.Qi[c][r];
+    }
+    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
+    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
+    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
+    if constexpr (REC) {
+        double* tp = tape + 12 * (size_t)i;
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) tp[3 * n + r] = x[n][r];
+    }
+    xf::solve_mixed_sim<R>(bp.energy == 7, C, x, w, Qi, QQ, QR, (R)t.comp0, (R)t.comp1, (R)t.a, (R)dt);
+    X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
+    X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
+    X[id.z] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
+    X[id.w] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+}
+
+// One launch of two independent sweeps of the pipelined reverse pass:
+// workgroups [0, rblocks) record colour [rbegin, rbegin + rcount) of a
+// window replay (on X, into tape_rec); the rest take the reverse of colour
+// [vbegin, vbegin + vcount) of the later window (tape_rev, inverse masses
+// from W, adjoint Xb, gacc). The two touch disjoint arrays.
+template <bool AUX, class R>
+__global__ __launch_bounds__(64) void xf_adj_pair(
+    uint32_t rbegin, uint32_t rcount, uint32_t rblocks, double4* __restrict__ X, double* __restrict__ tape_rec,
+    uint32_t vbegin, uint32_t vcount, const double* __restrict__ tape_rev, const double4* __restrict__ W,
+    double4* __restrict__ Xb, R* __restrict__ gacc, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
+    const float4* __restrict__ tg1, const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+    const float4* __restrict__ taux, const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u, float dt) {
+    if (blockIdx.x < rblocks) {
+        uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k < rcount)
+            project_one<AUX, R, true>(rbegin + k, tidx, tg0, tg1, tg2, tg3, taux, X, tape_rec, bodies, comp1_u, a_u,
+                                      dt);
+    } else {
+        uint32_t k = (blockIdx.x - rblocks) * blockDim.x + threadIdx.x;
+        if (k < vcount)
+            reverse_one<AUX, R>(vbegin + k, tidx, tg0, tg1, tg2, tg3, taux, tape_rev, W, Xb, gacc, bodies, comp1_u,
+                                a_u, dt);
+    }
 }
 
 // ---------------------------------------------- FP64 element model forward
@@ -336,54 +414,32 @@ __global__ __launch_bounds__(256) void xf_predict64(uint32_t n, double4* __restr
 // positions before its projection into tape (12 doubles per schedule
 // position), as xf_project_record does for the FP32 model.
 template <bool AUX, bool REC>
-__global__ __launch_bounds__(256) void xf_project64(uint32_t begin, uint32_t count, const uint4* __restrict__ tidx,
-                                                    const float4* __restrict__ tg0, const float4* __restrict__ tg1,
-                                                    const float4* __restrict__ tg2, const float4* __restrict__ tg3,
-                                                    const float4* __restrict__ taux, double4* __restrict__ X,
-                                                    double* __restrict__ tape, const XfBodyParams* __restrict__ bodies,
-                                                    float comp1_u, float a_u, float dt) {
+__global__ __launch_bounds__(64) void xf_project64(uint32_t begin, uint32_t count, const uint4* __restrict__ tidx,
+                                                   const float4* __restrict__ tg0, const float4* __restrict__ tg1,
+                                                   const float4* __restrict__ tg2, const float4* __restrict__ tg3,
+                                                   const float4* __restrict__ taux, double4* __restrict__ X,
+                                                   double* __restrict__ tape, const XfBodyParams* __restrict__ bodies,
+                                                   float comp1_u, float a_u, float dt) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    uint32_t i = begin + k;
-    uint4 id = tidx[i];
-    TetData t = load_tet<AUX>(i, tg0, tg1, tg2, tg3, taux, comp1_u, a_u);
-    const XfBodyParams& bp = bodies[t.body];
-    double C[3], Qi[3][3], QQ[3], QR[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        C[c] = (double)bp.yeoh[c];
-        QQ[c] = (double)t.QQ[c];
-        QR[c] = (double)t.QR[c];
-#pragma unroll
-        for (int r = 0; r < 3; r++) Qi[c][r] = (double)t.Qi[c][r];
-    }
-    double4 na = X[id.x], nb = X[id.y], nc = X[id.z], nd = X[id.w];
-    double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
-    float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
-    if constexpr (REC) {
-        double* tp = tape + 12 * (size_t)i;
-#pragma unroll
-        for (int n = 0; n < 4; n++)
-#pragma unroll
-            for (int r = 0; r < 3; r++) tp[3 * n + r] = x[n][r];
-    }
-    xf::solve_mixed_sim<double>(bp.energy == 7, C, x, w, Qi, QQ, QR, (double)t.comp0, (double)t.comp1, (double)t.a,
-                                (double)dt);
-    X[id.x] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
-    X[id.y] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
-    X[id.z] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
-    X[id.w] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+    if (k < count)
+        project_one<AUX, double, REC>(begin + k, tidx, tg0, tg1, tg2, tg3, taux, X, tape, bodies, comp1_u, a_u, dt);
 }
 
 // End of an FP64 substep for one node: a pinned node returns to O; other
 // nodes take the ground contact in double (static friction when the
 // tangential displacement is within mu * depth, else it shrinks by mu *
 // depth); then V = (X - O) / dt with 1 / dt in double.
+// xs_out / oo_out (nullable): X after the sweep and O are copied there first.
 __global__ __launch_bounds__(256) void xf_finish64(uint32_t n, double4* __restrict__ X, const double4* __restrict__ O,
-                                                   double4* __restrict__ V, const XfBodyParams* __restrict__ bodies) {
+                                                   double4* __restrict__ V, const XfBodyParams* __restrict__ bodies,
+                                                   double4* __restrict__ xs_out, double4* __restrict__ oo_out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 xi = X[i], oi = O[i];
+    if (xs_out) {
+        xs_out[i] = xi;
+        oo_out[i] = oi;
+    }
     const uint32_t flags = (uint32_t)oi.w;
     const XfBodyParams& p = bodies[flags >> 1];
     double x[3] = {xi.x, xi.y, xi.z};
@@ -425,21 +481,14 @@ __device__ __forceinline__ double rsqrt_of(double x) { return sqrt(x); }
 // Xb += lw_i * (x_loss - target). xs / o: the node's position before the
 // contact pass and at the substep start (recorded by the forward replay).
 // R: element precision of the forward (float: FP32 contact and 1.0f / dt;
-// double: the FP64 element model).
+// double: the FP64 element model). xbv / vbv: the node's Xb, Vb in, Xb out;
+// ob receives Ob.
 template <class R>
-__global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4* __restrict__ O,
-                                                       const double4* __restrict__ xs, const double4* __restrict__ o,
-                                                       double4* __restrict__ Xb, double4* __restrict__ Vb,
-                                                       double4* __restrict__ Ob, const XfBodyParams* __restrict__ bodies,
-                                                       const double4* __restrict__ xloss,
-                                                       const double4* __restrict__ target,
-                                                       const double* __restrict__ lw) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t flags = (uint32_t)O[i].w;
-    const XfBodyParams& p = bodies[flags >> 1];
-    double4 xb4 = Xb[i], vb4 = Vb[i];
-    double xbv[3] = {xb4.x, xb4.y, xb4.z};
+__device__ __forceinline__ void node_end_one(uint32_t i, uint32_t flags, const XfBodyParams& p,
+                                             const double4* __restrict__ xs, const double4* __restrict__ o,
+                                             const double4* __restrict__ xloss, const double4* __restrict__ target,
+                                             const double* __restrict__ lw, double xbv[3], const double vbv[3],
+                                             double ob[3]) {
     if (xloss) {
         double4 xl = xloss[i], tg = target[i];
         double wl = lw[i];
@@ -447,9 +496,8 @@ __global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4
         xbv[1] = xbv[1] + wl * (xl.y - tg.y);
         xbv[2] = xbv[2] + wl * (xl.z - tg.z);
     }
-    double vbv[3] = {vb4.x, vb4.y, vb4.z};
     const double invdt = sizeof(R) == 4 ? (double)p.invdt : 1.0 / (double)p.dt;
-    double xcb[3], ob[3];
+    double xcb[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         xcb[r] = xbv[r] + vbv[r] * invdt;
@@ -493,26 +541,56 @@ __global__ __launch_bounds__(256) void xf_adj_node_end(uint32_t n, const double4
             }
         }
     }
-    Xb[i] = make_double4(out[0], out[1], out[2], 0.0);
-    Ob[i] = make_double4(ob[0], ob[1], ob[2], 0.0);
+#pragma unroll
+    for (int r = 0; r < 3; r++) xbv[r] = out[r];
 }
 
 // Reverse of the prediction (Geo.cpp:307-312) for one node: Xb <- Xb + Ob,
 // Vb <- Xb_pred * dt * (1 - drag) (1 - drag: the FP32 value, or formed in
 // double for the FP64 element model).
 template <class R>
-__global__ __launch_bounds__(256) void xf_adj_node_start(uint32_t n, const double4* __restrict__ O,
-                                                         double4* __restrict__ Xb, double4* __restrict__ Vb,
-                                                         const double4* __restrict__ Ob,
-                                                         const XfBodyParams* __restrict__ bodies) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const XfBodyParams& p = bodies[(uint32_t)O[i].w >> 1];
-    double4 xb = Xb[i], obv = Ob[i];
+__device__ __forceinline__ void node_start_one(const XfBodyParams& p, double xb[3], const double ob[3],
+                                               double vb[3]) {
     const double dt = (double)p.dt;
     const double dragf = sizeof(R) == 4 ? (double)p.dragf : 1.0 - (double)p.drag;
-    Xb[i] = make_double4(xb.x + obv.x, xb.y + obv.y, xb.z + obv.z, 0.0);
-    Vb[i] = make_double4((xb.x * dt) * dragf, (xb.y * dt) * dragf, (xb.z * dt) * dragf, 0.0);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const double x0b = xb[r];
+        xb[r] = x0b + ob[r];
+        vb[r] = (x0b * dt) * dragf;
+    }
+}
+
+// Reverse of the node steps around one substep boundary, per node:
+// START: the prediction of the later substep (node_start_one, from Ob);
+// END: then the end of the earlier substep (node_end_one, recorded xs / o,
+// loss at xloss when it ends a loss frame), writing Ob for its sweep.
+template <class R, bool START, bool END>
+__global__ __launch_bounds__(256) void xf_adj_nodes(uint32_t n, const double4* __restrict__ O,
+                                                    const double4* __restrict__ xs, const double4* __restrict__ o,
+                                                    double4* __restrict__ Xb, double4* __restrict__ Vb,
+                                                    double4* __restrict__ Ob, const XfBodyParams* __restrict__ bodies,
+                                                    const double4* __restrict__ xloss,
+                                                    const double4* __restrict__ target,
+                                                    const double* __restrict__ lw) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t flags = (uint32_t)O[i].w;
+    const XfBodyParams& p = bodies[flags >> 1];
+    const double4 xb4 = Xb[i], vb4 = Vb[i];
+    double xb[3] = {xb4.x, xb4.y, xb4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
+    if constexpr (START) {
+        const double4 o4 = Ob[i];
+        const double ob[3] = {o4.x, o4.y, o4.z};
+        node_start_one<R>(p, xb, ob, vb);
+    }
+    if constexpr (END) {
+        double ob[3];
+        node_end_one<R>(i, flags, p, xs, o, xloss, target, lw, xb, vb, ob);
+        Ob[i] = make_double4(ob[0], ob[1], ob[2], 0.0);
+    }
+    Xb[i] = make_double4(xb[0], xb[1], xb[2], 0.0);
+    if constexpr (START) Vb[i] = make_double4(vb[0], vb[1], vb[2], 0.0);
 }
 
 // Window boundary: glog += gacc (the window sum in element precision into
@@ -625,7 +703,7 @@ __global__ __launch_bounds__(256) void xf_copy2_d4(uint32_t n, double4* __restri
 extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, bool fp64, uint32_t begin, uint32_t count,
                                                 const uint4* tidx, const float4* tg0, const float4* tg1,
                                                 const float4* tg2, const float4* tg3, const float4* taux,
-                                                const double* tape, const double4* X, double4* Xb, void* gacc,
+                                                const double* tape, const double4* W, double4* Xb, void* gacc,
                                                 const XfBodyParams* bodies, float comp1_u, float a_u, float dt) {
     if (count == 0) return hipSuccess;
     dim3 grid((count + 63) / 64), block(64);
@@ -633,42 +711,67 @@ extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, bool fp
     double* g64 = static_cast<double*>(gacc);
     if (fp64 && aux)
         hipLaunchKernelGGL((xf_project_reverse<true, double>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
-                           tg3, taux, tape, X, Xb, g64, bodies, comp1_u, a_u, dt);
+                           tg3, taux, tape, W, Xb, g64, bodies, comp1_u, a_u, dt);
     else if (fp64)
         hipLaunchKernelGGL((xf_project_reverse<false, double>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
-                           tg3, taux, tape, X, Xb, g64, bodies, comp1_u, a_u, dt);
+                           tg3, taux, tape, W, Xb, g64, bodies, comp1_u, a_u, dt);
     else if (aux)
         hipLaunchKernelGGL((xf_project_reverse<true, float>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
-                           tg3, taux, tape, X, Xb, g32, bodies, comp1_u, a_u, dt);
+                           tg3, taux, tape, W, Xb, g32, bodies, comp1_u, a_u, dt);
     else
         hipLaunchKernelGGL((xf_project_reverse<false, float>), grid, block, 0, s, begin, count, tidx, tg0, tg1, tg2,
-                           tg3, taux, tape, X, Xb, g32, bodies, comp1_u, a_u, dt);
+                           tg3, taux, tape, W, Xb, g32, bodies, comp1_u, a_u, dt);
     return hipGetLastError();
 }
 
-extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, bool fp64, uint32_t n, const double4* O,
-                                             const double4* xs, const double4* o, double4* Xb, double4* Vb,
-                                             double4* Ob, const XfBodyParams* bodies, const double4* xloss,
-                                             const double4* target, const double* lw) {
-    if (n == 0) return hipSuccess;
-    if (fp64)
-        hipLaunchKernelGGL(xf_adj_node_end<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, xs, o, Xb, Vb, Ob,
-                           bodies, xloss, target, lw);
-    else
-        hipLaunchKernelGGL(xf_adj_node_end<float>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, xs, o, Xb, Vb, Ob,
-                           bodies, xloss, target, lw);
+// Record of colour [rbegin, rbegin + rcount) and reverse of colour
+// [vbegin, vbegin + vcount) in one launch (xf_adj_pair); either count may
+// be 0.
+extern "C" hipError_t xf_launch_adj_pair(hipStream_t s, bool aux, bool fp64, uint32_t rbegin, uint32_t rcount,
+                                         double4* X, double* tape_rec, uint32_t vbegin, uint32_t vcount,
+                                         const double* tape_rev, const double4* W, double4* Xb, void* gacc,
+                                         const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
+                                         const float4* tg3, const float4* taux, const XfBodyParams* bodies,
+                                         float comp1_u, float a_u, float dt) {
+    const uint32_t rblocks = (rcount + 63) / 64, vblocks = (vcount + 63) / 64;
+    if (rblocks + vblocks == 0) return hipSuccess;
+    dim3 grid(rblocks + vblocks), block(64);
+#define XF_PAIR(A, R, G)                                                                                         \
+    hipLaunchKernelGGL((xf_adj_pair<A, R>), grid, block, 0, s, rbegin, rcount, rblocks, X, tape_rec, vbegin,   \
+                       vcount, tape_rev, W, Xb, static_cast<R*>(G), tidx, tg0, tg1, tg2, tg3, taux, bodies, comp1_u, \
+                       a_u, dt)
+    if (fp64) {
+        if (aux) XF_PAIR(true, double, gacc);
+        else XF_PAIR(false, double, gacc);
+    } else {
+        if (aux) XF_PAIR(true, float, gacc);
+        else XF_PAIR(false, float, gacc);
+    }
+#undef XF_PAIR
     return hipGetLastError();
 }
 
-extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, bool fp64, uint32_t n, const double4* O, double4* Xb,
-                                               double4* Vb, const double4* Ob, const XfBodyParams* bodies) {
-    if (n == 0) return hipSuccess;
-    if (fp64)
-        hipLaunchKernelGGL(xf_adj_node_start<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, Xb, Vb, Ob,
-                           bodies);
-    else
-        hipLaunchKernelGGL(xf_adj_node_start<float>, dim3((n + 255) / 256), dim3(256), 0, s, n, O, Xb, Vb, Ob,
-                           bodies);
+// start / end: the reverse of the later substep's prediction and of the
+// earlier substep's end (xs, o, xloss, target, lw are read only with end).
+extern "C" hipError_t xf_launch_adj_nodes(hipStream_t s, bool fp64, bool start, bool end, uint32_t n,
+                                          const double4* O, const double4* xs, const double4* o, double4* Xb,
+                                          double4* Vb, double4* Ob, const XfBodyParams* bodies,
+                                          const double4* xloss, const double4* target, const double* lw) {
+    if (n == 0 || (!start && !end)) return hipSuccess;
+    dim3 grid((n + 255) / 256), block(256);
+#define XF_NODES(R, S, E)                                                                                    \
+    hipLaunchKernelGGL((xf_adj_nodes<R, S, E>), grid, block, 0, s, n, O, xs, o, Xb, Vb, Ob, bodies, xloss, \
+                       target, lw)
+    if (fp64) {
+        if (start && end) XF_NODES(double, true, true);
+        else if (start) XF_NODES(double, true, false);
+        else XF_NODES(double, false, true);
+    } else {
+        if (start && end) XF_NODES(float, true, true);
+        else if (start) XF_NODES(float, true, false);
+        else XF_NODES(float, false, true);
+    }
+#undef XF_NODES
     return hipGetLastError();
 }
 
@@ -711,9 +814,9 @@ extern "C" hipError_t xf_launch_project64(hipStream_t s, bool aux, bool rec, uin
 }
 
 extern "C" hipError_t xf_launch_finish64(hipStream_t s, uint32_t n, double4* X, const double4* O, double4* V,
-                                         const XfBodyParams* bodies) {
+                                         const XfBodyParams* bodies, double4* xs_out, double4* oo_out) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(xf_finish64, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies);
+    hipLaunchKernelGGL(xf_finish64, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, xs_out, oo_out);
     return hipGetLastError();
 }
 

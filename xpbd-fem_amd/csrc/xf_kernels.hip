@@ -1147,14 +1147,21 @@ __global__ __launch_bounds__(256) void xf_predict(uint32_t n, double4* __restric
 // SINGLE: the scene has one body, whose parameters every node reads (loaded
 // once per wave, alongside the node loads, instead of through the node's
 // body id).
-template <bool PREDICT, bool SINGLE>
+// SNAP: X after the sweep and O are first copied to xs_out / oo_out (the
+// adjoint's replay records them for the reverse of the contact pass).
+template <bool PREDICT, bool SINGLE, bool SNAP>
 __global__ __launch_bounds__(256) void xf_finish(uint32_t n, double4* __restrict__ X, double4* __restrict__ O,
                                                  double4* __restrict__ V, const XfBodyParams* __restrict__ bodies,
-                                                 const float* __restrict__ clock) {
+                                                 const float* __restrict__ clock, double4* __restrict__ xs_out,
+                                                 double4* __restrict__ oo_out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 xi = X[i];
     double4 oi = O[i];
+    if constexpr (SNAP) {
+        xs_out[i] = xi;
+        oo_out[i] = oi;
+    }
     uint32_t flags = (uint32_t)oi.w;
     const XfBodyParams& p = bodies[SINGLE ? 0u : flags >> 1];
     double x[3] = {xi.x, xi.y, xi.z};
@@ -1664,9 +1671,21 @@ extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, d
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
                                        double4* O, double4* V, const XfBodyParams* bodies, const float* clock) {
     if (n == 0) return hipSuccess;
-    auto fn = predict_next ? (single_body ? xf_finish<true, true> : xf_finish<true, false>)
-                           : (single_body ? xf_finish<false, true> : xf_finish<false, false>);
-    hipLaunchKernelGGL(fn, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, clock);
+    auto fn = predict_next ? (single_body ? xf_finish<true, true, false> : xf_finish<true, false, false>)
+                           : (single_body ? xf_finish<false, true, false> : xf_finish<false, false, false>);
+    hipLaunchKernelGGL(fn, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, clock, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+// xf_launch_finish that first copies X (after the sweep) and O to xs_out /
+// oo_out.
+extern "C" hipError_t xf_launch_finish_snap(hipStream_t s, bool predict_next, bool single_body, uint32_t n,
+                                            double4* X, double4* O, double4* V, const XfBodyParams* bodies,
+                                            const float* clock, double4* xs_out, double4* oo_out) {
+    if (n == 0) return hipSuccess;
+    auto fn = predict_next ? (single_body ? xf_finish<true, true, true> : xf_finish<true, false, true>)
+                           : (single_body ? xf_finish<false, true, true> : xf_finish<false, false, true>);
+    hipLaunchKernelGGL(fn, dim3((n + 255) / 256), dim3(256), 0, s, n, X, O, V, bodies, clock, xs_out, oo_out);
     return hipGetLastError();
 }
 

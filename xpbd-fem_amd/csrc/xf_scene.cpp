@@ -74,14 +74,21 @@ extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool a
 extern "C" hipError_t xf_launch_project_reverse(hipStream_t s, bool aux, bool fp64, uint32_t begin, uint32_t count,
                                                 const uint4* tidx, const float4* tg0, const float4* tg1,
                                                 const float4* tg2, const float4* tg3, const float4* taux,
-                                                const double* tape, const double4* X, double4* Xb, void* gacc,
+                                                const double* tape, const double4* W, double4* Xb, void* gacc,
                                                 const XfBodyParams* bodies, float comp1_u, float a_u, float dt);
-extern "C" hipError_t xf_launch_adj_node_end(hipStream_t s, bool fp64, uint32_t n, const double4* O,
-                                             const double4* xs, const double4* o, double4* Xb, double4* Vb,
-                                             double4* Ob, const XfBodyParams* bodies, const double4* xloss,
-                                             const double4* target, const double* lw);
-extern "C" hipError_t xf_launch_adj_node_start(hipStream_t s, bool fp64, uint32_t n, const double4* O, double4* Xb,
-                                               double4* Vb, const double4* Ob, const XfBodyParams* bodies);
+extern "C" hipError_t xf_launch_adj_pair(hipStream_t s, bool aux, bool fp64, uint32_t rbegin, uint32_t rcount,
+                                         double4* X, double* tape_rec, uint32_t vbegin, uint32_t vcount,
+                                         const double* tape_rev, const double4* W, double4* Xb, void* gacc,
+                                         const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2,
+                                         const float4* tg3, const float4* taux, const XfBodyParams* bodies,
+                                         float comp1_u, float a_u, float dt);
+extern "C" hipError_t xf_launch_adj_nodes(hipStream_t s, bool fp64, bool start, bool end, uint32_t n,
+                                          const double4* O, const double4* xs, const double4* o, double4* Xb,
+                                          double4* Vb, double4* Ob, const XfBodyParams* bodies,
+                                          const double4* xloss, const double4* target, const double* lw);
+extern "C" hipError_t xf_launch_finish_snap(hipStream_t s, bool predict_next, bool single_body, uint32_t n,
+                                            double4* X, double4* O, double4* V, const XfBodyParams* bodies,
+                                            const float* clock, double4* xs_out, double4* oo_out);
 extern "C" hipError_t xf_launch_adj_flush(hipStream_t s, bool fp64, uint32_t t, void* gacc, double* glog);
 extern "C" hipError_t xf_launch_predict64(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                           const XfBodyParams* bodies, float dt, float* clock);
@@ -90,7 +97,7 @@ extern "C" hipError_t xf_launch_project64(hipStream_t s, bool aux, bool rec, uin
                                           const float4* tg3, const float4* taux, double4* X, double* tape,
                                           const XfBodyParams* bodies, float comp1_u, float a_u, float dt);
 extern "C" hipError_t xf_launch_finish64(hipStream_t s, uint32_t n, double4* X, const double4* O, double4* V,
-                                         const XfBodyParams* bodies);
+                                         const XfBodyParams* bodies, double4* xs_out, double4* oo_out);
 extern "C" hipError_t xf_launch_zero_d4(hipStream_t s, uint32_t n, double4* a);
 extern "C" hipError_t xf_launch_zero_u32(hipStream_t s, uint32_t n, uint32_t* a);
 extern "C" hipError_t xf_launch_expand_targets(hipStream_t s, uint32_t n, const double* raw, double4* out);
@@ -2079,29 +2086,22 @@ int AdjointCheck(xf_scene* s, uint32_t substeps, uint32_t n_loss, const uint32_t
 }
 
 // `substeps` substeps of the FP64 element model from (X, V): prediction,
-// the colours in sweep order, pins / ground contact / velocity. With tape,
-// each substep k records its sweep tape at tape + 12 T k and its node state
-// before the contact pass and O at xs / oo + N k (the replay of a window).
-// advance: the scene clock advances as under xf_scene_step (not for a
-// replay).
-int Forward64(xf_scene* s, uint32_t substeps, float dt, bool advance, double* tape, double4* xs, double4* oo) {
+// the colours in sweep order, pins / ground contact / velocity. Advances
+// the clock as xf_scene_step does.
+int Forward64(xf_scene* s, uint32_t substeps, float dt) {
     xf_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
-    const uint32_t N = s->N, T = s->T;
+    const uint32_t N = s->N;
     for (uint32_t k = 0; k < substeps; k++) {
-        XF_HIP(ctx, xf_launch_predict64(st, N, s->X, s->O, s->V, s->bp, dt, advance ? s->clock : nullptr));
-        double* tp = tape ? tape + 12 * (size_t)T * k : nullptr;
+        XF_HIP(ctx, xf_launch_predict64(st, N, s->X, s->O, s->V, s->bp, dt, s->clock));
         for (uint32_t c = 0; c < s->ncol; c++) {
             uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-            XF_HIP(ctx, xf_launch_project64(st, s->aux, tp != nullptr, begin, count, s->tidx, s->tg[0], s->tg[1],
-                                            s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp, s->comp1_u, s->a_u, dt));
+            XF_HIP(ctx, xf_launch_project64(st, s->aux, false, begin, count, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                                            s->tg[3], s->taux, s->X, nullptr, s->bp, s->comp1_u, s->a_u, dt));
         }
-        if (tape) XF_HIP(ctx, xf_launch_copy2_d4(st, N, xs + (size_t)N * k, s->X, oo + (size_t)N * k, s->O));
-        XF_HIP(ctx, xf_launch_finish64(st, N, s->X, s->O, s->V, s->bp));
-        if (advance) {
-            s->time_host = s->time_host + dt;
-            s->tick_host += 1;
-        }
+        XF_HIP(ctx, xf_launch_finish64(st, N, s->X, s->O, s->V, s->bp, nullptr, nullptr));
+        s->time_host = s->time_host + dt;
+        s->tick_host += 1;
     }
     return XF_OK;
 }
@@ -2128,7 +2128,7 @@ int AdjointForward(xf_scene* s, uint32_t substeps, float dt, int energy, bool si
         uint32_t next = std::min(substeps, (k / window + 1) * window);
         if (j < n_loss) next = std::min(next, loss_at[j]);
         if (fp64) {
-            if ((rc = Forward64(s, next - k, dt, true, nullptr, nullptr, nullptr))) return rc;
+            if ((rc = Forward64(s, next - k, dt))) return rc;
         } else if (graph) {
             if ((rc = xf_scene_step(s, next - k, dt))) return rc;
         } else {
@@ -2149,63 +2149,133 @@ int AdjointForward(xf_scene* s, uint32_t substeps, float dt, int energy, bool si
     return CheckBarrier(s);
 }
 
-// Launches of the reverse pass, window by window from the end: each window is replayed from
-// its checkpoint (recording the tape and node states), then swept back.
-// Leaves the scene at the end of the forward rollout and glog holding
-// -dL/dlog mu per schedule position.
+// Launches of the reverse pass, window by window from the end: each window
+// is replayed from its checkpoint (recording the sweep tape and the node
+// states before the contact pass), then swept back. pipelined: the replay
+// of window w - 1 runs interleaved with the reverse of window w (their
+// colour launches paired into one launch each, xf_launch_adj_pair), with
+// the tape and node records double-buffered by window parity; else one
+// window after the other in single buffers. glog receives -dL/dlog mu per
+// schedule position.
 int ReverseLaunches(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_t n_loss, const uint32_t* loss_at,
-                    uint32_t window, const AdjBufs& b, bool fp64) {
+                    uint32_t window, const AdjBufs& b, bool fp64, bool pipelined) {
     xf_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     const uint32_t N = s->N, T = s->T;
     const uint32_t nwin = (substeps + window - 1) / window;
+    const uint32_t wlen = std::min(window, substeps);
+    auto k0_of = [&](uint32_t w) { return w * window; };
+    auto len_of = [&](uint32_t w) { return std::min(substeps, (w + 1) * window) - w * window; };
+    // per-window records: tape (12 doubles a tet), X after the sweep and O (a node each), per substep
+    auto slot = [&](uint32_t w) { return pipelined ? (w & 1u) : 0u; };
+    auto tape_of = [&](uint32_t w, uint32_t i) { return b.tape + 12 * (size_t)T * (slot(w) * (size_t)wlen + i); };
+    auto xs_of = [&](uint32_t w, uint32_t i) { return b.xs + (size_t)N * (slot(w) * (size_t)wlen + i); };
+    auto oo_of = [&](uint32_t w, uint32_t i) { return b.oo + (size_t)N * (slot(w) * (size_t)wlen + i); };
+    const double4* W = b.ckx;  // inverse masses (.w) for the reverse sweeps, constant over the rollout
     XF_HIP(ctx, xf_launch_zero_d4(st, N, b.Xb));
     XF_HIP(ctx, xf_launch_zero_d4(st, N, b.Vb));
     XF_HIP(ctx, xf_launch_zero_u32(st, fp64 ? 2 * T : T, reinterpret_cast<uint32_t*>(b.gacc)));
     XF_HIP(ctx, xf_launch_zero_u32(st, 2 * T, reinterpret_cast<uint32_t*>(b.glog)));
-    uint32_t jl = n_loss;
-    for (uint32_t w = nwin; w-- > 0;) {
-        const uint32_t k0 = w * window, k1 = std::min(substeps, k0 + window);
+    const bool single = s->bodies.size() == 1;
+    // replay of window w: start (state from the checkpoint, first prediction)
+    auto replay_start = [&](uint32_t w) -> int {
         XF_HIP(ctx, xf_launch_copy2_d4(st, N, s->X, b.ckx + (size_t)N * w, s->V, b.ckv + (size_t)N * w));
-        // replay of the window, recording the sweep tape and the node states
+        if (fp64) XF_HIP(ctx, xf_launch_predict64(st, N, s->X, s->O, s->V, s->bp, dt, nullptr));
+        else XF_HIP(ctx, xf_launch_predict(st, N, s->X, s->O, s->V, s->bp));
+        return XF_OK;
+    };
+    // end of replayed substep i of window w (records X and O, then the next prediction)
+    auto replay_end = [&](uint32_t w, uint32_t i) -> int {
+        const bool next = i + 1 < len_of(w);
         if (fp64) {
-            int rc = Forward64(s, k1 - k0, dt, false, b.tape, b.xs, b.oo);
-            if (rc) return rc;
+            XF_HIP(ctx, xf_launch_finish64(st, N, s->X, s->O, s->V, s->bp, xs_of(w, i), oo_of(w, i)));
+            if (next) XF_HIP(ctx, xf_launch_predict64(st, N, s->X, s->O, s->V, s->bp, dt, nullptr));
         } else {
-            XF_HIP(ctx, xf_launch_predict(st, N, s->X, s->O, s->V, s->bp));
-            for (uint32_t kk = k0; kk < k1; kk++) {
-                double* tp = b.tape + 12 * (size_t)T * (kk - k0);
-                for (uint32_t c = 0; c < s->ncol; c++) {
-                    uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                    XF_HIP(ctx, xf_launch_project_record(st, energy, s->aux, begin, count, s->tidx, s->tg[0],
-                                                         s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, tp, s->bp,
-                                                         s->comp1_u, s->a_u, dt, nullptr));
-                }
-                XF_HIP(ctx, xf_launch_copy2_d4(st, N, b.xs + (size_t)N * (kk - k0), s->X,
-                                               b.oo + (size_t)N * (kk - k0), s->O));
-                XF_HIP(ctx, xf_launch_finish(st, kk + 1 < k1, s->bodies.size() == 1, N, s->X, s->O, s->V, s->bp,
-                                             s->clock));
-            }
+            XF_HIP(ctx, xf_launch_finish_snap(st, next, single, N, s->X, s->O, s->V, s->bp, s->clock, xs_of(w, i),
+                                              oo_of(w, i)));
         }
-        for (uint32_t kk = k1; kk-- > k0;) {
-            const double4* xl = nullptr;
-            const double4* tg = nullptr;
-            if (jl > 0 && loss_at[jl - 1] == kk + 1) {
-                jl--;
-                xl = b.xloss + (size_t)N * jl;
-                tg = b.tgt + (size_t)N * jl;
+        return XF_OK;
+    };
+    // colour c of a replay substep (record) and colour v of a reverse substep, either absent (nullptr tape)
+    auto colours = [&](double* rec_tape, uint32_t c, const double* rev_tape, uint32_t v) -> int {
+        const uint32_t rb = s->colour_begin[c], rc = rec_tape ? s->colour_begin[c + 1] - rb : 0;
+        const uint32_t vb = s->colour_begin[v], vc = rev_tape ? s->colour_begin[v + 1] - vb : 0;
+        if (rc && vc) {
+            XF_HIP(ctx, xf_launch_adj_pair(st, s->aux, fp64, rb, rc, s->X, rec_tape, vb, vc, rev_tape, W, b.Xb,
+                                           b.gacc, s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->bp,
+                                           s->comp1_u, s->a_u, dt));
+        } else if (rc) {
+            if (fp64)
+                XF_HIP(ctx, xf_launch_project64(st, s->aux, true, rb, rc, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                                                s->tg[3], s->taux, s->X, rec_tape, s->bp, s->comp1_u, s->a_u, dt));
+            else
+                XF_HIP(ctx, xf_launch_project_record(st, energy, s->aux, rb, rc, s->tidx, s->tg[0], s->tg[1],
+                                                     s->tg[2], s->tg[3], s->taux, s->X, rec_tape, s->bp, s->comp1_u,
+                                                     s->a_u, dt, nullptr));
+        } else if (vc) {
+            XF_HIP(ctx, xf_launch_project_reverse(st, s->aux, fp64, vb, vc, s->tidx, s->tg[0], s->tg[1], s->tg[2],
+                                                  s->tg[3], s->taux, rev_tape, W, b.Xb, b.gacc, s->bp, s->comp1_u,
+                                                  s->a_u, dt));
+        }
+        return XF_OK;
+    };
+    uint32_t jl = n_loss;
+    // the loss injected at the end of substep kk, if kk ends a loss frame
+    auto loss_of = [&](uint32_t kk, const double4** xl, const double4** tg) {
+        *xl = nullptr;
+        *tg = nullptr;
+        if (jl > 0 && loss_at[jl - 1] == kk + 1) {
+            jl--;
+            *xl = b.xloss + (size_t)N * jl;
+            *tg = b.tgt + (size_t)N * jl;
+        }
+    };
+    int rc;
+    if (pipelined) {
+        // the last window's replay runs alone
+        const uint32_t wl = nwin - 1;
+        if ((rc = replay_start(wl))) return rc;
+        for (uint32_t i = 0; i < len_of(wl); i++) {
+            for (uint32_t c = 0; c < s->ncol; c++)
+                if ((rc = colours(tape_of(wl, i), c, nullptr, 0))) return rc;
+            if ((rc = replay_end(wl, i))) return rc;
+        }
+    }
+    for (uint32_t w = nwin; w-- > 0;) {
+        // the window to replay alongside this reverse (pipelined: the previous one), or this one first
+        const bool rep = pipelined ? w > 0 : true;
+        const uint32_t wr = pipelined ? w - 1 : w;
+        if (!pipelined) {
+            if ((rc = replay_start(w))) return rc;
+            for (uint32_t i = 0; i < len_of(w); i++) {
+                for (uint32_t c = 0; c < s->ncol; c++)
+                    if ((rc = colours(tape_of(w, i), c, nullptr, 0))) return rc;
+                if ((rc = replay_end(w, i))) return rc;
             }
-            XF_HIP(ctx, xf_launch_adj_node_end(st, fp64, N, s->O, b.xs + (size_t)N * (kk - k0),
-                                               b.oo + (size_t)N * (kk - k0),
-                                               b.Xb, b.Vb, b.Ob, s->bp, xl, tg, b.lw));
-            const double* tp = b.tape + 12 * (size_t)T * (kk - k0);
-            for (uint32_t c = s->ncol; c-- > 0;) {
-                uint32_t begin = s->colour_begin[c], count = s->colour_begin[c + 1] - begin;
-                XF_HIP(ctx, xf_launch_project_reverse(st, s->aux, fp64, begin, count, s->tidx, s->tg[0], s->tg[1],
-                                                      s->tg[2], s->tg[3], s->taux, tp, s->X, b.Xb, b.gacc, s->bp,
-                                                      s->comp1_u, s->a_u, dt));
+        } else if (rep && (rc = replay_start(wr))) {
+            return rc;
+        }
+        const uint32_t lv = len_of(w), lr = pipelined && rep ? len_of(wr) : 0, k0 = k0_of(w);
+        const double4 *xl, *tg;
+        loss_of(k0 + lv - 1, &xl, &tg);
+        XF_HIP(ctx, xf_launch_adj_nodes(st, fp64, false, true, N, s->O, xs_of(w, lv - 1), oo_of(w, lv - 1), b.Xb,
+                                        b.Vb, b.Ob, s->bp, xl, tg, b.lw));
+        for (uint32_t i = 0; i < std::max(lv, lr); i++) {
+            const bool von = i < lv, ron = i < lr;
+            const uint32_t kk = k0 + lv - 1 - i;  // reverse substep (when von)
+            for (uint32_t c = 0; c < s->ncol; c++)
+                if ((rc = colours(ron ? tape_of(wr, i) : nullptr, c, von ? tape_of(w, lv - 1 - i) : nullptr,
+                                  s->ncol - 1 - c)))
+                    return rc;
+            if (ron && (rc = replay_end(wr, i))) return rc;
+            if (von) {
+                const bool more = kk > k0;
+                xl = tg = nullptr;
+                if (more) loss_of(kk - 1, &xl, &tg);
+                const uint32_t ip = more ? lv - 2 - i : 0;
+                XF_HIP(ctx, xf_launch_adj_nodes(st, fp64, true, more, N, s->O, xs_of(w, ip), oo_of(w, ip), b.Xb,
+                                                b.Vb, b.Ob, s->bp, xl, tg, b.lw));
             }
-            XF_HIP(ctx, xf_launch_adj_node_start(st, fp64, N, s->O, b.Xb, b.Vb, b.Ob, s->bp));
         }
         XF_HIP(ctx, xf_launch_adj_flush(st, fp64, T, b.gacc, b.glog));
     }
@@ -2217,12 +2287,13 @@ int ReverseLaunches(xf_scene* s, uint32_t substeps, float dt, int energy, uint32
 // shape. Leaves the scene at the end of the forward rollout and glog
 // holding -dL/dlog mu per schedule position.
 int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_t n_loss, const uint32_t* loss_at,
-                   uint32_t window, const AdjBufs& b, bool fp64, bool graph, const float* clock_end) {
+                   uint32_t window, const AdjBufs& b, bool fp64, bool graph, bool pipelined,
+                   const float* clock_end) {
     xf_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     const size_t n4 = sizeof(double4) * s->N;
     if (!graph) {
-        int rc = ReverseLaunches(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64);
+        int rc = ReverseLaunches(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64, pipelined);
         if (rc) return rc;
     } else {
         auto bits = [](float f) {
@@ -2230,7 +2301,7 @@ int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_
             memcpy(&u, &f, 4);
             return (uint64_t)u;
         };
-        std::vector<uint64_t> key = {substeps, window, n_loss, fp64, (uint64_t)energy, s->aux, bits(dt),
+        std::vector<uint64_t> key = {substeps, window, n_loss, fp64, pipelined, (uint64_t)energy, s->aux, bits(dt),
                                      bits(s->comp1_u), bits(s->a_u), s->ncol, s->N, s->T};
         key.insert(key.end(), loss_at, loss_at + n_loss);
         for (const void* q : {(const void*)b.ckx, (const void*)b.ckv, (const void*)b.tape, (const void*)b.xs,
@@ -2243,7 +2314,7 @@ int AdjointReverse(xf_scene* s, uint32_t substeps, float dt, int energy, uint32_
         if (!s->adj_exec || s->adj_key != key) {
             DropAdjGraph(s);
             XF_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            int rc = ReverseLaunches(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64);
+            int rc = ReverseLaunches(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64, pipelined);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(st, &g);
             if (rc) {
@@ -2288,9 +2359,12 @@ int AdjointRun(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const 
     const uint32_t nwin = (substeps + window - 1) / window;
     const uint32_t wlen = std::min(window, substeps);
     const size_t n4 = sizeof(double4) * N;
+    // more than one window: the replay of each window runs beside the reverse of the next
+    const bool pipelined = nwin > 1;
+    const size_t nbuf = pipelined ? 2 : 1;
     if ((rc = Reserve(s, &s->adj_ck_x, n4 * nwin)) || (rc = Reserve(s, &s->adj_ck_v, n4 * nwin)) ||
-        (rc = Reserve(s, &s->adj_tape, sizeof(double) * 12 * (size_t)T * wlen)) ||
-        (rc = Reserve(s, &s->adj_xs, n4 * wlen)) || (rc = Reserve(s, &s->adj_o, n4 * wlen)) ||
+        (rc = Reserve(s, &s->adj_tape, sizeof(double) * 12 * (size_t)T * wlen * nbuf)) ||
+        (rc = Reserve(s, &s->adj_xs, n4 * wlen * nbuf)) || (rc = Reserve(s, &s->adj_o, n4 * wlen * nbuf)) ||
         (rc = Reserve(s, &s->adj_xb, n4)) || (rc = Reserve(s, &s->adj_vb, n4)) || (rc = Reserve(s, &s->adj_ob, n4)) ||
         (rc = Reserve(s, &s->adj_gacc, (fp64 ? sizeof(double) : sizeof(float)) * T)) ||
         (rc = Reserve(s, &s->adj_glog, sizeof(double) * T)) ||
@@ -2358,7 +2432,8 @@ int AdjointRun(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const 
         if (loss) *loss = L;
     }
 
-    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64, true, &clock_end)))
+    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64, s->opts.use_graph != 0,
+                             pipelined, &clock_end)))
         return rc;
     std::vector<double> hg(T);
     XF_HIP(ctx, hipMemcpyAsync(hg.data(), b.glog, sizeof(double) * T, hipMemcpyDeviceToHost, st));
@@ -2473,7 +2548,7 @@ XF_API int xf_scene_adjoint_noalloc(xf_scene* s, uint32_t substeps, float dt, ui
     if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, false, false, &clock_end)))
         return rc;
     XF_HIP(ctx, xf_launch_loss(st, n_loss, B, N, b.xloss, b.tgt, s->bp, eterm, lout));
-    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, false, false, &clock_end)))
+    if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, false, false, false, &clock_end)))
         return rc;
     XF_HIP(ctx, xf_launch_grad_out(st, T, b.glog, s->tout, gout));
     if (loss) XF_HIP(ctx, hipMemcpyAsync(loss, lout, sizeof(double), hipMemcpyDeviceToHost, st));
