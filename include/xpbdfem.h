@@ -258,6 +258,10 @@ typedef struct xf_scene_info {
                                 the energy is not checked here) */
     uint32_t tet_lanes;      /* lanes per tet of sweep-block launches in use (xf_scene_opts.tet_lanes,
                                 or the autotuner's choice) */
+    uint32_t block_lds_limit; /* LDS bytes one sweep-block workgroup may use: the device's
+                                 per-workgroup limit (160 KiB on gfx950); it caps a slot at
+                                 block_lds_limit / 32 nodes and decides whether the node phase
+                                 fits the last launch */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */

@@ -303,6 +303,38 @@ def test_blocks_autotune_slot_caps(ctx):
     assert np.array_equal(x, ref[0]) and np.array_equal(v, ref[1])
 
 
+def separate_tets(k):
+    """k unit tets 0.3 m apart along x, no pins (positions, tets, density)."""
+    one, t4 = xf.tet_block(1, 1, 1, (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))
+    p = one.reshape(-1, 3)[t4[:4]]
+    xyz = np.concatenate([p + np.float32([0.3 * j, 0.0, 0.0]) for j in range(k)]).ravel().astype(np.float32)
+    return xyz, np.arange(4 * k, dtype=np.uint32), 1000.0
+
+
+def test_blocks_beyond_64k_lds(ctx):
+    """A slot of 2,800 nodes (700 separate tets in one block: 89.6 KB of LDS,
+    more than 64 KiB) runs on a device whose workgroups may use more, bit
+    for bit as the oracle's serial sweep."""
+    xyz, tets, rho = separate_tets(700)
+    wl = dict(name="tets700", xyz=xyz, tets=tets, density=rho, pinned=None, velocity=None, tet_mu=None,
+              settings=dict(energy=xf.ENERGY_MIXED_SEL, simultaneous=1, compliance=1e-5, poissons_ratio=0.5,
+                            gravity=(0.0, -9.81)),
+              dt=np.float32(1.0) / np.float32(1200.0), substeps=20)
+    sc = workloads.make_scene(ctx, wl, sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS)
+    info = sc.info()
+    assert info["block_lds_limit"] >= 65536
+    if info["block_lds_limit"] < 2800 * 32:
+        sc.close()
+        pytest.skip("the device's workgroups hold at most 64 KiB of LDS")
+    assert info["n_block_nodes"] == 2800
+    sc.step(wl["substeps"], float(wl["dt"]))
+    x, v = sc.get_state(0)
+    order, _ = sc.order(0)
+    sc.close()
+    ob = oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
+
+
 def test_blocks_rejections(ctx):
     wl = workloads.beam_neo_12x4x4()
     cases = [dict(launch_mode=xf.LAUNCH_BLOCKS), dict(sweep_blocks=True, xcd_slabs=True),
@@ -318,14 +350,15 @@ def test_blocks_rejections(ctx):
     with pytest.raises(xf.XfError):
         sc.finalize(sweep_blocks=True)
     sc.close()
-    # 700 separate tets: every component's levels restart at 0, so all fall
-    # in one block of 2,800 nodes (LDS holds 2,048)
-    one, t4 = xf.tet_block(1, 1, 1, (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))
-    p = one.reshape(-1, 3)[t4[:4]]
-    xyz = np.concatenate([p + np.float32([0.3 * k, 0.0, 0.0]) for k in range(700)]).ravel().astype(np.float32)
-    tets = np.arange(4 * 700, dtype=np.uint32)
+    # separate tets: every component's levels restart at 0, so all fall in
+    # one block; more nodes than a workgroup's LDS holds (32 B each)
+    probe = xf.Scene(ctx)
+    probe.add_body(*separate_tets(1))
+    probe.finalize()
+    limit = probe.info()["block_lds_limit"]
+    probe.close()
     sc = xf.Scene(ctx)
-    sc.add_body(xyz, tets, 1000.0)
+    sc.add_body(*separate_tets(limit // 32 // 4 + 1))
     with pytest.raises(xf.XfError, match="LDS"):
         sc.finalize(sweep_blocks=True)
     sc.close()

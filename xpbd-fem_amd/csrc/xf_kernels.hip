@@ -1564,7 +1564,9 @@ extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax) { return 
 // nodes and fmax extra nodes; tend: the clock, not advanced by this launch;
 // single_body: the scene has one body); not with damp or energy 30. quad:
 // four lanes per tet (simultaneous MixedSel / YeohSkinFast without damping,
-// energies 0, 4 and 7; anything else is hipErrorInvalidValue).
+// energies 0, 4 and 7; anything else is hipErrorInvalidValue). lds_limit:
+// the device's LDS bytes per workgroup (more than 64 KiB is granted to the
+// kernel with hipFuncAttributeMaxDynamicSharedMemorySize).
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
                                        uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
                                        const uint32_t* rec,
@@ -1573,7 +1575,7 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
                                        const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
-                                       const float* tend, bool single_body, bool quad) {
+                                       const float* tend, bool single_body, bool quad, size_t lds_limit) {
     XfBlocksFn fn = nullptr;
     const bool over = pmax > (quad ? threads / 4 : threads);
     switch (energy) {
@@ -1589,10 +1591,15 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
         default: return hipErrorInvalidValue;
     }
     size_t lds = fin ? xf_block_fin_lds_bytes(nmax, fmax) : xf_block_lds_bytes(nmax);
-    if (fn == nullptr || threads == 0 || threads > 256 || threads % 64 || nph == 0 || ntet == 0 || lds > 65536)
+    if (fn == nullptr || threads == 0 || threads > 256 || threads % 64 || nph == 0 || ntet == 0 || lds > lds_limit)
         return hipErrorInvalidValue;
     if (fin && (tend == nullptr || clock != nullptr || fextra == nullptr)) return hipErrorInvalidValue;
     if (nslot == 0) return hipSuccess;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(fn, dim3(nslot), dim3(threads), lds, s, nph, nmax, ntet, rec, bnode, tloc, tidx, tg0, tg1,
                        tg2, tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock, fextra, fmax, tend,
                        single_body ? 1u : 0u);

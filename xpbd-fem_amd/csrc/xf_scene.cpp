@@ -56,7 +56,7 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
                                        const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
-                                       const float* tend, bool single_body, bool quad);
+                                       const float* tend, bool single_body, bool quad, size_t lds_limit);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
@@ -234,8 +234,6 @@ constexpr int kRingMode = XF_ENERGY_CONTINUOUS_PIXAR;
 
 // LDS budget of one ring workgroup.
 constexpr size_t kRingLdsLimit = 65536;
-// LDS budget of one sweep-block workgroup.
-constexpr size_t kBlockLdsLimit = 65536;
 
 // XCD-slab order of a colour-major sweep order: inside each colour the items
 // are sorted by x (ties keep their order) and cut into the eighths the
@@ -308,6 +306,7 @@ struct xf_scene {
     uint32_t* blk_rec = nullptr;
     uint32_t* blk_node = nullptr;
     size_t blk_rec_n = 0, blk_node_n = 0, blk_fin_n = 0;  // their allocated lengths
+    size_t lds_limit = 65536;  // LDS bytes one workgroup may use on the device
     // the blocks of every body and the first colour of each launch colour,
     // kept for repacking the slots (xf_scene_autotune)
     std::vector<xf::SweepBlocks> sweep;
@@ -812,7 +811,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                                          s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O,
                                          s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr,
                                          fin && q == s->blk_last, s->blk_fin, s->blk_fmax, s->clock,
-                                         s->bodies.size() == 1, s->opts.tet_lanes == 4));
+                                         s->bodies.size() == 1, s->opts.tet_lanes == 4, s->lds_limit));
             first = false;
             if (timer) {
                 XF_HIP(ctx, hipEventRecord(e1, st));
@@ -935,7 +934,8 @@ struct BlockTables {
 int PackBlocks(xf_scene* s, uint32_t lanes_opt, BlockTables* bt) {
     xf_ctx* ctx = s->ctx;
     *bt = BlockTables();
-    constexpr uint32_t kSlotNodes = 2048;
+    // a slot's nodes (32 B each in LDS) within the device's per-workgroup LDS
+    const uint32_t kSlotNodes = (uint32_t)std::min<size_t>(s->lds_limit / 32, 65535);
     uint32_t nblk[4] = {0, 0, 0, 0};
     std::vector<std::vector<uint32_t>> lay(s->bodies.size());
     for (size_t bi = 0; bi < s->bodies.size(); bi++) {
@@ -1097,7 +1097,7 @@ int PackBlocks(xf_scene* s, uint32_t lanes_opt, BlockTables* bt) {
             if (!covered[v]) extra.push_back(v);
         s->blk_fmax = (uint32_t)((extra.size() + nslot[q] - 1) / nslot[q]);
         // staged with O and ids in LDS; else the node phase stays separate
-        if (xf_block_fin_lds_bytes(s->blk_nmax, s->blk_fmax) > kBlockLdsLimit) {
+        if (xf_block_fin_lds_bytes(s->blk_nmax, s->blk_fmax) > s->lds_limit) {
             s->blk_last = -1;
             s->blk_fmax = 0;
         }
@@ -1514,6 +1514,11 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     if (pos != s->T) return Fail(ctx, XF_E_INVALID, "internal: schedule covers %u of %u tets", pos, s->T);
 
     BlockTables bt;
+    {
+        int lds = 65536;
+        XF_HIP(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+        s->lds_limit = (size_t)lds;
+    }
     if (s->opts.sweep_blocks) {
         int prc = PackBlocks(s, s->opts.block_lanes, &bt);
         if (prc) return prc;
@@ -1621,6 +1626,7 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     if (s->nkin || s->manip_on) fused = false;
     info->fused_finish = fused ? 1u : 0u;
     info->tet_lanes = s->opts.tet_lanes;
+    info->block_lds_limit = (uint32_t)s->lds_limit;
     return XF_OK;
 }
 

@@ -85,7 +85,8 @@ class SceneInfo(C.Structure):
                 ("n_colours", C.c_uint32), ("device_bytes", C.c_uint64), ("n_rings", C.c_uint32),
                 ("n_ring_colours", C.c_uint32), ("n_blocks", C.c_uint32), ("block_launches", C.c_uint32),
                 ("n_block_nodes", C.c_uint32), ("block_lanes", C.c_uint32), ("n_block_slots", C.c_uint32),
-                ("fused_finish", C.c_uint32), ("tet_lanes", C.c_uint32)]
+                ("fused_finish", C.c_uint32), ("tet_lanes", C.c_uint32),
+                ("block_lds_limit", C.c_uint32)]
 
 
 _lib = None
@@ -464,7 +465,7 @@ class Scene:
                     device_bytes=i.device_bytes, n_rings=i.n_rings, n_ring_colours=i.n_ring_colours,
                     n_blocks=i.n_blocks, block_launches=i.block_launches, n_block_nodes=i.n_block_nodes,
                     block_lanes=i.block_lanes, n_block_slots=i.n_block_slots,
-                    fused_finish=i.fused_finish, tet_lanes=i.tet_lanes)
+                    fused_finish=i.fused_finish, tet_lanes=i.tet_lanes, block_lds_limit=i.block_lds_limit)
 
     def set_settings(self, body, settings=None, **kw):
         s = settings if settings is not None else default_settings(**kw)
