@@ -17,11 +17,12 @@
 //   1  per ring tet (lanes in parallel): I1 and J of the tet and their
 //      weighted gradient contributions at its 4 corners;
 //   2  per ring node (lanes in parallel): the contributions of the node's
-//      tets added in ring-tet order (host-built incidence lists); two lanes
-//      add the two energy terms in ring-tet order;
-//   3  per ring node: SolveSimplex's scaling; then one lane per constraint
-//      sum forms that sum in ring-node order, and one lane the Lagrange
-//      multipliers;
+//      tets added in ring-tet order (host-built incidence lists; phase 1
+//      wrote them in that order); two lanes add the two energy terms in
+//      ring-tet order;
+//   3  per ring node: SolveSimplex's scaling; then the terms of the
+//      constraint sums (all lanes), one lane per sum adding them in ring-node
+//      order, and one lane the Lagrange multipliers;
 //   4  per ring node: the position (or velocity) update.
 // Every sum keeps the reference's order (ring tets in OneRing order, ring
 // nodes in OneRing order), compiled with -ffp-contract=off: a serial CPU
@@ -34,40 +35,44 @@
 
 namespace {
 
-// Floats per ring tet in LDS: weighted I1, weighted J, then the I1 and J
-// gradient contributions at the 4 corners (4 x 3 each).
+// Floats per ring tet in LDS: weighted I1 and J, then the I1 and J gradient
+// contributions at its 4 corners (4 x 6), stored at the corners' entries of
+// the incidence lists.
 constexpr uint32_t kTetFloats = 2 + 12 + 12;
 constexpr uint32_t kScalars = 16;
 
 // LDS of one ring team: positions [3][nmax] double, inverse masses [nmax],
 // gradients [6][nmax] (g0 xyz, g1 xyz), per-node velocity-like vectors
-// [3][nmax] (X - O, or V), per-tet contributions [tmax][kTetFloats],
-// scalars [kScalars], incidence lists: per-node starts [nmax + 1] and
-// entries [4 tmax] (uint16).
+// [3][nmax] (X - O, or V), per-tet contributions [tmax * kTetFloats]
+// (energy terms [tmax][2], then gradient contributions [4 tmax][6] in
+// incidence-entry order; reused for the constraint-sum terms), scalars
+// [kScalars], incidence lists: per-node starts [nmax + 1], entries [4 tmax]
+// and their inverse [4 tmax] (uint16).
 __host__ __device__ inline size_t ring_team_bytes(uint32_t nmax, uint32_t tmax) {
     size_t b = 8 * 3 * (size_t)nmax + 4 * ((size_t)nmax * (1 + 6 + 3) + (size_t)tmax * kTetFloats + kScalars) +
-               2 * ((size_t)nmax + 1 + 4 * (size_t)tmax);
+               2 * ((size_t)nmax + 1 + 8 * (size_t)tmax);
     return (b + 7) & ~(size_t)7;
 }
 
 struct Team {
     double* xs;
     float *ws, *g, *v, *c, *sc;
-    uint16_t *incb, *inc;
+    uint16_t *incb, *inc, *inv;
     uint32_t nmax;
     __device__ double& x(uint32_t r, uint32_t n) const { return xs[r * nmax + n]; }
     __device__ float& gr(uint32_t k, uint32_t r, uint32_t n) const { return g[(k * 3 + r) * nmax + n]; }
     __device__ float& vr(uint32_t r, uint32_t n) const { return v[r * nmax + n]; }
-    __device__ float* tet(uint32_t k) const { return c + k * kTetFloats; }
 };
 
 using xf::adjdet;
 using xf::dot3;
 
-// Phase 1 for one ring tet (CalculateElementEnergyAndGradients for a linear
-// tet, Fem.cpp:941-973): out[0] = w_t I1, out[1] = w_t J, out[2 + 3 j + r]
-// = w_t dI1/dx_j, out[14 + 3 j + r] = w_t dJ/dx_j with w_t = (V_t / 4) / V.
-__device__ void ring_tet(const Team& T, float4 a, float4 b, float vsum, float* out) {
+// Phase 1 for ring tet k (CalculateElementEnergyAndGradients for a linear
+// tet, Fem.cpp:941-973): c[2 k] = w_t I1, c[2 k + 1] = w_t J and, for corner
+// j at incidence entry e = inv[4 k + j], c[2 tmax + 6 e + r] = w_t dI1/dx_j
+// and c[2 tmax + 6 e + 3 + r] = w_t dJ/dx_j, with w_t = (V_t / 4) / V. The
+// node sums of phase 2 then read each node's entries contiguously.
+__device__ void ring_tet(const Team& T, float4 a, float4 b, float vsum, uint32_t k, uint32_t tmax) {
     uint32_t bits = __float_as_uint(b.w);
     uint32_t li[4] = {bits & 255u, (bits >> 8) & 255u, (bits >> 16) & 255u, bits >> 24};
     float P[3][3];
@@ -91,20 +96,24 @@ __device__ void ring_tet(const Team& T, float4 a, float4 b, float vsum, float* o
             ++q;
         }
     for (int r = 0; r < 3; r++) gI[3][r] = ((0.0f - gI[0][r]) - gI[1][r]) - gI[2][r];
-    out[0] = ew * I1;
+    // energy terms at c[2 k + q]; corner j's gradients at the entry the
+    // incidence lists give (tet k, corner j): c[2 tmax + 6 e + 0..5]
+    float* slot[4];
+    for (int n = 0; n < 4; n++) slot[n] = T.c + 2 * tmax + 6 * (uint32_t)T.inv[4 * k + n];
+    T.c[2 * k] = ew * I1;
     for (int n = 0; n < 4; n++)
-        for (int r = 0; r < 3; r++) out[2 + 3 * n + r] = ew * gI[n][r];
+        for (int r = 0; r < 3; r++) slot[n][r] = ew * gI[n][r];
     // J of the tet from mat3 PP with rows P_i (Fem.cpp:960-970)
     float M[3][3], adj[3][3];
     for (int j = 0; j < 3; j++)
         for (int r = 0; r < 3; r++) M[j][r] = P[r][j];
     float det = adjdet(M, adj);
     float J = ((1.0f / vt) * (1.0f / 6.0f)) * det;
-    out[1] = ew * J;
+    T.c[2 * k + 1] = ew * J;
     float sJ = (ew / vt) * (1.0f / 6.0f);
     for (int n = 0; n < 3; n++)
-        for (int r = 0; r < 3; r++) out[14 + 3 * n + r] = sJ * adj[n][r];
-    for (int r = 0; r < 3; r++) out[23 + r] = sJ * (((-adj[0][r]) - adj[1][r]) - adj[2][r]);
+        for (int r = 0; r < 3; r++) slot[n][3 + r] = sJ * adj[n][r];
+    for (int r = 0; r < 3; r++) slot[3][3 + r] = sJ * (((-adj[0][r]) - adj[1][r]) - adj[2][r]);
 }
 
 // One colour of the ring sweep. POST = false: project positions
@@ -137,6 +146,7 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
     T.sc = T.c + tmax * kTetFloats;
     T.incb = reinterpret_cast<uint16_t*>(T.sc + kScalars);
     T.inc = T.incb + nmax + 1;
+    T.inv = T.inc + 4 * tmax;
 
     // Every lane reaches every barrier; a team without a ring (colour tail,
     // or a body the damping pass skips) has nn = nt = 0.
@@ -182,28 +192,28 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
         T.incb[n] = rincb[m.x + n];
     }
     if (lane == 0) T.incb[nn] = (uint16_t)(4u * nt);
-    for (uint32_t e = lane; e < 4u * nt; e += LANES) T.inc[e] = rinc[4 * m.z + e];
+    for (uint32_t e = lane; e < 4u * nt; e += LANES) {
+        const uint16_t code = rinc[4 * m.z + e];
+        T.inc[e] = code;
+        T.inv[code] = (uint16_t)e;  // entry e of the lists holds (tet, corner) = code
+    }
     __syncthreads();
 
     // Phase 1: per ring tet.
     for (uint32_t k = lane; k < nt; k += LANES)
-        ring_tet(T, rtet[2 * (m.z + k)], rtet[2 * (m.z + k) + 1], volume, T.tet(k));
+        ring_tet(T, rtet[2 * (m.z + k)], rtet[2 * (m.z + k) + 1], volume, k, tmax);
     __syncthreads();
 
-    // Phase 2: ring-node gradients in ring-tet order; energy terms.
+    // Phase 2: ring-node gradients in ring-tet order (a node's entries are
+    // contiguous: no dependent index load); energy terms.
     for (uint32_t n = lane; n < nn; n += LANES) {
-        uint32_t e0 = T.incb[n], e1 = T.incb[n + 1];
+        const uint32_t e0 = T.incb[n], e1 = T.incb[n + 1];
+        const float* sl = T.c + 2 * tmax;
         float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 4
-        for (uint32_t e = e0; e < e1; e++) {
-            uint32_t code = T.inc[e];
-            const float* t = T.tet(code >> 2);
-            uint32_t j = code & 3u;
-            for (uint32_t r = 0; r < 3; r++) {
-                a[r] = a[r] + t[2 + 3 * j + r];
-                a[3 + r] = a[3 + r] + t[14 + 3 * j + r];
-            }
-        }
+        for (uint32_t e = e0; e < e1; e++)
+#pragma unroll
+            for (uint32_t r = 0; r < 6; r++) a[r] = a[r] + sl[6 * e + r];
         for (uint32_t r = 0; r < 3; r++) {
             T.gr(0, r, n) = a[r];
             T.gr(1, r, n) = a[3 + r];
@@ -213,7 +223,7 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
         uint32_t q = lane - (LANES - 2);  // 0: I1 term, 1: J term
         float u = 0.0f;
 #pragma unroll 4
-        for (uint32_t k = 0; k < nt; k++) u = u + T.tet(k)[q];
+        for (uint32_t k = 0; k < nt; k++) u = u + T.c[2 * k + q];
         T.sc[1 + q] = u;
     }
     __syncthreads();
@@ -272,26 +282,32 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
     }
     __syncthreads();
 
-    // Phase 3b: one lane per constraint sum, each in ring-node order, all
-    // lanes running the same loop on their own operands (lane q: a = A_q,
-    // b = B_q, weighted by w or not, added or subtracted):
+    // Phase 3b: the constraint sums, each in ring-node order (the sums of
+    // Xpbd.h:87-263):
     //   sc[5] = 1e-22 + sum w g0.g0     sc[6] = 1e-22 + sum w g1.g0
     //   sc[7] = 1e-22 + sum w g1.g1     sc[8] = sum +-g0.v, sc[9] = sum +-g1.v
     // with v = X - O (projection) or V (damping pass, where the two-constraint
-    // RayleighDamp sums -g.v), the sums of Xpbd.h:87-263. Multiplying by 1
-    // and negating are exact, so each sum rounds as the reference's does.
-    if (nt > 0 && lane < 5) {
-        const float* a = T.g + (lane == 1 || lane == 2 || lane == 4 ? 3 * nmax : 0);
-        const float* b = lane <= 1 ? T.g : lane == 2 ? T.g + 3 * nmax : T.v;
-        bool weighted = lane <= 2;
-        float sign = lane >= 3 && POST && two ? -1.0f : 1.0f;
-        float acc = weighted ? 1.0e-22f : 0.0f;
-#pragma unroll 4
-        for (uint32_t n = 0; n < nn; n++) {
+    // RayleighDamp sums -g.v). Their terms are formed by all lanes (one
+    // (sum, node) pair at a time) into the per-tet area, free after phase 2;
+    // then one lane per sum adds them in order. Multiplying by 1 and negating
+    // are exact, so each sum rounds as the reference's does.
+    const uint32_t nsum = POST || damped_proj ? 5u : 3u;
+    float* terms = T.c;  // nsum x nn floats (nn <= 3 nt + 1 fits the 26 nt floats)
+    if (nt > 0)
+        for (uint32_t it = lane; it < nsum * nn; it += LANES) {
+            const uint32_t q = it / nn, n = it - q * nn;
+            const float* a = T.g + (q == 1 || q == 2 || q == 4 ? 3 * nmax : 0);
+            const float* b = q <= 1 ? T.g : q == 2 ? T.g + 3 * nmax : T.v;
             float d = a[n] * b[n] + a[nmax + n] * b[nmax + n] + a[2 * nmax + n] * b[2 * nmax + n];
-            float term = weighted ? T.ws[n] * d : d;
-            acc = acc + sign * term;
+            float term = q <= 2 ? T.ws[n] * d : d;
+            terms[it] = q >= 3 && POST && two ? -1.0f * term : term;
         }
+    __syncthreads();
+    if (nt > 0 && lane < nsum) {
+        const float* tq = terms + lane * nn;
+        float acc = lane <= 2 ? 1.0e-22f : 0.0f;
+#pragma unroll 8
+        for (uint32_t n = 0; n < nn; n++) acc = acc + tq[n];
         T.sc[5 + lane] = acc;
     }
     __syncthreads();
