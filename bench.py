@@ -183,7 +183,8 @@ def batched_leg(ctx, frames=3, schedule="blocks"):
     sc.sync()
     el_b = time.perf_counter() - t0
     finite = sc.count_nonfinite() == 0
-    ncol = sc.info()["n_colours"]
+    info = sc.info()
+    ncol = info["n_colours"]
     sc.close()
     singles = [workloads.make_batch_scene(ctx, b, bodies=[i]) for i in range(len(b["bodies"]))]
     for s in singles:
@@ -204,16 +205,18 @@ def batched_leg(ctx, frames=3, schedule="blocks"):
     out = {"workload": f"{b['name']}: {len(b['bodies'])} bodies, {T} tets, {S} substeps/frame, {ncol} colours",
            "batched_tet_substeps_per_s": batched, "plain_loop_tet_substeps_per_s": plain,
            "batched_over_plain": batched / plain, "frames": frames,
-           "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": schedule},
+           "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": schedule,
+                      "block_lanes": info["block_lanes"], "tet_lanes": info["tet_lanes"]},
            "finite": finite}
-    out["check"] = batch_check(ctx, b, schedule, mode, size)
+    out["check"] = batch_check(ctx, b, schedule, mode, size, block_lanes=info["block_lanes"],
+                               tet_lanes=info["tet_lanes"])
     out["cpu_baseline"] = batch_cpu(b)
     if "value" in out["cpu_baseline"]:
         out["gpu_over_cpu_all_cores"] = batched / out["cpu_baseline"]["value"]
     return out
 
 
-def batch_check(ctx, b, schedule, mode, size, sample=(0, 85, 170, 255)):
+def batch_check(ctx, b, schedule, mode, size, sample=(0, 85, 170, 255), block_lanes=0, tet_lanes=1):
     """One frame of the batched scene (the timed launch shape) against the
     oracle's serial sweep of sampled bodies in the scene's order: at the
     reference's precision (rel. position error 0 = bit-exact) and in FP64."""
@@ -222,6 +225,8 @@ def batch_check(ctx, b, schedule, mode, size, sample=(0, 85, 170, 255)):
     S, dt = b["substeps"], b["dt"]
     opts = dict(SCHEDULES[schedule], launch_mode=mode,
                 **({"persistent_blocks": size} if mode == xf.LAUNCH_PERSISTENT else {"block_size": size}))
+    if mode == xf.LAUNCH_BLOCKS:
+        opts.update(block_lanes=block_lanes, tet_lanes=tet_lanes)
     sc = workloads.make_batch_scene(ctx, b, **opts)
     sc.step(S, float(dt))
     err_ref, err_64 = 0.0, 0.0
@@ -372,9 +377,9 @@ def large_leg(ctx, frames=2, cells=(288, 72, 72), schedule="blocks"):
     wl = workloads.beam_yeoh(*cells)
     S, dt = wl["substeps"], float(wl["dt"])
     sc = workloads.make_scene(ctx, wl, **SCHEDULES[schedule])
-    info = sc.info()
-    T = info["n_tets"]
     mode, size, _ = sc.autotune(S, dt, reps=1)
+    info = sc.info()  # after tuning: the slot packing follows the chosen cap
+    T = info["n_tets"]
     ms = timed_frames(sc, S, dt, frames)
     finite = sc.count_nonfinite() == 0
     ms_proj, _, n_launch = sc.profile(S, dt)
@@ -384,7 +389,9 @@ def large_leg(ctx, frames=2, cells=(288, 72, 72), schedule="blocks"):
     out = {"workload": f"{wl['name']}: {T} tets, {info['n_nodes']} nodes, {info['n_colours']} colours, "
                        f"{info['n_blocks']} sweep blocks, {S} substeps/frame",
            "tet_substeps_per_s": T * S / (ms * 1e-3), "ms_per_frame": ms, "finite": finite,
-           "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": schedule},
+           "launch": {"mode": MODE_NAMES[mode], "size": size, "schedule": schedule,
+                      "block_lanes": info["block_lanes"], "tet_lanes": info["tet_lanes"],
+                      "node_phase": "in the last projection launch" if info["fused_finish"] else "own launch"},
            "roofline": {"bound": "hbm", "kernel": KERNELS[mode], "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
                         "launches": n_launch, "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
