@@ -351,7 +351,7 @@ struct xf_scene {
         size_t bytes = 0;
     };
     AdjBuf adj_ck_x, adj_ck_v, adj_tape, adj_xs, adj_o, adj_xb, adj_vb, adj_ob, adj_gacc, adj_glog, adj_xloss,
-        adj_tgt, adj_lw, adj_save_x, adj_save_v;
+        adj_tgt, adj_lw, adj_save_x, adj_save_v, adj_traw, adj_eterm, adj_lout, adj_gout;
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -1268,7 +1268,8 @@ XF_API void xf_scene_destroy(xf_scene* s) {
         if (p) (void)hipFree(p);
     for (xf_scene::AdjBuf* b : {&s->adj_ck_x, &s->adj_ck_v, &s->adj_tape, &s->adj_xs, &s->adj_o, &s->adj_xb,
                                 &s->adj_vb, &s->adj_ob, &s->adj_gacc, &s->adj_glog, &s->adj_xloss, &s->adj_tgt,
-                                &s->adj_lw, &s->adj_save_x, &s->adj_save_v})
+                                &s->adj_lw, &s->adj_save_x, &s->adj_save_v, &s->adj_traw, &s->adj_eterm,
+                                &s->adj_lout, &s->adj_gout})
         if (b->p) (void)hipFree(b->p);
     delete s;
 }
@@ -2376,7 +2377,11 @@ int AdjointRun(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const 
         (rc = Reserve(s, &s->adj_glog, sizeof(double) * T)) ||
         (rc = Reserve(s, &s->adj_xloss, n4 * n_loss)) || (rc = Reserve(s, &s->adj_tgt, n4 * n_loss)) ||
         (rc = Reserve(s, &s->adj_lw, sizeof(double) * N)) || (rc = Reserve(s, &s->adj_save_x, n4)) ||
-        (rc = Reserve(s, &s->adj_save_v, n4)))
+        (rc = Reserve(s, &s->adj_save_v, n4)) ||
+        (rc = Reserve(s, &s->adj_traw, sizeof(double) * 3 * (size_t)N * n_loss)) ||
+        (rc = Reserve(s, &s->adj_eterm, sizeof(double) * s->bodies.size() * n_loss)) ||
+        (rc = Reserve(s, &s->adj_lout, sizeof(double) * (1 + s->bodies.size()))) ||
+        (rc = Reserve(s, &s->adj_gout, sizeof(double) * T)))
         return rc;
     AdjBufs b;
     b.ckx = (double4*)s->adj_ck_x.p;
@@ -2395,62 +2400,31 @@ int AdjointRun(xf_scene* s, uint32_t substeps, float dt, uint32_t n_loss, const 
     b.save_x = (double4*)s->adj_save_x.p;
     b.save_v = (double4*)s->adj_save_v.p;
     hipStream_t st = ctx->stream;
+    const uint32_t B = (uint32_t)s->bodies.size();
+    double* traw = (double*)s->adj_traw.p;
+    double* eterm = (double*)s->adj_eterm.p;
+    double* lout = (double*)s->adj_lout.p;
+    double* gout = (double*)s->adj_gout.p;
 
-    // Targets (body-concatenated xyz) and per-node loss-gradient weights 2/n_b.
-    {
-        std::vector<double4> ht((size_t)N * n_loss);
-        for (size_t k = 0; k < ht.size(); k++)
-            ht[k] = make_double4(targets[3 * k], targets[3 * k + 1], targets[3 * k + 2], 0.0);
-        std::vector<double> hw(N);
-        for (const Body& bd : s->bodies)
-            for (uint32_t i = 0; i < bd.n; i++) hw[bd.node_base + i] = 2.0 / (double)bd.n;
-        XF_HIP(ctx, hipMemcpyAsync(b.tgt, ht.data(), ht.size() * sizeof(double4), hipMemcpyHostToDevice, st));
-        XF_HIP(ctx, hipMemcpyAsync(b.lw, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        XF_HIP(ctx, hipStreamSynchronize(st));
-    }
+    // Targets (body-concatenated xyz) to the node layout and per-node
+    // loss-gradient weights 2/n_b, on the device.
+    XF_HIP(ctx, hipMemcpyAsync(traw, targets, sizeof(double) * 3 * (size_t)N * n_loss, hipMemcpyHostToDevice, st));
+    XF_HIP(ctx, xf_launch_expand_targets(st, N * n_loss, traw, b.tgt));
+    XF_HIP(ctx, xf_launch_loss_weights(st, N, s->O, s->bp, b.lw));
 
     float clock_end = 0.0f;
     if ((rc = AdjointForward(s, substeps, dt, energy, sim, n_loss, loss_at, window, b, true, fp64, &clock_end)))
         return rc;
-
-    // Loss: sum over loss substeps and bodies of (1/n_b) sum |x - target|^2.
-    {
-        std::vector<double4> hx((size_t)N * n_loss);
-        XF_HIP(ctx, hipMemcpy(hx.data(), b.xloss, hx.size() * sizeof(double4), hipMemcpyDeviceToHost));
-        double L = 0.0;
-        if (body_loss)
-            for (size_t bi = 0; bi < s->bodies.size(); bi++) body_loss[bi] = 0.0;
-        for (uint32_t jj = 0; jj < n_loss; jj++)
-            for (size_t bi = 0; bi < s->bodies.size(); bi++) {
-                const Body& bd = s->bodies[bi];
-                double e = 0.0;
-                for (uint32_t i = 0; i < bd.n; i++) {
-                    size_t g = (size_t)N * jj + bd.node_base + i;
-                    const double* tg = targets + 3 * g;
-                    double d0 = hx[g].x - tg[0], d1 = hx[g].y - tg[1], d2 = hx[g].z - tg[2];
-                    e += d0 * d0;
-                    e += d1 * d1;
-                    e += d2 * d2;
-                }
-                L += e / (double)bd.n;
-                if (body_loss) body_loss[bi] += e / (double)bd.n;
-            }
-        if (loss) *loss = L;
-    }
-
+    // Loss: sum over loss substeps, then bodies, of (1/n_b) sum |x - target|^2.
+    XF_HIP(ctx, xf_launch_loss(st, n_loss, B, N, b.xloss, b.tgt, s->bp, eterm, lout));
     if ((rc = AdjointReverse(s, substeps, dt, energy, n_loss, loss_at, window, b, fp64, s->opts.use_graph != 0,
                              pipelined, &clock_end)))
         return rc;
-    std::vector<double> hg(T);
-    XF_HIP(ctx, hipMemcpyAsync(hg.data(), b.glog, sizeof(double) * T, hipMemcpyDeviceToHost, st));
+    XF_HIP(ctx, xf_launch_grad_out(st, T, b.glog, s->tout, gout));
+    if (loss) XF_HIP(ctx, hipMemcpyAsync(loss, lout, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (body_loss) XF_HIP(ctx, hipMemcpyAsync(body_loss, lout + 1, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    if (grad_log_mu) XF_HIP(ctx, hipMemcpyAsync(grad_log_mu, gout, sizeof(double) * T, hipMemcpyDeviceToHost, st));
     XF_HIP(ctx, hipStreamSynchronize(st));
-    if (grad_log_mu) {
-        size_t off = 0;
-        for (const Body& bd : s->bodies) {
-            for (uint32_t t = 0; t < bd.t; t++) grad_log_mu[off + t] = -hg[bd.sched_pos[t]];
-            off += bd.t;
-        }
-    }
     return XF_OK;
 }
 }  // namespace

@@ -34,12 +34,15 @@ import xpbdfem as xf
 class BatchFit:
     """Targets from the hidden-mu rollout, then Adam on log mu."""
 
-    def __init__(self, ctx, batch, loss_frames=(10, 20, 30), window=None, lr=0.05, betas=(0.9, 0.999), eps=1e-8,
+    def __init__(self, ctx, batch, loss_frames=None, window=None, lr=0.05, betas=(0.9, 0.999), eps=1e-8,
                  smooth=0.0):
         self.batch = batch
         self.dt = float(batch["dt"])
         S = batch["substeps"]
         self.n_sub = batch["frames"] * S
+        if loss_frames is None:
+            loss_frames = range(1, batch["frames"] + 1)
+        self.loss_frames = tuple(loss_frames)
         self.loss_at = np.array([f * S for f in loss_frames], np.uint32)
         self.window = int(window or S)
         self.lr, self.betas, self.eps = lr, betas, eps
@@ -149,6 +152,13 @@ class BatchFit:
         self.scene.close()
 
 
+def frames_text(frames):
+    f = list(frames)
+    if len(f) > 3 and f == list(range(f[0], f[-1] + 1)):
+        return f"{f[0]}-{f[-1]}"
+    return "/".join(str(x) for x in f)
+
+
 def face_pairs(tets):
     """Pairs (i, j), i < j, of tets sharing a face."""
     t = np.asarray(tets, np.int64).reshape(-1, 4)
@@ -161,11 +171,11 @@ def face_pairs(tets):
     return np.stack([np.minimum(a, b), np.maximum(a, b)], axis=1)
 
 
-def run(iters=200, n_bodies=64, seed=4, timing_forward=True, log=None, smooth=0.0):
+def run(iters=200, n_bodies=64, seed=4, timing_forward=True, log=None, smooth=0.0, loss_frames=None):
     ctx = xf.Context(0)
     batch = workloads.fit_batch(n_bodies=n_bodies, seed=seed)
     t0 = time.perf_counter()
-    fit = BatchFit(ctx, batch, smooth=smooth)
+    fit = BatchFit(ctx, batch, loss_frames=loss_frames, smooth=smooth)
     setup_s = time.perf_counter() - t0
     err0 = fit.mu_error()
     fwd_s = None
@@ -188,7 +198,8 @@ def run(iters=200, n_bodies=64, seed=4, timing_forward=True, log=None, smooth=0.
     tets = int(sum(fit.tets_per_body))
     out = dict(
         workload=f"{batch['name']}: {n_bodies} beams, {tets} tets, {fit.n_sub} substeps "
-                 f"({batch['frames']} frames x {batch['substeps']}), loss at frames 10/20/30, window {fit.window}",
+                 f"({batch['frames']} frames x {batch['substeps']}), loss at frames {frames_text(fit.loss_frames)}, "
+                 f"window {fit.window}",
         iterations=iters, smooth=smooth, setup_s=setup_s,
         adjoint_s_per_iter=t_grad / max(iters, 1),
         body_gradients_per_s=n_bodies * iters / t_grad if iters else None,
@@ -212,9 +223,12 @@ if __name__ == "__main__":
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--bodies", type=int, default=64)
     ap.add_argument("--smooth", type=float, default=0.0, help="weight of the log-mu smoothness prior")
+    ap.add_argument("--loss-frames", default="all",
+                    help="frames whose node positions enter the loss: comma list, or 'all' for every frame")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
-    res = run(a.iters, a.bodies, log=lambda s: print(s, flush=True), smooth=a.smooth)
+    frames = None if a.loss_frames == "all" else tuple(int(x) for x in a.loss_frames.split(","))
+    res = run(a.iters, a.bodies, log=lambda s: print(s, flush=True), smooth=a.smooth, loss_frames=frames)
     print(json.dumps(res), flush=True)
     if a.out:
         with open(a.out, "w") as f:
