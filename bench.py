@@ -368,6 +368,34 @@ def timed_frames(sc, S, dt, frames):
     return 1e3 * (time.perf_counter() - t0) / frames
 
 
+def contracted_leg(ctx, wl, shape, frames=10):
+    """The headline beam with the contracted sweep-block kernels
+    (xf_scene_opts.contract: fused multiply-adds, not bit-exact) at the tuned
+    launch shape: frame time, and the node positions after one frame against
+    the exact kernels' (bit-identical to the reference's serial sweep), per
+    node relative to |x| and relative to the largest displacement."""
+    S, dt = wl["substeps"], float(wl["dt"])
+    states = {}
+    for c in (False, True):
+        sc = workloads.make_scene(ctx, wl, contract=c, **shape)
+        x0, _ = sc.get_state(0)
+        sc.step(S, dt)
+        states[c] = (x0.reshape(-1, 3), sc.get_state(0)[0].reshape(-1, 3))
+        if c:
+            ms = timed_frames(sc, S, dt, frames)
+            used = sc.info()["contract"]
+        sc.close()
+    x0, xe = states[False]
+    xc = states[True][1]
+    d = np.linalg.norm(xc - xe, axis=1)
+    return {"ms_per_frame": ms, "tet_substeps_per_s": (wl["tets"].size // 4) * S / (ms * 1e-3),
+            "contract": used,
+            "one_frame_vs_exact": {"rel_position_error": float(np.linalg.norm(xc - xe) / np.linalg.norm(xe)),
+                                   "max_over_max_displacement": float(d.max() / np.linalg.norm(xe - x0, axis=1).max()),
+                                   "max_abs_m": float(d.max())},
+            "note": "exact kernels = reference rounding (bit-identical); contracted kernels fuse multiply-adds"}
+
+
 def large_leg(ctx, frames=2, cells=(288, 72, 72), schedule="blocks"):
     """The same Yeoh beam at 288x72x72 cells (8,957,952 tets, 1,540,081
     nodes): colours large enough that the projection is bound by throughput
@@ -779,6 +807,13 @@ def main():
         ms_pc = timed_frames(pc, S, dt, 10)
         pc.close()
         out["per_colour_slabs"] = {"ms_per_frame": ms_pc, "tet_substeps_per_s": T * S / (ms_pc * 1e-3)}
+        if mode == xf.LAUNCH_BLOCKS:
+            try:
+                out["contracted"] = contracted_leg(
+                    ctx, wl, dict(opts, launch_mode=mode, use_graph=True, block_size=size,
+                                  tet_lanes=info["tet_lanes"], block_lanes=info["block_lanes"]))
+            except xf.XfError as e:
+                out["contracted"] = {"error": str(e)}
     legs = []
     if not args.no_batch:
         legs.append(("batched", lambda c: batched_leg(c, schedule=args.schedule)))

@@ -236,6 +236,14 @@ typedef struct xf_scene_opts {
                                    split four ways, for launches whose phases are latency-bound; the
                                    simultaneous MixedSel / YeohSkinFast projection without damping,
                                    else step calls fail with XF_E_UNSUPPORTED). Same results. */
+    uint32_t contract;          /* sweep-block launches: 1 runs kernels compiled with multiply-add
+                                   contraction (fused multiply-adds in the element chain and the
+                                   node phase: about a quarter fewer instructions per tet). Results
+                                   are then no longer bit-identical to the reference's serial FP32
+                                   sweep (each product-sum rounds once); positions stay within a
+                                   relative 1e-4 of it after a frame. 0 (default): the reference's
+                                   rounding. Other launch modes are unaffected; the adjoint entry
+                                   points reject a scene with it set in XF_LAUNCH_BLOCKS. */
 } xf_scene_opts;
 
 typedef struct xf_scene_info {
@@ -262,6 +270,8 @@ typedef struct xf_scene_info {
                                  per-workgroup limit (160 KiB on gfx950); it caps a slot at
                                  block_lds_limit / 32 nodes and decides whether the node phase
                                  fits the last launch */
+    uint32_t contract;        /* 1: step calls run the contracted sweep-block kernels
+                                 (xf_scene_opts.contract in XF_LAUNCH_BLOCKS) */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */

@@ -12,11 +12,22 @@
 // added back as Dvec in Xpbd.h:212), in the reference's operation order,
 // compiled with -ffp-contract=off and IEEE division/sqrt: a serial sweep of
 // the same colour order on the CPU reproduces the result bit-for-bit.
+//
+// The file is compiled twice. As itself it holds every kernel and launcher
+// of this part of the library. xf_blocks_contract.hip includes it inside
+// namespace xf_contract with XF_BLOCKS_ONLY defined, compiled with
+// -ffp-contract=fast: that copy keeps only the sweep-block launcher, named
+// XF_BLOCKS_LAUNCH, whose kernels then fuse multiplies and adds (v_fma) and
+// no longer round like the reference.
 #include <hip/hip_runtime.h>
 
 #include "xf_device.h"
 #include "xf_element.h"
 #include "xf_sched.h"
+
+#ifndef XF_BLOCKS_LAUNCH
+#define XF_BLOCKS_LAUNCH xf_launch_blocks
+#endif
 
 using xf::colour_grid;
 using xf::colour_slot;
@@ -585,6 +596,7 @@ __device__ __forceinline__ void pbd_one(uint32_t i, const uint4* __restrict__ ti
 
 // ---------------------------------------------------------------- kernels
 
+#ifndef XF_BLOCKS_ONLY
 // Volume pass `pass` over one colour: tets [begin, begin+count).
 __global__ __launch_bounds__(256) void xf_volume_colour(uint32_t begin, uint32_t count, uint32_t slabs, uint32_t pass,
                                                         const uint4* __restrict__ tidx,
@@ -597,6 +609,7 @@ __global__ __launch_bounds__(256) void xf_volume_colour(uint32_t begin, uint32_t
     if (i == xf::kNoSlot) return;
     volume_one(i, pass, tidx, tg0, tg1, tg2, taux, X, bodies, dt);
 }
+#endif  // XF_BLOCKS_ONLY
 
 // Projects one colour: tets [begin, begin+count) of the sweep order.
 // AUX: per-tet (comp1, a, body) from taux; otherwise comp1/a are uniform and
@@ -1126,6 +1139,7 @@ __device__ __forceinline__ void finish_predict(double4 xi, double4 oi, uint32_t 
 
 }  // namespace
 
+#ifndef XF_BLOCKS_ONLY
 // First prediction of a step call from the stored velocity.
 __global__ __launch_bounds__(256) void xf_predict(uint32_t n, double4* __restrict__ X, double4* __restrict__ O,
                                                   const double4* __restrict__ V,
@@ -1508,6 +1522,7 @@ extern "C" hipError_t xf_launch_project(hipStream_t s, int energy, bool sim, boo
                        tg1, tg2, tg3, taux, X, O, bodies, comp1_u, a_u, dt, clock);
     return hipGetLastError();
 }
+#endif  // XF_BLOCKS_ONLY
 
 typedef void (*XfBlocksFn)(uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*, const uint2*,
                            const uint4*, const float4*, const float4*, const float4*, const float4*, const float4*,
@@ -1550,12 +1565,17 @@ XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over, bool fin, bool 
 }
 }  // namespace
 
+#ifndef XF_BLOCKS_ONLY
 // LDS bytes of a sweep-block workgroup: its nodes in FP64 (32 B each).
 extern "C" size_t xf_block_lds_bytes(uint32_t nmax) { return 32 * (size_t)nmax; }
 
 // LDS bytes of a workgroup of the launch that ends the substep: its nodes and
 // extra nodes, each with O (32 B) and id (4 B).
 extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax) { return 68 * ((size_t)nmax + fmax); }
+#else
+extern "C" size_t xf_block_lds_bytes(uint32_t nmax);
+extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax);
+#endif
 
 // Host-callable: one sweep-block launch (nslot workgroups of `threads`);
 // rec / bnode point at the launch colour's first block; pmax: the most tets
@@ -1567,7 +1587,7 @@ extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax) { return 
 // energies 0, 4 and 7; anything else is hipErrorInvalidValue). lds_limit:
 // the device's LDS bytes per workgroup (more than 64 KiB is granted to the
 // kernel with hipFuncAttributeMaxDynamicSharedMemorySize).
-extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
+extern "C" hipError_t XF_BLOCKS_LAUNCH(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
                                        uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
                                        const uint32_t* rec,
                                        const uint32_t* bnode, const uint2* tloc, const uint4* tidx,
@@ -1606,6 +1626,7 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
     return hipGetLastError();
 }
 
+#ifndef XF_BLOCKS_ONLY
 typedef void (*XfDampFn)(uint32_t, uint32_t, uint32_t, const uint4*, const float4*, const float4*, const float4*,
                          const float4*, const float4*, const float2*, const double4*, double4*, const XfBodyParams*, float, float,
                          float, const float*);
@@ -1788,3 +1809,4 @@ extern "C" hipError_t xf_launch_project_record(hipStream_t s, int energy, bool a
 #undef XF_REC
     return hipGetLastError();
 }
+#endif  // XF_BLOCKS_ONLY

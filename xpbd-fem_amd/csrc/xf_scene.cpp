@@ -57,6 +57,14 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
                                        const float* tend, bool single_body, bool quad, size_t lds_limit);
+// the same launch with the kernels compiled with multiply-add contraction
+// (xf_blocks_contract.hip)
+extern "C" hipError_t xf_launch_blocks_contract(
+    hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads, uint32_t nslot, uint32_t nph,
+    uint32_t nmax, uint32_t ntet, uint32_t pmax, const uint32_t* rec, const uint32_t* bnode, const uint2* tloc,
+    const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3, const float4* taux,
+    double4* X, double4* O, const XfBodyParams* bodies, float comp1_u, float a_u, float dt, float* clock, bool fin,
+    const uint32_t* fextra, uint32_t fmax, const float* tend, bool single_body, bool quad, size_t lds_limit);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
@@ -806,12 +814,13 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                 XF_HIP(ctx, hipEventCreate(&e1));
                 XF_HIP(ctx, hipEventRecord(e0, st));
             }
-            XF_HIP(ctx, xf_launch_blocks(st, energy, sim, s->aux, s->dproj, block, L.nslot, L.nph, s->blk_nmax,
-                                         s->T, s->blk_pmax, s->blk_rec + L.rec_off, s->blk_node + L.node_off, s->tloc,
-                                         s->tidx, s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O,
-                                         s->bp, s->comp1_u, s->a_u, dt, first ? s->clock : nullptr,
-                                         fin && q == s->blk_last, s->blk_fin, s->blk_fmax, s->clock,
-                                         s->bodies.size() == 1, s->opts.tet_lanes == 4, s->lds_limit));
+            auto launch = s->opts.contract ? xf_launch_blocks_contract : xf_launch_blocks;
+            XF_HIP(ctx, launch(st, energy, sim, s->aux, s->dproj, block, L.nslot, L.nph, s->blk_nmax, s->T,
+                               s->blk_pmax, s->blk_rec + L.rec_off, s->blk_node + L.node_off, s->tloc, s->tidx,
+                               s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O, s->bp, s->comp1_u,
+                               s->a_u, dt, first ? s->clock : nullptr, fin && q == s->blk_last, s->blk_fin,
+                               s->blk_fmax, s->clock, s->bodies.size() == 1, s->opts.tet_lanes == 4,
+                               s->lds_limit));
             first = false;
             if (timer) {
                 XF_HIP(ctx, hipEventRecord(e1, st));
@@ -1378,6 +1387,7 @@ XF_API int xf_scene_finalize(xf_scene* s, const xf_scene_opts* opts) {
     if (s->opts.tet_lanes == 0) s->opts.tet_lanes = 1;
     if (s->opts.tet_lanes != 1 && s->opts.tet_lanes != 4) return Fail(ctx, XF_E_INVALID, "tet_lanes must be 0, 1 or 4");
     if (s->opts.block_lanes > 256) return Fail(ctx, XF_E_INVALID, "block_lanes must be at most 256");
+    if (s->opts.contract > 1) return Fail(ctx, XF_E_INVALID, "contract must be 0 or 1");
     if (s->opts.sweep_blocks && s->opts.xcd_slabs)
         return Fail(ctx, XF_E_INVALID, "sweep_blocks and xcd_slabs are exclusive");
     if (s->opts.launch_mode == XF_LAUNCH_BLOCKS && !s->opts.sweep_blocks)
@@ -1628,6 +1638,7 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     info->fused_finish = fused ? 1u : 0u;
     info->tet_lanes = s->opts.tet_lanes;
     info->block_lds_limit = (uint32_t)s->lds_limit;
+    info->contract = s->opts.contract && s->opts.launch_mode == XF_LAUNCH_BLOCKS ? 1u : 0u;
     return XF_OK;
 }
 
@@ -2077,6 +2088,10 @@ int AdjointCheck(xf_scene* s, uint32_t substeps, uint32_t n_loss, const uint32_t
     for (uint32_t j = 0; j < n_loss; j++)
         if (loss_at[j] < 1 || loss_at[j] > substeps || (j && loss_at[j] <= loss_at[j - 1]))
             return Fail(ctx, XF_E_INVALID, "loss_at must ascend within [1, substeps]");
+    if (s->opts.contract && s->opts.launch_mode == XF_LAUNCH_BLOCKS)
+        return Fail(ctx, XF_E_UNSUPPORTED,
+                    "contract rounds the forward sweep differently from the adjoint's exact replay; "
+                    "turn contract off for the adjoint");
     for (const Body& b : s->bodies) {
         const xf_settings& st = b.settings;
         if (!st.simultaneous || (st.energy != 4 && st.energy != 7) || (st.collide & XF_COLLIDE_SPHERE) ||

@@ -220,7 +220,7 @@ def fit_batch(n_bodies=64, seed=4, cells=(4, 16), h=0.08, frames=30, substeps=40
 
 def make_batch_scene(ctx, batch, bodies=None, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64,
                      launch_mode=xf.LAUNCH_PER_COLOUR, persistent_blocks=0, xcd_slabs=False, ring_lanes=0,
-                     sweep_blocks=False, block_lanes=0, separate_finish=False, tet_lanes=0):
+                     sweep_blocks=False, block_lanes=0, separate_finish=False, tet_lanes=0, contract=False):
     """One scene holding the given bodies of a batch (all by default). With
     sweep_blocks the scene colours the tets itself (tet_colour is dropped)."""
     sc = xf.Scene(ctx)
@@ -231,7 +231,7 @@ def make_batch_scene(ctx, batch, bodies=None, colour_order=xf.COLOUR_NATURAL, us
     sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size, launch_mode=launch_mode,
                 persistent_blocks=persistent_blocks, xcd_slabs=xcd_slabs, ring_lanes=ring_lanes,
                 sweep_blocks=sweep_blocks, block_lanes=block_lanes, separate_finish=separate_finish,
-                tet_lanes=tet_lanes)
+                tet_lanes=tet_lanes, contract=contract)
     for i, wl in enumerate(chosen):
         sc.set_settings(i, **wl["settings"])
     return sc
@@ -239,7 +239,7 @@ def make_batch_scene(ctx, batch, bodies=None, colour_order=xf.COLOUR_NATURAL, us
 
 def make_scene(ctx, wl, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_size=64,
                launch_mode=xf.LAUNCH_PER_COLOUR, persistent_blocks=0, xcd_slabs=False, ring_lanes=0,
-               sweep_blocks=False, block_lanes=0, separate_finish=False, tet_lanes=0):
+               sweep_blocks=False, block_lanes=0, separate_finish=False, tet_lanes=0, contract=False):
     """Builds and finalizes a one-body scene of a workload. With sweep_blocks
     the scene colours the tets itself (the workload's tet_colour is dropped)."""
     sc = xf.Scene(ctx)
@@ -248,6 +248,6 @@ def make_scene(ctx, wl, colour_order=xf.COLOUR_NATURAL, use_graph=True, block_si
     sc.finalize(colour_order=colour_order, use_graph=use_graph, block_size=block_size, launch_mode=launch_mode,
                 persistent_blocks=persistent_blocks, xcd_slabs=xcd_slabs, ring_lanes=ring_lanes,
                 sweep_blocks=sweep_blocks, block_lanes=block_lanes, separate_finish=separate_finish,
-                tet_lanes=tet_lanes)
+                tet_lanes=tet_lanes, contract=contract)
     sc.set_settings(0, **wl["settings"])
     return sc

@@ -77,7 +77,7 @@ class SceneOpts(C.Structure):
     _fields_ = [("colour_order", C.c_uint32), ("block_size", C.c_uint32), ("use_graph", C.c_uint32),
                 ("launch_mode", C.c_uint32), ("persistent_blocks", C.c_uint32), ("xcd_slabs", C.c_uint32),
                 ("ring_lanes", C.c_uint32), ("sweep_blocks", C.c_uint32), ("block_lanes", C.c_uint32),
-                ("separate_finish", C.c_uint32), ("tet_lanes", C.c_uint32)]
+                ("separate_finish", C.c_uint32), ("tet_lanes", C.c_uint32), ("contract", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -86,7 +86,7 @@ class SceneInfo(C.Structure):
                 ("n_ring_colours", C.c_uint32), ("n_blocks", C.c_uint32), ("block_launches", C.c_uint32),
                 ("n_block_nodes", C.c_uint32), ("block_lanes", C.c_uint32), ("n_block_slots", C.c_uint32),
                 ("fused_finish", C.c_uint32), ("tet_lanes", C.c_uint32),
-                ("block_lds_limit", C.c_uint32)]
+                ("block_lds_limit", C.c_uint32), ("contract", C.c_uint32)]
 
 
 _lib = None
@@ -443,9 +443,10 @@ class Scene:
 
     def finalize(self, colour_order=COLOUR_NATURAL, block_size=64, use_graph=True, launch_mode=LAUNCH_PER_COLOUR,
                  persistent_blocks=0, xcd_slabs=False, ring_lanes=0, sweep_blocks=False, block_lanes=0,
-                 separate_finish=False, tet_lanes=0):
+                 separate_finish=False, tet_lanes=0, contract=False):
         o = SceneOpts()
         o.tet_lanes = tet_lanes
+        o.contract = int(bool(contract))
         o.sweep_blocks = int(sweep_blocks)
         o.block_lanes = block_lanes
         o.separate_finish = int(bool(separate_finish))
@@ -465,7 +466,8 @@ class Scene:
                     device_bytes=i.device_bytes, n_rings=i.n_rings, n_ring_colours=i.n_ring_colours,
                     n_blocks=i.n_blocks, block_launches=i.block_launches, n_block_nodes=i.n_block_nodes,
                     block_lanes=i.block_lanes, n_block_slots=i.n_block_slots,
-                    fused_finish=i.fused_finish, tet_lanes=i.tet_lanes, block_lds_limit=i.block_lds_limit)
+                    fused_finish=i.fused_finish, tet_lanes=i.tet_lanes, block_lds_limit=i.block_lds_limit,
+                    contract=i.contract)
 
     def set_settings(self, body, settings=None, **kw):
         s = settings if settings is not None else default_settings(**kw)
