@@ -611,14 +611,15 @@ def rings_profile(frames=1, xcd_slabs=True):
                       "frames": frames, "xcd_slabs": xcd_slabs}))
 
 
-def beam_profile(schedule="blocks", block=64, tet_lanes=1, frames=1, graph=False):
+def beam_profile(schedule="blocks", block=64, tet_lanes=1, frames=1, graph=False, contract=False):
     """Frames of the 96x24x24 beam in a fixed launch shape: block launches or
     per-colour launches of `block` threads, `tet_lanes` lanes per tet in
     block launches; eager launches unless graph (short runs under rocprofv3
     counter and trace passes)."""
     ctx = xf.Context(0)
     wl = workloads.beam_yeoh_96x24x24()
-    sc = workloads.make_scene(ctx, wl, block_size=block, tet_lanes=tet_lanes, use_graph=graph, **SCHEDULES[schedule])
+    sc = workloads.make_scene(ctx, wl, block_size=block, tet_lanes=tet_lanes, use_graph=graph, contract=contract,
+                              **SCHEDULES[schedule])
     for _ in range(frames):
         sc.step(wl["substeps"], float(wl["dt"]))
     sc.sync()
@@ -627,7 +628,8 @@ def beam_profile(schedule="blocks", block=64, tet_lanes=1, frames=1, graph=False
     ctx.close()
     print(json.dumps({"beam_profile": wl["name"], "tets": info["n_tets"], "colours": info["n_colours"],
                       "blocks": info["n_blocks"], "block_nodes": info["n_block_nodes"], "schedule": schedule,
-                      "block": block, "tet_lanes": tet_lanes, "frames": frames, "graph": graph}))
+                      "block": block, "tet_lanes": tet_lanes, "frames": frames, "graph": graph,
+                      "contract": info["contract"]}))
 
 
 def large_profile(schedule="blocks", block=256):
@@ -679,6 +681,8 @@ def main():
     ap.add_argument("--tet-lanes", type=int, default=1, help="--profile: lanes per tet of block launches")
     ap.add_argument("--frames", type=int, default=1, help="--profile: frames")
     ap.add_argument("--graph", action="store_true", help="--profile: replay each frame from a hipGraph")
+    ap.add_argument("--contract", action="store_true",
+                    help="--profile: the sweep-block kernels with multiply-add contraction")
     ap.add_argument("--adjoint-profile", action="store_true",
                     help="only run the adjoint leg's forward and adjoint with eager launches (rocprofv3 kernel "
                          "traces)")
@@ -703,7 +707,7 @@ def main():
         return
     if args.profile:
         beam_profile(schedule=args.schedule, block=args.block, tet_lanes=args.tet_lanes, frames=args.frames,
-                     graph=args.graph)
+                     graph=args.graph, contract=args.contract)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is not built",
