@@ -834,6 +834,7 @@ def main():
                      **({"persistent_blocks": size} if mode == xf.LAUNCH_PERSISTENT else {"block_size": size}))
         legs.append(("cpu_baseline", lambda c: cpu_baseline(c, wl, opts=shape)))
     for name, leg in legs:
+        print(f"bench: {name} leg", file=sys.stderr, flush=True)
         try:
             out[name] = leg(ctx)
         except (xf.XfError, FloatingPointError) as e:  # a side leg must not hide the main line
