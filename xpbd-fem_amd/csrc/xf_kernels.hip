@@ -920,6 +920,44 @@ __device__ __forceinline__ void block_tet_quad(const BlockTet& t, uint32_t q, do
     xs[own] = make_double4(xn[0], xn[1], xn[2], xo.w);
 }
 
+// A slot's nodes into LDS (stage_nodes) and back (write_nodes), kStage list
+// entries per lane at a time: the batch's id loads are issued together,
+// then its node loads (at clamped in-range positions, so that no branch
+// separates them), so a batch costs two memory round trips whatever its
+// size, where one entry per iteration waits for each id before its node.
+constexpr uint32_t kStage = 4;
+
+__device__ __forceinline__ void stage_nodes(uint32_t tid, uint32_t bs, uint32_t n, const uint32_t* __restrict__ bn,
+                                            const double4* __restrict__ X, double4* xs) {
+    for (uint32_t k0 = tid; k0 < n; k0 += kStage * bs) {
+        uint32_t g[kStage];
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; u++) g[u] = bn[min(k0 + u * bs, n - 1u)];
+        double4 v[kStage];
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; u++) v[u] = X[g[u] != xf::kNoNode ? g[u] : 0u];
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; u++) {
+            const uint32_t k = k0 + u * bs;
+            if (k < n && g[u] != xf::kNoNode) xs[k] = v[u];
+        }
+    }
+}
+
+__device__ __forceinline__ void write_nodes(uint32_t tid, uint32_t bs, uint32_t n, const uint32_t* __restrict__ bn,
+                                            double4* __restrict__ X, const double4* xs) {
+    for (uint32_t k0 = tid; k0 < n; k0 += kStage * bs) {
+        uint32_t g[kStage];
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; u++) g[u] = bn[min(k0 + u * bs, n - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; u++) {
+            const uint32_t k = k0 + u * bs;
+            if (k < n && g[u] != xf::kNoNode) X[g[u]] = xs[k];
+        }
+    }
+}
+
 // ntet: tets in the schedule. Element loads are issued by every lane for
 // every phase pair, at clamped in-range positions (lanes without a tet, and
 // the phase after the last, load data they do not use), so that the loads in
@@ -996,10 +1034,7 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
             }
         }
     } else {
-        for (uint32_t k = tid; k < nmax; k += bs) {
-            uint32_t g = bn[k];
-            if (g != xf::kNoNode) xs[k] = X[g];
-        }
+        stage_nodes(tid, bs, nmax, bn, X, xs);
     }
     __syncthreads();
     // phases in pairs, element data of the next phase loaded before solving
@@ -1027,10 +1062,7 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
         else
             finish_all([&](uint32_t flags) -> const XfBodyParams& { return bodies[flags >> 1]; });
     } else {
-        for (uint32_t k = tid; k < nmax; k += bs) {
-            uint32_t g = bn[k];
-            if (g != xf::kNoNode) X[g] = xs[k];
-        }
+        write_nodes(tid, bs, nmax, bn, X, xs);
     }
 }
 
