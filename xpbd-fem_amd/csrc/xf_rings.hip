@@ -40,6 +40,8 @@ namespace {
 // the incidence lists.
 constexpr uint32_t kTetFloats = 2 + 12 + 12;
 constexpr uint32_t kScalars = 16;
+// Incidence entries per lane whose loads phase 0 issues up front.
+constexpr uint32_t kIncLoads = 4;
 
 // LDS of one ring team: positions [3][nmax] double, inverse masses [nmax],
 // gradients [6][nmax] (g0 xyz, g1 xyz), per-node velocity-like vectors
@@ -182,8 +184,30 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
     const uint32_t nn = m.y, nt = m.w;
     const uint32_t* nodes = rnodes + m.x;
 
-    // Phase 0: positions, inverse masses and incidence lists.
-    for (uint32_t n = lane; n < nn; n += LANES) {
+    // Phase 0: positions, inverse masses and incidence lists. The loads of a
+    // lane's first node, its first kIncLoads incidence entries and its first
+    // ring tet (for phase 1) are issued before any of them is used, at
+    // clamped in-range positions (a team without a ring reads entry 0), so
+    // the phase waits for two dependent round trips (node id, position), not
+    // one per loop iteration; rings beyond those counts loop as before.
+    const uint32_t ne = 4u * nt;
+    const uint32_t n0 = min(lane, nn > 0u ? nn - 1u : 0u);
+    const uint32_t g0 = nodes[n0];
+    const uint16_t b0 = rincb[m.x + n0];
+    uint16_t code[kIncLoads];
+#pragma unroll
+    for (uint32_t u = 0; u < kIncLoads; u++) code[u] = rinc[4 * m.z + min(lane + u * LANES, ne > 0u ? ne - 1u : 0u)];
+    const uint32_t k0 = min(lane, nt > 0u ? nt - 1u : 0u);
+    float4 ta = rtet[2 * (m.z + k0)], tb = rtet[2 * (m.z + k0) + 1];
+    const double4 x0 = X[g0];
+    if (lane < nn) {
+        T.x(0, lane) = x0.x;
+        T.x(1, lane) = x0.y;
+        T.x(2, lane) = x0.z;
+        T.ws[lane] = (float)x0.w;
+        T.incb[lane] = b0;
+    }
+    for (uint32_t n = lane + LANES; n < nn; n += LANES) {
         double4 x = X[nodes[n]];
         T.x(0, n) = x.x;
         T.x(1, n) = x.y;
@@ -191,17 +215,30 @@ __global__ __launch_bounds__(256) void xf_ring_colour(
         T.ws[n] = (float)x.w;
         T.incb[n] = rincb[m.x + n];
     }
-    if (lane == 0) T.incb[nn] = (uint16_t)(4u * nt);
-    for (uint32_t e = lane; e < 4u * nt; e += LANES) {
-        const uint16_t code = rinc[4 * m.z + e];
-        T.inc[e] = code;
-        T.inv[code] = (uint16_t)e;  // entry e of the lists holds (tet, corner) = code
+    if (lane == 0) T.incb[nn] = (uint16_t)ne;
+#pragma unroll
+    for (uint32_t u = 0; u < kIncLoads; u++) {
+        const uint32_t e = lane + u * LANES;
+        if (e < ne) {
+            T.inc[e] = code[u];
+            T.inv[code[u]] = (uint16_t)e;  // entry e of the lists holds (tet, corner) = code
+        }
+    }
+    for (uint32_t e = lane + kIncLoads * LANES; e < ne; e += LANES) {
+        const uint16_t c = rinc[4 * m.z + e];
+        T.inc[e] = c;
+        T.inv[c] = (uint16_t)e;
     }
     __syncthreads();
 
-    // Phase 1: per ring tet.
-    for (uint32_t k = lane; k < nt; k += LANES)
-        ring_tet(T, rtet[2 * (m.z + k)], rtet[2 * (m.z + k) + 1], volume, k, tmax);
+    // Phase 1: per ring tet (the lane's first from the phase-0 loads).
+    for (uint32_t k = lane; k < nt; k += LANES) {
+        if (k != lane) {
+            ta = rtet[2 * (m.z + k)];
+            tb = rtet[2 * (m.z + k) + 1];
+        }
+        ring_tet(T, ta, tb, volume, k, tmax);
+    }
     __syncthreads();
 
     // Phase 2: ring-node gradients in ring-tet order (a node's entries are

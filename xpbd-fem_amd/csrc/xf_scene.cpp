@@ -644,6 +644,14 @@ int UploadRings(xf_scene* s) {
     }
     s->ring_colour_begin[s->rcol] = pos;
     if (pos != s->R) return Fail(ctx, XF_E_INVALID, "internal: ring schedule covers %u of %u rings", pos, s->R);
+    // One padding entry past the end of each list: the ring kernel loads a
+    // lane's first entries at clamped positions, which for a ring without
+    // nodes or tets (or a lane without a ring) is the ring's start offset.
+    nodes.push_back(0u);
+    incb.push_back(0u);
+    inc.insert(inc.end(), 4, (uint16_t)0u);
+    rtet.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    rtet.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     rc = DevAlloc(ctx, &s->rmeta, s->R, &s->device_bytes);
     if (!rc) rc = DevAlloc(ctx, &s->rinfo, s->R, &s->device_bytes);
     if (!rc) rc = DevAlloc(ctx, &s->rnodes, nodes.size(), &s->device_bytes);
