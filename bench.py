@@ -30,6 +30,13 @@ import xpbdfem as xf  # noqa: E402
 
 METRIC = "tet-substeps/s plain vs batched at ≤1e-4 rel pos err; adjoint:forward time"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# FP32 VALU operations of one tet projection (YeohSkinFast, simultaneous
+# solve; add/sub, mul and transcendental count one, FMA two): the
+# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F32 counters of a sweep-block launch per
+# working wave and phase, 182 + 260 + 2 x 28 + 5, the same for the exact and
+# the contracted kernels (profiles/xf_project_blocks_valu_flops.md).
+FP32_FLOP_PER_TET = 503
 # Algorithmic bytes per tet projection with the reference's precision (FP64
 # positions, FP32 inverse mass and constants): node ids 16 + constants (Qi 36,
 # QQ 12, QR 12, material 4) 64 + gather 4 x (xyz FP64 24 + w 4) 112 +
@@ -53,6 +60,14 @@ SCHEDULES = {"blocks": dict(sweep_blocks=True, launch_mode=xf.LAUNCH_BLOCKS), "s
 MODE_NAMES = {xf.LAUNCH_PER_COLOUR: "per-colour", xf.LAUNCH_PERSISTENT: "persistent", xf.LAUNCH_BLOCKS: "blocks"}
 KERNELS = {xf.LAUNCH_PER_COLOUR: "xf_project_colour", xf.LAUNCH_PERSISTENT: "xf_frame_persistent",
            xf.LAUNCH_BLOCKS: "xf_project_blocks"}
+
+
+def fp32_valu(tet_projections, ms):
+    """Algorithmic FP32 VALU rate of `tet_projections` tet projections done
+    in `ms` milliseconds of projection launches, against the FP32 peak."""
+    tf = FP32_FLOP_PER_TET * tet_projections / (ms * 1e-3) / 1e12
+    return {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP32_PEAK_TFLOPS,
+            "flop_per_tet": FP32_FLOP_PER_TET}
 
 
 def projection_bytes(info, mode, substeps, per_tet_data=False):
@@ -423,7 +438,8 @@ def large_leg(ctx, frames=2, cells=(288, 72, 72), schedule="blocks"):
            "roofline": {"bound": "hbm", "kernel": KERNELS[mode], "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
                         "launches": n_launch, "avg_launch_us": 1e3 * ms_proj / max(n_launch, 1),
-                        "bytes_per_launch": bytes_proj / max(n_launch, 1)}}
+                        "bytes_per_launch": bytes_proj / max(n_launch, 1),
+                        "fp32_valu": fp32_valu(T * S, ms_proj)}}
     if schedule != "slabs":
         sc = workloads.make_scene(ctx, wl, **SCHEDULES["slabs"])
         out["per_colour_slabs_ms_per_frame"] = timed_frames(sc, S, dt, frames)
@@ -802,7 +818,8 @@ def main():
                      "bytes_per_launch": bytes_proj / max(n_launch, 1),
                      "frac_196B_model": bytes_model / (ms_proj * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "ms_frame_events": ms_frame_dev, "ms_node_launches": ms_node,
-                     "avg_launch_us_eager": 1e3 * ms_proj_eager / max(n_launch, 1)},
+                     "avg_launch_us_eager": 1e3 * ms_proj_eager / max(n_launch, 1),
+                     "fp32_valu": fp32_valu(T * S, ms_proj)},
     }
     if args.schedule == "blocks":
         # the per-colour schedule on the same beam (structured 24 colours, XCD slabs)
