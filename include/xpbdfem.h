@@ -272,6 +272,11 @@ typedef struct xf_scene_info {
                                  fits the last launch */
     uint32_t contract;        /* 1: step calls run the contracted sweep-block kernels
                                  (xf_scene_opts.contract in XF_LAUNCH_BLOCKS) */
+    uint32_t block_node_stride; /* LDS bytes per slot node in the last block launch run: 48 where
+                                   every block launch has at most two workgroups per compute unit,
+                                   the projection is simultaneous MixedSel / YeohSkinFast without
+                                   damping, one lane per tet, and the nodes fit (fewer LDS bank
+                                   conflicts); else 32 (also before the first step) */
 } xf_scene_info;
 
 /* ------------------------------------------------------------ context */

@@ -395,6 +395,7 @@ def test_bench_launch_shape_bit_exact(ctx, sphere_dy):
     sc.step(S, float(wl["dt"]))
     x2, v2 = sc.get_state(0)
     order = sc.order(0)[0]
+    assert sc.info()["block_node_stride"] == 48  # 264 workgroups per launch: latency-bound layout
     sc.close()
     ob = oracle(wl, order, S)
     print(f"\nbench shape, frame 1: max|dx| vs oracle {np.abs(x1 - ob.X).max():.3e}")
@@ -406,3 +407,31 @@ def test_bench_launch_shape_bit_exact(ctx, sphere_dy):
         c = np.float32(c0) + np.float32(s["sphere_v"]) * np.float32(2 * S * wl["dt"])
         d = np.linalg.norm(x2.reshape(-1, 3) - c, axis=1)
         assert np.sum(np.abs(d - s["sphere_r"]) < 1e-4) >= 3
+
+
+@pytest.mark.parametrize("case", ["wide", "damped", "quad", "serial", "many_slots"])
+def test_blocks_node_stride(ctx, case):
+    """48 bytes per slot node in LDS where every block launch has at most two
+    workgroups per compute unit and the projection is simultaneous MixedSel /
+    YeohSkinFast without damping with one lane per tet; 32 otherwise. Both
+    layouts bit-exact against the oracle."""
+    wl = workloads.beam_neo_12x4x4(energy=7, simultaneous=case != "serial")
+    opts = dict(block_size=64)
+    if case == "damped":
+        wl["settings"] = dict(wl["settings"], damping=0.01)
+    if case == "quad":
+        opts["tet_lanes"] = 4
+    if case == "many_slots":
+        # 300 x 8 x 4 cells of 1 cm, one block per workgroup (block_lanes 1):
+        # about 900 workgroups per launch colour, more than two per compute unit
+        xyz, tets = xf.tet_block(300, 8, 4, (0.01, 0.01, 0.01), (0.0, 0.0, 0.0))
+        wl = dict(wl, xyz=xyz, tets=tets, pinned=(xyz.reshape(-1, 3)[:, 0] == 0).astype(np.uint8),
+                  velocity=None, tet_colour=None)
+        opts["block_lanes"] = 1
+    x, v, order, _, info, _, _ = run(ctx, wl, frames=1, **opts)
+    want = 48 if case == "wide" else 32
+    assert info["block_node_stride"] == want, info
+    if case == "many_slots":
+        assert info["n_block_slots"] > 4 * 2 * 256
+    ob = oracle(wl, order, wl["substeps"])
+    assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)

@@ -750,23 +750,51 @@ __device__ __forceinline__ BlockTet load_block_tet(uint32_t i, const uint2* __re
     return t;
 }
 
+// A slot's nodes in LDS, node k at byte k * NS: (x, y) in its first 16
+// bytes, (z, w) in the next. NS = 32 packs them; NS = 48 spreads their
+// 16-byte halves over every bank group of a 128-byte LDS row (with 32 only
+// every other group is reached), so the four-node gathers of a phase meet
+// fewer bank conflicts, for half as much LDS again.
+template <uint32_t NS>
+struct NodeLds {
+    static_assert(NS == 32u || NS == 48u, "32 or 48 bytes per node");
+    double2* base;  // 16-byte units
+    __device__ __forceinline__ double2* at(uint32_t k) const { return base + k * (NS / 16u); }
+    __device__ __forceinline__ double4 get(uint32_t k) const {
+        const double2* p = at(k);
+        const double2 a = p[0], b = p[1];
+        return make_double4(a.x, a.y, b.x, b.y);
+    }
+    __device__ __forceinline__ void put(uint32_t k, double4 v) const {
+        double2* p = at(k);
+        p[0] = make_double2(v.x, v.y);
+        p[1] = make_double2(v.z, v.w);
+    }
+    // position only (w, the inverse mass, is not changed by a projection)
+    __device__ __forceinline__ void put_xyz(uint32_t k, double x, double y, double z) const {
+        double2* p = at(k);
+        p[0] = make_double2(x, y);
+        p[1].x = z;
+    }
+};
+
 // One tet of a block phase: nodes from LDS, element data loaded, solved,
 // nodes written back to LDS.
-template <int E, bool SIM, bool AUX, bool DAMP>
-__device__ __forceinline__ void block_tet(const BlockTet& t, uint32_t i, double4* xs, const uint4* __restrict__ tidx,
+template <int E, bool SIM, bool AUX, bool DAMP, uint32_t NS>
+__device__ __forceinline__ void block_tet(const BlockTet& t, uint32_t i, NodeLds<NS> xs, const uint4* __restrict__ tidx,
                                           const double4* __restrict__ O, const XfBodyParams* __restrict__ bodies,
                                           float comp1_u, float a_u, float dt) {
     uint32_t ia = t.loc.x & 0xFFFFu, ib = t.loc.x >> 16, ic = t.loc.y & 0xFFFFu, id = t.loc.y >> 16;
-    double4 na = xs[ia], nb = xs[ib], nc = xs[ic], nd = xs[id];
+    double4 na = xs.get(ia), nb = xs.get(ib), nc = xs.get(ic), nd = xs.get(id);
     double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
     float w[4] = {(float)na.w, (float)nb.w, (float)nc.w, (float)nd.w};
     uint4 gid = make_uint4(0u, 0u, 0u, 0u);
     if constexpr (DAMP) gid = tidx[i];
     project_loaded<E, SIM, AUX, DAMP>(t.q0, t.q1, t.q2, t.q3, t.ax, gid, x, w, O, bodies, comp1_u, a_u, dt);
-    xs[ia] = make_double4(x[0][0], x[0][1], x[0][2], na.w);
-    xs[ib] = make_double4(x[1][0], x[1][1], x[1][2], nb.w);
-    xs[ic] = make_double4(x[2][0], x[2][1], x[2][2], nc.w);
-    xs[id] = make_double4(x[3][0], x[3][1], x[3][2], nd.w);
+    xs.put_xyz(ia, x[0][0], x[0][1], x[0][2]);
+    xs.put_xyz(ib, x[1][0], x[1][1], x[1][2]);
+    xs.put_xyz(ic, x[2][0], x[2][1], x[2][2]);
+    xs.put_xyz(id, x[3][0], x[3][1], x[3][2]);
 }
 
 // Value of v in lane L of this lane's quad (lanes 4k .. 4k + 3), by DPP
@@ -786,12 +814,12 @@ __device__ __forceinline__ float quad_lane(float v) {
 // node order from the other lanes by DPP. Results equal block_tet bit for
 // bit.
 template <int E, bool AUX>
-__device__ __forceinline__ void block_tet_quad(const BlockTet& t, uint32_t q, double4* xs,
+__device__ __forceinline__ void block_tet_quad(const BlockTet& t, uint32_t q, NodeLds<32> xs,
                                                const XfBodyParams* __restrict__ bodies, float comp1_u, float a_u,
                                                float dt) {
     static_assert(E == 0 || E == 4 || E == 7, "quad lanes take MixedSel / YeohSkinFast");
     const uint32_t ia = t.loc.x & 0xFFFFu, ib = t.loc.x >> 16, ic = t.loc.y & 0xFFFFu, id = t.loc.y >> 16;
-    const double4 na = xs[ia], nb = xs[ib], nc = xs[ic], nd = xs[id];
+    const double4 na = xs.get(ia), nb = xs.get(ib), nc = xs.get(ic), nd = xs.get(id);
     const double x[4][3] = {{na.x, na.y, na.z}, {nb.x, nb.y, nb.z}, {nc.x, nc.y, nc.z}, {nd.x, nd.y, nd.z}};
     const bool q0 = q == 0u, q1 = q == 1u, q2 = q == 2u, q3 = q == 3u;
     const uint32_t own = q0 ? ia : (q1 ? ib : (q2 ? ic : id));
@@ -917,7 +945,7 @@ __device__ __forceinline__ void block_tet_quad(const BlockTet& t, uint32_t q, do
         acc = acc + l1 * g1[r];
         xn[r] = xn[r] + (double)(w * acc);
     }
-    xs[own] = make_double4(xn[0], xn[1], xn[2], xo.w);
+    xs.put_xyz(own, xn[0], xn[1], xn[2]);
 }
 
 // A slot's nodes into LDS (stage_nodes) and back (write_nodes), kStage list
@@ -927,8 +955,9 @@ __device__ __forceinline__ void block_tet_quad(const BlockTet& t, uint32_t q, do
 // size, where one entry per iteration waits for each id before its node.
 constexpr uint32_t kStage = 4;
 
+template <uint32_t NS>
 __device__ __forceinline__ void stage_nodes(uint32_t tid, uint32_t bs, uint32_t n, const uint32_t* __restrict__ bn,
-                                            const double4* __restrict__ X, double4* xs) {
+                                            const double4* __restrict__ X, NodeLds<NS> xs) {
     for (uint32_t k0 = tid; k0 < n; k0 += kStage * bs) {
         uint32_t g[kStage];
 #pragma unroll
@@ -939,13 +968,14 @@ __device__ __forceinline__ void stage_nodes(uint32_t tid, uint32_t bs, uint32_t 
 #pragma unroll
         for (uint32_t u = 0; u < kStage; u++) {
             const uint32_t k = k0 + u * bs;
-            if (k < n && g[u] != xf::kNoNode) xs[k] = v[u];
+            if (k < n && g[u] != xf::kNoNode) xs.put(k, v[u]);
         }
     }
 }
 
+template <uint32_t NS>
 __device__ __forceinline__ void write_nodes(uint32_t tid, uint32_t bs, uint32_t n, const uint32_t* __restrict__ bn,
-                                            double4* __restrict__ X, const double4* xs) {
+                                            double4* __restrict__ X, NodeLds<NS> xs) {
     for (uint32_t k0 = tid; k0 < n; k0 += kStage * bs) {
         uint32_t g[kStage];
 #pragma unroll
@@ -953,7 +983,7 @@ __device__ __forceinline__ void write_nodes(uint32_t tid, uint32_t bs, uint32_t 
 #pragma unroll
         for (uint32_t u = 0; u < kStage; u++) {
             const uint32_t k = k0 + u * bs;
-            if (k < n && g[u] != xf::kNoNode) X[g[u]] = xs[k];
+            if (k < n && g[u] != xf::kNoNode) X[g[u]] = xs.get(k);
         }
     }
 }
@@ -973,7 +1003,7 @@ __device__ __forceinline__ void write_nodes(uint32_t tid, uint32_t bs, uint32_t 
 // those nodes, and O is not read by the projection (FIN excludes DAMP).
 // tend: the clock, read for the end time of the substep (advanced by an
 // earlier launch). fsingle: the scene has one body.
-template <int E, bool SIM, bool AUX, bool DAMP, bool OVER, bool FIN, bool QUAD = false>
+template <int E, bool SIM, bool AUX, bool DAMP, bool OVER, bool FIN, bool QUAD = false, uint32_t NS = 32>
 __global__ __launch_bounds__(256) void xf_project_blocks(
     uint32_t nph, uint32_t nmax, uint32_t ntet, const uint32_t* __restrict__ rec, const uint32_t* __restrict__ bnode,
     const uint2* __restrict__ tloc, const uint4* __restrict__ tidx, const float4* __restrict__ tg0,
@@ -984,6 +1014,7 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     static_assert(!(FIN && (DAMP || E == 30)), "the fused node phase needs a projection that leaves O unread");
     static_assert(!QUAD || (SIM && !DAMP && (E == 0 || E == 4 || E == 7)),
                   "quad lanes take the simultaneous MixedSel / YeohSkinFast projection without damping");
+    static_assert(NS == 32u || !QUAD, "quad lanes take 32-byte nodes");
     if (clock != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
         clock[0] = clock[0] + dt;
         uint32_t* tick = reinterpret_cast<uint32_t*>(clock) + 1;
@@ -991,15 +1022,15 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     }
     if constexpr (E == 30) return;  // CubeNeo / CubeSkin: no constraint on tets (Fem.cpp:567-760)
     extern __shared__ double4 xf_block_nodes[];
-    double4* xs = xf_block_nodes;
+    const NodeLds<NS> xs{reinterpret_cast<double2*>(xf_block_nodes)};
     const uint32_t tid = threadIdx.x, bs = blockDim.x, last = ntet - 1u;
     const uint32_t* r = rec + (size_t)blockIdx.x * 2 * nph;
     const uint32_t* bn = bnode + (size_t)blockIdx.x * nmax;
     const uint32_t jl = nph - 1u;
-    // FIN: LDS holds ftot nodes (the block nodes, then the extra nodes), then
-    // their O, then their ids
+    // FIN: LDS holds ftot nodes (the block nodes, then the extra nodes, NS
+    // bytes each), then their O, then their ids
     const uint32_t ftot = FIN ? nmax + fmax : 0u;
-    double4* os = FIN ? xs + ftot : nullptr;
+    double4* os = FIN ? reinterpret_cast<double4*>(xs.base + (NS / 16u) * ftot) : nullptr;
     uint32_t* fs = FIN ? reinterpret_cast<uint32_t*>(os + ftot) : nullptr;
     // QUAD: four lanes per tet (lane tid & 3 owns node tid & 3), bs / 4 tets
     // at a time; else one lane per tet
@@ -1009,7 +1040,7 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
     };
     auto solve = [&](const BlockTet& t, uint32_t i) {
         if constexpr (QUAD) block_tet_quad<E, AUX>(t, tid & 3u, xs, bodies, comp1_u, a_u, dt);
-        else block_tet<E, SIM, AUX, DAMP>(t, i, xs, tidx, O, bodies, comp1_u, a_u, dt);
+        else block_tet<E, SIM, AUX, DAMP, NS>(t, i, xs, tidx, O, bodies, comp1_u, a_u, dt);
     };
     // each lane's first tet of the phase is t; with OVER, lanes go on to the
     // tets beyond the workgroup's lanes
@@ -1029,7 +1060,7 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
             uint32_t g = fl[k];
             fs[k] = g;
             if (g != xf::kNoNode) {
-                xs[k] = X[g];
+                xs.put(k, X[g]);
                 os[k] = O[g];
             }
         }
@@ -1053,7 +1084,7 @@ __global__ __launch_bounds__(256) void xf_project_blocks(
                 uint32_t g = fs[k];
                 if (g != xf::kNoNode) {
                     double4 oi = os[k];
-                    finish_predict(xs[k], oi, g, X, O, body_of((uint32_t)oi.w), t_end);
+                    finish_predict(xs.get(k), oi, g, X, O, body_of((uint32_t)oi.w), t_end);
                 }
             }
         };
@@ -1585,6 +1616,21 @@ XfBlocksFn pick_blocks_ad(bool aux, bool damp, bool fin) {
         return damp ? xf_project_blocks<E, SIM, true, true, OVER, false> : xf_project_blocks<E, SIM, true, false, OVER, false>;
     return damp ? xf_project_blocks<E, SIM, false, true, OVER, false> : xf_project_blocks<E, SIM, false, false, OVER, false>;
 }
+// 48-byte nodes: the simultaneous MixedSel / YeohSkinFast projection without
+// damping, one lane per tet, no phase beyond the workgroup's lanes; else
+// nullptr (the launch keeps 32-byte nodes)
+template <int E>
+XfBlocksFn pick_blocks_wide(bool sim, bool aux, bool damp, bool over, bool fin, bool quad) {
+    if constexpr (E == 0 || E == 4 || E == 7) {
+        if (!sim || damp || over || quad) return nullptr;
+        if (fin)
+            return aux ? xf_project_blocks<E, true, true, false, false, true, false, 48>
+                       : xf_project_blocks<E, true, false, false, false, true, false, 48>;
+        return aux ? xf_project_blocks<E, true, true, false, false, false, false, 48>
+                   : xf_project_blocks<E, true, false, false, false, false, false, 48>;
+    }
+    return nullptr;
+}
 template <int E>
 XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over, bool fin, bool quad) {
     if (quad) {
@@ -1598,15 +1644,18 @@ XfBlocksFn pick_blocks(bool sim, bool aux, bool damp, bool over, bool fin, bool 
 }  // namespace
 
 #ifndef XF_BLOCKS_ONLY
-// LDS bytes of a sweep-block workgroup: its nodes in FP64 (32 B each).
-extern "C" size_t xf_block_lds_bytes(uint32_t nmax) { return 32 * (size_t)nmax; }
+// LDS bytes of a sweep-block workgroup: its nodes in FP64, ns (32 or 48)
+// bytes each.
+extern "C" size_t xf_block_lds_bytes(uint32_t nmax, uint32_t ns) { return (size_t)ns * nmax; }
 
 // LDS bytes of a workgroup of the launch that ends the substep: its nodes and
 // extra nodes, each with O (32 B) and id (4 B).
-extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax) { return 68 * ((size_t)nmax + fmax); }
+extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax, uint32_t ns) {
+    return ((size_t)ns + 36) * ((size_t)nmax + fmax);
+}
 #else
-extern "C" size_t xf_block_lds_bytes(uint32_t nmax);
-extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax);
+extern "C" size_t xf_block_lds_bytes(uint32_t nmax, uint32_t ns);
+extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax, uint32_t ns);
 #endif
 
 // Host-callable: one sweep-block launch (nslot workgroups of `threads`);
@@ -1618,7 +1667,10 @@ extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax);
 // four lanes per tet (simultaneous MixedSel / YeohSkinFast without damping,
 // energies 0, 4 and 7; anything else is hipErrorInvalidValue). lds_limit:
 // the device's LDS bytes per workgroup (more than 64 KiB is granted to the
-// kernel with hipFuncAttributeMaxDynamicSharedMemorySize).
+// kernel with hipFuncAttributeMaxDynamicSharedMemorySize). wide: the launch
+// may keep 48 bytes per node in LDS (NodeLds) where a 48-byte kernel exists
+// for it and its LDS fits; *nstride (if not null) receives the bytes per
+// node used.
 extern "C" hipError_t XF_BLOCKS_LAUNCH(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
                                        uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
                                        const uint32_t* rec,
@@ -1627,10 +1679,22 @@ extern "C" hipError_t XF_BLOCKS_LAUNCH(hipStream_t s, int energy, bool sim, bool
                                        const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
-                                       const float* tend, bool single_body, bool quad, size_t lds_limit) {
+                                       const float* tend, bool single_body, bool quad, size_t lds_limit,
+                                       bool wide, uint32_t* nstride) {
     XfBlocksFn fn = nullptr;
     const bool over = pmax > (quad ? threads / 4 : threads);
-    switch (energy) {
+    uint32_t ns = 32;
+    if (wide && (fin ? xf_block_fin_lds_bytes(nmax, fmax, 48) : xf_block_lds_bytes(nmax, 48)) <= lds_limit) {
+        switch (energy) {
+            case 0: fn = pick_blocks_wide<0>(sim, aux, damp, over, fin, quad); break;
+            case 4: fn = pick_blocks_wide<4>(sim, aux, damp, over, fin, quad); break;
+            case 7: fn = pick_blocks_wide<7>(sim, aux, damp, over, fin, quad); break;
+            default: break;
+        }
+        if (fn != nullptr) ns = 48;
+    }
+    if (nstride != nullptr) *nstride = ns;
+    if (fn == nullptr) switch (energy) {
         case 0: fn = pick_blocks<0>(sim, aux, damp, over, fin, quad); break;
         case 3: fn = pick_blocks<3>(sim, aux, damp, over, fin, quad); break;
         case 4: fn = pick_blocks<4>(sim, aux, damp, over, fin, quad); break;
@@ -1642,7 +1706,7 @@ extern "C" hipError_t XF_BLOCKS_LAUNCH(hipStream_t s, int energy, bool sim, bool
         case 30: fn = pick_blocks<30>(sim, aux, damp, over, fin, quad); break;
         default: return hipErrorInvalidValue;
     }
-    size_t lds = fin ? xf_block_fin_lds_bytes(nmax, fmax) : xf_block_lds_bytes(nmax);
+    size_t lds = fin ? xf_block_fin_lds_bytes(nmax, fmax, ns) : xf_block_lds_bytes(nmax, ns);
     if (fn == nullptr || threads == 0 || threads > 256 || threads % 64 || nph == 0 || ntet == 0 || lds > lds_limit)
         return hipErrorInvalidValue;
     if (fin && (tend == nullptr || clock != nullptr || fextra == nullptr)) return hipErrorInvalidValue;

@@ -46,8 +46,8 @@ extern "C" hipError_t xf_launch_rings(hipStream_t s, bool post, uint32_t lanes, 
                                       const double4* O, double4* V, const XfBodyParams* bodies, float dt,
                                       float* clock);
 extern "C" size_t xf_ring_team_bytes(uint32_t nmax, uint32_t tmax);
-extern "C" size_t xf_block_lds_bytes(uint32_t nmax);
-extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax);
+extern "C" size_t xf_block_lds_bytes(uint32_t nmax, uint32_t ns);
+extern "C" size_t xf_block_fin_lds_bytes(uint32_t nmax, uint32_t fmax, uint32_t ns);
 extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool aux, bool damp, uint32_t threads,
                                        uint32_t nslot, uint32_t nph, uint32_t nmax, uint32_t ntet, uint32_t pmax,
                                        const uint32_t* rec,
@@ -56,7 +56,8 @@ extern "C" hipError_t xf_launch_blocks(hipStream_t s, int energy, bool sim, bool
                                        const float4* taux, double4* X, double4* O,
                                        const XfBodyParams* bodies, float comp1_u, float a_u, float dt,
                                        float* clock, bool fin, const uint32_t* fextra, uint32_t fmax,
-                                       const float* tend, bool single_body, bool quad, size_t lds_limit);
+                                       const float* tend, bool single_body, bool quad, size_t lds_limit,
+                                       bool wide, uint32_t* nstride);
 // the same launch with the kernels compiled with multiply-add contraction
 // (xf_blocks_contract.hip)
 extern "C" hipError_t xf_launch_blocks_contract(
@@ -64,7 +65,8 @@ extern "C" hipError_t xf_launch_blocks_contract(
     uint32_t nmax, uint32_t ntet, uint32_t pmax, const uint32_t* rec, const uint32_t* bnode, const uint2* tloc,
     const uint4* tidx, const float4* tg0, const float4* tg1, const float4* tg2, const float4* tg3, const float4* taux,
     double4* X, double4* O, const XfBodyParams* bodies, float comp1_u, float a_u, float dt, float* clock, bool fin,
-    const uint32_t* fextra, uint32_t fmax, const float* tend, bool single_body, bool quad, size_t lds_limit);
+    const uint32_t* fextra, uint32_t fmax, const float* tend, bool single_body, bool quad, size_t lds_limit,
+    bool wide, uint32_t* nstride);
 extern "C" hipError_t xf_launch_predict(hipStream_t s, uint32_t n, double4* X, double4* O, const double4* V,
                                         const XfBodyParams* bodies);
 extern "C" hipError_t xf_launch_finish(hipStream_t s, bool predict_next, bool single_body, uint32_t n, double4* X,
@@ -327,6 +329,11 @@ struct xf_scene {
     // blocks touches
     int blk_last = -1;
     uint32_t blk_fmax = 0;
+    // 48 bytes per slot node in LDS allowed (fewer bank conflicts): every
+    // block launch has at most two workgroups per compute unit, so LDS is not
+    // what limits them; blk_nstride: the bytes per node of the last launch
+    bool blk_wide = false;
+    uint32_t blk_nstride = 32;
     uint32_t* blk_fin = nullptr;
     uint2* tloc = nullptr;
     uint32_t* tout = nullptr;  // per schedule position: the tet's slot in body-concatenated, body-local order
@@ -828,7 +835,7 @@ int Enqueue(xf_scene* s, uint32_t substeps, float dt, int energy, bool sim, Laun
                                s->tg[0], s->tg[1], s->tg[2], s->tg[3], s->taux, s->X, s->O, s->bp, s->comp1_u,
                                s->a_u, dt, first ? s->clock : nullptr, fin && q == s->blk_last, s->blk_fin,
                                s->blk_fmax, s->clock, s->bodies.size() == 1, s->opts.tet_lanes == 4,
-                               s->lds_limit));
+                               s->lds_limit, s->blk_wide, &s->blk_nstride));
             first = false;
             if (timer) {
                 XF_HIP(ctx, hipEventRecord(e1, st));
@@ -1114,7 +1121,7 @@ int PackBlocks(xf_scene* s, uint32_t lanes_opt, BlockTables* bt) {
             if (!covered[v]) extra.push_back(v);
         s->blk_fmax = (uint32_t)((extra.size() + nslot[q] - 1) / nslot[q]);
         // staged with O and ids in LDS; else the node phase stays separate
-        if (xf_block_fin_lds_bytes(s->blk_nmax, s->blk_fmax) > s->lds_limit) {
+        if (xf_block_fin_lds_bytes(s->blk_nmax, s->blk_fmax, 32) > s->lds_limit) {
             s->blk_last = -1;
             s->blk_fmax = 0;
         }
@@ -1131,6 +1138,9 @@ int PackBlocks(xf_scene* s, uint32_t lanes_opt, BlockTables* bt) {
                 bt->fin[b * row + s->blk_nmax + j] = extra[(size_t)b * fmax + j];
         }
     }
+    s->blk_wide = true;
+    for (int q = 0; q < 4; q++)
+        if (nslot[q] > 2 * (uint32_t)cus) s->blk_wide = false;
     for (int q = 0; q < 4; q++) {
         xf_scene::BlockLaunch& L = s->blk[q];
         L.nslot = nslot[q];
@@ -1646,6 +1656,7 @@ XF_API int xf_scene_info_get(const xf_scene* s, xf_scene_info* info) {
     info->fused_finish = fused ? 1u : 0u;
     info->tet_lanes = s->opts.tet_lanes;
     info->block_lds_limit = (uint32_t)s->lds_limit;
+    info->block_node_stride = s->blk_nstride;
     info->contract = s->opts.contract && s->opts.launch_mode == XF_LAUNCH_BLOCKS ? 1u : 0u;
     return XF_OK;
 }

@@ -86,7 +86,7 @@ class SceneInfo(C.Structure):
                 ("n_ring_colours", C.c_uint32), ("n_blocks", C.c_uint32), ("block_launches", C.c_uint32),
                 ("n_block_nodes", C.c_uint32), ("block_lanes", C.c_uint32), ("n_block_slots", C.c_uint32),
                 ("fused_finish", C.c_uint32), ("tet_lanes", C.c_uint32),
-                ("block_lds_limit", C.c_uint32), ("contract", C.c_uint32)]
+                ("block_lds_limit", C.c_uint32), ("contract", C.c_uint32), ("block_node_stride", C.c_uint32)]
 
 
 _lib = None
@@ -467,7 +467,7 @@ class Scene:
                     n_blocks=i.n_blocks, block_launches=i.block_launches, n_block_nodes=i.n_block_nodes,
                     block_lanes=i.block_lanes, n_block_slots=i.n_block_slots,
                     fused_finish=i.fused_finish, tet_lanes=i.tet_lanes, block_lds_limit=i.block_lds_limit,
-                    contract=i.contract)
+                    contract=i.contract, block_node_stride=i.block_node_stride)
 
     def set_settings(self, body, settings=None, **kw):
         s = settings if settings is not None else default_settings(**kw)
