@@ -784,10 +784,11 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             rec = json.load(f)
-        if mode == xf.LAUNCH_BLOCKS and "blocks" in rec and rec["blocks"].get("n_blocks") == info["n_blocks"] and \
-                rec["blocks"].get("block_size") == size and info["tet_lanes"] == 1 and \
-                rec["blocks"].get("block_lanes", 64) == info["block_lanes"]:
-            traffic = float(rec["blocks"]["traffic_bytes_per_dispatch"])
+        shapes = rec.get("blocks_shapes", [])
+        match = [b for b in shapes if b.get("n_blocks") == info["n_blocks"] and b.get("block_size") == size and
+                 b.get("block_lanes") == info["block_lanes"]]
+        if mode == xf.LAUNCH_BLOCKS and match and info["tet_lanes"] == 1:
+            traffic = float(match[0]["traffic_bytes_per_dispatch"])
         elif mode == xf.LAUNCH_PER_COLOUR and args.schedule != "blocks" and \
                 rec.get("tets_per_dispatch") == T // max(info["n_colours"], 1):
             traffic = float(rec["traffic_bytes_per_dispatch"] if args.schedule == "slabs" else
