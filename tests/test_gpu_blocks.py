@@ -136,6 +136,47 @@ def test_blocks_big_beam_bit_exact(ctx):
     assert np.array_equal(x, ob.X) and np.array_equal(v, ob.V)
 
 
+def test_blocks_large_beam_equals_per_colour(ctx):
+    """The 288x72x72 Yeoh beam (8,957,952 tets), the throughput-bound shape
+    the bench's large leg runs: 256-lane slots on 256-thread workgroups, more
+    than two per compute unit (32-byte LDS nodes, the node phase in its own
+    launch). One frame of block launches equals one frame of per-colour
+    launches over the same colouring, bit for bit, and stays finite (the
+    oracle would take minutes at this size; the 96x24x24 test pins the
+    colouring's serial sweep against it)."""
+    wl = workloads.beam_yeoh(288, 72, 72)
+    a = run(ctx, wl, frames=1, block_size=256, block_lanes=256)
+    b = run(ctx, wl, frames=1, block_size=256, block_lanes=256, launch_mode=xf.LAUNCH_PER_COLOUR)
+    info = a[4]
+    assert info["n_tets"] == 288 * 72 * 72 * 6 and info["n_colours"] == 24
+    assert info["block_node_stride"] == 32 and info["fused_finish"] == 0
+    assert info["n_block_slots"] > 4 * 2 * 256
+    assert np.array_equal(a[2], b[2])
+    assert np.all(np.isfinite(a[0])) and np.all(np.isfinite(a[1]))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_blocks_batch256_equals_per_colour(ctx):
+    """The 256-box batch the bench's batched leg runs (3,790,950 tets, MixedSel
+    and YeohSkinFast, per-tet material slots): one frame of block launches
+    (128-lane slots, 256-thread workgroups) equals per-colour launches over
+    the same colouring for every body, bit for bit."""
+    b = workloads.boxes_batch()
+    dt, S = float(b["dt"]), b["substeps"]
+    states = []
+    for mode in (xf.LAUNCH_BLOCKS, xf.LAUNCH_PER_COLOUR):
+        sc = workloads.make_batch_scene(ctx, b, sweep_blocks=True, launch_mode=mode, block_size=256,
+                                        block_lanes=128)
+        sc.step(S, dt)
+        states.append([sc.get_state(i) for i in range(len(b["bodies"]))] + [sc.order(0)[0]])
+        sc.close()
+    assert np.array_equal(states[0][-1], states[1][-1])
+    for i in range(len(b["bodies"])):
+        (xa, va), (xb, vb) = states[0][i], states[1][i]
+        assert np.all(np.isfinite(xa))
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb), i
+
+
 @pytest.mark.parametrize("energy", [7, 4])
 def test_blocks_fused_node_phase(ctx, energy):
     """The end of each substep (pins, ground and moving-sphere contact with
