@@ -88,16 +88,18 @@ def test_contract_within_tolerance(ctx, energy, block_size):
     assert np.all(np.isfinite(v))
 
 
-def test_contract_bench_beam(ctx):
+@pytest.mark.parametrize("block_size", [128, 256])
+def test_contract_bench_beam(ctx, block_size):
     """The 96x24x24 Yeoh beam with ground and moving sphere (raised 5 cm so
-    contacts start at once), 256-thread workgroups, graph replay, one frame."""
+    contacts start at once), the bench's workgroup sizes, graph replay, one
+    frame."""
     wl = workloads.beam_yeoh_96x24x24()
     s = dict(wl["settings"])
     s["sphere_c0"] = (s["sphere_c0"][0], s["sphere_c0"][1] + 0.05, s["sphere_c0"][2])
     wl["settings"] = s
-    x0, x, _, order, info = run(ctx, wl, 1, block_size=256, use_graph=True, contract=True)
+    x0, x, _, order, info = run(ctx, wl, 1, block_size=block_size, use_graph=True, contract=True)
     assert info["contract"] == 1 and info["n_colours"] == 24
-    check(wl, x, x0, order, wl["substeps"], "96x24x24, one frame")
+    check(wl, x, x0, order, wl["substeps"], f"96x24x24, {block_size} threads, one frame")
 
 
 def test_contract_batch(ctx):
